@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Benchmark of the FP64 matrix-free PolyDG leapfrog step on one MI355X.
+
+Workload (default): the unit cube split into 40^3 cubes x 6 tets (384,000
+tets), each cube's 6 tets agglomerated into one polyhedron (64,000
+polyhedra), p = 3 (Np = 20, 3Np = 60 -> 3,840,000 DOFs), standing-wave data
+u_i = sin(pi x)sin(pi y)sin(pi z)cos(sqrt3 pi t), Dirichlet on all faces,
+rho = lambda = mu = 1, SIPG alpha = 10, dt = 0.2 h / (c_p p^2) with
+c_p = sqrt(3), 1000 timed leapfrog steps.  Synthetic mesh built in-process.
+
+One "step" = one leapfrog step over the whole mesh (one launch of the fused
+step kernel).  value = DOF-updates/s = 3 Np N_poly / t_step with all data
+resident in HBM.  The timed region is bracketed by device synchronisation
+and measured with HIP events recorded on the stream the steps run on.
+
+roofline: algorithmic bytes per step B = 8 [n_blk (3Np)^2 + N_poly Np(Np+1)/2
++ 5 N_poly 3Np] (operator blocks, packed mass factor, five vectors) divided by
+the average step-kernel time, against the MI355X HBM peak of 8 TB/s.
+cpu_baseline: the C oracle (a scalar restatement of the reference's
+algorithm, OpenMP over polyhedra) timed on this host's cores on a bounded
+sample of the same workload (a smaller cube of the same structure).
+
+Usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--n 40] [--p 3]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "DOF-updates/s + %HBM roofline at p=3; ensemble solves/s vs plain at fixed L2"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def stream_copy_gbps():
+    path = os.path.join(ROOT, "lymph3d_apsc_amd", "liblymph_bwprobe.so")
+    if not os.path.exists(path):
+        return None
+    L = C.CDLL(path)
+    L.lymph_bw_copy.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    g, ms = C.c_double(), C.c_double()
+    if L.lymph_bw_copy(2 << 30, 20, C.byref(g), C.byref(ms)) != 0:
+        return None
+    return g.value
+
+
+def cpu_baseline(p, n_sample, target_s):
+    """Time the oracle's leapfrog step on an n_sample^3-cube sample of the
+    same structure, using the host's cores (OpenMP)."""
+    from lymph3d_apsc_amd import mesh as M
+    from oracle import oracle
+    m = M.cube_mesh(n_sample, agglomerate=True)
+    kinds = np.zeros(8, np.int32)
+    kinds[[2, 3, 4]] = -1
+    kinds[[5, 6, 7]] = -2
+    space = np.zeros(8, np.int32)
+    space[2] = space[5] = 1
+    threads = min(16, os.cpu_count() or 1)
+    oracle.lib().o_num_threads(threads)
+    o = oracle.Oracle(m.xyz, m.tet_vert, m.tria_vert, m.tria_tag, kinds, space, p,
+                      tet_poly=m.tet_poly)
+    o.build_operator()
+    rs = o.rhs(oracle.PROB_NULL)
+    rd = o.rhs(oracle.PROB_CUBE_WAVE)
+    u0 = o.project(oracle.FN_CUBE_U)
+    u1 = u0.copy()
+    dt = 0.2 * (1.0 / n_sample) / (np.sqrt(3.0) * p * p)
+    # one untimed step, then as many as fit in target_s
+    u2 = o.step(0.5, dt, u0, u1, rs, rd)
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        u0, u1 = u1, o.step(0.5, dt, u0, u1, rs, rd)
+        k += 1
+        el = time.perf_counter() - t0
+        if el >= target_s:
+            break
+    dofs = 3 * o.Np * o.n_poly
+    return dict(value=dofs * k / el, unit="DOF-updates/s", cores=threads, kind="port",
+                sample=f"C oracle, {n_sample}^3 cubes x 6 tets agglomerated per cube "
+                       f"({o.n_poly} polyhedra, p={p}, {dofs} DOFs), {k} FP64 leapfrog steps "
+                       f"in {el:.1f} s on {threads} OpenMP threads", steps=k, seconds=el,
+                _unused=float(u2[0]))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=40, help="cubes per side")
+    ap.add_argument("--p", type=int, default=3)
+    ap.add_argument("--cpu-sample", type=int, default=12, help="cubes per side of the CPU sample")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs on one GPU only: the whole mesh lives on one "
+                                   "MI355X and nothing is sharded; run with --gpus 1"}))
+        return 2
+
+    import lymph3d_apsc_amd as L
+    from lymph3d_apsc_amd import _lib, mesh as M
+
+    if L.device_count() == 0:
+        print(json.dumps({"error": "no HIP device visible"}))
+        return 1
+    t_setup = time.perf_counter()
+    m = M.cube_mesh(a.n, agglomerate=True)
+    log(f"mesh: {m.n_tet} tets, {m.n_poly} polyhedra ({time.perf_counter() - t_setup:.1f}s)")
+    p = a.p
+    h = 1.0 / a.n
+    dt = 0.2 * h / (np.sqrt(3.0) * p * p)
+    prob = L.PROB_CUBE_WAVE
+    ctx = L.Context(m, p)
+    t0 = time.perf_counter()
+    # operator, RHS, IC projections, first step and the warm-up steps
+    r0 = ctx.run(prob, dt, 1e30, max_steps=1 + a.warmup)
+    t_build = time.perf_counter() - t0
+    log(f"setup + {1 + a.warmup} steps: {t_build:.2f}s; device bytes {ctx.device_bytes / 1e9:.2f} GB")
+    t_start = r0["t_final"]
+    # timed region: K device-resident steps, HIP events on the context stream
+    w0 = time.perf_counter()
+    ms = ctx.time_steps(prob, a.steps, t_start, dt)
+    wall = time.perf_counter() - w0
+    # final time and errors vs the projected exact solution
+    t = t_start
+    for _ in range(a.steps):
+        t = t + dt
+    u = ctx.get_slot(_lib.SLOT_U_CUR)
+    uex = ctx.compute_modal_coefficients_free(L.FN_CUBE_U) * L.time_function(prob, t)
+    e_l2 = np.sqrt(ctx.compute_error_l2_matrix_free(u, uex))
+    e_dg = np.sqrt(ctx.compute_error_dg_matrix_free(u, uex))
+    ms_step = ms / a.steps
+    dofs = 3 * ctx.Np * ctx.n_poly
+    value = dofs / (ms_step * 1e-3)
+    B = ctx.step_bytes
+    achieved = B / (ms_step * 1e-3) / 1e9
+    copy = stream_copy_gbps()
+    res = {
+        "metric": METRIC, "value": value, "unit": "DOF-updates/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {
+            "workload": f"unit cube {a.n}^3 cubes x 6 tets ({m.n_tet} tets) agglomerated per cube "
+                        f"to {m.n_poly} polyhedra, p={p} ({dofs} DOFs), FP64 matrix-free PolyDG "
+                        f"leapfrog, SIPG alpha=10, standing-wave data, dt=CFL 0.2 ({dt:.3e})",
+            "n_poly": m.n_poly, "n_tet": m.n_tet, "p": p, "dofs": dofs, "n_blk": ctx.n_blk,
+            "dt": dt, "t_final": t, "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "bytes_per_step": B, "kernel": "k_leapfrog<3,0>",
+            "stream_copy_GBps": copy,
+            "frac_of_stream_copy": (achieved / copy) if copy else None,
+            "fp64_tflops": ctx.step_flops / (ms_step * 1e-3) / 1e12,
+            "fp64_frac": ctx.step_flops / (ms_step * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+        },
+        "wall_ms_per_step": wall * 1e3 / a.steps,
+        "err_l2": e_l2, "err_dg": e_dg,
+        "setup_s": t_build,
+    }
+    del ctx
+    if not a.no_cpu_baseline:
+        try:
+            cb = cpu_baseline(p, a.cpu_sample, a.cpu_seconds)
+            cb.pop("_unused", None)
+            res["cpu_baseline"] = cb
+        except Exception as e:  # reported, never fatal
+            res["cpu_baseline"] = {"error": repr(e)}
+    print(json.dumps(res))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

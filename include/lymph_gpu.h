@@ -1,0 +1,201 @@
+/*
+ * lymph_gpu.h -- C ABI of liblymph_gpu.so, the MI355X (gfx950) implementation
+ * of LYMPH3D's matrix-free PolyDG leapfrog path for 3-D linear elastodynamics.
+ *
+ * The Fortran host keeps its mesh pipeline (READ_INPUT_FILES, partitioning,
+ * agglomeration, normals/bounding boxes) and hands the flattened mesh to
+ * lymph_ctx_create; every numerical routine of the path below then runs as
+ * hand-written HIP kernels on one GPU.  Each entry point cites the reference
+ * routine it replaces (paths relative to the reference repository root).
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success or a nonzero code; codes reuse the
+ *    reference's exit codes (src/Poly_global.f90:130-173) where one exists:
+ *      LYMPH_EXIT_SETUP 4, LYMPH_EXIT_SINGULARMTX 5, LYMPH_EXIT_NEIGHBOUR 18;
+ *    plus LYMPH_ERR_ARG 101 (bad argument), LYMPH_ERR_HIP 102 (HIP failure),
+ *    LYMPH_ERR_STATE 103 (call order).  lymph_last_error() gives a message.
+ *    Nothing aborts across the ABI.
+ *  - All arrays are 0-based.  Solution-like vectors hold 3*Np doubles per
+ *    polyhedron, component-major inside a polyhedron (index comp*Np + mode,
+ *    the reference's (i-1)*Np+m, src/matrix_free.f90:247-252).  Host vectors
+ *    use the layout chosen with lymph_ctx_set_layout: LYMPH_LAYOUT_POLY
+ *    (u[P*3Np + dof], default) or LYMPH_LAYOUT_FORTRAN (the reference's
+ *    column-major u(npoly, 3Np), u[dof*npoly + P]).
+ *  - Ownership: the library owns all device memory of a context for its
+ *    lifetime; host arrays are only read/written during a call.
+ *  - Threading: one host thread per context; calls on one context are not
+ *    re-entrant.  Contexts are independent.  All work of a context runs on its
+ *    HIP stream (its own, or the one given to lymph_ctx_set_stream); entry
+ *    points taking host pointers synchronise that stream before returning,
+ *    lymph_step_n / lymph_*_dev entry points do not.
+ *  - Time and the time factor g(t) are evaluated by the caller in FP64 and
+ *    passed in (the reference's time_function, problem_data :14-23).
+ */
+#ifndef LYMPH_GPU_H
+#define LYMPH_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LYMPH_EXIT_SETUP 4
+#define LYMPH_EXIT_SINGULARMTX 5
+#define LYMPH_EXIT_NEIGHBOUR 18
+#define LYMPH_ERR_ARG 101
+#define LYMPH_ERR_HIP 102
+#define LYMPH_ERR_STATE 103
+
+#define LYMPH_LAYOUT_POLY 0
+#define LYMPH_LAYOUT_FORTRAN 1
+
+/* Built-in data sets replacing MAKE_RHS_FREE's procedure arguments
+ * (src/matrix_free.f90:474-493; src/problem_data_and_properties.f90). */
+#define LYMPH_PROB_NULL 0      /* f_null, gd_null, gn_null */
+#define LYMPH_PROB_REF_MMS 1   /* f_time, gd, gn: the reference's built-in manufactured solution */
+#define LYMPH_PROB_CUBE_WAVE 2 /* u_i = sin(pi x)sin(pi y)sin(pi z) cos(sqrt3 pi t), all components */
+
+/* Fields for lymph_project (COMPUTE_MODAL_COEFFICIENTS_FREE's f_analytic). */
+#define LYMPH_FN_ZERO 0    /* ic_displacement, uex_stat */
+#define LYMPH_FN_REF_VEL 1 /* ic_velocity */
+#define LYMPH_FN_REF_UEX 2 /* uex */
+#define LYMPH_FN_CUBE_U 3  /* sin(pi x)sin(pi y)sin(pi z) on every component */
+
+/* Device-resident vector slots of a context (used by the *_dev / step_n calls). */
+#define LYMPH_SLOT_RHS_STAT 0
+#define LYMPH_SLOT_RHS_DYN 1
+#define LYMPH_SLOT_U_PREV 2 /* u^{n-1} */
+#define LYMPH_SLOT_U_CUR 3  /* u^n     */
+#define LYMPH_SLOT_U_NEXT 4 /* u^{n+1} (scratch for the rotation) */
+#define LYMPH_SLOT_V0 5
+#define LYMPH_SLOT_UEX 6
+#define LYMPH_NUM_SLOTS 7
+
+typedef struct LymphCtx LymphCtx;
+
+/* Flattened mesh: the data contract of the reference's Mesh_Structure
+ * (src/Poly_mesh.f90:13-104) restricted to what the path reads.
+ *  neigh[t*4+f]: neighbouring tet across face f (0-based), or -1 (Dirichlet,
+ *   BC_DIRI) / -2 (Neumann, BC_NEUM) -- Element%neigh_el(face,2);
+ *  faces f=0..3 are the reference's faces 1..4: (v3,v2,v1), (v1,v2,v4),
+ *   (v2,v3,v4), (v3,v1,v4) (src/MAKE_PARTITION_AND_MPI_FILES.f90:1087-1095);
+ *  normal/area as CREATE_NORMAL_FACE (:1429-1504): area = |P x Q| (twice the
+ *   triangle area), bbox/hk of the polyhedron as CREATE_BBOX_EL (:1510-1580). */
+typedef struct LymphMesh {
+  int32_t n_node;
+  int32_t n_tet;
+  int32_t n_poly;
+  const double* xyz;        /* n_node*3 */
+  const int32_t* tet_vert;  /* n_tet*4 */
+  const int32_t* tet_poly;  /* n_tet, or NULL: polyhedron == tet */
+  const int32_t* neigh;     /* n_tet*4 */
+  const int32_t* bc_space;  /* n_tet*4, space_fun_tag of boundary faces */
+  const double* normal;     /* n_tet*12 */
+  const double* area;       /* n_tet*4 */
+  const double* bbox;       /* n_poly*6: xmin,xmax,ymin,ymax,zmin,zmax */
+  const double* hk;         /* n_poly */
+  const double* rho;        /* n_tet */
+  const double* lam;        /* n_tet */
+  const double* mu;         /* n_tet */
+} LymphMesh;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* lymph_last_error(void);
+/* Library version string. */
+const char* lymph_version(void);
+/* Number of visible HIP devices (0 without a GPU); never fails. */
+int lymph_device_count(void);
+
+/* Create a context on the current HIP device: uploads the mesh, builds the
+ * block-row pattern (self block + one block per neighbouring polyhedron) and
+ * the quadrature tables.  Replaces the allocations and set-up of
+ * src/Lymph3D.f90:219-252 (p from Elem_loc(1)%Degree; alpha, theta from
+ * set_properties, src/problem_data_and_properties.f90:304-315). */
+int lymph_ctx_create(const LymphMesh* mesh, int p, double alpha, double theta,
+                     LymphCtx** out);
+int lymph_ctx_destroy(LymphCtx* ctx);
+/* info[0..7] = n_tet, n_poly, Np, n_blk, operator bytes, device bytes total,
+ *              algorithmic bytes of one FP64 leapfrog step, flops per step. */
+int lymph_ctx_info(const LymphCtx* ctx, int64_t* info);
+int lymph_ctx_set_layout(LymphCtx* ctx, int layout);
+/* Run all work on `stream` (a hipStream_t) instead of the context's own. */
+int lymph_ctx_set_stream(LymphCtx* ctx, void* stream);
+void* lymph_ctx_get_stream(const LymphCtx* ctx);
+/* Block pattern: row_ptr[n_poly+1], col[n_blk]. */
+int lymph_get_pattern(const LymphCtx* ctx, int32_t* row_ptr, int32_t* col);
+
+/* MAKE_MATRICES_FREE (src/matrix_free.f90:27-464) followed by the mass-block
+ * Cholesky factorisations (src/Lymph3D.f90:264-271), on the GPU.
+ * Returns LYMPH_EXIT_SINGULARMTX if a mass block is not SPD. */
+int lymph_build_operator(LymphCtx* ctx);
+/* Copy the assembled data to the host for checking: K and A_dg as row-major
+ * 3Np x 3Np blocks (n_blk of them, pattern order); M, M_modal as full Np x Np
+ * per polyhedron; any pointer may be NULL. */
+int lymph_export_operator(const LymphCtx* ctx, double* K, double* A_dg,
+                          double* M, double* M_modal);
+
+/* MAKE_RHS_FREE (src/matrix_free.f90:467-843) for a built-in data set into a
+ * device slot (LYMPH_SLOT_RHS_STAT / _DYN); copied to rhs_out if non-NULL. */
+int lymph_build_rhs(LymphCtx* ctx, int problem, int slot, double* rhs_out);
+/* COMPUTE_MODAL_COEFFICIENTS_FREE (src/matrix_free.f90:847-997): L2
+ * projection of field `fn` into a device slot; copied to out if non-NULL. */
+int lymph_project(LymphCtx* ctx, int fn, int slot, double* out);
+
+/* Host <-> device slot copies (layout as set). */
+int lymph_set_slot(LymphCtx* ctx, int slot, const double* host);
+int lymph_get_slot(LymphCtx* ctx, int slot, double* host);
+/* Raw device pointer of a slot (poly layout), for device-resident callers. */
+double* lymph_slot_ptr(LymphCtx* ctx, int slot);
+
+/* FIRST_TIME_STEP_MATRIX_FREE (src/matrix_free.f90:1100-1227):
+ * u1 = u0 + dt v0 + dt^2/2 M^-1 (F_s + F_d g0 - K u0); host pointers, uses
+ * the RHS slots built before. */
+int lymph_first_step(LymphCtx* ctx, double g0, double dt, const double* u0,
+                     const double* v0, double* u1);
+/* TIME_STEP_MATRIX_FREE (src/matrix_free.f90:1230-1385) -- the drop-in:
+ * u^{n+1} = dt^2 M^-1 (F_s + F_d g - K u^n) + 2u^n - u^{n-1}; host pointers. */
+int lymph_step(LymphCtx* ctx, double g, double dt, const double* um1,
+               const double* u, double* up1);
+/* Same two steps on device pointers (poly layout), asynchronous on the
+ * context stream. */
+int lymph_first_step_dev(LymphCtx* ctx, double g0, double dt, const double* u0,
+                         const double* v0, double* u1);
+int lymph_step_dev(LymphCtx* ctx, double g, double dt, const double* um1,
+                   const double* u, double* up1);
+/* Device-resident time loop body of src/Lymph3D.f90:389-418: `nsteps` steps
+ * starting at time t0 (g(t) = time factor of `problem`, evaluated on the host
+ * per step with t accumulated as t = t + dt) on slots U_PREV/U_CUR, rotating
+ * them; asynchronous.  On return *t_out holds the accumulated time. */
+int lymph_step_n(LymphCtx* ctx, int problem, int nsteps, double t0, double dt,
+                 double* t_out);
+/* Time `nsteps` device-resident steps with HIP events recorded on the context
+ * stream around the launches: *ms = elapsed milliseconds (synchronises). */
+int lymph_time_steps(LymphCtx* ctx, int problem, int nsteps, double t0,
+                     double dt, double* ms);
+
+/* COMPUTE_ERROR_L2_MATRIX_FREE / _DG_ (src/matrix_free.f90:1388-1504):
+ * squared norms of uh - uex; host pointers or slots. */
+int lymph_error_l2(LymphCtx* ctx, const double* uh, const double* uex,
+                   double* err2);
+int lymph_error_dg(LymphCtx* ctx, const double* uh, const double* uex,
+                   double* err2);
+int lymph_error_slots(LymphCtx* ctx, int slot_uh, int slot_uex, double* l2sq,
+                      double* dgsq);
+
+/* Whole path of src/Lymph3D.f90:262-518 on the device: operator, RHS,
+ * IC projections, first step, `while (t <= stop_time)` loop (at most
+ * max_steps steps if max_steps >= 0), exact-solution projection scaled by
+ * g(t_final), errors.  out[0..3] = t_final, steps, err_L2, err_DG (the
+ * square roots printed at src/Lymph3D.f90:516-517); u_final may be NULL. */
+int lymph_run(LymphCtx* ctx, int problem, double dt, double stop_time,
+              int max_steps, double* u_final, double* out);
+
+/* Time factor g(t) of a data set: sin(sqrt2 pi t) for LYMPH_PROB_REF_MMS
+ * (time_function), cos(sqrt3 pi t) for LYMPH_PROB_CUBE_WAVE. */
+double lymph_time_function(int problem, double t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

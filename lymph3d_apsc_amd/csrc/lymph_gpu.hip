@@ -1,0 +1,654 @@
+// lymph_gpu.hip -- context management, host-side block pattern and the C ABI
+// declared in include/lymph_gpu.h.  All numerics run in the kernels of
+// lymph_kernels.hpp; the host only builds index arrays and quadrature tables
+// and evaluates the scalar time factor g(t).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lymph_gpu.h"
+#include "lymph_kernels.hpp"
+
+using namespace lymph;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+  do {                                                                                    \
+    hipError_t _e = (expr);                                                               \
+    if (_e != hipSuccess)                                                                 \
+      return fail(LYMPH_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));      \
+  } while (0)
+
+template <class T>
+int dalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess)
+    return fail(LYMPH_ERR_HIP, "hipMalloc(" + std::to_string(n * sizeof(T)) + " B): " +
+                                   hipGetErrorString(e));
+  return 0;
+}
+
+template <class T>
+int upload(T** p, const T* h, size_t n) {
+  int rc = dalloc(p, n);
+  if (rc) return rc;
+  if (n) HIP_TRY(hipMemcpy(*p, h, n * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+}  // namespace
+
+struct LymphCtx {
+  int p = 0, Np = 0, n3 = 0, n3c = 0, nq3 = 0, nq2 = 0;
+  double alpha = 10.0, theta = -1.0;
+  int n_tet = 0, n_poly = 0, n_blk = 0;
+  int layout = LYMPH_LAYOUT_POLY;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  bool built = false;
+  std::vector<int> h_row_ptr, h_col;
+  // device
+  MeshDev md{};
+  std::vector<void*> owned;  // everything hipMalloc'ed for the mesh
+  double *K = nullptr, *A = nullptr, *M = nullptr, *Mm = nullptr, *RM = nullptr, *RMm = nullptr;
+  double* slot[LYMPH_NUM_SLOTS] = {};
+  double* stage[4] = {};
+  double* part = nullptr;
+  double* dsum = nullptr;
+  int* status = nullptr;
+  size_t bytes = 0;
+};
+
+namespace {
+
+size_t vec_n(const LymphCtx* c) { return (size_t)c->n_poly * c->n3; }
+size_t blk_sz(const LymphCtx* c) { return (size_t)c->n3 * c->n3c; }
+
+template <class F>
+int dispatch_p(int p, F&& f) {
+  switch (p) {
+    case 1: return f(std::integral_constant<int, 1>{});
+    case 2: return f(std::integral_constant<int, 2>{});
+    case 3: return f(std::integral_constant<int, 3>{});
+    case 4: return f(std::integral_constant<int, 4>{});
+  }
+  return fail(LYMPH_ERR_ARG, "polynomial degree must be 1..4");
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(LYMPH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return 0;
+}
+
+unsigned grid1(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+// host vector (user layout) -> device staging (poly layout)
+int h2d_vec(LymphCtx* c, const double* h, double* d) {
+  const size_t n = vec_n(c);
+  if (c->layout == LYMPH_LAYOUT_POLY) {
+    HIP_TRY(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  } else {
+    HIP_TRY(hipMemcpyAsync(c->stage[3], h, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    k_to_poly<<<grid1(n, 256), 256, 0, c->stream>>>(c->stage[3], d, c->n_poly, c->n3);
+    if (int rc = check_launch("k_to_poly")) return rc;
+  }
+  return 0;
+}
+
+int d2h_vec(LymphCtx* c, const double* d, double* h) {
+  const size_t n = vec_n(c);
+  if (c->layout == LYMPH_LAYOUT_POLY) {
+    HIP_TRY(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  } else {
+    k_to_fortran<<<grid1(n, 256), 256, 0, c->stream>>>(d, c->stage[3], c->n_poly, c->n3);
+    if (int rc = check_launch("k_to_fortran")) return rc;
+    HIP_TRY(hipMemcpyAsync(h, c->stage[3], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int launch_step(LymphCtx* c, int mode, double g, double c1, double c2, const double* u,
+                const double* wa, double* out) {
+  return dispatch_p(c->p, [&](auto pc) -> int {
+    constexpr int P = decltype(pc)::value;
+    if (mode == 0)
+      k_leapfrog<P, 0><<<c->n_poly, 64, 0, c->stream>>>(c->K, c->md.row_ptr, c->md.col, c->RM,
+                                                         c->slot[LYMPH_SLOT_RHS_STAT],
+                                                         c->slot[LYMPH_SLOT_RHS_DYN], u, wa, out, g,
+                                                         c1, c2);
+    else
+      k_leapfrog<P, 1><<<c->n_poly, 64, 0, c->stream>>>(c->K, c->md.row_ptr, c->md.col, c->RM,
+                                                         c->slot[LYMPH_SLOT_RHS_STAT],
+                                                         c->slot[LYMPH_SLOT_RHS_DYN], u, wa, out, g,
+                                                         c1, c2);
+    return check_launch("k_leapfrog");
+  });
+}
+
+int errors_dev(LymphCtx* c, const double* uh, const double* ue, double* l2sq, double* dgsq) {
+  const size_t n = vec_n(c);
+  double res[2];
+  int rc = dispatch_p(c->p, [&](auto pc) -> int {
+    constexpr int P = decltype(pc)::value;
+    if (l2sq) {
+      k_l2_terms<P><<<grid1(3 * (size_t)c->n_poly, 128), 128, 0, c->stream>>>(c->Mm, uh, ue, c->part,
+                                                                               c->n_poly);
+      k_seq_sum<<<1, 64, 0, c->stream>>>(c->part, 3 * c->n_poly, c->dsum);
+    }
+    if (dgsq) {
+      k_sub<<<grid1(n, 256), 256, 0, c->stream>>>(uh, ue, c->stage[2], n);
+      k_leapfrog<P, 2><<<c->n_poly, 64, 0, c->stream>>>(c->A, c->md.row_ptr, c->md.col, nullptr,
+                                                         nullptr, nullptr, c->stage[2], nullptr,
+                                                         c->part, 0.0, 0.0, 0.0);
+      k_seq_sum<<<1, 64, 0, c->stream>>>(c->part, c->n_poly, c->dsum + 1);
+    }
+    return check_launch("error kernels");
+  });
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(res, c->dsum, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (l2sq) *l2sq = res[0];
+  if (dgsq) *dgsq = res[1];
+  return 0;
+}
+
+int need_ctx(const LymphCtx* c) {
+  if (!c) return fail(LYMPH_ERR_ARG, "null context");
+  return 0;
+}
+int need_built(const LymphCtx* c) {
+  if (int rc = need_ctx(c)) return rc;
+  if (!c->built) return fail(LYMPH_ERR_STATE, "operator not built: call lymph_build_operator first");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lymph_last_error(void) { return g_err.c_str(); }
+const char* lymph_version(void) { return "lymph3d_apsc_amd 0.1 (gfx950)"; }
+
+int lymph_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+double lymph_time_function(int problem, double t) { return time_factor(problem, t); }
+
+int lymph_ctx_destroy(LymphCtx* c) {
+  if (!c) return 0;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : c->owned) (void)hipFree(p);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+int lymph_ctx_create(const LymphMesh* mesh, int p, double alpha, double theta, LymphCtx** out) {
+  if (!out) return fail(LYMPH_ERR_ARG, "out is null");
+  *out = nullptr;
+  if (!mesh || !mesh->xyz || !mesh->tet_vert || !mesh->neigh || !mesh->normal || !mesh->area ||
+      !mesh->bbox || !mesh->hk || !mesh->rho || !mesh->lam || !mesh->mu || !mesh->bc_space)
+    return fail(LYMPH_ERR_ARG, "mesh has a null array");
+  if (p < 1 || p > kMaxP) return fail(LYMPH_ERR_ARG, "polynomial degree must be 1..4");
+  if (mesh->n_tet <= 0 || mesh->n_poly <= 0 || mesh->n_node <= 0)
+    return fail(LYMPH_ERR_ARG, "empty mesh");
+  const int nt = mesh->n_tet, np = mesh->n_poly;
+  std::vector<int> tpoly(nt);
+  for (int t = 0; t < nt; ++t) {
+    tpoly[t] = mesh->tet_poly ? mesh->tet_poly[t] : t;
+    if (tpoly[t] < 0 || tpoly[t] >= np) return fail(LYMPH_ERR_ARG, "tet_poly out of range");
+  }
+  for (int i = 0; i < 4 * nt; ++i) {
+    const int e = mesh->neigh[i];
+    if (e >= nt || (e < 0 && e != -1 && e != -2))
+      return fail(LYMPH_EXIT_NEIGHBOUR, "neigh entry out of range at " + std::to_string(i));
+  }
+  for (int i = 0; i < 4 * nt; ++i)
+    if (mesh->tet_vert[i] < 0 || mesh->tet_vert[i] >= mesh->n_node)
+      return fail(LYMPH_ERR_ARG, "tet_vert out of range");
+
+  // polyhedron -> tets (ascending), block pattern in first-appearance order
+  std::vector<int> ptr(np + 1, 0), ptet(nt);
+  for (int t = 0; t < nt; ++t) ptr[tpoly[t] + 1]++;
+  for (int i = 0; i < np; ++i) ptr[i + 1] += ptr[i];
+  for (int i = 0; i < np; ++i)
+    if (ptr[i + 1] == ptr[i]) return fail(LYMPH_EXIT_SETUP, "empty polyhedron " + std::to_string(i));
+  {
+    std::vector<int> fill(np, 0);
+    for (int t = 0; t < nt; ++t) ptet[ptr[tpoly[t]] + fill[tpoly[t]]++] = t;
+  }
+  std::vector<int> row_ptr(np + 1), col, face_blk(4 * (size_t)nt, -1);
+  col.reserve((size_t)np * 8);
+  for (int P = 0; P < np; ++P) {
+    row_ptr[P] = (int)col.size();
+    col.push_back(P);
+    for (int a = ptr[P]; a < ptr[P + 1]; ++a) {
+      const int t = ptet[a];
+      for (int f = 0; f < 4; ++f) {
+        const int e2 = mesh->neigh[t * 4 + f];
+        if (e2 < 0) continue;
+        const int Q = tpoly[e2];
+        if (Q == P) continue;
+        int found = -1;
+        for (int b = row_ptr[P] + 1; b < (int)col.size(); ++b)
+          if (col[b] == Q) { found = b; break; }
+        if (found < 0) {
+          found = (int)col.size();
+          col.push_back(Q);
+        }
+        face_blk[(size_t)t * 4 + f] = found;
+      }
+    }
+  }
+  row_ptr[np] = (int)col.size();
+
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(LYMPH_ERR_HIP, "no HIP device: liblymph_gpu needs an MI355X (gfx950)");
+
+  LymphCtx* c = new LymphCtx();
+  c->p = p;
+  c->Np = np_of(p);
+  c->n3 = 3 * c->Np;
+  c->n3c = (c->n3 + 1) & ~1;
+  c->nq3 = (p + 1) * (p + 1) * (p + 1);
+  c->nq2 = (p + 1) * (p + 1);
+  c->alpha = alpha;
+  c->theta = theta;
+  c->n_tet = nt;
+  c->n_poly = np;
+  c->n_blk = (int)col.size();
+  c->h_row_ptr = row_ptr;
+  c->h_col = col;
+
+  std::vector<double> txyz((size_t)nt * 12);
+  for (int t = 0; t < nt; ++t)
+    for (int v = 0; v < 4; ++v)
+      for (int d = 0; d < 3; ++d)
+        txyz[(size_t)t * 12 + v * 3 + d] = mesh->xyz[(size_t)mesh->tet_vert[t * 4 + v] * 3 + d];
+  std::vector<double> qtet(c->nq3 * 4), wtet(c->nq3), qtri(c->nq2 * 4), wtri(c->nq2);
+  quad_tables(p, qtet.data(), wtet.data(), qtri.data(), wtri.data());
+  std::vector<int> bl(c->Np * 3);
+  basis_list(p, bl.data());
+
+  int rc = 0;
+  auto up = [&](auto** dst, const auto* src, size_t n) {
+    if (rc) return;
+    rc = upload(dst, src, n);
+    if (!rc) {
+      c->owned.push_back((void*)*dst);
+      c->bytes += n * sizeof(**dst);
+    }
+  };
+  auto al = [&](auto** dst, size_t n) {
+    if (rc) return;
+    rc = dalloc(dst, n);
+    if (!rc) {
+      c->owned.push_back((void*)*dst);
+      c->bytes += n * sizeof(**dst);
+    }
+  };
+  MeshDev& m = c->md;
+  double *d_txyz, *d_normal, *d_area, *d_rho, *d_lam, *d_mu, *d_bbox, *d_hk, *d_qtet, *d_wtet,
+      *d_qtri, *d_wtri;
+  int *d_tpoly, *d_neigh, *d_bc, *d_ptr, *d_ptet, *d_rowp, *d_col, *d_fblk, *d_bl;
+  up(&d_txyz, txyz.data(), txyz.size());
+  up(&d_tpoly, tpoly.data(), tpoly.size());
+  up(&d_neigh, mesh->neigh, 4 * (size_t)nt);
+  up(&d_bc, mesh->bc_space, 4 * (size_t)nt);
+  up(&d_normal, mesh->normal, 12 * (size_t)nt);
+  up(&d_area, mesh->area, 4 * (size_t)nt);
+  up(&d_rho, mesh->rho, nt);
+  up(&d_lam, mesh->lam, nt);
+  up(&d_mu, mesh->mu, nt);
+  up(&d_bbox, mesh->bbox, 6 * (size_t)np);
+  up(&d_hk, mesh->hk, np);
+  up(&d_ptr, ptr.data(), ptr.size());
+  up(&d_ptet, ptet.data(), ptet.size());
+  up(&d_rowp, row_ptr.data(), row_ptr.size());
+  up(&d_col, col.data(), col.size());
+  up(&d_fblk, face_blk.data(), face_blk.size());
+  up(&d_qtet, qtet.data(), qtet.size());
+  up(&d_wtet, wtet.data(), wtet.size());
+  up(&d_qtri, qtri.data(), qtri.size());
+  up(&d_wtri, wtri.data(), wtri.size());
+  up(&d_bl, bl.data(), bl.size());
+  const size_t bs = blk_sz(c), npk = (size_t)c->Np * (c->Np + 1) / 2;
+  al(&c->K, (size_t)c->n_blk * bs);
+  al(&c->A, (size_t)c->n_blk * bs);
+  al(&c->M, (size_t)np * c->Np * c->Np);
+  al(&c->Mm, (size_t)np * c->Np * c->Np);
+  al(&c->RM, (size_t)np * npk);
+  al(&c->RMm, (size_t)np * npk);
+  for (int s = 0; s < LYMPH_NUM_SLOTS; ++s) al(&c->slot[s], vec_n(c));
+  for (int s = 0; s < 4; ++s) al(&c->stage[s], vec_n(c));
+  al(&c->part, 3 * (size_t)np);
+  al(&c->dsum, 2);
+  al(&c->status, 1);
+  if (rc) {
+    lymph_ctx_destroy(c);
+    return rc;
+  }
+  m.tet_xyz = d_txyz; m.tet_poly = d_tpoly; m.neigh = d_neigh; m.bc_space = d_bc;
+  m.normal = d_normal; m.area = d_area; m.rho = d_rho; m.lam = d_lam; m.mu = d_mu;
+  m.bbox = d_bbox; m.hk = d_hk; m.ptet_ptr = d_ptr; m.ptet = d_ptet; m.row_ptr = d_rowp;
+  m.col = d_col; m.face_blk = d_fblk; m.qtet = d_qtet; m.wtet = d_wtet; m.qtri = d_qtri;
+  m.wtri = d_wtri; m.blist = d_bl;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    lymph_ctx_destroy(c);
+    return fail(LYMPH_ERR_HIP, "hipStreamCreate failed");
+  }
+  c->own_stream = true;
+  for (int s = 0; s < LYMPH_NUM_SLOTS; ++s)
+    if (hipMemsetAsync(c->slot[s], 0, vec_n(c) * sizeof(double), c->stream) != hipSuccess) {
+      lymph_ctx_destroy(c);
+      return fail(LYMPH_ERR_HIP, "hipMemsetAsync failed");
+    }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    lymph_ctx_destroy(c);
+    return fail(LYMPH_ERR_HIP, "stream sync failed");
+  }
+  *out = c;
+  return 0;
+}
+
+int lymph_ctx_info(const LymphCtx* c, int64_t* info) {
+  if (int rc = need_ctx(c)) return rc;
+  const int64_t Np = c->Np, n3 = c->n3;
+  info[0] = c->n_tet;
+  info[1] = c->n_poly;
+  info[2] = Np;
+  info[3] = c->n_blk;
+  info[4] = (int64_t)c->n_blk * (int64_t)blk_sz(c) * 8;
+  info[5] = (int64_t)c->bytes;
+  // B = 8 [ n_blk (3Np)^2 + N_poly Np(Np+1)/2 + 5 N_poly 3Np ]
+  info[6] = 8 * ((int64_t)c->n_blk * n3 * n3 + (int64_t)c->n_poly * Np * (Np + 1) / 2 +
+                 5 * (int64_t)c->n_poly * n3);
+  // flops: 2 per operator entry + 2 triangular solves per component
+  info[7] = 2 * (int64_t)c->n_blk * n3 * n3 + (int64_t)c->n_poly * 3 * 2 * Np * Np +
+            (int64_t)c->n_poly * n3 * 6;
+  return 0;
+}
+
+int lymph_ctx_set_layout(LymphCtx* c, int layout) {
+  if (int rc = need_ctx(c)) return rc;
+  if (layout != LYMPH_LAYOUT_POLY && layout != LYMPH_LAYOUT_FORTRAN)
+    return fail(LYMPH_ERR_ARG, "unknown layout");
+  c->layout = layout;
+  return 0;
+}
+
+int lymph_ctx_set_stream(LymphCtx* c, void* stream) {
+  if (int rc = need_ctx(c)) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (c->own_stream) (void)hipStreamDestroy(c->stream);
+  c->own_stream = false;
+  c->stream = (hipStream_t)stream;
+  return 0;
+}
+
+void* lymph_ctx_get_stream(const LymphCtx* c) { return c ? (void*)c->stream : nullptr; }
+
+int lymph_get_pattern(const LymphCtx* c, int32_t* row_ptr, int32_t* col) {
+  if (int rc = need_ctx(c)) return rc;
+  if (row_ptr) std::memcpy(row_ptr, c->h_row_ptr.data(), sizeof(int) * (c->n_poly + 1));
+  if (col) std::memcpy(col, c->h_col.data(), sizeof(int) * c->n_blk);
+  return 0;
+}
+
+int lymph_build_operator(LymphCtx* c) {
+  if (int rc = need_ctx(c)) return rc;
+  const size_t bs = blk_sz(c);
+  HIP_TRY(hipMemsetAsync(c->K, 0, (size_t)c->n_blk * bs * sizeof(double), c->stream));
+  HIP_TRY(hipMemsetAsync(c->A, 0, (size_t)c->n_blk * bs * sizeof(double), c->stream));
+  HIP_TRY(hipMemsetAsync(c->M, 0, (size_t)c->n_poly * c->Np * c->Np * sizeof(double), c->stream));
+  HIP_TRY(hipMemsetAsync(c->Mm, 0, (size_t)c->n_poly * c->Np * c->Np * sizeof(double), c->stream));
+  HIP_TRY(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
+  int rc = dispatch_p(c->p, [&](auto pc) -> int {
+    constexpr int P = decltype(pc)::value;
+    const int chunk = 1 << 16;
+    for (int p0 = 0; p0 < c->n_poly; p0 += chunk) {
+      const int n = std::min(chunk, c->n_poly - p0);
+      k_build_operator<P><<<n, 256, 0, c->stream>>>(c->md, c->alpha, c->theta, c->K, c->A, c->M,
+                                                   c->Mm, p0);
+      if (int r = check_launch("k_build_operator")) return r;
+    }
+    k_cholesky<P><<<grid1(c->n_poly, 64), 64, 0, c->stream>>>(c->M, c->Mm, c->RM, c->RMm,
+                                                              c->n_poly, c->status);
+    return check_launch("k_cholesky");
+  });
+  if (rc) return rc;
+  int st = 0;
+  HIP_TRY(hipMemcpyAsync(&st, c->status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (st) return fail(LYMPH_EXIT_SINGULARMTX, "mass matrix is not positive definite");
+  c->built = true;
+  return 0;
+}
+
+int lymph_export_operator(const LymphCtx* cc, double* K, double* A, double* M, double* Mm) {
+  LymphCtx* c = const_cast<LymphCtx*>(cc);
+  if (int rc = need_built(c)) return rc;
+  const size_t bs = blk_sz(c), n3 = c->n3;
+  std::vector<double> tmp((size_t)c->n_blk * bs);
+  for (int which = 0; which < 2; ++which) {
+    double* dst = which ? A : K;
+    if (!dst) continue;
+    HIP_TRY(hipMemcpyAsync(tmp.data(), which ? c->A : c->K, tmp.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (size_t b = 0; b < (size_t)c->n_blk; ++b)
+      for (size_t r = 0; r < n3; ++r)
+        for (size_t cl = 0; cl < n3; ++cl)
+          dst[(b * n3 + r) * n3 + cl] = tmp[b * bs + kpos((int)n3, (int)r, (int)cl)];
+  }
+  const size_t mb = (size_t)c->n_poly * c->Np * c->Np * sizeof(double);
+  if (M) HIP_TRY(hipMemcpyAsync(M, c->M, mb, hipMemcpyDeviceToHost, c->stream));
+  if (Mm) HIP_TRY(hipMemcpyAsync(Mm, c->Mm, mb, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int lymph_build_rhs(LymphCtx* c, int problem, int slot, double* rhs_out) {
+  if (int rc = need_ctx(c)) return rc;
+  if (slot < 0 || slot >= LYMPH_NUM_SLOTS) return fail(LYMPH_ERR_ARG, "bad slot");
+  if (problem < LYMPH_PROB_NULL || problem > LYMPH_PROB_CUBE_WAVE)
+    return fail(LYMPH_ERR_ARG, "unknown problem");
+  int rc = dispatch_p(c->p, [&](auto pc) -> int {
+    constexpr int P = decltype(pc)::value;
+    k_rhs<P><<<c->n_poly, 128, 0, c->stream>>>(c->md, problem, c->alpha, c->theta, c->slot[slot]);
+    return check_launch("k_rhs");
+  });
+  if (rc) return rc;
+  if (rhs_out) return d2h_vec(c, c->slot[slot], rhs_out);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int lymph_project(LymphCtx* c, int fn, int slot, double* out) {
+  if (int rc = need_built(c)) return rc;
+  if (slot < 0 || slot >= LYMPH_NUM_SLOTS) return fail(LYMPH_ERR_ARG, "bad slot");
+  if (fn < LYMPH_FN_ZERO || fn > LYMPH_FN_CUBE_U) return fail(LYMPH_ERR_ARG, "unknown field");
+  int rc = dispatch_p(c->p, [&](auto pc) -> int {
+    constexpr int P = decltype(pc)::value;
+    k_project<P><<<c->n_poly, 128, 0, c->stream>>>(c->md, fn, c->RMm, c->slot[slot]);
+    return check_launch("k_project");
+  });
+  if (rc) return rc;
+  if (out) return d2h_vec(c, c->slot[slot], out);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int lymph_set_slot(LymphCtx* c, int slot, const double* host) {
+  if (int rc = need_ctx(c)) return rc;
+  if (slot < 0 || slot >= LYMPH_NUM_SLOTS || !host) return fail(LYMPH_ERR_ARG, "bad slot/pointer");
+  if (int rc = h2d_vec(c, host, c->slot[slot])) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int lymph_get_slot(LymphCtx* c, int slot, double* host) {
+  if (int rc = need_ctx(c)) return rc;
+  if (slot < 0 || slot >= LYMPH_NUM_SLOTS || !host) return fail(LYMPH_ERR_ARG, "bad slot/pointer");
+  return d2h_vec(c, c->slot[slot], host);
+}
+
+double* lymph_slot_ptr(LymphCtx* c, int slot) {
+  if (!c || slot < 0 || slot >= LYMPH_NUM_SLOTS) return nullptr;
+  return c->slot[slot];
+}
+
+int lymph_first_step_dev(LymphCtx* c, double g0, double dt, const double* u0, const double* v0,
+                         double* u1) {
+  if (int rc = need_built(c)) return rc;
+  const double dt2 = dt * dt, half_dt2 = 0.5 * dt2;
+  return launch_step(c, 1, g0, half_dt2, dt, u0, v0, u1);
+}
+
+int lymph_step_dev(LymphCtx* c, double g, double dt, const double* um1, const double* u,
+                   double* up1) {
+  if (int rc = need_built(c)) return rc;
+  return launch_step(c, 0, g, dt * dt, 0.0, u, um1, up1);
+}
+
+int lymph_first_step(LymphCtx* c, double g0, double dt, const double* u0, const double* v0,
+                     double* u1) {
+  if (int rc = need_built(c)) return rc;
+  if (!u0 || !v0 || !u1) return fail(LYMPH_ERR_ARG, "null vector");
+  if (int rc = h2d_vec(c, u0, c->stage[0])) return rc;
+  if (int rc = h2d_vec(c, v0, c->stage[1])) return rc;
+  if (int rc = lymph_first_step_dev(c, g0, dt, c->stage[0], c->stage[1], c->stage[2])) return rc;
+  return d2h_vec(c, c->stage[2], u1);
+}
+
+int lymph_step(LymphCtx* c, double g, double dt, const double* um1, const double* u, double* up1) {
+  if (int rc = need_built(c)) return rc;
+  if (!um1 || !u || !up1) return fail(LYMPH_ERR_ARG, "null vector");
+  if (int rc = h2d_vec(c, um1, c->stage[0])) return rc;
+  if (int rc = h2d_vec(c, u, c->stage[1])) return rc;
+  if (int rc = lymph_step_dev(c, g, dt, c->stage[0], c->stage[1], c->stage[2])) return rc;
+  return d2h_vec(c, c->stage[2], up1);
+}
+
+int lymph_step_n(LymphCtx* c, int problem, int nsteps, double t0, double dt, double* t_out) {
+  if (int rc = need_built(c)) return rc;
+  if (nsteps < 0) return fail(LYMPH_ERR_ARG, "nsteps < 0");
+  double t = t0;
+  const double dt2 = dt * dt;
+  for (int s = 0; s < nsteps; ++s) {
+    if (int rc = launch_step(c, 0, time_factor(problem, t), dt2, 0.0, c->slot[LYMPH_SLOT_U_CUR],
+                             c->slot[LYMPH_SLOT_U_PREV], c->slot[LYMPH_SLOT_U_NEXT]))
+      return rc;
+    double* prev = c->slot[LYMPH_SLOT_U_PREV];
+    c->slot[LYMPH_SLOT_U_PREV] = c->slot[LYMPH_SLOT_U_CUR];
+    c->slot[LYMPH_SLOT_U_CUR] = c->slot[LYMPH_SLOT_U_NEXT];
+    c->slot[LYMPH_SLOT_U_NEXT] = prev;
+    t = t + dt;
+  }
+  if (t_out) *t_out = t;
+  return 0;
+}
+
+int lymph_time_steps(LymphCtx* c, int problem, int nsteps, double t0, double dt, double* ms) {
+  if (int rc = need_built(c)) return rc;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, c->stream));
+  int rc = lymph_step_n(c, problem, nsteps, t0, dt, nullptr);
+  HIP_TRY(hipEventRecord(e1, c->stream));
+  HIP_TRY(hipEventSynchronize(e1));
+  float f = 0.f;
+  HIP_TRY(hipEventElapsedTime(&f, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if (ms) *ms = f;
+  return 0;
+}
+
+int lymph_error_l2(LymphCtx* c, const double* uh, const double* uex, double* err2) {
+  if (int rc = need_built(c)) return rc;
+  if (int rc = h2d_vec(c, uh, c->stage[0])) return rc;
+  if (int rc = h2d_vec(c, uex, c->stage[1])) return rc;
+  return errors_dev(c, c->stage[0], c->stage[1], err2, nullptr);
+}
+
+int lymph_error_dg(LymphCtx* c, const double* uh, const double* uex, double* err2) {
+  if (int rc = need_built(c)) return rc;
+  if (int rc = h2d_vec(c, uh, c->stage[0])) return rc;
+  if (int rc = h2d_vec(c, uex, c->stage[1])) return rc;
+  return errors_dev(c, c->stage[0], c->stage[1], nullptr, err2);
+}
+
+int lymph_error_slots(LymphCtx* c, int su, int se, double* l2sq, double* dgsq) {
+  if (int rc = need_built(c)) return rc;
+  if (su < 0 || su >= LYMPH_NUM_SLOTS || se < 0 || se >= LYMPH_NUM_SLOTS)
+    return fail(LYMPH_ERR_ARG, "bad slot");
+  return errors_dev(c, c->slot[su], c->slot[se], l2sq, dgsq);
+}
+
+int lymph_run(LymphCtx* c, int problem, double dt, double stop_time, int max_steps,
+              double* u_final, double* out) {
+  if (int rc = need_ctx(c)) return rc;
+  if (!out) return fail(LYMPH_ERR_ARG, "out is null");
+  if (!c->built)
+    if (int rc = lymph_build_operator(c)) return rc;
+  if (int rc = lymph_build_rhs(c, LYMPH_PROB_NULL, LYMPH_SLOT_RHS_STAT, nullptr)) return rc;
+  if (int rc = lymph_build_rhs(c, problem, LYMPH_SLOT_RHS_DYN, nullptr)) return rc;
+  const int fn_u0 = problem == LYMPH_PROB_CUBE_WAVE ? LYMPH_FN_CUBE_U : LYMPH_FN_ZERO;
+  const int fn_v0 = problem == LYMPH_PROB_REF_MMS ? LYMPH_FN_REF_VEL : LYMPH_FN_ZERO;
+  const int fn_ex = problem == LYMPH_PROB_REF_MMS
+                        ? LYMPH_FN_REF_UEX
+                        : (problem == LYMPH_PROB_CUBE_WAVE ? LYMPH_FN_CUBE_U : LYMPH_FN_ZERO);
+  if (int rc = lymph_project(c, fn_u0, LYMPH_SLOT_U_PREV, nullptr)) return rc;
+  if (int rc = lymph_project(c, fn_v0, LYMPH_SLOT_V0, nullptr)) return rc;
+  double t = 0.0;
+  int nsteps = 0;
+  if (int rc = lymph_first_step_dev(c, time_factor(problem, t), dt, c->slot[LYMPH_SLOT_U_PREV],
+                                    c->slot[LYMPH_SLOT_V0], c->slot[LYMPH_SLOT_U_CUR]))
+    return rc;
+  nsteps++;
+  t = t + dt;
+  while (t <= stop_time && (max_steps < 0 || nsteps < max_steps)) {
+    double tn;
+    if (int rc = lymph_step_n(c, problem, 1, t, dt, &tn)) return rc;
+    nsteps++;
+    t = tn;
+  }
+  if (int rc = lymph_project(c, fn_ex, LYMPH_SLOT_UEX, nullptr)) return rc;
+  k_scale<<<grid1(vec_n(c), 256), 256, 0, c->stream>>>(c->slot[LYMPH_SLOT_UEX],
+                                                       time_factor(problem, t), vec_n(c));
+  if (int rc = check_launch("k_scale")) return rc;
+  double l2, dg;
+  if (int rc = errors_dev(c, c->slot[LYMPH_SLOT_U_CUR], c->slot[LYMPH_SLOT_UEX], &l2, &dg)) return rc;
+  if (u_final)
+    if (int rc = d2h_vec(c, c->slot[LYMPH_SLOT_U_CUR], u_final)) return rc;
+  out[0] = t;
+  out[1] = nsteps;
+  out[2] = std::sqrt(l2);
+  out[3] = std::sqrt(dg);
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,625 @@
+// lymph_kernels.hpp -- gfx950 kernels of the PolyDG leapfrog path.
+//
+// Data layout in HBM (one context):
+//  * operator K and A_dg: one block row per polyhedron, blocks of 3Np x 3Np
+//    FP64 in "column-pair" order: entry (r, c) of block b sits at
+//      b*N3*N3C + ((c>>1)*N3 + r)*2 + (c&1),     N3 = 3Np, N3C = N3 rounded up
+//    to even (padding column zero).  A 64-lane wave with lane = row r reads one
+//    column pair of a block as a single 16-byte load per lane: N3*16 contiguous
+//    bytes per wave instruction, every byte of the block read exactly once.
+//  * mass factors R (upper, M = R^T R): packed row-major upper triangle,
+//    Np(Np+1)/2 doubles per polyhedron (the bytes the step must read).
+//  * vectors: 3Np doubles per polyhedron, component-major inside.
+#pragma once
+#include "lymph_math.hpp"
+
+namespace lymph {
+
+template <int P>
+struct Dims {
+  static constexpr int NP = np_of(P);
+  static constexpr int N3 = 3 * NP;
+  static constexpr int N3C = (N3 + 1) & ~1;
+  static constexpr int NPAIR = N3C / 2;
+  static constexpr int NQ1 = P + 1;
+  static constexpr int NQ3 = NQ1 * NQ1 * NQ1;
+  static constexpr int NQ2 = NQ1 * NQ1;
+  static constexpr int NPK = NP * (NP + 1) / 2;
+  static constexpr size_t BS = (size_t)N3 * N3C;
+};
+
+LYMPH_HD size_t kpos(int N3, int r, int c) {
+  return ((size_t)(c >> 1) * N3 + r) * 2 + (c & 1);
+}
+LYMPH_HD int pk_off(int NP, int i) { return i * NP - i * (i - 1) / 2; }
+
+struct MeshDev {
+  const double* tet_xyz;  // n_tet*12
+  const int* tet_poly;
+  const int* neigh;       // n_tet*4
+  const int* bc_space;    // n_tet*4
+  const double* normal;   // n_tet*12
+  const double* area;     // n_tet*4
+  const double* rho;
+  const double* lam;
+  const double* mu;
+  const double* bbox;     // n_poly*6
+  const double* hk;       // n_poly
+  const int* ptet_ptr;    // n_poly+1
+  const int* ptet;        // n_tet
+  const int* row_ptr;     // n_poly+1
+  const int* col;         // n_blk
+  const int* face_blk;    // n_tet*4 block index of the neighbour block, -1 if none
+  const double* qtet;     // NQ3*4
+  const double* wtet;     // NQ3
+  const double* qtri;     // NQ2*4
+  const double* wtri;     // NQ2
+  const int* blist;       // NP*3
+};
+
+// ---------------------------------------------------------------------------
+// Operator build: MAKE_MATRICES_FREE (src/matrix_free.f90:27-464).
+// One 256-thread workgroup per polyhedron; its tets are processed in order
+// (volume term, then faces 1..4), each output entry owned by one thread for
+// the whole kernel so every entry accumulates in the reference's order.
+// ---------------------------------------------------------------------------
+template <int P>
+__global__ __launch_bounds__(256) void k_build_operator(MeshDev m, double alpha, double theta,
+                                                         double* __restrict__ K,
+                                                         double* __restrict__ A,
+                                                         double* __restrict__ M,
+                                                         double* __restrict__ Mm, int poly0) {
+  using D = Dims<P>;
+  constexpr int NP = D::NP, N3 = D::N3, NQ3 = D::NQ3, NQ2 = D::NQ2;
+  constexpr int VOLSZ = 4 * NP * NQ3, FACESZ = 8 * NP * NQ2;
+  constexpr int SHSZ = VOLSZ > FACESZ ? VOLSZ : FACESZ;
+  __shared__ double sh[SHSZ];
+  __shared__ double spt[NQ3 * 3];
+  __shared__ int sbl[NP * 3];
+  const int tid = threadIdx.x;
+  const int Pp = poly0 + blockIdx.x;
+  for (int i = tid; i < NP * 3; i += blockDim.x) sbl[i] = m.blist[i];
+  double bbP[6];
+  for (int d = 0; d < 6; ++d) bbP[d] = m.bbox[Pp * 6 + d];
+  const double hkP = m.hk[Pp];
+  double* Ks = K + (size_t)m.row_ptr[Pp] * D::BS;
+  double* As = A + (size_t)m.row_ptr[Pp] * D::BS;
+  double* MP = M + (size_t)Pp * NP * NP;
+  double* MmP = Mm + (size_t)Pp * NP * NP;
+  // volume-phase views
+  double* phi = sh;                // [NP][NQ3]
+  double* dphi = sh + NP * NQ3;    // [3][NP][NQ3]
+  // face-phase views
+  double* p1 = sh;                      // [NP][NQ2]
+  double* p2 = sh + NP * NQ2;           // [NP][NQ2]
+  double* g1 = sh + 2 * NP * NQ2;       // [3][NP][NQ2]
+  double* g2 = sh + 5 * NP * NQ2;       // [3][NP][NQ2]
+
+  for (int a = m.ptet_ptr[Pp]; a < m.ptet_ptr[Pp + 1]; ++a) {
+    const int t = m.ptet[a];
+    double v[12], Fk[12], Jdet;
+    for (int i = 0; i < 12; ++i) v[i] = m.tet_xyz[(size_t)t * 12 + i];
+    affine_map(v, Fk, Jdet);
+    const double aJ = fabs(Jdet);
+    const double rho = m.rho[t], lam = m.lam[t], mu = m.mu[t];
+    const double l2m = lam + 2 * mu;
+    __syncthreads();
+    for (int q = tid; q < NQ3; q += blockDim.x) map_point(Fk, &m.qtet[q * 4], &spt[q * 3]);
+    __syncthreads();
+    for (int e = tid; e < NP * NQ3; e += blockDim.x) {
+      const int f = e / NQ3, q = e % NQ3;
+      double ph, d0, d1, d2;
+      basis_at(&sbl[f * 3], bbP, &spt[q * 3], ph, d0, d1, d2);
+      phi[e] = ph;
+      dphi[(0 * NP + f) * NQ3 + q] = d0;
+      dphi[(1 * NP + f) * NQ3 + q] = d1;
+      dphi[(2 * NP + f) * NQ3 + q] = d2;
+    }
+    __syncthreads();
+    // mass (MAKE_MATRICES_FREE :224-233 with MAKE_MASS_VOLUME)
+    for (int e = tid; e < NP * NP; e += blockDim.x) {
+      const int mm = e / NP, nn = e % NP;
+      double s = 0.0;
+      for (int q = 0; q < NQ3; ++q)
+        s = s + 1.0 * m.wtet[q] * aJ * phi[mm * NQ3 + q] * phi[nn * NQ3 + q];
+      MmP[e] = MmP[e] + s;
+      MP[e] = MP[e] + s * rho;
+    }
+    // volume stiffness (MAKE_STIFFNESS_VOLUME, assemble_element :24-77)
+    for (int e = tid; e < 9 * NP * NP; e += blockDim.x) {
+      const int i = e / (3 * NP * NP), j = (e / (NP * NP)) % 3;
+      const int mm = (e / NP) % NP, nn = e % NP;
+      const int oi = i == 0 ? 1 : 0, oi2 = i == 2 ? 1 : 2;  // the other two directions
+      double s = 0.0;
+      const double* Dm = dphi + mm * NQ3;
+      const double* Dn = dphi + nn * NQ3;
+      constexpr int SD = NP * NQ3;
+      for (int q = 0; q < NQ3; ++q) {
+        const double wj = m.wtet[q] * aJ;
+        double tv;
+        if (i == j)
+          tv = l2m * Dm[i * SD + q] * Dn[i * SD + q] + mu * Dm[oi * SD + q] * Dn[oi * SD + q] +
+               mu * Dm[oi2 * SD + q] * Dn[oi2 * SD + q];
+        else
+          tv = lam * Dm[i * SD + q] * Dn[j * SD + q] + mu * Dm[j * SD + q] * Dn[i * SD + q];
+        s = s + wj * tv;
+      }
+      const size_t o = kpos(N3, i * NP + mm, j * NP + nn);
+      Ks[o] = Ks[o] + s;
+      As[o] = As[o] + s;
+    }
+    // faces (MAKE_STIFFNESS_FACE, assemble_element :404-661; insertion
+    // src/matrix_free.f90:371-441)
+    for (int f = 0; f < 4; ++f) {
+      const int e2 = m.neigh[t * 4 + f];
+      int Q = -1;
+      if (e2 >= 0) {
+        Q = m.tet_poly[e2];
+        if (Q == Pp) continue;  // intra-polyhedron face
+      }
+      if (e2 == -2) continue;  // Neumann: no stiffness contribution
+      __syncthreads();
+      for (int q = tid; q < NQ2; q += blockDim.x) face_point(Fk, f, &m.qtri[q * 4], &spt[q * 3]);
+      __syncthreads();
+      double bbQ[6];
+      if (Q >= 0)
+        for (int d = 0; d < 6; ++d) bbQ[d] = m.bbox[Q * 6 + d];
+      for (int e = tid; e < NP * NQ2; e += blockDim.x) {
+        const int fb = e / NQ2, q = e % NQ2;
+        double ph, d0, d1, d2;
+        basis_at(&sbl[fb * 3], bbP, &spt[q * 3], ph, d0, d1, d2);
+        p1[e] = ph;
+        g1[(0 * NP + fb) * NQ2 + q] = d0;
+        g1[(1 * NP + fb) * NQ2 + q] = d1;
+        g1[(2 * NP + fb) * NQ2 + q] = d2;
+        if (Q >= 0) {
+          basis_at(&sbl[fb * 3], bbQ, &spt[q * 3], ph, d0, d1, d2);
+          p2[e] = ph;
+          g2[(0 * NP + fb) * NQ2 + q] = d0;
+          g2[(1 * NP + fb) * NQ2 + q] = d1;
+          g2[(2 * NP + fb) * NQ2 + q] = d2;
+        }
+      }
+      __syncthreads();
+      const double* nrm = &m.normal[(t * 4 + f) * 3];
+      const double n1 = nrm[0], n2 = nrm[1], n3 = nrm[2];
+      const double ar = m.area[t * 4 + f];
+      const double Dbar = lam + 2 * mu;
+      const double hmin = (Q >= 0) ? (hkP < m.hk[Q] ? hkP : m.hk[Q]) : hkP;
+      const double sigma = alpha * (P * P) / hmin * Dbar;
+      const double d11 = fabs(n1) * fabs(n1) + 0.5 * (fabs(n2) * fabs(n2)) + 0.5 * (fabs(n3) * fabs(n3));
+      const double d22 = fabs(n2) * fabs(n2) + 0.5 * (fabs(n1) * fabs(n1)) + 0.5 * (fabs(n3) * fabs(n3));
+      const double d33 = fabs(n3) * fabs(n3) + 0.5 * (fabs(n1) * fabs(n1)) + 0.5 * (fabs(n2) * fabs(n2));
+      const double dd[3] = {d11, d22, d33};
+      const bool interior = Q >= 0;
+      double* Kb = interior ? K + (size_t)m.face_blk[t * 4 + f] * D::BS : nullptr;
+      double* Ab = interior ? A + (size_t)m.face_blk[t * 4 + f] * D::BS : nullptr;
+      constexpr int SG = NP * NQ2;
+      for (int e = tid; e < 9 * NP * NP; e += blockDim.x) {
+        const int i = e / (3 * NP * NP), j = (e / (NP * NP)) % 3;
+        const int mm = (e / NP) % NP, nn = e % NP;
+        const double ni = nrm[i < j ? i : j], nj = nrm[i < j ? j : i];  // reference order n(min)*n(max)
+        double s1 = 0.0, i1 = 0.0, i1t = 0.0, i2 = 0.0, it2 = 0.0, s2 = 0.0;
+        for (int q = 0; q < NQ2; ++q) {
+          const double w = m.wtri[q];
+          // S_E1 (:476-490)
+          double ts = (i == j) ? (p1[mm * NQ2 + q] * p1[nn * NQ2 + q]) * dd[i]
+                               : 0.5 * p1[mm * NQ2 + q] * p1[nn * NQ2 + q] * ni * nj;
+          s1 = s1 + sigma * w * ts * ar;
+          // I(i,j,m,n) = sigma(grad_m) : (phi_n (x) n) with the component pattern
+          // of :504-515; a = grad side, b = phi side.
+          auto itemp = [&](const double* G, int gm, double phv, int ii, int jj) -> double {
+            if (ii == jj) {
+              if (ii == 0)
+                return l2m * G[0 * SG + gm * NQ2 + q] * phv * n1 + mu * G[1 * SG + gm * NQ2 + q] * phv * n2 +
+                       mu * G[2 * SG + gm * NQ2 + q] * phv * n3;
+              if (ii == 1)
+                return l2m * G[1 * SG + gm * NQ2 + q] * phv * n2 + mu * G[0 * SG + gm * NQ2 + q] * phv * n1 +
+                       mu * G[2 * SG + gm * NQ2 + q] * phv * n3;
+              return l2m * G[2 * SG + gm * NQ2 + q] * phv * n3 + mu * G[1 * SG + gm * NQ2 + q] * phv * n2 +
+                     mu * G[0 * SG + gm * NQ2 + q] * phv * n1;
+            }
+            return lam * G[ii * SG + gm * NQ2 + q] * phv * nrm[jj] +
+                   mu * G[jj * SG + gm * NQ2 + q] * phv * nrm[ii];
+          };
+          const double tI = itemp(g1, mm, p1[nn * NQ2 + q], i, j);
+          const double tIt = itemp(g1, nn, p1[mm * NQ2 + q], j, i);
+          if (!interior) {
+            i1 = i1 + w * ar * tI;
+            i1t = i1t + w * ar * tIt;
+          } else {
+            i1 = i1 + 0.5 * w * ar * tI;
+            i1t = i1t + 0.5 * w * ar * tIt;
+            const double tI2 = itemp(g1, mm, p2[nn * NQ2 + q], i, j);
+            i2 = i2 - 0.5 * w * ar * tI2;
+            const double tIT2 = itemp(g2, nn, p1[mm * NQ2 + q], i, j);
+            it2 = it2 + 0.5 * w * ar * tIT2;
+            double t3 = (i == j) ? (p1[mm * NQ2 + q] * p2[nn * NQ2 + q]) * dd[i]
+                                 : 0.5 * p1[mm * NQ2 + q] * p2[nn * NQ2 + q] * ni * nj;
+            s2 = s2 - sigma * w * t3 * ar;
+          }
+        }
+        const size_t o = kpos(N3, i * NP + mm, j * NP + nn);
+        Ks[o] = Ks[o] + s1;
+        As[o] = As[o] + s1;
+        Ks[o] = Ks[o] + theta * i1 - i1t;
+        if (interior) {
+          Kb[o] = Kb[o] + theta * i2 + s2;
+          Ab[o] = Ab[o] + s2;
+          Kb[o] = Kb[o] - it2;
+        }
+      }
+    }
+  }
+}
+
+// Cholesky of both mass blocks of a polyhedron (cholesky,
+// src/vet_mat_operations.f90:84-120), one thread per polyhedron; writes the
+// packed upper factors.  status[0] set to 1 if a block is not SPD.
+template <int P>
+__global__ void k_cholesky(const double* __restrict__ M, const double* __restrict__ Mm,
+                           double* __restrict__ RM, double* __restrict__ RMm, int n_poly,
+                           int* __restrict__ status) {
+  using D = Dims<P>;
+  constexpr int NP = D::NP;
+  const int Pp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (Pp >= n_poly) return;
+  for (int which = 0; which < 2; ++which) {
+    const double* A = (which ? Mm : M) + (size_t)Pp * NP * NP;
+    double* R = (which ? RMm : RM) + (size_t)Pp * D::NPK;
+    for (int j = 0; j < NP; ++j) {
+      double s = A[j * NP + j];
+      for (int k = 0; k < j; ++k) {
+        const double rkj = R[pk_off(NP, k) + (j - k)];
+        s = s - rkj * rkj;
+      }
+      if (!(s > 0.0)) {
+        atomicOr(status, 1);
+        return;
+      }
+      const double rjj = sqrt(s);
+      R[pk_off(NP, j)] = rjj;
+      for (int i = j + 1; i < NP; ++i) {
+        double s2 = A[j * NP + i];
+        for (int k = 0; k < j; ++k)
+          s2 = s2 - R[pk_off(NP, k) + (j - k)] * R[pk_off(NP, k) + (i - k)];
+        R[pk_off(NP, j) + (i - j)] = s2 / rjj;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// RHS (MAKE_RHS_FREE, src/matrix_free.f90:467-843) and L2 projection
+// (COMPUTE_MODAL_COEFFICIENTS_FREE, :847-997).  One 128-thread workgroup per
+// polyhedron, entry (i, m) owned by one thread.
+// ---------------------------------------------------------------------------
+template <int P>
+__global__ __launch_bounds__(128) void k_rhs(MeshDev m, int prob, double alpha, double theta,
+                                             double* __restrict__ rhs) {
+  using D = Dims<P>;
+  constexpr int NP = D::NP, N3 = D::N3, NQ3 = D::NQ3, NQ2 = D::NQ2;
+  __shared__ double phi[NP * NQ3];
+  __shared__ double gb[3 * NP * NQ2];
+  __shared__ double spt[NQ3 * 3];
+  __shared__ double sval[NQ3 * 3];
+  __shared__ int sbl[NP * 3];
+  const int tid = threadIdx.x;
+  const int Pp = blockIdx.x;
+  for (int i = tid; i < NP * 3; i += blockDim.x) sbl[i] = m.blist[i];
+  double bbP[6];
+  for (int d = 0; d < 6; ++d) bbP[d] = m.bbox[Pp * 6 + d];
+  double* r = rhs + (size_t)Pp * N3;
+  double racc = 0.0;  // running rhs entry of this thread
+  for (int a = m.ptet_ptr[Pp]; a < m.ptet_ptr[Pp + 1]; ++a) {
+    const int t = m.ptet[a];
+    double v[12], Fk[12], Jdet;
+    for (int i = 0; i < 12; ++i) v[i] = m.tet_xyz[(size_t)t * 12 + i];
+    affine_map(v, Fk, Jdet);
+    const double aJ = fabs(Jdet);
+    const double rho = m.rho[t], lam = m.lam[t], mu = m.mu[t];
+    __syncthreads();
+    for (int q = tid; q < NQ3; q += blockDim.x) {
+      map_point(Fk, &m.qtet[q * 4], &spt[q * 3]);
+      forcing(prob, lam, mu, &spt[q * 3], rho, &sval[q * 3]);
+    }
+    __syncthreads();
+    for (int e = tid; e < NP * NQ3; e += blockDim.x) {
+      const int f = e / NQ3, q = e % NQ3;
+      double d0, d1, d2;
+      basis_at(&sbl[f * 3], bbP, &spt[q * 3], phi[e], d0, d1, d2);
+    }
+    __syncthreads();
+    if (tid < N3) {
+      const int i = tid / NP, mm = tid % NP;
+      double s = 0.0;
+      for (int q = 0; q < NQ3; ++q) s = s + aJ * m.wtet[q] * sval[q * 3 + i] * phi[mm * NQ3 + q];
+      racc = racc + s;
+    }
+    for (int f = 0; f < 4; ++f) {
+      const int e2 = m.neigh[t * 4 + f];
+      if (e2 >= 0) continue;  // only boundary faces carry data
+      __syncthreads();
+      const double* nrm = &m.normal[(t * 4 + f) * 3];
+      const int tag = m.bc_space[t * 4 + f];
+      for (int q = tid; q < NQ2; q += blockDim.x) {
+        double xr[3];
+        face_point(Fk, f, &m.qtri[q * 4], &spt[q * 3]);
+        face_point_rhs(Fk, f, &m.qtri[q * 4], xr);
+        if (e2 == -1) dirichlet(prob, xr, tag, &sval[q * 3]);
+        else neumann(prob, lam, mu, nrm, xr, tag, &sval[q * 3]);
+      }
+      __syncthreads();
+      for (int e = tid; e < NP * NQ2; e += blockDim.x) {
+        const int fb = e / NQ2, q = e % NQ2;
+        basis_at(&sbl[fb * 3], bbP, &spt[q * 3], phi[e], gb[(0 * NP + fb) * NQ2 + q],
+                 gb[(1 * NP + fb) * NQ2 + q], gb[(2 * NP + fb) * NQ2 + q]);
+      }
+      __syncthreads();
+      if (tid < N3) {
+        const int i = tid / NP, mm = tid % NP;
+        const double n1 = nrm[0], n2 = nrm[1], n3 = nrm[2];
+        const double ar = m.area[t * 4 + f];
+        const double l2m = lam + 2 * mu;
+        const double sigma = alpha * (P * P) / m.hk[Pp] * (lam + 2 * mu);
+        double s = 0.0;
+        for (int q = 0; q < NQ2; ++q) {
+          const double w = m.wtri[q];
+          const double* dd = &sval[q * 3];
+          const double ph = phi[mm * NQ2 + q];
+          if (e2 == -1) {
+            const double gg1 = gb[(0 * NP + mm) * NQ2 + q], gg2 = gb[(1 * NP + mm) * NQ2 + q],
+                         gg3 = gb[(2 * NP + mm) * NQ2 + q];
+            double tm, t2;
+            if (i == 0) {
+              tm = l2m * gg1 * dd[0] * n1 + mu * gg2 * dd[0] * n2 + mu * gg3 * dd[0] * n3 +
+                   lam * gg1 * dd[1] * n2 + mu * gg2 * dd[1] * n1 + lam * gg1 * dd[2] * n3 +
+                   mu * gg3 * dd[2] * n1;
+              t2 = ph * dd[0] * (fabs(n1) * fabs(n1) + 0.5 * (fabs(n2) * fabs(n2)) + 0.5 * (fabs(n3) * fabs(n3))) +
+                   0.5 * ph * dd[1] * n1 * n2 + 0.5 * ph * dd[2] * n1 * n3;
+            } else if (i == 1) {
+              tm = l2m * gg2 * dd[1] * n2 + mu * gg1 * dd[1] * n1 + mu * gg3 * dd[1] * n3 +
+                   lam * gg2 * dd[0] * n1 + mu * gg1 * dd[0] * n2 + lam * gg2 * dd[2] * n3 +
+                   mu * gg3 * dd[2] * n2;
+              t2 = ph * dd[1] * (fabs(n2) * fabs(n2) + 0.5 * (fabs(n1) * fabs(n1)) + 0.5 * (fabs(n3) * fabs(n3))) +
+                   0.5 * ph * dd[0] * n1 * n2 + 0.5 * ph * dd[2] * n2 * n3;
+            } else {
+              tm = l2m * gg3 * dd[2] * n3 + mu * gg1 * dd[2] * n1 + mu * gg2 * dd[2] * n2 +
+                   lam * gg3 * dd[0] * n1 + mu * gg1 * dd[0] * n3 + lam * gg3 * dd[1] * n2 +
+                   mu * gg2 * dd[1] * n3;
+              t2 = ph * dd[2] * (fabs(n3) * fabs(n3) + 0.5 * (fabs(n1) * fabs(n1)) + 0.5 * (fabs(n2) * fabs(n2))) +
+                   0.5 * ph * dd[0] * n1 * n3 + 0.5 * ph * dd[1] * n2 * n3;
+            }
+            s = s + theta * w * tm * ar + sigma * w * t2 * ar;
+          } else {
+            s = s + w * ph * dd[i] * ar;
+          }
+        }
+        if (s != 0.0) racc = racc + s;
+      }
+    }
+  }
+  if (tid < N3) r[tid] = racc;
+}
+
+// Forward/backward substitution with the packed factor in the reference's
+// order (solve_LU with L = R^T, src/vet_mat_operations.f90:123-153); x in place.
+template <int NP>
+__device__ __forceinline__ void solve_packed(const double* __restrict__ R, double* x) {
+  x[0] = x[0] / R[0];
+  for (int i = 1; i < NP; ++i) {
+    double s = x[i];
+    for (int j = 0; j < i; ++j) s = s - R[pk_off(NP, j) + (i - j)] * x[j];
+    x[i] = s / R[pk_off(NP, i)];
+  }
+  x[NP - 1] = x[NP - 1] / R[pk_off(NP, NP - 1)];
+  for (int i = NP - 2; i >= 0; --i) {
+    double s = x[i];
+    for (int j = i + 1; j < NP; ++j) s = s - R[pk_off(NP, i) + (j - i)] * x[j];
+    x[i] = s / R[pk_off(NP, i)];
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(128) void k_project(MeshDev m, int fn, const double* __restrict__ RMm,
+                                                 double* __restrict__ out) {
+  using D = Dims<P>;
+  constexpr int NP = D::NP, N3 = D::N3, NQ3 = D::NQ3;
+  __shared__ double phi[NP * NQ3];
+  __shared__ double spt[NQ3 * 3];
+  __shared__ double sval[NQ3 * 3];
+  __shared__ double sb[N3];
+  __shared__ int sbl[NP * 3];
+  const int tid = threadIdx.x;
+  const int Pp = blockIdx.x;
+  for (int i = tid; i < NP * 3; i += blockDim.x) sbl[i] = m.blist[i];
+  double bbP[6];
+  for (int d = 0; d < 6; ++d) bbP[d] = m.bbox[Pp * 6 + d];
+  double acc = 0.0;
+  for (int a = m.ptet_ptr[Pp]; a < m.ptet_ptr[Pp + 1]; ++a) {
+    const int t = m.ptet[a];
+    double v[12], Fk[12], Jdet;
+    for (int i = 0; i < 12; ++i) v[i] = m.tet_xyz[(size_t)t * 12 + i];
+    affine_map(v, Fk, Jdet);
+    const double aJ = fabs(Jdet);
+    __syncthreads();
+    for (int q = tid; q < NQ3; q += blockDim.x) {
+      map_point(Fk, &m.qtet[q * 4], &spt[q * 3]);
+      field(fn, &spt[q * 3], &sval[q * 3]);
+    }
+    __syncthreads();
+    for (int e = tid; e < NP * NQ3; e += blockDim.x) {
+      const int f = e / NQ3, q = e % NQ3;
+      double d0, d1, d2;
+      basis_at(&sbl[f * 3], bbP, &spt[q * 3], phi[e], d0, d1, d2);
+    }
+    __syncthreads();
+    if (tid < N3) {
+      const int i = tid / NP, mm = tid % NP;
+      for (int q = 0; q < NQ3; ++q) acc = acc + aJ * m.wtet[q] * sval[q * 3 + i] * phi[mm * NQ3 + q];
+    }
+  }
+  if (tid < N3) sb[tid] = acc;
+  __syncthreads();
+  if (tid < 3) solve_packed<NP>(RMm + (size_t)Pp * D::NPK, sb + tid * NP);
+  __syncthreads();
+  if (tid < N3) out[(size_t)Pp * N3 + tid] = sb[tid];
+}
+
+// ---------------------------------------------------------------------------
+// The leapfrog step (TIME_STEP_MATRIX_FREE, src/matrix_free.f90:1230-1385;
+// FIRST_TIME_STEP_MATRIX_FREE :1100-1227) fused in one kernel: one wave per
+// polyhedron, lane r = row r of the block row.  Per block the wave streams
+// the block's column pairs (16 B per lane per load), with the neighbour's
+// coefficients as wave-uniform (scalar) loads: the halo exchange of
+// src/exchange_data.f90 becomes the on-device gather col[b].  Then
+// w = -Ku + F_s + F_d g, three triangular-solve pairs with the packed mass
+// factor, and the update.
+//   MODE 0: out = (c1*x + 2*u) - wa            (c1 = dt^2, u = x_apply, wa = u^{n-1})
+//   MODE 1: out = (c1*x + u) + c2*wa           (c1 = dt^2/2, u = u^0, wa = v^0, c2 = dt)
+//   MODE 2: part[P] = sum_r u_r (K u)_r        (DG error: K = A_dg, u = uh - uex)
+// ---------------------------------------------------------------------------
+template <int P, int MODE>
+__global__ __launch_bounds__(64) void k_leapfrog(const double* __restrict__ K,
+                                                 const int* __restrict__ row_ptr,
+                                                 const int* __restrict__ col,
+                                                 const double* __restrict__ Rp,
+                                                 const double* __restrict__ rs,
+                                                 const double* __restrict__ rd,
+                                                 const double* __restrict__ u,
+                                                 const double* __restrict__ wa,
+                                                 double* __restrict__ out, double g, double c1,
+                                                 double c2) {
+  using D = Dims<P>;
+  constexpr int NP = D::NP, N3 = D::N3, NPAIR = D::NPAIR;
+  constexpr int RPL = (N3 + 63) / 64;
+  __shared__ double sw[N3];
+  __shared__ double sR[D::NPK];
+  const int lane = threadIdx.x;
+  const int Pp = blockIdx.x;
+  if (MODE != 2)
+    for (int k = lane; k < D::NPK; k += 64) sR[k] = Rp[(size_t)Pp * D::NPK + k];
+  double acc[RPL];
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) acc[k] = 0.0;
+  const int b0 = row_ptr[Pp], b1 = row_ptr[Pp + 1];
+  for (int b = b0; b < b1; ++b) {
+    const int cb = __builtin_amdgcn_readfirstlane(col[b]);
+    const double* __restrict__ xv = u + (size_t)cb * N3;
+    const double* __restrict__ Kb = K + (size_t)b * D::BS;
+    double s[RPL];
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) s[k] = 0.0;
+#pragma unroll
+    for (int c2i = 0; c2i < NPAIR; ++c2i) {
+      const double x0 = xv[2 * c2i];
+      const double x1 = (2 * c2i + 1 < N3) ? xv[2 * c2i + 1] : 0.0;
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) {
+        const int r = lane + 64 * k;
+        if (r < N3) {
+          const double2 kv = *reinterpret_cast<const double2*>(Kb + ((size_t)c2i * N3 + r) * 2);
+          s[k] = s[k] + kv.x * x0;
+          if (2 * c2i + 1 < N3) s[k] = s[k] + kv.y * x1;
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) acc[k] = (b == b0) ? s[k] : acc[k] + s[k];
+  }
+  const size_t o = (size_t)Pp * N3;
+  if (MODE == 2) {
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) {
+      const int r = lane + 64 * k;
+      if (r < N3) sw[r] = acc[k];
+    }
+    __syncthreads();
+    if (lane == 0) {
+      double el = 0.0;
+      for (int r = 0; r < N3; ++r) el = el + u[o + r] * sw[r];
+      out[Pp] = el;
+    }
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) {
+    const int r = lane + 64 * k;
+    if (r < N3) sw[r] = -acc[k] + rs[o + r] + rd[o + r] * g;
+  }
+  __syncthreads();
+  if (lane < 3) solve_packed<NP>(sR, sw + lane * NP);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) {
+    const int r = lane + 64 * k;
+    if (r < N3) {
+      if (MODE == 0)
+        out[o + r] = c1 * sw[r] + 2.0 * u[o + r] - wa[o + r];
+      else
+        out[o + r] = c1 * sw[r] + u[o + r] + c2 * wa[o + r];
+    }
+  }
+}
+
+// L2 error terms du_i^T M_modal du_i per (polyhedron, component)
+// (COMPUTE_ERROR_L2_MATRIX_FREE, src/matrix_free.f90:1406-1416).
+template <int P>
+__global__ void k_l2_terms(const double* __restrict__ Mm, const double* __restrict__ uh,
+                           const double* __restrict__ ue, double* __restrict__ part, int n_poly) {
+  using D = Dims<P>;
+  constexpr int NP = D::NP;
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= 3 * n_poly) return;
+  const int Pp = id / 3, i = id % 3;
+  const size_t o = (size_t)Pp * D::N3 + i * NP;
+  double du[NP];
+  for (int mm = 0; mm < NP; ++mm) du[mm] = uh[o + mm] - ue[o + mm];
+  const double* A = Mm + (size_t)Pp * NP * NP;
+  double el = 0.0;
+  for (int mm = 0; mm < NP; ++mm) {
+    double s = 0.0;
+    for (int nn = 0; nn < NP; ++nn) s = s + A[mm * NP + nn] * du[nn];
+    el = el + du[mm] * s;
+  }
+  part[id] = el;
+}
+
+// Sequential sum in index order (the reference accumulates the error terms
+// element by element); one thread: called once per error evaluation.
+__global__ void k_seq_sum(const double* __restrict__ part, int n, double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s = s + part[i];
+    out[0] = s;
+  }
+}
+
+__global__ void k_sub(const double* __restrict__ a, const double* __restrict__ b,
+                      double* __restrict__ c, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) c[i] = a[i] - b[i];
+}
+
+__global__ void k_scale(double* __restrict__ a, double g, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0.0 * 0.0 + a[i] * g;
+}
+
+// poly-major u[P*N3 + d]  <->  reference column-major u(npoly, N3): u[d*npoly + P]
+__global__ void k_to_poly(const double* __restrict__ f, double* __restrict__ p, int n_poly, int n3) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < (size_t)n_poly * n3) {
+    const int P = (int)(i / n3), d = (int)(i % n3);
+    p[i] = f[(size_t)d * n_poly + P];
+  }
+}
+__global__ void k_to_fortran(const double* __restrict__ p, double* __restrict__ f, int n_poly, int n3) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < (size_t)n_poly * n3) {
+    const int d = (int)(i / n_poly), P = (int)(i % n_poly);
+    f[i] = p[(size_t)P * n3 + d];
+  }
+}
+
+}  // namespace lymph

@@ -1,0 +1,169 @@
+"""Host-side mirror of the reference's `matrix_free` module
+(src/matrix_free.f90) on top of liblymph_gpu.so.
+
+`Context` owns one GPU context (one mesh, one degree).  Its methods carry the
+reference routine names and argument meaning:
+
+  make_matrices_free()                 MAKE_MATRICES_FREE + mass Cholesky
+                                       (src/matrix_free.f90:27-464, src/Lymph3D.f90:264-271)
+  make_rhs_free(problem, slot)         MAKE_RHS_FREE (:467-843)
+  compute_modal_coefficients_free(fn)  COMPUTE_MODAL_COEFFICIENTS_FREE (:847-997)
+  first_time_step_matrix_free(...)     FIRST_TIME_STEP_MATRIX_FREE (:1100-1227)
+  time_step_matrix_free(...)           TIME_STEP_MATRIX_FREE (:1230-1385)
+  compute_error_l2/dg_matrix_free(...) COMPUTE_ERROR_*_MATRIX_FREE (:1388-1504)
+
+Errors raise LymphError whose .code reuses the reference's exit codes
+(EXIT_SINGULARMTX=5 for a non-SPD mass block, EXIT_SETUP=4, ...).  Vectors are
+numpy float64 arrays of n_poly*3*Np entries in the layout selected with
+`layout` (polyhedron-major by default; "fortran" = the reference's column-major
+u(npoly, 3Np)).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib, ptr
+
+
+class Context:
+    def __init__(self, mesh, p, alpha=10.0, theta=-1.0, layout="poly"):
+        L = lib()
+        self.mesh = mesh
+        self.p = int(p)
+        self.Np = (p + 1) * (p + 2) * (p + 3) // 6
+        self.n3 = 3 * self.Np
+        self.n_poly = mesh.n_poly
+        cm, self._keep = mesh.to_c()
+        h = C.c_void_p()
+        check(L.lymph_ctx_create(C.byref(cm), self.p, alpha, theta, C.byref(h)), "lymph_ctx_create")
+        self.h = h
+        self.built = False
+        if layout == "fortran":
+            check(L.lymph_ctx_set_layout(h, _lib.LAYOUT_FORTRAN), "lymph_ctx_set_layout")
+        info = np.zeros(8, np.int64)
+        check(L.lymph_ctx_info(h, ptr(info)), "lymph_ctx_info")
+        (self.n_tet, _, _, self.n_blk, self.operator_bytes, self.device_bytes,
+         self.step_bytes, self.step_flops) = (int(v) for v in info)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and _lib._LIB is not None:
+            _lib._LIB.lymph_ctx_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- helpers --------------------------------------------------------
+    def vec(self):
+        return np.zeros(self.n_poly * self.n3)
+
+    def _c(self, a):
+        a = np.ascontiguousarray(a, np.float64)
+        if a.size != self.n_poly * self.n3:
+            raise ValueError(f"vector of {a.size} entries, expected {self.n_poly * self.n3}")
+        return a
+
+    def pattern(self):
+        rp = np.zeros(self.n_poly + 1, np.int32)
+        col = np.zeros(self.n_blk, np.int32)
+        check(lib().lymph_get_pattern(self.h, ptr(rp), ptr(col)), "lymph_get_pattern")
+        return rp, col
+
+    def set_stream(self, stream_handle):
+        check(lib().lymph_ctx_set_stream(self.h, C.c_void_p(stream_handle)), "lymph_ctx_set_stream")
+
+    def stream(self):
+        return lib().lymph_ctx_get_stream(self.h)
+
+    # ---- the reference routines ----------------------------------------
+    def make_matrices_free(self):
+        check(lib().lymph_build_operator(self.h), "lymph_build_operator")
+        self.built = True
+
+    def export_operator(self):
+        """(K, A_dg) as (n_blk, 3Np, 3Np) row-major blocks, (M, M_modal) as
+        (n_poly, Np, Np)."""
+        K = np.zeros((self.n_blk, self.n3, self.n3))
+        A = np.zeros_like(K)
+        M = np.zeros((self.n_poly, self.Np, self.Np))
+        Mm = np.zeros_like(M)
+        check(lib().lymph_export_operator(self.h, ptr(K), ptr(A), ptr(M), ptr(Mm)),
+              "lymph_export_operator")
+        return K, A, M, Mm
+
+    def make_rhs_free(self, problem, slot=_lib.SLOT_RHS_DYN):
+        out = self.vec()
+        check(lib().lymph_build_rhs(self.h, problem, slot, ptr(out)), "lymph_build_rhs")
+        return out
+
+    def compute_modal_coefficients_free(self, fn, slot=_lib.SLOT_UEX):
+        out = self.vec()
+        check(lib().lymph_project(self.h, fn, slot, ptr(out)), "lymph_project")
+        return out
+
+    def set_slot(self, slot, v):
+        v = self._c(v)
+        check(lib().lymph_set_slot(self.h, slot, ptr(v)), "lymph_set_slot")
+
+    def get_slot(self, slot):
+        out = self.vec()
+        check(lib().lymph_get_slot(self.h, slot, ptr(out)), "lymph_get_slot")
+        return out
+
+    def first_time_step_matrix_free(self, g0, dt, u0, v0):
+        out = self.vec()
+        u0, v0 = self._c(u0), self._c(v0)
+        check(lib().lymph_first_step(self.h, g0, dt, ptr(u0), ptr(v0), ptr(out)), "lymph_first_step")
+        return out
+
+    def time_step_matrix_free(self, g, dt, u0, un):
+        """u^{n+1} from u^{n-1}=u0 and u^n=un (argument order of the reference)."""
+        out = self.vec()
+        u0, un = self._c(u0), self._c(un)
+        check(lib().lymph_step(self.h, g, dt, ptr(u0), ptr(un), ptr(out)), "lymph_step")
+        return out
+
+    def step_n(self, problem, nsteps, t0, dt):
+        t = C.c_double()
+        check(lib().lymph_step_n(self.h, problem, nsteps, t0, dt, C.byref(t)), "lymph_step_n")
+        return t.value
+
+    def time_steps(self, problem, nsteps, t0, dt):
+        ms = C.c_double()
+        check(lib().lymph_time_steps(self.h, problem, nsteps, t0, dt, C.byref(ms)), "lymph_time_steps")
+        return ms.value
+
+    def compute_error_l2_matrix_free(self, uh, uex):
+        e = C.c_double()
+        uh, uex = self._c(uh), self._c(uex)
+        check(lib().lymph_error_l2(self.h, ptr(uh), ptr(uex), C.byref(e)), "lymph_error_l2")
+        return e.value
+
+    def compute_error_dg_matrix_free(self, uh, uex):
+        e = C.c_double()
+        uh, uex = self._c(uh), self._c(uex)
+        check(lib().lymph_error_dg(self.h, ptr(uh), ptr(uex), C.byref(e)), "lymph_error_dg")
+        return e.value
+
+    def error_slots(self, slot_uh, slot_uex):
+        l2, dg = C.c_double(), C.c_double()
+        check(lib().lymph_error_slots(self.h, slot_uh, slot_uex, C.byref(l2), C.byref(dg)),
+              "lymph_error_slots")
+        return l2.value, dg.value
+
+    def run(self, problem, dt, stop_time, max_steps=-1, want_u=False):
+        """The whole path of src/Lymph3D.f90:262-518 on the device."""
+        out = np.zeros(4)
+        u = self.vec() if want_u else None
+        check(lib().lymph_run(self.h, problem, dt, stop_time, max_steps, ptr(u), ptr(out)), "lymph_run")
+        self.built = True
+        res = dict(t_final=out[0], nsteps=int(out[1]), err_l2=out[2], err_dg=out[3])
+        if want_u:
+            res["u"] = u
+        return res

@@ -1,0 +1,370 @@
+"""Host-side mesh pipeline: the part of LYMPH3D's set-up that produces the
+flattened mesh the GPU path consumes (struct LymphMesh of include/lymph_gpu.h).
+
+It stands in for the reference's Fortran host (READ_INPUT_FILES,
+MAKE_PARTITION_AND_MPI_FILES) in standalone runs:
+  * deck reader: Poly.input / .mate / .mesh (src/Poly_readfile.f90:9-187,
+    327-707, 715-1049);
+  * structured cube generator: n^3 cubes x 6 tets (Kuhn split, positively
+    oriented), optional agglomeration of each cube's 6 tets into one
+    polyhedron, or seeded region growing into polyhedra of a target size;
+  * face matching by sorted vertex triples, vectorised (replaces the O(F^2)
+    search of CREATE_NEIGH_EL_TRIA, src/MAKE_PARTITION_AND_MPI_FILES.f90:
+    2030-2145) with boundary tags mapped as :2105-2109 (2,3,4 -> Dirichlet -1;
+    5,6,7 -> Neumann -2);
+  * face normals/areas (CREATE_NORMAL_FACE :1429-1504) and polyhedron
+    bounding boxes/diameters (CREATE_BBOX_EL :1510-1580).
+Everything is vectorised numpy; nothing here is on the per-step path.
+"""
+import os
+
+import numpy as np
+
+# tet faces 1..4 of the reference: vertex sets for matching and ordered
+# triplets (a, b, c) for n = (c-b) x (a-b)
+FACE_SETS = np.array([[0, 1, 2], [0, 1, 3], [1, 2, 3], [2, 0, 3]])
+FACE_TRI = np.array([[2, 1, 0], [0, 1, 3], [1, 2, 3], [2, 0, 3]])
+
+DIRICHLET_TAGS = (2, 3, 4)
+NEUMANN_TAGS = (5, 6, 7)
+
+
+class MeshError(RuntimeError):
+    """Mesh set-up failure; .code follows the reference's exit codes."""
+
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+class Mesh:
+    """Flattened mesh (0-based).  Arrays are C-contiguous numpy arrays matching
+    the fields of LymphMesh."""
+
+    def __init__(self, xyz, tet_vert, tet_poly, tria_vert, tria_tag, tag_space,
+                 rho=None, lam=None, mu=None, degree=None, dt=None, stop_time=None):
+        self.xyz = np.ascontiguousarray(xyz, np.float64)
+        self.tet_vert = np.ascontiguousarray(tet_vert, np.int32)
+        nt = self.tet_vert.shape[0]
+        if tet_poly is None:
+            tet_poly = np.arange(nt)
+        self.tet_poly = np.ascontiguousarray(tet_poly, np.int32)
+        self.n_tet = nt
+        self.n_poly = int(self.tet_poly.max()) + 1
+        self.tria_vert = np.ascontiguousarray(tria_vert, np.int32).reshape(-1, 3)
+        self.tria_tag = np.ascontiguousarray(tria_tag, np.int32)
+        self.tag_space = dict(tag_space)
+        ones = np.ones(nt)
+        self.rho = np.ascontiguousarray(ones if rho is None else rho, np.float64)
+        self.lam = np.ascontiguousarray(ones if lam is None else lam, np.float64)
+        self.mu = np.ascontiguousarray(ones if mu is None else mu, np.float64)
+        self.degree, self.dt, self.stop_time = degree, dt, stop_time
+        self._build()
+
+    # --- connectivity and geometry --------------------------------------
+    def _build(self):
+        xyz, tv = self.xyz, self.tet_vert
+        nt = self.n_tet
+        v = xyz[tv]  # (nt, 4, 3)
+        J = np.stack([v[:, 1] - v[:, 0], v[:, 2] - v[:, 0], v[:, 3] - v[:, 0]], axis=2)
+        det = np.linalg.det(J)
+        if np.any(det <= 0):
+            raise MeshError(11, f"{int((det <= 0).sum())} tetrahedra are degenerate or "
+                                "negatively oriented (EXIT_ELEM_ORIENT)")
+        nn = np.int64(xyz.shape[0])
+
+        def key(tri):
+            s = np.sort(tri, axis=-1).astype(np.int64)
+            return (s[..., 0] * nn + s[..., 1]) * nn + s[..., 2]
+
+        fk = key(tv[:, FACE_SETS]).reshape(-1)  # (4 nt,)
+        order = np.argsort(fk, kind="stable")
+        sk = fk[order]
+        neigh = np.full(4 * nt, -100, np.int32)
+        nface = np.full(4 * nt, -1, np.int32)
+        same = sk[1:] == sk[:-1]
+        if np.any(same[1:] & same[:-1]):
+            raise MeshError(18, "a face is shared by more than two tetrahedra")
+        i = np.nonzero(same)[0]
+        a, b = order[i], order[i + 1]
+        neigh[a], neigh[b] = b // 4, a // 4
+        nface[a], nface[b] = b % 4, a % 4
+        bc = np.zeros(4 * nt, np.int32)
+        free = np.nonzero(neigh == -100)[0]
+        if free.size:
+            if self.tria_vert.shape[0] == 0:
+                raise MeshError(18, "boundary faces without boundary triangles")
+            tk = key(self.tria_vert)
+            to = np.argsort(tk, kind="stable")
+            pos = np.searchsorted(tk[to], fk[free])
+            pos = np.minimum(pos, tk.size - 1)
+            hit = tk[to][pos] == fk[free]
+            if not np.all(hit):
+                raise MeshError(18, f"{int((~hit).sum())} boundary faces have no tagged triangle")
+            tag = self.tria_tag[to][pos]
+            kind = np.where(np.isin(tag, DIRICHLET_TAGS), -1,
+                            np.where(np.isin(tag, NEUMANN_TAGS), -2, 0))
+            if np.any(kind == 0):
+                raise MeshError(18, "boundary triangle tag is neither Dirichlet nor Neumann")
+            neigh[free] = kind
+            nface[free] = free % 4
+            bc[free] = np.array([self.tag_space.get(int(t), 0) for t in tag], np.int32) \
+                if len(self.tag_space) else 0
+        self.neigh = neigh.reshape(nt, 4)
+        self.neigh_face = nface.reshape(nt, 4)
+        self.bc_space = bc.reshape(nt, 4)
+        # normals / areas
+        A = xyz[tv[:, FACE_TRI[:, 0]]]
+        B = xyz[tv[:, FACE_TRI[:, 1]]]
+        Cc = xyz[tv[:, FACE_TRI[:, 2]]]
+        Q = A - B
+        P = Cc - B
+        n1 = P[..., 1] * Q[..., 2] - P[..., 2] * Q[..., 1]
+        n2 = -(P[..., 0] * Q[..., 2]) + P[..., 2] * Q[..., 0]
+        n3 = P[..., 0] * Q[..., 1] - P[..., 1] * Q[..., 0]
+        norm = np.sqrt(n1 * n1 + n2 * n2 + n3 * n3)
+        self.normal = np.ascontiguousarray(np.stack([n1 / norm, n2 / norm, n3 / norm], axis=-1))
+        self.area = np.ascontiguousarray(norm)
+        self._poly_geometry(v)
+
+    def _poly_geometry(self, v):
+        npoly = self.n_poly
+        order = np.argsort(self.tet_poly, kind="stable")
+        counts = np.bincount(self.tet_poly, minlength=npoly)
+        if np.any(counts == 0):
+            raise MeshError(4, "empty polyhedron id")
+        self.poly_ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+        self.poly_tets = order.astype(np.int32)
+        bbox = np.zeros((npoly, 6))
+        hk = np.zeros(npoly)
+        for size in np.unique(counts):
+            polys = np.nonzero(counts == size)[0]
+            tets = np.stack([order[self.poly_ptr[p]:self.poly_ptr[p] + size] for p in polys]) \
+                if size > 1 else order[self.poly_ptr[polys]][:, None]
+            pts = v[tets].reshape(len(polys), 4 * size, 3)  # vertices with duplicates
+            bbox[polys, 0::2] = pts.min(axis=1)
+            bbox[polys, 1::2] = pts.max(axis=1)
+            nv = 4 * size
+            ii, jj = np.triu_indices(nv, 1)
+            h = np.zeros(len(polys))
+            step = max(1, int(4e6 // max(1, len(ii))))
+            for s in range(0, len(polys), step):
+                d = pts[s:s + step, ii] - pts[s:s + step, jj]
+                dist = np.sqrt(d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1] + d[..., 2] * d[..., 2])
+                h[s:s + step] = dist.max(axis=1)
+            hk[polys] = h
+        self.bbox = np.ascontiguousarray(bbox)
+        self.hk = np.ascontiguousarray(hk)
+
+    def hmax(self):
+        return float(self.hk.max())  # compute_hmax, src/solution_processing.f90:197-214
+
+    def n_dof(self, p):
+        return 3 * self.n_poly * (p + 1) * (p + 2) * (p + 3) // 6
+
+    def to_c(self):
+        """Return (LymphMesh struct, keep-alive list)."""
+        from ._lib import LymphMesh, ptr
+        arrs = [self.xyz, self.tet_vert, self.tet_poly,
+                np.ascontiguousarray(self.neigh, np.int32),
+                np.ascontiguousarray(self.bc_space, np.int32),
+                self.normal, self.area, self.bbox, self.hk, self.rho, self.lam, self.mu]
+        m = LymphMesh(self.xyz.shape[0], self.n_tet, self.n_poly, *(ptr(a) for a in arrs))
+        return m, arrs
+
+
+# ----------------------------------------------------------------------------
+def read_deck(folder, force_unit_material=True):
+    """Read a LYMPH3D deck (Poly.input + .mate + .mesh).  With
+    force_unit_material the driver's override rho=lambda=mu=1 is applied
+    (src/Lymph3D.f90:196-203); otherwise lambda=rho(vp^2-2vs^2), mu=rho vs^2
+    from the MATE line (src/Poly_readfile.f90:408-412)."""
+    hdr = {}
+    with open(os.path.join(folder, "Poly.input")) as f:
+        for line in f:
+            line = line.rstrip("\n")
+            if not line or line[0] == " ":
+                continue  # comment lines start with a blank
+            parts = line.split()
+            if len(parts) < 2:
+                raise MeshError(8, f"no argument given for command {parts[0]!r} (EXIT_SYNTAX_ERROR)")
+            hdr[parts[0]] = parts[1]
+    for k in ("GRIDFILE", "MATFILE"):
+        if k not in hdr:
+            raise MeshError(9, f"Poly.input lacks {k}")
+    mate = {}
+    tag_space = {}
+    matfile = os.path.join(folder, hdr["MATFILE"] + ".mate")
+    if not os.path.exists(matfile):
+        raise MeshError(9, f"missing file {matfile} (EXIT_MISSING_FILE)")
+    with open(matfile) as f:
+        for line in f:
+            key, parts = line[:4], line.split()
+            if key == "MATE":
+                tag, deg = int(parts[1]), int(parts[2])
+                rho, vs, vp = float(parts[3]), float(parts[4]), float(parts[5])
+                mate[tag] = (deg, rho, rho * (vp ** 2 - 2 * vs ** 2), rho * vs ** 2)
+            elif key in ("DIRI", "NEUM"):
+                tag_space[int(parts[1])] = int(parts[2])
+    gridfile = os.path.join(folder, hdr["GRIDFILE"] + ".mesh")
+    if not os.path.exists(gridfile):
+        raise MeshError(9, f"missing file {gridfile} (EXIT_MISSING_FILE)")
+    with open(gridfile) as f:
+        lines = f.read().splitlines()
+    i = 0
+    while lines[i].startswith("#"):
+        i += 1
+    nnode, nelem = (int(x) for x in lines[i].split()[:2])
+    i += 1
+    xyz = np.loadtxt(lines[i:i + nnode], usecols=(1, 2, 3), ndmin=2)
+    i += nnode
+    tets, tmat, trias, ttag, polys = [], [], [], [], []
+    for line in lines[i:i + nelem]:
+        parts = line.split()
+        tag, kind = int(parts[1]), parts[2].lower()
+        if kind == "tetra" and tag in mate:
+            tets.append([int(x) - 1 for x in parts[3:7]])
+            tmat.append(tag)
+        elif kind == "tria" and tag in tag_space:
+            trias.append([int(x) - 1 for x in parts[3:6]])
+            ttag.append(tag)
+        elif kind == "poly":
+            polys.append(int(parts[3]) - 1)
+    if not tets:
+        raise MeshError(13, "no tetrahedra (EXIT_NO_ELEMENTS)")
+    tmat = np.asarray(tmat)
+    degs = {mate[t][0] for t in tmat}
+    deg = mate[int(tmat[0])][0]  # Elem_loc(1)%Degree (src/Lymph3D.f90:171-172)
+    if force_unit_material:
+        rho = lam = mu = np.ones(len(tets))
+    else:
+        rho = np.array([mate[t][1] for t in tmat])
+        lam = np.array([mate[t][2] for t in tmat])
+        mu = np.array([mate[t][3] for t in tmat])
+    tet_poly = np.asarray(polys) if len(polys) == len(tets) else None
+    m = Mesh(xyz, np.asarray(tets), tet_poly, np.asarray(trias).reshape(-1, 3),
+             np.asarray(ttag), tag_space, rho, lam, mu, degree=deg,
+             dt=float(hdr.get("TIMESTEP", "nan")), stop_time=float(hdr.get("STOPTIME", "nan")))
+    m.mixed_degree = len(degs) > 1
+    return m
+
+
+# ----------------------------------------------------------------------------
+# Kuhn (Freudenthal) split of the unit cube along its main diagonal: one tet
+# per permutation of the axes, made positively oriented.
+_PERMS = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+
+
+def _kuhn_tets():
+    tets = []
+    for p in _PERMS:
+        c = [0, 0, 0]
+        path = [tuple(c)]
+        for ax in p:
+            c[ax] = 1
+            path.append(tuple(c))
+        ids = [a + 2 * b + 4 * cc for a, b, cc in path]  # corner id in the cube
+        pts = np.array(path, float)
+        det = np.linalg.det(np.stack([pts[1] - pts[0], pts[2] - pts[0], pts[3] - pts[0]], axis=1))
+        if det < 0:
+            ids[1], ids[2] = ids[2], ids[1]
+        tets.append(ids)
+    return np.array(tets)
+
+
+def cube_mesh(n, agglomerate=False, dirichlet_faces="all", poly_size=None, seed=0,
+              lengths=(1.0, 1.0, 1.0)):
+    """Unit cube split into n^3 cubes x 6 tets.
+
+    agglomerate: each cube's 6 tets form one polyhedron (n^3 polyhedra).
+    poly_size=(lo, hi): instead, seeded region growing over the tet adjacency
+      into polyhedra of lo..hi tets (uneven elements).
+    dirichlet_faces: "all" (tag 2 on all six faces) or a set of face names
+      among x0,x1,y0,y1,z0,z1 that get Dirichlet tag 2; others get Neumann tag 5.
+    """
+    L = np.asarray(lengths, float)
+    g = np.arange(n + 1)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    xyz = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1) / n * L
+    nid = lambda i, j, k: (i * (n + 1) + j) * (n + 1) + k  # noqa: E731
+    ci, cj, ck = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    ci, cj, ck = ci.ravel(), cj.ravel(), ck.ravel()
+    corners = np.stack([nid(ci + a, cj + b, ck + c) for c in (0, 1) for b in (0, 1) for a in (0, 1)],
+                       axis=1)  # corner id a + 2b + 4c
+    kt = _kuhn_tets()
+    tets = corners[:, kt].reshape(-1, 4)
+    cube_of_tet = np.repeat(np.arange(n ** 3), 6)
+    # boundary triangles: every tet face on the box boundary
+    keys = np.sort(tets[:, FACE_SETS], axis=2).reshape(-1, 3)
+    fx = xyz[keys]  # (F, 3, 3)
+    names = ["x0", "x1", "y0", "y1", "z0", "z1"]
+    tri, tag = [], []
+    if dirichlet_faces == "all":
+        dset = set(names)
+    else:
+        dset = set(dirichlet_faces)
+    for d in range(3):
+        for side, val in ((0, 0.0), (1, L[d])):
+            on = np.all(np.isclose(fx[:, :, d], val), axis=1)
+            tri.append(keys[on])
+            tag.append(np.full(on.sum(), 2 if names[2 * d + side] in dset else 5))
+    tri = np.concatenate(tri)
+    tag = np.concatenate(tag)
+    if poly_size is not None:
+        tet_poly = grow_polyhedra(tets, n, poly_size, seed)
+    elif agglomerate:
+        tet_poly = cube_of_tet
+    else:
+        tet_poly = None
+    return Mesh(xyz, tets, tet_poly, tri, tag, {2: 1, 5: 1})
+
+
+def grow_polyhedra(tets, n, size, seed):
+    """Seeded region growing over face adjacency into polyhedra whose tet
+    counts are drawn uniformly in [lo, hi] (leftover small regions merge into
+    a neighbour)."""
+    lo, hi = size
+    rng = np.random.default_rng(seed)
+    nt = tets.shape[0]
+    fk = np.sort(tets[:, FACE_SETS], axis=2).reshape(-1, 3).astype(np.int64)
+    m = int(tets.max()) + 1
+    key = (fk[:, 0] * m + fk[:, 1]) * m + fk[:, 2]
+    order = np.argsort(key, kind="stable")
+    sk = key[order]
+    i = np.nonzero(sk[1:] == sk[:-1])[0]
+    a, b = order[i] // 4, order[i + 1] // 4
+    adj = [[] for _ in range(nt)]
+    for x, y in zip(a.tolist(), b.tolist()):
+        adj[x].append(y)
+        adj[y].append(x)
+    poly = np.full(nt, -1, np.int64)
+    pid = 0
+    for start in rng.permutation(nt):
+        if poly[start] >= 0:
+            continue
+        target = int(rng.integers(lo, hi + 1))
+        front = [start]
+        poly[start] = pid
+        cnt = 1
+        k = 0
+        while k < len(front) and cnt < target:
+            for nb in adj[front[k]]:
+                if poly[nb] < 0 and cnt < target:
+                    poly[nb] = pid
+                    front.append(nb)
+                    cnt += 1
+            k += 1
+        pid += 1
+    # merge regions smaller than lo into a neighbouring region
+    counts = np.bincount(poly, minlength=pid)
+    for t in range(nt):
+        if counts[poly[t]] < lo:
+            for nb in adj[t]:
+                if poly[nb] != poly[t] and counts[poly[nb]] + counts[poly[t]] <= hi:
+                    old = poly[t]
+                    poly[poly == old] = poly[nb]
+                    counts[poly[nb]] += counts[old]
+                    counts[old] = 0
+                    break
+    _, dense = np.unique(poly, return_inverse=True)
+    return dense.astype(np.int32)

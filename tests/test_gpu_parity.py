@@ -1,0 +1,207 @@
+"""GPU parity: every kernel of liblymph_gpu.so against the C oracle on the same
+inputs, through the C ABI (lymph3d_apsc_amd.matrix_free.Context).
+
+Tolerances (FP64): operator/RHS/projection/step entries within 1e-13 of the
+oracle relative to the block's (vector's) max magnitude -- the kernels keep
+the reference's summation order with FMA contraction off, so the remaining
+differences are ulp-level (device sin/cos/sqrt); final solution vectors after
+full runs within 1e-11 relative; printed L2/DG errors of
+the reference decks within 5e-13 (p=1) / 5e-12 (tet_100, 3183 steps at p=3) of
+the reference's own known answers."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import lymph3d_apsc_amd as L
+from lymph3d_apsc_amd import _lib, mesh as M
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+DECKS = os.path.join(GOLD, "decks")
+KA = json.load(open(os.path.join(GOLD, "known_answers.json")))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if L.device_count() == 0:
+        pytest.skip("no HIP device")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    s = np.abs(b).max()
+    return 0.0 if s == 0 and np.abs(a).max() == 0 else np.abs(a - b).max() / max(s, 1e-300)
+
+
+def oracle_for(m, p):
+    kinds = np.zeros(8, np.int32)
+    kinds[[2, 3, 4]] = -1
+    kinds[[5, 6, 7]] = -2
+    space = np.zeros(8, np.int32)
+    for t, s in m.tag_space.items():
+        space[t] = s
+    return oracle.Oracle(m.xyz, m.tet_vert, m.tria_vert, m.tria_tag, kinds, space, p,
+                         tet_poly=m.tet_poly, rho=m.rho, lam=m.lam, mu=m.mu)
+
+
+def hetero_mesh(n=3, seed=5):
+    m = M.cube_mesh(n, dirichlet_faces={"x0", "x1", "y0", "y1", "z0"}, poly_size=(4, 12), seed=seed)
+    rng = np.random.default_rng(seed)
+    m.lam = np.exp(rng.normal(0, 0.5, m.n_tet))
+    m.mu = np.exp(rng.normal(0, 0.5, m.n_tet))
+    m.rho = rng.uniform(0.8, 1.2, m.n_tet)
+    return m
+
+
+CASES = [
+    ("tet_5", 1), ("tet_5", 2), ("tet_5", 3), ("tet_5", 4), ("tet_25", 2), ("tet_100", 3),
+    ("cube3", 2), ("cube3agg", 3), ("cube2agg", 4), ("hetero", 2), ("hetero", 3),
+]
+
+
+def make_mesh(name):
+    if name.startswith("tet_"):
+        return M.read_deck(os.path.join(DECKS, name))
+    if name == "cube3":
+        return M.cube_mesh(3)
+    if name == "cube3agg":
+        return M.cube_mesh(3, agglomerate=True)
+    if name == "cube2agg":
+        return M.cube_mesh(2, agglomerate=True)
+    if name == "hetero":
+        return hetero_mesh()
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("name,p", CASES)
+def test_operator_rhs_projection(name, p):
+    say(f"[operator] {name} p={p}")
+    m = make_mesh(name)
+    o = oracle_for(m, p)
+    o.build_operator()
+    with L.Context(m, p) as ctx:
+        rp, col = ctx.pattern()
+        orp, ocol = o.pattern()
+        assert np.array_equal(rp, orp) and np.array_equal(col, ocol)
+        ctx.make_matrices_free()
+        K, A, Mf, Mm = ctx.export_operator()
+        oK, oA = o.K(), o.Adg()
+        oM, oMm, _, _ = o.mass()
+        for b in range(K.shape[0]):
+            assert rel(K[b], oK[b]) < 1e-13, (b, rel(K[b], oK[b]))
+            assert rel(A[b], oA[b]) < 1e-13
+        assert rel(Mf, oM) < 1e-13 and rel(Mm, oMm) < 1e-13
+        for prob in (L.PROB_NULL, L.PROB_REF_MMS, L.PROB_CUBE_WAVE):
+            r = ctx.make_rhs_free(prob)
+            orr = o.rhs(prob)
+            assert rel(r, orr) < 1e-13, (prob, rel(r, orr))
+        for fn in (L.FN_ZERO, L.FN_REF_VEL, L.FN_REF_UEX, L.FN_CUBE_U):
+            c = ctx.compute_modal_coefficients_free(fn)
+            assert rel(c, o.project(fn)) < 1e-13, fn
+
+
+@pytest.mark.parametrize("name,p", [("tet_25", 2), ("cube3agg", 3), ("hetero", 2), ("cube2agg", 4)])
+def test_steps_match_oracle(name, p):
+    say(f"[steps] {name} p={p}")
+    m = make_mesh(name)
+    o = oracle_for(m, p)
+    o.build_operator()
+    rng = np.random.default_rng(1)
+    n = m.n_poly * 3 * o.Np
+    with L.Context(m, p) as ctx:
+        ctx.make_matrices_free()
+        rs = ctx.make_rhs_free(L.PROB_NULL, _lib.SLOT_RHS_STAT)
+        rd = ctx.make_rhs_free(L.PROB_REF_MMS, _lib.SLOT_RHS_DYN)
+        ors, ord_ = o.rhs(L.PROB_NULL), o.rhs(L.PROB_REF_MMS)
+        u0, v0, u1 = rng.standard_normal(n), rng.standard_normal(n), rng.standard_normal(n)
+        dt, g = 1e-3, 0.37
+        a = ctx.first_time_step_matrix_free(g, dt, u0, v0)
+        b = o.first_step(g, dt, u0, v0, ors, ord_)
+        assert rel(a, b) < 1e-13
+        a = ctx.time_step_matrix_free(g, dt, u0, u1)
+        b = o.step(g, dt, u0, u1, ors, ord_)
+        assert rel(a, b) < 1e-13
+        # the same step through the reference's column-major layout
+        with L.Context(m, p, layout="fortran") as cf:
+            cf.make_matrices_free()
+            cf.make_rhs_free(L.PROB_NULL, _lib.SLOT_RHS_STAT)
+            cf.make_rhs_free(L.PROB_REF_MMS, _lib.SLOT_RHS_DYN)
+            tof = lambda v: v.reshape(m.n_poly, -1).T.ravel()  # noqa: E731
+            af = cf.time_step_matrix_free(g, dt, tof(u0), tof(u1))
+            assert np.array_equal(af, tof(a))
+        # device-resident loop == repeated host steps
+        ctx.set_slot(_lib.SLOT_U_PREV, u0)
+        ctx.set_slot(_lib.SLOT_U_CUR, u1)
+        t = ctx.step_n(L.PROB_REF_MMS, 5, 0.1, dt)
+        up, uc = u0, u1
+        tt = 0.1
+        for _ in range(5):
+            un = o.step(oracle.time_function(oracle.PROB_REF_MMS, tt), dt, up, uc, ors, ord_)
+            up, uc = uc, un
+            tt = tt + dt
+        assert t == tt
+        assert rel(ctx.get_slot(_lib.SLOT_U_CUR), uc) < 1e-12
+        # errors
+        l2 = ctx.compute_error_l2_matrix_free(u0, u1)
+        dg = ctx.compute_error_dg_matrix_free(u0, u1)
+        assert abs(l2 / o.error_l2_sq(u0, u1) - 1) < 1e-13
+        assert abs(dg / o.error_dg_sq(u0, u1) - 1) < 1e-12
+        assert ctx.time_steps(L.PROB_REF_MMS, 3, 0.0, dt) > 0
+
+
+@pytest.mark.parametrize("name", ["tet_5", "tet_25", "tet_125", "tet_100"])
+def test_reference_known_answers(name):
+    """Full runs of the reference's decks on the GPU: printed errors vs the
+    reference's known answers and the final vector vs the oracle's."""
+    say(f"[known answer] {name}")
+    ka = KA[name]
+    m = M.read_deck(os.path.join(DECKS, name))
+    with L.Context(m, m.degree) as ctx:
+        r = ctx.run(L.PROB_REF_MMS, m.dt, m.stop_time, want_u=True)
+    assert r["nsteps"] == ka["steps"]
+    tol = 5e-12 if name == "tet_100" else 5e-13
+    assert abs(r["err_l2"] / ka["err_l2"] - 1) < tol, r
+    assert abs(r["err_dg"] / ka["err_dg"] - 1) < tol, r
+    o = oracle_for(m, m.degree)
+    ro = o.run(oracle.PROB_REF_MMS, m.dt, m.stop_time, want_u=True)
+    assert ro["t_final"] == r["t_final"]
+    assert rel(r["u"], ro["u"]) < 1e-11, rel(r["u"], ro["u"])
+
+
+@pytest.mark.parametrize("n,p,steps", [(4, 2, 300), (3, 3, 200), (2, 4, 100)])
+def test_agglomerated_cube_wave_run(n, p, steps):
+    """Agglomerated polyhedra (one per cube), the standing-wave data set, a
+    CFL-derived dt: final vector and errors vs the oracle."""
+    say(f"[cube wave] n={n} p={p}")
+    m = M.cube_mesh(n, agglomerate=True)
+    dt = 0.2 * (1.0 / n) / (np.sqrt(3.0) * p * p)
+    with L.Context(m, p) as ctx:
+        r = ctx.run(L.PROB_CUBE_WAVE, dt, 1e9, max_steps=steps, want_u=True)
+    ro = oracle_for(m, p).run(oracle.PROB_CUBE_WAVE, dt, 1e9, max_steps=steps, want_u=True)
+    assert r["nsteps"] == steps == ro["nsteps"]
+    assert rel(r["u"], ro["u"]) < 1e-11
+    assert abs(r["err_l2"] / ro["err_l2"] - 1) < 1e-9
+    assert abs(r["err_dg"] / ro["err_dg"] - 1) < 1e-9
+    assert np.isfinite(r["err_l2"]) and r["err_l2"] < 0.5
+
+
+def test_singular_mass_reports_exit_code():
+    m = M.cube_mesh(1)
+    m.rho = -np.ones(m.n_tet)  # host-forced: M = rho * mass is negative definite
+    with L.Context(m, 2) as ctx:
+        with pytest.raises(L.LymphError) as e:
+            ctx.make_matrices_free()
+        assert e.value.code == _lib.EXIT_SINGULARMTX
+        with pytest.raises(L.LymphError) as e:
+            ctx.time_step_matrix_free(0.0, 1e-3, ctx.vec(), ctx.vec())
+        assert e.value.code == _lib.ERR_STATE
