@@ -131,6 +131,8 @@ def main():
     t_build = time.perf_counter() - t0
     log(f"setup + {1 + a.warmup} steps: {t_build:.2f}s; device bytes {ctx.device_bytes / 1e9:.2f} GB")
     t_start = r0["t_final"]
+    chosen, tune = ctx.autotune(reps=20)
+    log(f"autotune: group size {chosen}; ms/step by group size {tune}")
     # timed region: K device-resident steps, HIP events on the context stream
     w0 = time.perf_counter()
     ms = ctx.time_steps(prob, a.steps, t_start, dt)
@@ -170,6 +172,7 @@ def main():
             "fp64_frac": ctx.step_flops / (ms_step * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
         },
         "wall_ms_per_step": wall * 1e3 / a.steps,
+        "autotune": {"chosen_unroll": chosen, "ms_per_step": tune},
         "err_l2": e_l2, "err_dg": e_dg,
         "setup_s": t_build,
     }

@@ -174,6 +174,15 @@ int lymph_step_n(LymphCtx* ctx, int problem, int nsteps, double t0, double dt,
 int lymph_time_steps(LymphCtx* ctx, int problem, int nsteps, double t0,
                      double dt, double* ms);
 
+/* Time every variant of the step kernel (how many block column pairs are
+ * loaded per unrolled group) for `reps` steps on copies of the state slots,
+ * select the fastest for this context and report them: n_variants, unr[i],
+ * ms[i] (milliseconds per step) -- arrays of at least 8 entries -- and the
+ * chosen group size.  No reference counterpart (the reference has one CPU
+ * loop); results do not depend on the variant. */
+int lymph_autotune(LymphCtx* ctx, int reps, int* n_variants, int* unr, double* ms,
+                   int* chosen);
+
 /* COMPUTE_ERROR_L2_MATRIX_FREE / _DG_ (src/matrix_free.f90:1388-1504):
  * squared norms of uh - uex; host pointers or slots. */
 int lymph_error_l2(LymphCtx* ctx, const double* uh, const double* uex,

@@ -36,7 +36,7 @@ EXPORTS = [
     "lymph_get_slot", "lymph_slot_ptr", "lymph_first_step", "lymph_step",
     "lymph_first_step_dev", "lymph_step_dev", "lymph_step_n", "lymph_time_steps",
     "lymph_error_l2", "lymph_error_dg", "lymph_error_slots", "lymph_run",
-    "lymph_time_function",
+    "lymph_time_function", "lymph_autotune",
 ]
 
 
@@ -99,6 +99,7 @@ def lib():
         "lymph_error_slots": (_i, [_vp, _i, _i, _vp, _vp]),
         "lymph_run": (_i, [_vp, _i, _d, _d, _i, _vp, _vp]),
         "lymph_time_function": (_d, [_i, _d]),
+        "lymph_autotune": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -57,6 +57,7 @@ struct LymphCtx {
   double alpha = 10.0, theta = -1.0;
   int n_tet = 0, n_poly = 0, n_blk = 0;
   int layout = LYMPH_LAYOUT_POLY;
+  int step_variant = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
   bool built = false;
@@ -87,6 +88,30 @@ int dispatch_p(int p, F&& f) {
     case 4: return f(std::integral_constant<int, 4>{});
   }
   return fail(LYMPH_ERR_ARG, "polynomial degree must be 1..4");
+}
+
+// Step-kernel variants (column pairs per unrolled load group) per degree;
+// index 0 is the default until lymph_autotune picks one.
+template <int P> struct StepUnr;
+template <> struct StepUnr<1> { static constexpr int v[] = {6, 1, 2, 3}; };
+template <> struct StepUnr<2> { static constexpr int v[] = {5, 3, 15}; };
+template <> struct StepUnr<3> { static constexpr int v[] = {10, 3, 5, 6, 15, 30}; };
+template <> struct StepUnr<4> { static constexpr int v[] = {4, 1, 2, 8}; };
+template <int P> constexpr int n_unr() { return (int)(sizeof(StepUnr<P>::v) / sizeof(int)); }
+
+template <int P, int MODE, int I = 0>
+void launch_leapfrog(int variant, int grid, hipStream_t st, const double* K, const int* rp,
+                     const int* col, const double* Rp, const double* rs, const double* rd,
+                     const double* u, const double* wa, double* out, double g, double c1,
+                     double c2) {
+  if constexpr (I < n_unr<P>()) {
+    if (variant == I) {
+      k_leapfrog<P, MODE, StepUnr<P>::v[I]><<<grid, 64, 0, st>>>(K, rp, col, Rp, rs, rd, u, wa, out,
+                                                                 g, c1, c2);
+      return;
+    }
+    launch_leapfrog<P, MODE, I + 1>(variant, grid, st, K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2);
+  }
 }
 
 int check_launch(const char* what) {
@@ -127,16 +152,14 @@ int launch_step(LymphCtx* c, int mode, double g, double c1, double c2, const dou
                 const double* wa, double* out) {
   return dispatch_p(c->p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
+    const double* rs = c->slot[LYMPH_SLOT_RHS_STAT];
+    const double* rd = c->slot[LYMPH_SLOT_RHS_DYN];
     if (mode == 0)
-      k_leapfrog<P, 0><<<c->n_poly, 64, 0, c->stream>>>(c->K, c->md.row_ptr, c->md.col, c->RM,
-                                                         c->slot[LYMPH_SLOT_RHS_STAT],
-                                                         c->slot[LYMPH_SLOT_RHS_DYN], u, wa, out, g,
-                                                         c1, c2);
+      launch_leapfrog<P, 0>(c->step_variant, c->n_poly, c->stream, c->K, c->md.row_ptr, c->md.col,
+                            c->RM, rs, rd, u, wa, out, g, c1, c2);
     else
-      k_leapfrog<P, 1><<<c->n_poly, 64, 0, c->stream>>>(c->K, c->md.row_ptr, c->md.col, c->RM,
-                                                         c->slot[LYMPH_SLOT_RHS_STAT],
-                                                         c->slot[LYMPH_SLOT_RHS_DYN], u, wa, out, g,
-                                                         c1, c2);
+      launch_leapfrog<P, 1>(c->step_variant, c->n_poly, c->stream, c->K, c->md.row_ptr, c->md.col,
+                            c->RM, rs, rd, u, wa, out, g, c1, c2);
     return check_launch("k_leapfrog");
   });
 }
@@ -153,9 +176,8 @@ int errors_dev(LymphCtx* c, const double* uh, const double* ue, double* l2sq, do
     }
     if (dgsq) {
       k_sub<<<grid1(n, 256), 256, 0, c->stream>>>(uh, ue, c->stage[2], n);
-      k_leapfrog<P, 2><<<c->n_poly, 64, 0, c->stream>>>(c->A, c->md.row_ptr, c->md.col, nullptr,
-                                                         nullptr, nullptr, c->stage[2], nullptr,
-                                                         c->part, 0.0, 0.0, 0.0);
+      launch_leapfrog<P, 2>(0, c->n_poly, c->stream, c->A, c->md.row_ptr, c->md.col, nullptr,
+                            nullptr, nullptr, c->stage[2], nullptr, c->part, 0.0, 0.0, 0.0);
       k_seq_sum<<<1, 64, 0, c->stream>>>(c->part, c->n_poly, c->dsum + 1);
     }
     return check_launch("error kernels");
@@ -585,6 +607,53 @@ int lymph_time_steps(LymphCtx* c, int problem, int nsteps, double t0, double dt,
   if (rc) return rc;
   if (ms) *ms = f;
   return 0;
+}
+
+int lymph_autotune(LymphCtx* c, int reps, int* n_variants, int* unr, double* ms, int* chosen) {
+  if (int rc = need_built(c)) return rc;
+  if (reps < 1) return fail(LYMPH_ERR_ARG, "reps < 1");
+  return dispatch_p(c->p, [&](auto pc) -> int {
+    constexpr int P = decltype(pc)::value;
+    constexpr int NV = n_unr<P>();
+    if (n_variants) *n_variants = NV;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    // scratch inputs: the current state slots are copied so nothing the
+    // caller holds is modified
+    const size_t nb = vec_n(c) * sizeof(double);
+    HIP_TRY(hipMemcpyAsync(c->stage[0], c->slot[LYMPH_SLOT_U_PREV], nb, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->stage[1], c->slot[LYMPH_SLOT_U_CUR], nb, hipMemcpyDeviceToDevice, c->stream));
+    const int saved = c->step_variant;
+    double best = 1e300;
+    int bi = 0;
+    for (int v = 0; v < NV; ++v) {
+      c->step_variant = v;
+      if (int rc = launch_step(c, 0, 0.5, 1e-8, 0.0, c->stage[1], c->stage[0], c->stage[2])) {
+        c->step_variant = saved;
+        return rc;
+      }
+      HIP_TRY(hipEventRecord(e0, c->stream));
+      for (int r = 0; r < reps; ++r)
+        if (int rc = launch_step(c, 0, 0.5, 1e-8, 0.0, c->stage[1], c->stage[0], c->stage[2])) {
+          c->step_variant = saved;
+          return rc;
+        }
+      HIP_TRY(hipEventRecord(e1, c->stream));
+      HIP_TRY(hipEventSynchronize(e1));
+      float f = 0.f;
+      HIP_TRY(hipEventElapsedTime(&f, e0, e1));
+      const double t = (double)f / reps;
+      if (unr) unr[v] = StepUnr<P>::v[v];
+      if (ms) ms[v] = t;
+      if (t < best) { best = t; bi = v; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->step_variant = bi;
+    if (chosen) *chosen = StepUnr<P>::v[bi];
+    return 0;
+  });
 }
 
 int lymph_error_l2(LymphCtx* c, const double* uh, const double* uex, double* err2) {

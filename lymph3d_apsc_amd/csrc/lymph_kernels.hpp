@@ -479,7 +479,9 @@ __global__ __launch_bounds__(128) void k_project(MeshDev m, int fn, const double
 //   MODE 1: out = (c1*x + u) + c2*wa           (c1 = dt^2/2, u = u^0, wa = v^0, c2 = dt)
 //   MODE 2: part[P] = sum_r u_r (K u)_r        (DG error: K = A_dg, u = uh - uex)
 // ---------------------------------------------------------------------------
-template <int P, int MODE>
+// UNR = column pairs whose loads are issued together (register footprint vs
+// loads in flight; chosen per degree by the autotuner in lymph_autotune).
+template <int P, int MODE, int UNR>
 __global__ __launch_bounds__(64) void k_leapfrog(const double* __restrict__ K,
                                                  const int* __restrict__ row_ptr,
                                                  const int* __restrict__ col,
@@ -510,17 +512,34 @@ __global__ __launch_bounds__(64) void k_leapfrog(const double* __restrict__ K,
     double s[RPL];
 #pragma unroll
     for (int k = 0; k < RPL; ++k) s[k] = 0.0;
+#pragma unroll 1
+    for (int c0 = 0; c0 < NPAIR; c0 += UNR) {
+      double2 kv[UNR][RPL];
 #pragma unroll
-    for (int c2i = 0; c2i < NPAIR; ++c2i) {
-      const double x0 = xv[2 * c2i];
-      const double x1 = (2 * c2i + 1 < N3) ? xv[2 * c2i + 1] : 0.0;
+      for (int uu = 0; uu < UNR; ++uu) {
+        const int c2i = c0 + uu;
 #pragma unroll
-      for (int k = 0; k < RPL; ++k) {
-        const int r = lane + 64 * k;
-        if (r < N3) {
-          const double2 kv = *reinterpret_cast<const double2*>(Kb + ((size_t)c2i * N3 + r) * 2);
-          s[k] = s[k] + kv.x * x0;
-          if (2 * c2i + 1 < N3) s[k] = s[k] + kv.y * x1;
+        for (int k = 0; k < RPL; ++k) {
+          const int r = lane + 64 * k;
+          if ((NPAIR % UNR == 0 || c2i < NPAIR) && r < N3)
+            kv[uu][k] = *reinterpret_cast<const double2*>(Kb + ((size_t)c2i * N3 + r) * 2);
+        }
+      }
+#pragma unroll
+      for (int uu = 0; uu < UNR; ++uu) {
+        const int c2i = c0 + uu;
+        if (NPAIR % UNR == 0 || c2i < NPAIR) {
+          const bool odd_tail = (N3 % 2 == 1) && (c2i == NPAIR - 1);
+          const double x0 = xv[2 * c2i];
+          const double x1 = odd_tail ? 0.0 : xv[2 * c2i + 1];
+#pragma unroll
+          for (int k = 0; k < RPL; ++k) {
+            const int r = lane + 64 * k;
+            if (r < N3) {
+              s[k] = s[k] + kv[uu][k].x * x0;
+              if (!odd_tail) s[k] = s[k] + kv[uu][k].y * x1;
+            }
+          }
         }
       }
     }

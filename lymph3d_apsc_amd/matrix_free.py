@@ -139,6 +139,17 @@ class Context:
         check(lib().lymph_time_steps(self.h, problem, nsteps, t0, dt, C.byref(ms)), "lymph_time_steps")
         return ms.value
 
+    def autotune(self, reps=20):
+        """Time the step-kernel variants and keep the fastest; returns
+        (chosen group size, {group size: ms per step})."""
+        nv = C.c_int()
+        unr = np.zeros(8, np.int32)
+        ms = np.zeros(8)
+        ch = C.c_int()
+        check(lib().lymph_autotune(self.h, reps, C.byref(nv), ptr(unr), ptr(ms), C.byref(ch)),
+              "lymph_autotune")
+        return ch.value, {int(unr[i]): float(ms[i]) for i in range(nv.value)}
+
     def compute_error_l2_matrix_free(self, uh, uex):
         e = C.c_double()
         uh, uex = self._c(uh), self._c(uex)
