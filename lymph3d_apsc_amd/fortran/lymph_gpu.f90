@@ -1,0 +1,150 @@
+! lymph_gpu.f90 -- ISO_C_BINDING interface to liblymph_gpu.so for LYMPH3D's
+! Fortran host (declarations of include/lymph_gpu.h, one to one).
+!
+! A Fortran caller keeps its own mesh set-up and passes the flattened mesh
+! (0-based ids, see type lymph_mesh) to lymph_ctx_create; the drop-ins for
+! MAKE_MATRICES_FREE, MAKE_RHS_FREE, COMPUTE_MODAL_COEFFICIENTS_FREE,
+! FIRST_TIME_STEP_MATRIX_FREE, TIME_STEP_MATRIX_FREE and
+! COMPUTE_ERROR_*_MATRIX_FREE (src/matrix_free.f90) are the functions below.
+! With lymph_ctx_set_layout(ctx, LYMPH_LAYOUT_FORTRAN) the solution arrays are
+! the reference's own u(npoly, 3*Np) column-major arrays, passed as they are.
+module lymph_gpu
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: LYMPH_EXIT_SETUP = 4, LYMPH_EXIT_SINGULARMTX = 5
+  integer(c_int), parameter :: LYMPH_EXIT_NEIGHBOUR = 18
+  integer(c_int), parameter :: LYMPH_ERR_ARG = 101, LYMPH_ERR_HIP = 102, LYMPH_ERR_STATE = 103
+  integer(c_int), parameter :: LYMPH_LAYOUT_POLY = 0, LYMPH_LAYOUT_FORTRAN = 1
+  integer(c_int), parameter :: LYMPH_PROB_NULL = 0, LYMPH_PROB_REF_MMS = 1, LYMPH_PROB_CUBE_WAVE = 2
+  integer(c_int), parameter :: LYMPH_FN_ZERO = 0, LYMPH_FN_REF_VEL = 1, LYMPH_FN_REF_UEX = 2
+  integer(c_int), parameter :: LYMPH_FN_CUBE_U = 3
+  integer(c_int), parameter :: LYMPH_SLOT_RHS_STAT = 0, LYMPH_SLOT_RHS_DYN = 1
+  integer(c_int), parameter :: LYMPH_SLOT_U_PREV = 2, LYMPH_SLOT_U_CUR = 3, LYMPH_SLOT_U_NEXT = 4
+  integer(c_int), parameter :: LYMPH_SLOT_V0 = 5, LYMPH_SLOT_UEX = 6
+
+  ! struct LymphMesh
+  type, bind(C) :: lymph_mesh
+    integer(c_int32_t) :: n_node, n_tet, n_poly
+    type(c_ptr) :: xyz, tet_vert, tet_poly, neigh, bc_space, normal, area, bbox, hk
+    type(c_ptr) :: rho, lam, mu
+  end type lymph_mesh
+
+  interface
+    function lymph_last_error() bind(C, name='lymph_last_error') result(s)
+      import :: c_ptr
+      type(c_ptr) :: s
+    end function
+    function lymph_device_count() bind(C, name='lymph_device_count') result(n)
+      import :: c_int
+      integer(c_int) :: n
+    end function
+    function lymph_ctx_create(mesh, p, alpha, theta, ctx) bind(C, name='lymph_ctx_create') result(rc)
+      import :: c_int, c_double, c_ptr, lymph_mesh
+      type(lymph_mesh), intent(in) :: mesh
+      integer(c_int), value :: p
+      real(c_double), value :: alpha, theta
+      type(c_ptr), intent(out) :: ctx
+      integer(c_int) :: rc
+    end function
+    function lymph_ctx_destroy(ctx) bind(C, name='lymph_ctx_destroy') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    function lymph_ctx_set_layout(ctx, layout) bind(C, name='lymph_ctx_set_layout') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: layout
+      integer(c_int) :: rc
+    end function
+    function lymph_build_operator(ctx) bind(C, name='lymph_build_operator') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    function lymph_build_rhs(ctx, problem, slot, rhs) bind(C, name='lymph_build_rhs') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: problem, slot
+      real(c_double) :: rhs(*)
+      integer(c_int) :: rc
+    end function
+    function lymph_project(ctx, fn, slot, coeff) bind(C, name='lymph_project') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: fn, slot
+      real(c_double) :: coeff(*)
+      integer(c_int) :: rc
+    end function
+    function lymph_first_step(ctx, g0, dt, u0, v0, u1) bind(C, name='lymph_first_step') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: g0, dt
+      real(c_double), intent(in) :: u0(*), v0(*)
+      real(c_double), intent(out) :: u1(*)
+      integer(c_int) :: rc
+    end function
+    function lymph_step(ctx, g, dt, um1, u, up1) bind(C, name='lymph_step') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: g, dt
+      real(c_double), intent(in) :: um1(*), u(*)
+      real(c_double), intent(out) :: up1(*)
+      integer(c_int) :: rc
+    end function
+    function lymph_error_l2(ctx, uh, uex, err2) bind(C, name='lymph_error_l2') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: uh(*), uex(*)
+      real(c_double), intent(out) :: err2
+      integer(c_int) :: rc
+    end function
+    function lymph_error_dg(ctx, uh, uex, err2) bind(C, name='lymph_error_dg') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: uh(*), uex(*)
+      real(c_double), intent(out) :: err2
+      integer(c_int) :: rc
+    end function
+    function lymph_run(ctx, problem, dt, stop_time, max_steps, u_final, out) &
+        bind(C, name='lymph_run') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: problem, max_steps
+      real(c_double), value :: dt, stop_time
+      real(c_double) :: u_final(*)
+      real(c_double), intent(out) :: out(4)
+      integer(c_int) :: rc
+    end function
+    function lymph_time_function(problem, t) bind(C, name='lymph_time_function') result(g)
+      import :: c_int, c_double
+      integer(c_int), value :: problem
+      real(c_double), value :: t
+      real(c_double) :: g
+    end function
+  end interface
+
+contains
+
+  ! Message of the last failing call, as a Fortran string.
+  function lymph_error_message() result(msg)
+    character(len=:), allocatable :: msg
+    character(kind=c_char), pointer :: p(:)
+    integer :: n
+    interface
+      function c_strlen(s) bind(C, name='strlen') result(n)
+        import :: c_ptr, c_size_t
+        type(c_ptr), value :: s
+        integer(c_size_t) :: n
+      end function
+    end interface
+    type(c_ptr) :: s
+    s = lymph_last_error()
+    n = int(c_strlen(s))
+    call c_f_pointer(s, p, [n])
+    allocate(character(len=n) :: msg)
+    msg = transfer(p(1:n), msg)
+  end function
+
+end module lymph_gpu

@@ -162,6 +162,21 @@ class Mesh:
     def n_dof(self, p):
         return 3 * self.n_poly * (p + 1) * (p + 2) * (p + 3) // 6
 
+    def write_flat(self, path, p, dt, stop_time):
+        """Write the flattened mesh as the stream file read by the Fortran
+        drop-in driver (fortran/drive_mesh.f90): int32 n_node, n_tet, n_poly,
+        p, 0; float64 dt, stop_time; then the LymphMesh arrays in field order."""
+        with open(path, "wb") as f:
+            np.array([self.xyz.shape[0], self.n_tet, self.n_poly, p, 0], np.int32).tofile(f)
+            np.array([dt, stop_time], np.float64).tofile(f)
+            for a, dt_ in ((self.xyz, np.float64), (self.tet_vert, np.int32),
+                           (self.tet_poly, np.int32), (self.neigh, np.int32),
+                           (self.bc_space, np.int32), (self.normal, np.float64),
+                           (self.area, np.float64), (self.bbox, np.float64),
+                           (self.hk, np.float64), (self.rho, np.float64),
+                           (self.lam, np.float64), (self.mu, np.float64)):
+                np.ascontiguousarray(a, dt_).tofile(f)
+
     def to_c(self):
         """Return (LymphMesh struct, keep-alive list)."""
         from ._lib import LymphMesh, ptr

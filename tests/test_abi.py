@@ -66,3 +66,21 @@ def test_argument_errors_before_device():
     assert rc == _lib.EXIT_NEIGHBOUR
     assert L.lymph_build_operator(None) == _lib.ERR_ARG
     assert abs(L.lymph_time_function(1, 0.25) - np.sin(np.sqrt(2) * np.pi * 0.25)) < 1e-15
+
+
+def test_fortran_dropin_links_and_fails_loudly_without_gpu(tmp_path):
+    """The ISO_C_BINDING module and the reference-loop driver
+    (lymph3d_apsc_amd/fortran) link against liblymph_gpu.so; with no device
+    the driver stops at lymph_ctx_create with the library's message."""
+    import subprocess
+    exe = os.path.join(ROOT, "lymph3d_apsc_amd", "fortran", "drive_mesh")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran driver not built (no amdflang)")
+    if _lib.lib().lymph_device_count() > 0:
+        pytest.skip("a GPU is visible")
+    m = mesh.read_deck(os.path.join(ROOT, "tests", "golden", "decks", "tet_5"))
+    f = tmp_path / "t5.bin"
+    m.write_flat(str(f), m.degree, m.dt, m.stop_time)
+    r = subprocess.run([exe, str(f)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 3, r.stdout + r.stderr
+    assert "no HIP device" in r.stdout
