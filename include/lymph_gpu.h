@@ -183,6 +183,21 @@ int lymph_time_steps(LymphCtx* ctx, int problem, int nsteps, double t0,
 int lymph_autotune(LymphCtx* ctx, int reps, int* n_variants, int* unr, double* ms,
                    int* chosen);
 
+/* Receivers: point values of the displacement, evaluated like EXPORT_SOLUTION
+ * (src/solution_processing.f90:150-170) in the tet given for each point
+ * (e.g. located like FIND_ELEM_FROM_POINT, src/find_tet.f90:82-124, nearest
+ * barycentre).  xyz[n*3], tet[n]; room for max_records trace rows. */
+int lymph_set_receivers(LymphCtx* ctx, int n, const double* xyz, const int32_t* tet,
+                        int max_records);
+/* out[n*3]: receiver values of the vector in `slot`. */
+int lymph_sample_receivers(LymphCtx* ctx, int slot, double* out);
+/* lymph_step_n that also records the receiver values of u^{n+1} after every
+ * `every`-th step into the trace buffer (device-resident). */
+int lymph_step_n_record(LymphCtx* ctx, int problem, int nsteps, double t0, double dt,
+                        int every, double* t_out);
+/* out[n_records][n][3] (may be NULL to query the count). */
+int lymph_get_traces(LymphCtx* ctx, double* out, int* n_records);
+
 /* COMPUTE_ERROR_L2_MATRIX_FREE / _DG_ (src/matrix_free.f90:1388-1504):
  * squared norms of uh - uex; host pointers or slots. */
 int lymph_error_l2(LymphCtx* ctx, const double* uh, const double* uex,

@@ -61,7 +61,7 @@ struct LymphCtx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   bool built = false;
-  std::vector<int> h_row_ptr, h_col;
+  std::vector<int> h_row_ptr, h_col, h_tet_poly;
   // device
   MeshDev md{};
   std::vector<void*> owned;  // everything hipMalloc'ed for the mesh
@@ -72,6 +72,11 @@ struct LymphCtx {
   double* dsum = nullptr;
   int* status = nullptr;
   size_t bytes = 0;
+  // receivers
+  int n_recv = 0, max_rec = 0, n_rec = 0;
+  int* rpoly = nullptr;
+  double* rxyz = nullptr;
+  double* trace = nullptr;
 };
 
 namespace {
@@ -218,6 +223,9 @@ double lymph_time_function(int problem, double t) { return time_factor(problem, 
 int lymph_ctx_destroy(LymphCtx* c) {
   if (!c) return 0;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->rpoly) (void)hipFree(c->rpoly);
+  if (c->rxyz) (void)hipFree(c->rxyz);
+  if (c->trace) (void)hipFree(c->trace);
   for (void* p : c->owned) (void)hipFree(p);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -300,6 +308,7 @@ int lymph_ctx_create(const LymphMesh* mesh, int p, double alpha, double theta, L
   c->n_poly = np;
   c->n_blk = (int)col.size();
   c->h_row_ptr = row_ptr;
+  c->h_tet_poly = tpoly;
   c->h_col = col;
 
   std::vector<double> txyz((size_t)nt * 12);
@@ -654,6 +663,80 @@ int lymph_autotune(LymphCtx* c, int reps, int* n_variants, int* unr, double* ms,
     if (chosen) *chosen = StepUnr<P>::v[bi];
     return 0;
   });
+}
+
+int lymph_set_receivers(LymphCtx* c, int n, const double* xyz, const int32_t* tet,
+                        int max_records) {
+  if (int rc = need_ctx(c)) return rc;
+  if (n < 0 || max_records < 0 || (n > 0 && (!xyz || !tet)))
+    return fail(LYMPH_ERR_ARG, "bad receiver arguments");
+  std::vector<int> rp(n);
+  for (int i = 0; i < n; ++i) {
+    if (tet[i] < 0 || tet[i] >= c->n_tet) return fail(LYMPH_ERR_ARG, "receiver tet out of range");
+    rp[i] = c->h_tet_poly[tet[i]];
+  }
+  if (c->rpoly) (void)hipFree(c->rpoly);
+  if (c->rxyz) (void)hipFree(c->rxyz);
+  if (c->trace) (void)hipFree(c->trace);
+  c->rpoly = nullptr; c->rxyz = nullptr; c->trace = nullptr;
+  c->n_recv = n; c->max_rec = max_records; c->n_rec = 0;
+  if (n == 0) return 0;
+  if (int rc = upload(&c->rpoly, rp.data(), rp.size())) return rc;
+  if (int rc = upload(&c->rxyz, xyz, 3 * (size_t)n)) return rc;
+  if (int rc = dalloc(&c->trace, 3 * (size_t)n * (size_t)(max_records > 0 ? max_records : 1))) return rc;
+  return 0;
+}
+
+namespace {
+int sample_into(LymphCtx* c, const double* u, double* d_out) {
+  return dispatch_p(c->p, [&](auto pc) -> int {
+    constexpr int P = decltype(pc)::value;
+    k_sample<P><<<grid1(3 * (size_t)c->n_recv, 128), 128, 0, c->stream>>>(
+        u, c->rpoly, c->rxyz, c->md.bbox, c->md.blist, c->n_recv, d_out);
+    return check_launch("k_sample");
+  });
+}
+}  // namespace
+
+int lymph_sample_receivers(LymphCtx* c, int slot, double* out) {
+  if (int rc = need_ctx(c)) return rc;
+  if (slot < 0 || slot >= LYMPH_NUM_SLOTS || !out) return fail(LYMPH_ERR_ARG, "bad slot/pointer");
+  if (c->n_recv == 0) return fail(LYMPH_ERR_STATE, "no receivers set");
+  if (int rc = sample_into(c, c->slot[slot], c->stage[3])) return rc;
+  HIP_TRY(hipMemcpyAsync(out, c->stage[3], 3 * (size_t)c->n_recv * sizeof(double),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int lymph_step_n_record(LymphCtx* c, int problem, int nsteps, double t0, double dt, int every,
+                        double* t_out) {
+  if (int rc = need_built(c)) return rc;
+  if (c->n_recv == 0) return fail(LYMPH_ERR_STATE, "no receivers set");
+  if (every < 1 || nsteps < 0) return fail(LYMPH_ERR_ARG, "bad nsteps/every");
+  double t = t0;
+  for (int s = 1; s <= nsteps; ++s) {
+    if (int rc = lymph_step_n(c, problem, 1, t, dt, &t)) return rc;
+    if (s % every == 0) {
+      if (c->n_rec >= c->max_rec) return fail(LYMPH_ERR_ARG, "receiver trace buffer full");
+      if (int rc = sample_into(c, c->slot[LYMPH_SLOT_U_CUR],
+                               c->trace + (size_t)c->n_rec * 3 * c->n_recv))
+        return rc;
+      c->n_rec++;
+    }
+  }
+  if (t_out) *t_out = t;
+  return 0;
+}
+
+int lymph_get_traces(LymphCtx* c, double* out, int* n_records) {
+  if (int rc = need_ctx(c)) return rc;
+  if (n_records) *n_records = c->n_rec;
+  if (out && c->n_rec > 0)
+    HIP_TRY(hipMemcpyAsync(out, c->trace, (size_t)c->n_rec * 3 * c->n_recv * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
 }
 
 int lymph_error_l2(LymphCtx* c, const double* uh, const double* uex, double* err2) {

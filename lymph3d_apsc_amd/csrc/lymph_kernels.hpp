@@ -641,4 +641,36 @@ __global__ void k_to_fortran(const double* __restrict__ p, double* __restrict__ 
   }
 }
 
+// Receivers: value of the modal field at a point, sum_i u_i Lx Ly Lz on the
+// polyhedron's bounding box, as EXPORT_SOLUTION evaluates it
+// (src/solution_processing.f90:150-170).  One thread per (receiver, comp);
+// out[r*3 + comp] of record row `row`.
+template <int P>
+__global__ void k_sample(const double* __restrict__ u, const int* __restrict__ rpoly,
+                         const double* __restrict__ rxyz, const double* __restrict__ bbox,
+                         const int* __restrict__ blist, int n, double* __restrict__ out) {
+  using D = Dims<P>;
+  constexpr int NP = D::NP;
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= 3 * n) return;
+  const int r = id / 3, comp = id % 3;
+  const int Pp = rpoly[r];
+  const double* bb = bbox + (size_t)Pp * 6;
+  double xp[3], sq[3];
+  for (int d = 0; d < 3; ++d) {
+    const double hb = (bb[2 * d + 1] - bb[2 * d]) / 2.0, mb = (bb[2 * d + 1] + bb[2 * d]) / 2.0;
+    xp[d] = (rxyz[r * 3 + d] - mb) / hb;
+    sq[d] = sqrt(hb);
+  }
+  const double* uu = u + (size_t)Pp * D::N3 + comp * NP;
+  double s = 0.0;
+  for (int i = 0; i < NP; ++i) {
+    const double vx = leg(blist[i * 3 + 0], xp[0]) / sq[0];
+    const double vy = leg(blist[i * 3 + 1], xp[1]) / sq[1];
+    const double vz = leg(blist[i * 3 + 2], xp[2]) / sq[2];
+    s = s + uu[i] * vx * vy * vz;
+  }
+  out[id] = s;
+}
+
 }  // namespace lymph

@@ -150,6 +150,35 @@ class Context:
               "lymph_autotune")
         return ch.value, {int(unr[i]): float(ms[i]) for i in range(nv.value)}
 
+    # ---- receivers ------------------------------------------------------
+    def set_receivers(self, points, max_records=0, tets=None):
+        """Receivers at `points` (n, 3), located like FIND_ELEM_FROM_POINT
+        (nearest tet barycentre) unless `tets` is given."""
+        pts = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+        tets = self.mesh.locate(pts) if tets is None else np.ascontiguousarray(tets, np.int32)
+        self.n_recv = pts.shape[0]
+        check(lib().lymph_set_receivers(self.h, self.n_recv, ptr(pts), ptr(tets), max_records),
+              "lymph_set_receivers")
+        return tets
+
+    def sample_receivers(self, slot=_lib.SLOT_U_CUR):
+        out = np.zeros((self.n_recv, 3))
+        check(lib().lymph_sample_receivers(self.h, slot, ptr(out)), "lymph_sample_receivers")
+        return out
+
+    def step_n_record(self, problem, nsteps, t0, dt, every=1):
+        t = C.c_double()
+        check(lib().lymph_step_n_record(self.h, problem, nsteps, t0, dt, every, C.byref(t)),
+              "lymph_step_n_record")
+        return t.value
+
+    def traces(self):
+        n = C.c_int()
+        check(lib().lymph_get_traces(self.h, None, C.byref(n)), "lymph_get_traces")
+        out = np.zeros((n.value, self.n_recv, 3))
+        check(lib().lymph_get_traces(self.h, ptr(out), C.byref(n)), "lymph_get_traces")
+        return out
+
     def compute_error_l2_matrix_free(self, uh, uex):
         e = C.c_double()
         uh, uex = self._c(uh), self._c(uex)

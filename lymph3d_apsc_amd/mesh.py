@@ -156,6 +156,19 @@ class Mesh:
         self.bbox = np.ascontiguousarray(bbox)
         self.hk = np.ascontiguousarray(hk)
 
+    def locate(self, points):
+        """Tet of each point as FIND_ELEM_FROM_POINT picks it (src/find_tet.f90:
+        82-124): nearest barycentre ((x1+x2+x3+x4)/4), first minimum on ties."""
+        pts = np.asarray(points, np.float64).reshape(-1, 3)
+        v = self.xyz[self.tet_vert]
+        bc = (((v[:, 0] + v[:, 1]) + v[:, 2]) + v[:, 3]) / 4
+        out = np.zeros(pts.shape[0], np.int32)
+        for k in range(pts.shape[0]):
+            d = pts[k] - bc
+            d2 = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]
+            out[k] = int(np.argmin(d2))
+        return out
+
     def hmax(self):
         return float(self.hk.max())  # compute_hmax, src/solution_processing.f90:197-214
 

@@ -1421,3 +1421,48 @@ int o_run(OCtx* c, int problem, double dt, double stop_time, int max_steps,
   (void)PI_;
   return 0;
 }
+
+/* FIND_ELEM_FROM_POINT (src/find_tet.f90:82-124): the tet whose barycentre
+ * (COMPUTE_BARYCENTER :151-181) is nearest, first minimum on ties. */
+void o_locate_points(const OCtx* c, int n, const double* pts, int* tet) {
+  for (int k = 0; k < n; ++k) {
+    double dmin = 1.0e15;
+    int best = -1;
+    for (int t = 0; t < c->n_tet; ++t) {
+      double b[3];
+      for (int d = 0; d < 3; ++d) {
+        double s = 0.0;
+        for (int v = 0; v < 4; ++v) s = s + c->xyz[3 * c->tet_vert[t * 4 + v] + d];
+        b[d] = s / 4;
+      }
+      double dx = pts[k * 3] - b[0], dy = pts[k * 3 + 1] - b[1], dz = pts[k * 3 + 2] - b[2];
+      double d2 = dx * dx + dy * dy + dz * dz;
+      if (d2 < dmin) { dmin = d2; best = t; }
+    }
+    tet[k] = best;
+  }
+}
+
+/* Point value of the modal field, as EXPORT_SOLUTION evaluates it at nodes
+ * (src/solution_processing.f90:150-170): sum_i u_i Lx Ly Lz on the
+ * polyhedron's bounding box.  out[k*3 + comp]. */
+void o_eval_points(const OCtx* c, int n, const double* pts, const int* tet, const double* u,
+                   double* out) {
+  int Np = c->Np;
+  for (int k = 0; k < n; ++k) {
+    int P = c->tet_poly[tet[k]];
+    const double* bb = &c->bbox[P * 6];
+    for (int comp = 0; comp < 3; ++comp) {
+      double s = 0.0;
+      for (int i = 0; i < Np; ++i) {
+        double v[3];
+        for (int d = 0; d < 3; ++d) {
+          double hb = (bb[2 * d + 1] - bb[2 * d]) / 2.0, mb = (bb[2 * d + 1] + bb[2 * d]) / 2.0;
+          v[d] = leg(c->blist[i * 3 + d], (pts[k * 3 + d] - mb) / hb) / sqrt(hb);
+        }
+        s = s + u[(size_t)P * c->n3 + comp * Np + i] * v[0] * v[1] * v[2];
+      }
+      out[k * 3 + comp] = s;
+    }
+  }
+}

@@ -108,6 +108,11 @@ int o_run(OCtx* c, int problem, double dt, double stop_time, int max_steps,
 
 double o_time_function(int problem, double t);
 
+/* receivers: nearest-barycentre location and point evaluation */
+void o_locate_points(const OCtx* c, int n, const double* pts, int* tet);
+void o_eval_points(const OCtx* c, int n, const double* pts, const int* tet, const double* u,
+                   double* out);
+
 /* ---- element-level restatements (checked against the reference's own
  *      Fortran routines by the tests) ---- */
 int o_np(int p);

@@ -57,6 +57,8 @@ def lib():
         L.o_run.restype = C.c_int
         L.o_time_function.argtypes = [C.c_int, C.c_double]
         L.o_time_function.restype = C.c_double
+        L.o_locate_points.argtypes = [C.c_void_p, C.c_int, _dp, _ip]
+        L.o_eval_points.argtypes = [C.c_void_p, C.c_int, _dp, _ip, _dp, _dp]
         L.o_np.argtypes = [C.c_int]
         L.o_np.restype = C.c_int
         L.o_basis_list.argtypes = [C.c_int, _ip]
@@ -185,6 +187,19 @@ class Oracle:
 
     def error_dg_sq(self, uh, uex):
         return lib().o_error_dg_sq(self.h, uh, uex)
+
+    def locate(self, pts):
+        pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+        tet = np.zeros(pts.shape[0], np.int32)
+        lib().o_locate_points(self.h, pts.shape[0], pts, tet)
+        return tet
+
+    def eval_points(self, pts, tet, u):
+        pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+        out = np.zeros((pts.shape[0], 3))
+        lib().o_eval_points(self.h, pts.shape[0], pts, np.ascontiguousarray(tet, np.int32),
+                            np.ascontiguousarray(u, np.float64), out)
+        return out
 
     def run(self, problem, dt, stop_time, max_steps=-1, want_u=False):
         out = np.zeros(4)

@@ -205,3 +205,28 @@ def test_singular_mass_reports_exit_code():
         with pytest.raises(L.LymphError) as e:
             ctx.time_step_matrix_free(0.0, 1e-3, ctx.vec(), ctx.vec())
         assert e.value.code == _lib.ERR_STATE
+
+
+@pytest.mark.parametrize("p", [2, 3])
+def test_receivers_and_traces(p):
+    """Receiver values (EXPORT_SOLUTION's point evaluation) of the GPU state vs
+    the oracle's evaluation of the same vector; recorded traces equal the
+    samples taken step by step."""
+    say(f"[receivers] p={p}")
+    m = M.cube_mesh(4, agglomerate=True)
+    rng = np.random.default_rng(3)
+    pts = rng.uniform(0.05, 0.95, (64, 3))
+    o = oracle_for(m, p)
+    dt = 0.2 * 0.25 / (np.sqrt(3.0) * p * p)
+    with L.Context(m, p) as ctx:
+        r = ctx.run(L.PROB_CUBE_WAVE, dt, 1e9, max_steps=10, want_u=True)
+        tets = ctx.set_receivers(pts, max_records=5)
+        assert np.array_equal(tets, o.locate(pts))
+        got = ctx.sample_receivers(_lib.SLOT_U_CUR)
+        ref = o.eval_points(pts, tets, r["u"])
+        assert rel(got, ref) < 1e-13
+        t = ctx.step_n_record(L.PROB_CUBE_WAVE, 10, r["t_final"], dt, every=2)
+        tr = ctx.traces()
+        assert tr.shape == (5, 64, 3)
+        assert np.array_equal(tr[-1], ctx.sample_receivers(_lib.SLOT_U_CUR))
+        assert t > r["t_final"]

@@ -95,3 +95,12 @@ def test_deck_syntax_error(tmp_path):
     with pytest.raises(M.MeshError) as e:
         M.read_deck(str(tmp_path))
     assert e.value.code == 8
+
+
+def test_point_location_matches_oracle():
+    """Receivers are located like FIND_ELEM_FROM_POINT (nearest barycentre)."""
+    m = M.cube_mesh(4, agglomerate=True)
+    rng = np.random.default_rng(11)
+    pts = rng.uniform(0, 1, (64, 3))
+    o = _oracle_for(m)
+    assert np.array_equal(m.locate(pts), o.locate(pts))
