@@ -132,7 +132,7 @@ def main():
     log(f"setup + {1 + a.warmup} steps: {t_build:.2f}s; device bytes {ctx.device_bytes / 1e9:.2f} GB")
     t_start = r0["t_final"]
     chosen, tune = ctx.autotune(reps=20)
-    log(f"autotune: group size {chosen}; ms/step by group size {tune}")
+    log(f"autotune: chosen (group size, broadcast mode) {chosen}; ms/step {tune}")
     # timed region: K device-resident steps, HIP events on the context stream
     w0 = time.perf_counter()
     ms = ctx.time_steps(prob, a.steps, t_start, dt)
@@ -172,7 +172,8 @@ def main():
             "fp64_frac": ctx.step_flops / (ms_step * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
         },
         "wall_ms_per_step": wall * 1e3 / a.steps,
-        "autotune": {"chosen_unroll": chosen, "ms_per_step": tune},
+        "autotune": {"chosen": list(chosen),
+                     "ms_per_step": {f"unr{k[0]}_x{k[1]}": v for k, v in tune.items()}},
         "err_l2": e_l2, "err_dg": e_dg,
         "setup_s": t_build,
     }

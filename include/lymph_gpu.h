@@ -174,14 +174,15 @@ int lymph_step_n(LymphCtx* ctx, int problem, int nsteps, double t0, double dt,
 int lymph_time_steps(LymphCtx* ctx, int problem, int nsteps, double t0,
                      double dt, double* ms);
 
-/* Time every variant of the step kernel (how many block column pairs are
- * loaded per unrolled group) for `reps` steps on copies of the state slots,
- * select the fastest for this context and report them: n_variants, unr[i],
- * ms[i] (milliseconds per step) -- arrays of at least 8 entries -- and the
- * chosen group size.  No reference counterpart (the reference has one CPU
- * loop); results do not depend on the variant. */
-int lymph_autotune(LymphCtx* ctx, int reps, int* n_variants, int* unr, double* ms,
-                   int* chosen);
+/* Time every variant of the step kernel for `reps` steps on copies of the
+ * state slots, keep the fastest for this context and report them: for
+ * variant i, unr[i] = column pairs loaded per unrolled group, xmode[i] = how
+ * the neighbour's coefficients reach the lanes (0 scalar loads, 1 vector load
+ * + lane broadcast), ms[i] = milliseconds per step; arrays of at least 16
+ * entries; *chosen = index of the variant kept.  No reference counterpart
+ * (the reference has one CPU loop); results do not depend on the variant. */
+int lymph_autotune(LymphCtx* ctx, int reps, int* n_variants, int* unr, int* xmode,
+                   double* ms, int* chosen);
 
 /* Receivers: point values of the displacement, evaluated like EXPORT_SOLUTION
  * (src/solution_processing.f90:150-170) in the tet given for each point

@@ -100,7 +100,7 @@ def lib():
         "lymph_error_slots": (_i, [_vp, _i, _i, _vp, _vp]),
         "lymph_run": (_i, [_vp, _i, _d, _d, _i, _vp, _vp]),
         "lymph_time_function": (_d, [_i, _d]),
-        "lymph_autotune": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
+        "lymph_autotune": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
         "lymph_set_receivers": (_i, [_vp, _i, _vp, _vp, _i]),
         "lymph_sample_receivers": (_i, [_vp, _i, _vp]),
         "lymph_step_n_record": (_i, [_vp, _i, _i, _d, _d, _i, _vp]),

@@ -95,24 +95,37 @@ int dispatch_p(int p, F&& f) {
   return fail(LYMPH_ERR_ARG, "polynomial degree must be 1..4");
 }
 
-// Step-kernel variants (column pairs per unrolled load group) per degree;
-// index 0 is the default until lymph_autotune picks one.
-template <int P> struct StepUnr;
-template <> struct StepUnr<1> { static constexpr int v[] = {6, 1, 2, 3}; };
-template <> struct StepUnr<2> { static constexpr int v[] = {5, 3, 15}; };
-template <> struct StepUnr<3> { static constexpr int v[] = {10, 3, 5, 6, 15, 30}; };
-template <> struct StepUnr<4> { static constexpr int v[] = {4, 1, 2, 8}; };
-template <int P> constexpr int n_unr() { return (int)(sizeof(StepUnr<P>::v) / sizeof(int)); }
+// Step-kernel variants per degree: (column pairs per unrolled load group,
+// neighbour-coefficient broadcast mode); index 0 is the default until
+// lymph_autotune picks one.
+template <int P> struct StepVar;
+template <> struct StepVar<1> {
+  static constexpr int unr[] = {6, 6, 3, 2};
+  static constexpr int xs[] = {1, 0, 1, 1};
+};
+template <> struct StepVar<2> {
+  static constexpr int unr[] = {5, 5, 15, 3, 15};
+  static constexpr int xs[] = {1, 0, 1, 1, 0};
+};
+template <> struct StepVar<3> {
+  static constexpr int unr[] = {10, 10, 5, 6, 15, 30, 30, 3};
+  static constexpr int xs[] = {1, 0, 1, 1, 1, 1, 0, 1};
+};
+template <> struct StepVar<4> {
+  static constexpr int unr[] = {4, 4, 2, 8, 1};
+  static constexpr int xs[] = {1, 0, 1, 1, 1};
+};
+template <int P> constexpr int n_var() { return (int)(sizeof(StepVar<P>::unr) / sizeof(int)); }
 
 template <int P, int MODE, int I = 0>
 void launch_leapfrog(int variant, int grid, hipStream_t st, const double* K, const int* rp,
                      const int* col, const double* Rp, const double* rs, const double* rd,
                      const double* u, const double* wa, double* out, double g, double c1,
                      double c2) {
-  if constexpr (I < n_unr<P>()) {
+  if constexpr (I < n_var<P>()) {
     if (variant == I) {
-      k_leapfrog<P, MODE, StepUnr<P>::v[I]><<<grid, 64, 0, st>>>(K, rp, col, Rp, rs, rd, u, wa, out,
-                                                                 g, c1, c2);
+      k_leapfrog<P, MODE, StepVar<P>::unr[I], StepVar<P>::xs[I]><<<grid, 64, 0, st>>>(
+          K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2);
       return;
     }
     launch_leapfrog<P, MODE, I + 1>(variant, grid, st, K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2);
@@ -618,12 +631,13 @@ int lymph_time_steps(LymphCtx* c, int problem, int nsteps, double t0, double dt,
   return 0;
 }
 
-int lymph_autotune(LymphCtx* c, int reps, int* n_variants, int* unr, double* ms, int* chosen) {
+int lymph_autotune(LymphCtx* c, int reps, int* n_variants, int* unr, int* xmode, double* ms,
+                   int* chosen) {
   if (int rc = need_built(c)) return rc;
   if (reps < 1) return fail(LYMPH_ERR_ARG, "reps < 1");
   return dispatch_p(c->p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
-    constexpr int NV = n_unr<P>();
+    constexpr int NV = n_var<P>();
     if (n_variants) *n_variants = NV;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
@@ -653,14 +667,15 @@ int lymph_autotune(LymphCtx* c, int reps, int* n_variants, int* unr, double* ms,
       float f = 0.f;
       HIP_TRY(hipEventElapsedTime(&f, e0, e1));
       const double t = (double)f / reps;
-      if (unr) unr[v] = StepUnr<P>::v[v];
+      if (unr) unr[v] = StepVar<P>::unr[v];
+      if (xmode) xmode[v] = StepVar<P>::xs[v];
       if (ms) ms[v] = t;
       if (t < best) { best = t; bi = v; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c->step_variant = bi;
-    if (chosen) *chosen = StepUnr<P>::v[bi];
+    if (chosen) *chosen = bi;
     return 0;
   });
 }

@@ -406,15 +406,21 @@ __global__ __launch_bounds__(128) void k_rhs(MeshDev m, int prob, double alpha, 
 // order (solve_LU with L = R^T, src/vet_mat_operations.f90:123-153); x in place.
 template <int NP>
 __device__ __forceinline__ void solve_packed(const double* __restrict__ R, double* x) {
+  // loops kept rolled: fully unrolled, the compiler hoists all Np(Np+1)/2
+  // factor entries into registers and the step kernel drops to one wave/SIMD
   x[0] = x[0] / R[0];
+#pragma unroll 1
   for (int i = 1; i < NP; ++i) {
     double s = x[i];
+#pragma unroll 1
     for (int j = 0; j < i; ++j) s = s - R[pk_off(NP, j) + (i - j)] * x[j];
     x[i] = s / R[pk_off(NP, i)];
   }
   x[NP - 1] = x[NP - 1] / R[pk_off(NP, NP - 1)];
+#pragma unroll 1
   for (int i = NP - 2; i >= 0; --i) {
     double s = x[i];
+#pragma unroll 1
     for (int j = i + 1; j < NP; ++j) s = s - R[pk_off(NP, i) + (j - i)] * x[j];
     x[i] = s / R[pk_off(NP, i)];
   }
@@ -479,9 +485,19 @@ __global__ __launch_bounds__(128) void k_project(MeshDev m, int fn, const double
 //   MODE 1: out = (c1*x + u) + c2*wa           (c1 = dt^2/2, u = u^0, wa = v^0, c2 = dt)
 //   MODE 2: part[P] = sum_r u_r (K u)_r        (DG error: K = A_dg, u = uh - uex)
 // ---------------------------------------------------------------------------
+// Broadcast lane `l`'s double to the whole wave (v_readlane on both halves).
+__device__ __forceinline__ double lane_bcast(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
 // UNR = column pairs whose loads are issued together (register footprint vs
-// loads in flight; chosen per degree by the autotuner in lymph_autotune).
-template <int P, int MODE, int UNR>
+// loads in flight).  XS = how the neighbour's coefficients reach the lanes:
+// 0 = wave-uniform scalar loads per pair, 1 = one coalesced vector load per
+// block (lane c holds x[c]) broadcast with v_readlane, so the inner loop waits
+// only on the block's own HBM loads.  Chosen per degree by lymph_autotune.
+template <int P, int MODE, int UNR, int XS>
 __global__ __launch_bounds__(64) void k_leapfrog(const double* __restrict__ K,
                                                  const int* __restrict__ row_ptr,
                                                  const int* __restrict__ col,
@@ -509,6 +525,14 @@ __global__ __launch_bounds__(64) void k_leapfrog(const double* __restrict__ K,
     const int cb = __builtin_amdgcn_readfirstlane(col[b]);
     const double* __restrict__ xv = u + (size_t)cb * N3;
     const double* __restrict__ Kb = K + (size_t)b * D::BS;
+    double xl[RPL];
+    if (XS == 1) {
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) {
+        const int r = lane + 64 * k;
+        xl[k] = r < N3 ? xv[r] : 0.0;
+      }
+    }
     double s[RPL];
 #pragma unroll
     for (int k = 0; k < RPL; ++k) s[k] = 0.0;
@@ -530,8 +554,17 @@ __global__ __launch_bounds__(64) void k_leapfrog(const double* __restrict__ K,
         const int c2i = c0 + uu;
         if (NPAIR % UNR == 0 || c2i < NPAIR) {
           const bool odd_tail = (N3 % 2 == 1) && (c2i == NPAIR - 1);
-          const double x0 = xv[2 * c2i];
-          const double x1 = odd_tail ? 0.0 : xv[2 * c2i + 1];
+          double x0, x1;
+          if (XS == 1) {
+            const int ca = 2 * c2i, cbb = 2 * c2i + 1;
+            const double xa = (RPL > 1 && (ca >> 6)) ? xl[RPL - 1] : xl[0];
+            const double xb = (RPL > 1 && (cbb >> 6)) ? xl[RPL - 1] : xl[0];
+            x0 = lane_bcast(xa, ca & 63);
+            x1 = odd_tail ? 0.0 : lane_bcast(xb, cbb & 63);
+          } else {
+            x0 = xv[2 * c2i];
+            x1 = odd_tail ? 0.0 : xv[2 * c2i + 1];
+          }
 #pragma unroll
           for (int k = 0; k < RPL; ++k) {
             const int r = lane + 64 * k;

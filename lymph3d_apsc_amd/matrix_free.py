@@ -141,14 +141,16 @@ class Context:
 
     def autotune(self, reps=20):
         """Time the step-kernel variants and keep the fastest; returns
-        (chosen group size, {group size: ms per step})."""
+        (chosen (group size, broadcast mode), {(group size, mode): ms/step})."""
         nv = C.c_int()
-        unr = np.zeros(8, np.int32)
-        ms = np.zeros(8)
+        unr = np.zeros(16, np.int32)
+        xm = np.zeros(16, np.int32)
+        ms = np.zeros(16)
         ch = C.c_int()
-        check(lib().lymph_autotune(self.h, reps, C.byref(nv), ptr(unr), ptr(ms), C.byref(ch)),
-              "lymph_autotune")
-        return ch.value, {int(unr[i]): float(ms[i]) for i in range(nv.value)}
+        check(lib().lymph_autotune(self.h, reps, C.byref(nv), ptr(unr), ptr(xm), ptr(ms),
+                                   C.byref(ch)), "lymph_autotune")
+        table = {(int(unr[i]), int(xm[i])): float(ms[i]) for i in range(nv.value)}
+        return (int(unr[ch.value]), int(xm[ch.value])), table
 
     # ---- receivers ------------------------------------------------------
     def set_receivers(self, points, max_records=0, tets=None):
