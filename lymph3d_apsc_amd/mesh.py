@@ -139,8 +139,7 @@ class Mesh:
         hk = np.zeros(npoly)
         for size in np.unique(counts):
             polys = np.nonzero(counts == size)[0]
-            tets = np.stack([order[self.poly_ptr[p]:self.poly_ptr[p] + size] for p in polys]) \
-                if size > 1 else order[self.poly_ptr[polys]][:, None]
+            tets = order[self.poly_ptr[polys][:, None] + np.arange(size)[None, :]]
             pts = v[tets].reshape(len(polys), 4 * size, 3)  # vertices with duplicates
             bbox[polys, 0::2] = pts.min(axis=1)
             bbox[polys, 1::2] = pts.max(axis=1)
