@@ -104,6 +104,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=12, help="cubes per side of the CPU sample")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-apply measurement")
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -177,6 +178,29 @@ def main():
         "err_l2": e_l2, "err_dg": e_dg,
         "setup_s": t_build,
     }
+    if not a.no_fp32:
+        # FP32 operator apply (FP64 RHS, mass solves, update): same start state
+        # and the same number of steps as the FP64 timed run
+        ctx.run(prob, dt, 1e30, max_steps=1 + a.warmup)
+        ctx.set_apply_precision(32)
+        ch32, tune32 = ctx.autotune(reps=20)
+        ms32 = ctx.time_steps(prob, a.steps, t_start, dt)
+        u32 = ctx.get_slot(_lib.SLOT_U_CUR)
+        ms32_step = ms32 / a.steps
+        B32 = ctx.step_bytes_for(32)
+        res["fp32_apply"] = {
+            "value": dofs / (ms32_step * 1e-3), "unit": "DOF-updates/s", "ms_per_step": ms32_step,
+            "speedup_vs_fp64": ms_step / ms32_step,
+            "roofline": {"bound": "hbm", "achieved": B32 / (ms32_step * 1e-3) / 1e9,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": B32 / (ms32_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "bytes_per_step": B32},
+            "rel_l2_diff_vs_fp64": float(np.linalg.norm(u32 - u) / np.linalg.norm(u)),
+            "err_l2": float(np.sqrt(ctx.compute_error_l2_matrix_free(u32, uex))),
+            "autotune": {"chosen": list(ch32),
+                         "ms_per_step": {f"unr{k[0]}": v for k, v in tune32.items()}},
+        }
+        log(f"fp32 apply: {ms32_step:.3f} ms/step, rel diff vs fp64 {res['fp32_apply']['rel_l2_diff_vs_fp64']:.2e}")
     del ctx
     if not a.no_cpu_baseline:
         try:

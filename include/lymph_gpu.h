@@ -184,6 +184,14 @@ int lymph_time_steps(LymphCtx* ctx, int problem, int nsteps, double t0,
 int lymph_autotune(LymphCtx* ctx, int reps, int* n_variants, int* unr, int* xmode,
                    double* ms, int* chosen);
 
+/* Operator precision of the step's block apply: 64 (default) or 32.  With 32
+ * the assembled FP64 blocks are rounded once to FP32 (a second copy, half
+ * the bytes per step); products and per-block sums are FP32, the sum over
+ * blocks, RHS, mass solves (FP64 Cholesky factors) and update stay FP64.
+ * No reference counterpart (the reference is FP64 only).  Call after
+ * lymph_build_operator; resets the autotuned variant. */
+int lymph_set_apply_precision(LymphCtx* ctx, int bits);
+
 /* Receivers: point values of the displacement, evaluated like EXPORT_SOLUTION
  * (src/solution_processing.f90:150-170) in the tet given for each point
  * (e.g. located like FIND_ELEM_FROM_POINT, src/find_tet.f90:82-124, nearest

@@ -38,6 +38,7 @@ EXPORTS = [
     "lymph_error_l2", "lymph_error_dg", "lymph_error_slots", "lymph_run",
     "lymph_time_function", "lymph_autotune", "lymph_set_receivers",
     "lymph_sample_receivers", "lymph_step_n_record", "lymph_get_traces",
+    "lymph_set_apply_precision",
 ]
 
 
@@ -105,6 +106,7 @@ def lib():
         "lymph_sample_receivers": (_i, [_vp, _i, _vp]),
         "lymph_step_n_record": (_i, [_vp, _i, _i, _d, _d, _i, _vp]),
         "lymph_get_traces": (_i, [_vp, _vp, _vp]),
+        "lymph_set_apply_precision": (_i, [_vp, _i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -58,6 +58,8 @@ struct LymphCtx {
   int n_tet = 0, n_poly = 0, n_blk = 0;
   int layout = LYMPH_LAYOUT_POLY;
   int step_variant = 0;
+  int apply_bits = 64;
+  float* K32 = nullptr;
   hipStream_t stream = nullptr;
   bool own_stream = false;
   bool built = false;
@@ -132,6 +134,29 @@ void launch_leapfrog(int variant, int grid, hipStream_t st, const double* K, con
   }
 }
 
+template <int P> struct StepVar32;
+template <> struct StepVar32<1> { static constexpr int unr[] = {3, 1}; };
+template <> struct StepVar32<2> { static constexpr int unr[] = {4, 2, 8}; };
+template <> struct StepVar32<3> { static constexpr int unr[] = {5, 3, 15}; };
+template <> struct StepVar32<4> { static constexpr int unr[] = {3, 1, 9}; };
+template <int P> constexpr int n_var32() { return (int)(sizeof(StepVar32<P>::unr) / sizeof(int)); }
+
+template <int P, int MODE, int I = 0>
+void launch_leapfrog32(int variant, int grid, hipStream_t st, const float* K, const int* rp,
+                       const int* col, const double* Rp, const double* rs, const double* rd,
+                       const double* u, const double* wa, double* out, double g, double c1,
+                       double c2) {
+  if constexpr (I < n_var32<P>()) {
+    if (variant == I) {
+      k_leapfrog32<P, MODE, StepVar32<P>::unr[I]><<<grid, 64, 0, st>>>(K, rp, col, Rp, rs, rd, u,
+                                                                       wa, out, g, c1, c2);
+      return;
+    }
+    launch_leapfrog32<P, MODE, I + 1>(variant, grid, st, K, rp, col, Rp, rs, rd, u, wa, out, g, c1,
+                                      c2);
+  }
+}
+
 int check_launch(const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(LYMPH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
@@ -172,12 +197,20 @@ int launch_step(LymphCtx* c, int mode, double g, double c1, double c2, const dou
     constexpr int P = decltype(pc)::value;
     const double* rs = c->slot[LYMPH_SLOT_RHS_STAT];
     const double* rd = c->slot[LYMPH_SLOT_RHS_DYN];
-    if (mode == 0)
+    if (c->apply_bits == 32) {
+      if (mode == 0)
+        launch_leapfrog32<P, 0>(c->step_variant, c->n_poly, c->stream, c->K32, c->md.row_ptr,
+                                c->md.col, c->RM, rs, rd, u, wa, out, g, c1, c2);
+      else
+        launch_leapfrog32<P, 1>(c->step_variant, c->n_poly, c->stream, c->K32, c->md.row_ptr,
+                                c->md.col, c->RM, rs, rd, u, wa, out, g, c1, c2);
+    } else if (mode == 0) {
       launch_leapfrog<P, 0>(c->step_variant, c->n_poly, c->stream, c->K, c->md.row_ptr, c->md.col,
                             c->RM, rs, rd, u, wa, out, g, c1, c2);
-    else
+    } else {
       launch_leapfrog<P, 1>(c->step_variant, c->n_poly, c->stream, c->K, c->md.row_ptr, c->md.col,
                             c->RM, rs, rd, u, wa, out, g, c1, c2);
+    }
     return check_launch("k_leapfrog");
   });
 }
@@ -239,6 +272,7 @@ int lymph_ctx_destroy(LymphCtx* c) {
   if (c->rpoly) (void)hipFree(c->rpoly);
   if (c->rxyz) (void)hipFree(c->rxyz);
   if (c->trace) (void)hipFree(c->trace);
+  if (c->K32) (void)hipFree(c->K32);
   for (void* p : c->owned) (void)hipFree(p);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -637,7 +671,7 @@ int lymph_autotune(LymphCtx* c, int reps, int* n_variants, int* unr, int* xmode,
   if (reps < 1) return fail(LYMPH_ERR_ARG, "reps < 1");
   return dispatch_p(c->p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
-    constexpr int NV = n_var<P>();
+    const int NV = c->apply_bits == 32 ? n_var32<P>() : n_var<P>();
     if (n_variants) *n_variants = NV;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
@@ -667,8 +701,8 @@ int lymph_autotune(LymphCtx* c, int reps, int* n_variants, int* unr, int* xmode,
       float f = 0.f;
       HIP_TRY(hipEventElapsedTime(&f, e0, e1));
       const double t = (double)f / reps;
-      if (unr) unr[v] = StepVar<P>::unr[v];
-      if (xmode) xmode[v] = StepVar<P>::xs[v];
+      if (unr) unr[v] = c->apply_bits == 32 ? StepVar32<P>::unr[v] : StepVar<P>::unr[v];
+      if (xmode) xmode[v] = c->apply_bits == 32 ? 1 : StepVar<P>::xs[v];
       if (ms) ms[v] = t;
       if (t < best) { best = t; bi = v; }
     }
@@ -678,6 +712,25 @@ int lymph_autotune(LymphCtx* c, int reps, int* n_variants, int* unr, int* xmode,
     if (chosen) *chosen = bi;
     return 0;
   });
+}
+
+int lymph_set_apply_precision(LymphCtx* c, int bits) {
+  if (int rc = need_built(c)) return rc;
+  if (bits != 32 && bits != 64) return fail(LYMPH_ERR_ARG, "precision must be 32 or 64");
+  if (bits == 32 && !c->K32) {
+    int rc = dispatch_p(c->p, [&](auto pc) -> int {
+      constexpr int P = decltype(pc)::value;
+      const size_t n = (size_t)c->n_blk * Dims32<P>::BS;
+      if (int r = dalloc(&c->K32, n)) return r;
+      k_to_fp32<P><<<grid1(n, 256), 256, 0, c->stream>>>(c->K, c->K32, (size_t)c->n_blk);
+      return check_launch("k_to_fp32");
+    });
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  c->apply_bits = bits;
+  c->step_variant = 0;
+  return 0;
 }
 
 int lymph_set_receivers(LymphCtx* c, int n, const double* xyz, const int32_t* tet,

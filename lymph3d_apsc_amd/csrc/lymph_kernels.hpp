@@ -614,6 +614,134 @@ __global__ __launch_bounds__(64) void k_leapfrog(const double* __restrict__ K,
   }
 }
 
+// ---------------------------------------------------------------------------
+// FP32-apply variant of the step: the operator blocks are held in FP32 in
+// "column-quad" order (entry (r, c) at ((c>>2)*N3 + r)*4 + (c&3); a lane
+// reads four columns of its row as one 16-byte load, half the bytes of the
+// FP64 stream), products and the per-block sum are FP32; the sum over blocks,
+// the RHS, the mass solves and the update stay FP64.
+// ---------------------------------------------------------------------------
+template <int P>
+struct Dims32 {
+  static constexpr int N3 = 3 * np_of(P);
+  static constexpr int NQUAD = (N3 + 3) / 4;
+  static constexpr size_t BS = (size_t)N3 * NQUAD * 4;
+};
+LYMPH_HD size_t kpos32(int N3, int r, int c) {
+  return ((size_t)(c >> 2) * N3 + r) * 4 + (c & 3);
+}
+
+template <int P>
+__global__ void k_to_fp32(const double* __restrict__ K, float* __restrict__ K32, size_t n_blk) {
+  using D = Dims<P>;
+  constexpr int N3 = D::N3;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_blk * Dims32<P>::BS) return;
+  const size_t b = i / Dims32<P>::BS;
+  const size_t w = i % Dims32<P>::BS;  // within block, quad layout
+  const int c = (int)(w / (N3 * 4)) * 4 + (int)(w % 4);
+  const int r = (int)((w / 4) % N3);
+  K32[i] = c < N3 ? (float)K[b * D::BS + kpos(N3, r, c)] : 0.0f;
+}
+
+template <int P, int MODE, int UNR>
+__global__ __launch_bounds__(64) void k_leapfrog32(const float* __restrict__ K,
+                                                   const int* __restrict__ row_ptr,
+                                                   const int* __restrict__ col,
+                                                   const double* __restrict__ Rp,
+                                                   const double* __restrict__ rs,
+                                                   const double* __restrict__ rd,
+                                                   const double* __restrict__ u,
+                                                   const double* __restrict__ wa,
+                                                   double* __restrict__ out, double g, double c1,
+                                                   double c2) {
+  using D = Dims<P>;
+  constexpr int NP = D::NP, N3 = D::N3, NQUAD = Dims32<P>::NQUAD;
+  constexpr int RPL = (N3 + 63) / 64;
+  __shared__ double sw[N3];
+  __shared__ double sR[D::NPK];
+  const int lane = threadIdx.x;
+  const int Pp = blockIdx.x;
+  for (int k = lane; k < D::NPK; k += 64) sR[k] = Rp[(size_t)Pp * D::NPK + k];
+  double acc[RPL];
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) acc[k] = 0.0;
+  const int b0 = row_ptr[Pp], b1 = row_ptr[Pp + 1];
+  for (int b = b0; b < b1; ++b) {
+    const int cb = __builtin_amdgcn_readfirstlane(col[b]);
+    const double* __restrict__ xv = u + (size_t)cb * N3;
+    const float* __restrict__ Kb = K + (size_t)b * Dims32<P>::BS;
+    float xl[RPL];
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) {
+      const int r = lane + 64 * k;
+      xl[k] = r < N3 ? (float)xv[r] : 0.0f;
+    }
+    float s[RPL];
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) s[k] = 0.0f;
+#pragma unroll 1
+    for (int q0 = 0; q0 < NQUAD; q0 += UNR) {
+      float4 kv[UNR][RPL];
+#pragma unroll
+      for (int uu = 0; uu < UNR; ++uu) {
+        const int q = q0 + uu;
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) {
+          const int r = lane + 64 * k;
+          if ((NQUAD % UNR == 0 || q < NQUAD) && r < N3)
+            kv[uu][k] = *reinterpret_cast<const float4*>(Kb + ((size_t)q * N3 + r) * 4);
+        }
+      }
+#pragma unroll
+      for (int uu = 0; uu < UNR; ++uu) {
+        const int q = q0 + uu;
+        if (NQUAD % UNR == 0 || q < NQUAD) {
+          float xq[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int c = 4 * q + j;
+            const float src = (RPL > 1 && (c >> 6)) ? xl[RPL - 1] : xl[0];
+            xq[j] = (c < N3) ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src), c & 63))
+                             : 0.0f;
+          }
+#pragma unroll
+          for (int k = 0; k < RPL; ++k) {
+            const int r = lane + 64 * k;
+            if (r < N3) {
+              s[k] = s[k] + kv[uu][k].x * xq[0];
+              s[k] = s[k] + kv[uu][k].y * xq[1];
+              s[k] = s[k] + kv[uu][k].z * xq[2];
+              s[k] = s[k] + kv[uu][k].w * xq[3];
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) acc[k] = (b == b0) ? (double)s[k] : acc[k] + (double)s[k];
+  }
+  const size_t o = (size_t)Pp * N3;
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) {
+    const int r = lane + 64 * k;
+    if (r < N3) sw[r] = -acc[k] + rs[o + r] + rd[o + r] * g;
+  }
+  __syncthreads();
+  if (lane < 3) solve_packed<NP>(sR, sw + lane * NP);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) {
+    const int r = lane + 64 * k;
+    if (r < N3) {
+      if (MODE == 0)
+        out[o + r] = c1 * sw[r] + 2.0 * u[o + r] - wa[o + r];
+      else
+        out[o + r] = c1 * sw[r] + u[o + r] + c2 * wa[o + r];
+    }
+  }
+}
+
 // L2 error terms du_i^T M_modal du_i per (polyhedron, component)
 // (COMPUTE_ERROR_L2_MATRIX_FREE, src/matrix_free.f90:1406-1416).
 template <int P>

@@ -152,6 +152,17 @@ class Context:
         table = {(int(unr[i]), int(xm[i])): float(ms[i]) for i in range(nv.value)}
         return (int(unr[ch.value]), int(xm[ch.value])), table
 
+    def set_apply_precision(self, bits):
+        """64 (default) or 32: precision of the step's operator apply."""
+        check(lib().lymph_set_apply_precision(self.h, bits), "lymph_set_apply_precision")
+        self.apply_bits = bits
+
+    def step_bytes_for(self, bits):
+        """Algorithmic bytes of one step with the operator in `bits` precision."""
+        n3, Np = self.n3, self.Np
+        return (bits // 8) * self.n_blk * n3 * n3 + 8 * (self.n_poly * Np * (Np + 1) // 2
+                                                          + 5 * self.n_poly * n3)
+
     # ---- receivers ------------------------------------------------------
     def set_receivers(self, points, max_records=0, tets=None):
         """Receivers at `points` (n, 3), located like FIND_ELEM_FROM_POINT

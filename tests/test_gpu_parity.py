@@ -230,3 +230,31 @@ def test_receivers_and_traces(p):
         assert tr.shape == (5, 64, 3)
         assert np.array_equal(tr[-1], ctx.sample_receivers(_lib.SLOT_U_CUR))
         assert t > r["t_final"]
+
+
+@pytest.mark.parametrize("name,p", [("cube3agg", 3), ("hetero", 2), ("cube2agg", 4)])
+def test_fp32_apply(name, p):
+    """FP32 operator apply: the step's acceleration term dt^2 M^-1(F - Ku)
+    within 1e-5 of the FP64 oracle (FP32 rounding of K and of the per-block
+    sums), the update itself in FP64."""
+    say(f"[fp32 apply] {name} p={p}")
+    m = make_mesh(name)
+    o = oracle_for(m, p)
+    o.build_operator()
+    rng = np.random.default_rng(2)
+    n = m.n_poly * 3 * o.Np
+    with L.Context(m, p) as ctx:
+        ctx.make_matrices_free()
+        ctx.make_rhs_free(L.PROB_NULL, _lib.SLOT_RHS_STAT)
+        ctx.make_rhs_free(L.PROB_REF_MMS, _lib.SLOT_RHS_DYN)
+        ctx.set_apply_precision(32)
+        u0, u1 = rng.standard_normal(n), rng.standard_normal(n)
+        dt, g = 1e-3, 0.37
+        a = ctx.time_step_matrix_free(g, dt, u0, u1)
+        b = o.step(g, dt, u0, u1, o.rhs(L.PROB_NULL), o.rhs(L.PROB_REF_MMS))
+        acc_gpu, acc_ref = a - (2 * u1 - u0), b - (2 * u1 - u0)
+        assert rel(acc_gpu, acc_ref) < 1e-5
+        assert rel(acc_gpu, acc_ref) > 0  # really the FP32 path
+        ctx.set_apply_precision(64)
+        a64 = ctx.time_step_matrix_free(g, dt, u0, u1)
+        assert rel(a64, b) < 1e-13
