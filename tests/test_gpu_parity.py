@@ -105,9 +105,19 @@ def test_operator_rhs_projection(name, p):
             r = ctx.make_rhs_free(prob)
             orr = o.rhs(prob)
             assert rel(r, orr) < 1e-13, (prob, rel(r, orr))
+        # projections: compared in the L2 norm of the functions (the mass
+        # matrix norm).  Coefficients alone are not a fair measure: the
+        # Legendre basis on a tet's bounding box is far from orthogonal on the
+        # tet, so ulp-level differences of the right-hand side (device vs
+        # glibc sin/cos) move the solve along poorly conditioned directions.
         for fn in (L.FN_ZERO, L.FN_REF_VEL, L.FN_REF_UEX, L.FN_CUBE_U):
             c = ctx.compute_modal_coefficients_free(fn)
-            assert rel(c, o.project(fn)) < 1e-13, fn
+            co = o.project(fn)
+            d = (c - co).reshape(m.n_poly, 3, -1)
+            cc = co.reshape(m.n_poly, 3, -1)
+            num = np.einsum("pim,pmn,pin->", d, oMm, d)
+            den = np.einsum("pim,pmn,pin->", cc, oMm, cc)
+            assert (den == 0 and num == 0) or np.sqrt(num / den) < 1e-13, (fn, num, den)
 
 
 @pytest.mark.parametrize("name,p", [("tet_25", 2), ("cube3agg", 3), ("hetero", 2), ("cube2agg", 4)])
