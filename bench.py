@@ -105,6 +105,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-apply measurement")
+    ap.add_argument("--no-autotune", action="store_true", help="use the default step variant")
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -132,7 +133,7 @@ def main():
     t_build = time.perf_counter() - t0
     log(f"setup + {1 + a.warmup} steps: {t_build:.2f}s; device bytes {ctx.device_bytes / 1e9:.2f} GB")
     t_start = r0["t_final"]
-    chosen, tune = ctx.autotune(reps=20)
+    chosen, tune = ((None, None), {}) if a.no_autotune else ctx.autotune(reps=20)
     log(f"autotune: chosen (group size, broadcast mode) {chosen}; ms/step {tune}")
     # timed region: K device-resident steps, HIP events on the context stream
     w0 = time.perf_counter()
@@ -183,7 +184,7 @@ def main():
         # and the same number of steps as the FP64 timed run
         ctx.run(prob, dt, 1e30, max_steps=1 + a.warmup)
         ctx.set_apply_precision(32)
-        ch32, tune32 = ctx.autotune(reps=20)
+        ch32, tune32 = ((None, None), {}) if a.no_autotune else ctx.autotune(reps=20)
         ms32 = ctx.time_steps(prob, a.steps, t_start, dt)
         u32 = ctx.get_slot(_lib.SLOT_U_CUR)
         ms32_step = ms32 / a.steps
