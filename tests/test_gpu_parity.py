@@ -268,3 +268,20 @@ def test_fp32_apply(name, p):
         ctx.set_apply_precision(64)
         a64 = ctx.time_step_matrix_free(g, dt, u0, u1)
         assert rel(a64, b) < 1e-13
+
+
+def test_vertex_export_matches_oracle(tmp_path):
+    """Vertex values for the VTK export (receiver kernel at tet vertices) vs
+    the oracle's point evaluation of the same vector."""
+    say("[export]")
+    from lymph3d_apsc_amd import export
+    m = M.cube_mesh(3, agglomerate=True)
+    p = 2
+    o = oracle_for(m, p)
+    with L.Context(m, p) as ctx:
+        r = ctx.run(L.PROB_CUBE_WAVE, 0.01, 1e9, max_steps=3, want_u=True)
+        vals = export.vertex_values(ctx)
+    pts = m.xyz[m.tet_vert].reshape(-1, 3)
+    ref = o.eval_points(pts, np.repeat(np.arange(m.n_tet, dtype=np.int32), 4), r["u"])
+    assert rel(vals.reshape(-1, 3), ref) < 1e-13
+    export.write_vtu(str(tmp_path / "u.vtu"), m, vals)

@@ -104,3 +104,18 @@ def test_point_location_matches_oracle():
     pts = rng.uniform(0, 1, (64, 3))
     o = _oracle_for(m)
     assert np.array_equal(m.locate(pts), o.locate(pts))
+
+
+def test_vtu_round_trip(tmp_path):
+    """The VTK writer (EXPORT_SOLUTION's discontinuous vertex values) writes
+    what it was given: read it back."""
+    from lymph3d_apsc_amd import export
+    m = M.cube_mesh(2, agglomerate=True)
+    vals = np.random.default_rng(0).standard_normal((m.n_tet, 4, 3))
+    f = tmp_path / "u.vtu"
+    export.write_vtu(str(f), m, vals)
+    pts, v, poly = export.read_vtu_arrays(str(f))
+    assert np.array_equal(v, vals.reshape(-1, 3))
+    assert np.array_equal(pts, m.xyz[m.tet_vert].reshape(-1, 3))
+    assert np.array_equal(poly, m.tet_poly)
+    assert open(f, "rb").read().startswith(b'<?xml version="1.0"?>')
