@@ -55,6 +55,8 @@ extern "C" {
 #define LYMPH_PROB_NULL 0      /* f_null, gd_null, gn_null */
 #define LYMPH_PROB_REF_MMS 1   /* f_time, gd, gn: the reference's built-in manufactured solution */
 #define LYMPH_PROB_CUBE_WAVE 2 /* u_i = sin(pi x)sin(pi y)sin(pi z) cos(sqrt3 pi t), all components */
+#define LYMPH_PROB_RICKER 3    /* no volume/boundary data; point forces (lymph_add_point_sources)
+                                  with the Ricker time factor set by lymph_set_ricker */
 
 /* Fields for lymph_project (COMPUTE_MODAL_COEFFICIENTS_FREE's f_analytic). */
 #define LYMPH_FN_ZERO 0    /* ic_displacement, uex_stat */
@@ -191,6 +193,19 @@ int lymph_autotune(LymphCtx* ctx, int reps, int* n_variants, int* unr, int* xmod
  * No reference counterpart (the reference is FP64 only).  Call after
  * lymph_build_operator; resets the autotuned variant. */
 int lymph_set_apply_precision(LymphCtx* ctx, int bits);
+
+/* Point forces f_k delta(x - x_k) added to a RHS slot (normally
+ * LYMPH_SLOT_RHS_DYN): rhs[P][i*Np+m] += f_k,i phi_m(x_k) in the polyhedron of
+ * tet[k].  No reference counterpart: the reference's point source is the
+ * double couple of MAKE_DOUBLE_COUPLE (src/assemble_element.f90:116-167) and
+ * its Ricker keyword is parsed but unused (src/Poly_readfile.f90:589-680). */
+int lymph_add_point_sources(LymphCtx* ctx, int n, const double* xyz, const int32_t* tet,
+                            const double* force, int slot);
+/* Time factor of LYMPH_PROB_RICKER for this context: Ricker wavelet of peak
+ * frequency f0 centred at t0 (default f0 = 5, t0 = 1.2/f0). */
+int lymph_set_ricker(LymphCtx* ctx, double f0, double t0);
+/* (1 - 2a) exp(-a), a = (pi f0 (t - t0))^2 */
+double lymph_ricker(double f0, double t0, double t);
 
 /* Receivers: point values of the displacement, evaluated like EXPORT_SOLUTION
  * (src/solution_processing.f90:150-170) in the tet given for each point

@@ -7,12 +7,12 @@ reference's `matrix_free` routines (`matrix_free.Context`) and the driver of
 src/Lymph3D.f90 (`driver`).
 """
 from ._lib import (FN_CUBE_U, FN_REF_UEX, FN_REF_VEL, FN_ZERO, PROB_CUBE_WAVE, PROB_NULL,
-                   PROB_REF_MMS, LymphError, device_count, time_function)
+                   PROB_REF_MMS, PROB_RICKER, LymphError, device_count, ricker, time_function)
 from .matrix_free import Context
 from .mesh import Mesh, MeshError, cube_mesh, read_deck
 
 __all__ = [
     "Context", "Mesh", "MeshError", "LymphError", "cube_mesh", "read_deck", "device_count",
-    "time_function", "PROB_NULL", "PROB_REF_MMS", "PROB_CUBE_WAVE", "FN_ZERO", "FN_REF_VEL",
+    "time_function", "ricker", "PROB_RICKER", "PROB_NULL", "PROB_REF_MMS", "PROB_CUBE_WAVE", "FN_ZERO", "FN_REF_VEL",
     "FN_REF_UEX", "FN_CUBE_U",
 ]

@@ -22,7 +22,7 @@ ERR_HIP = 102
 ERR_STATE = 103
 
 LAYOUT_POLY, LAYOUT_FORTRAN = 0, 1
-PROB_NULL, PROB_REF_MMS, PROB_CUBE_WAVE = 0, 1, 2
+PROB_NULL, PROB_REF_MMS, PROB_CUBE_WAVE, PROB_RICKER = 0, 1, 2, 3
 FN_ZERO, FN_REF_VEL, FN_REF_UEX, FN_CUBE_U = 0, 1, 2, 3
 (SLOT_RHS_STAT, SLOT_RHS_DYN, SLOT_U_PREV, SLOT_U_CUR, SLOT_U_NEXT, SLOT_V0,
  SLOT_UEX) = range(7)
@@ -38,7 +38,7 @@ EXPORTS = [
     "lymph_error_l2", "lymph_error_dg", "lymph_error_slots", "lymph_run",
     "lymph_time_function", "lymph_autotune", "lymph_set_receivers",
     "lymph_sample_receivers", "lymph_step_n_record", "lymph_get_traces",
-    "lymph_set_apply_precision",
+    "lymph_set_apply_precision", "lymph_add_point_sources", "lymph_set_ricker", "lymph_ricker",
 ]
 
 
@@ -107,6 +107,9 @@ def lib():
         "lymph_step_n_record": (_i, [_vp, _i, _i, _d, _d, _i, _vp]),
         "lymph_get_traces": (_i, [_vp, _vp, _vp]),
         "lymph_set_apply_precision": (_i, [_vp, _i]),
+        "lymph_add_point_sources": (_i, [_vp, _i, _vp, _vp, _vp, _i]),
+        "lymph_set_ricker": (_i, [_vp, _d, _d]),
+        "lymph_ricker": (_d, [_d, _d, _d]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -127,6 +130,10 @@ def check(rc, where):
 
 def device_count():
     return lib().lymph_device_count()
+
+
+def ricker(f0, t0, t):
+    return lib().lymph_ricker(f0, t0, t)
 
 
 def time_function(problem, t):

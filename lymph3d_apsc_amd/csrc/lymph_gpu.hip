@@ -59,6 +59,7 @@ struct LymphCtx {
   int layout = LYMPH_LAYOUT_POLY;
   int step_variant = 0;
   int apply_bits = 64;
+  double ricker_f0 = 5.0, ricker_t0 = 1.2 / 5.0;
   float* K32 = nullptr;
   hipStream_t stream = nullptr;
   bool own_stream = false;
@@ -164,6 +165,11 @@ int check_launch(const char* what) {
 }
 
 unsigned grid1(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+double g_of(const LymphCtx* c, int problem, double t) {
+  if (problem == LYMPH_PROB_RICKER) return ricker(c->ricker_f0, c->ricker_t0, t);
+  return time_factor(problem, t);
+}
 
 // host vector (user layout) -> device staging (poly layout)
 int h2d_vec(LymphCtx* c, const double* h, double* d) {
@@ -549,7 +555,7 @@ int lymph_export_operator(const LymphCtx* cc, double* K, double* A, double* M, d
 int lymph_build_rhs(LymphCtx* c, int problem, int slot, double* rhs_out) {
   if (int rc = need_ctx(c)) return rc;
   if (slot < 0 || slot >= LYMPH_NUM_SLOTS) return fail(LYMPH_ERR_ARG, "bad slot");
-  if (problem < LYMPH_PROB_NULL || problem > LYMPH_PROB_CUBE_WAVE)
+  if (problem < LYMPH_PROB_NULL || problem > LYMPH_PROB_RICKER)
     return fail(LYMPH_ERR_ARG, "unknown problem");
   int rc = dispatch_p(c->p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
@@ -634,7 +640,7 @@ int lymph_step_n(LymphCtx* c, int problem, int nsteps, double t0, double dt, dou
   double t = t0;
   const double dt2 = dt * dt;
   for (int s = 0; s < nsteps; ++s) {
-    if (int rc = launch_step(c, 0, time_factor(problem, t), dt2, 0.0, c->slot[LYMPH_SLOT_U_CUR],
+    if (int rc = launch_step(c, 0, g_of(c, problem, t), dt2, 0.0, c->slot[LYMPH_SLOT_U_CUR],
                              c->slot[LYMPH_SLOT_U_PREV], c->slot[LYMPH_SLOT_U_NEXT]))
       return rc;
     double* prev = c->slot[LYMPH_SLOT_U_PREV];
@@ -731,6 +737,46 @@ int lymph_set_apply_precision(LymphCtx* c, int bits) {
   c->apply_bits = bits;
   c->step_variant = 0;
   return 0;
+}
+
+int lymph_set_ricker(LymphCtx* c, double f0, double t0) {
+  if (int rc = need_ctx(c)) return rc;
+  if (!(f0 > 0)) return fail(LYMPH_ERR_ARG, "f0 must be positive");
+  c->ricker_f0 = f0;
+  c->ricker_t0 = t0;
+  return 0;
+}
+
+double lymph_ricker(double f0, double t0, double t) { return ricker(f0, t0, t); }
+
+int lymph_add_point_sources(LymphCtx* c, int n, const double* xyz, const int32_t* tet,
+                            const double* force, int slot) {
+  if (int rc = need_ctx(c)) return rc;
+  if (n < 0 || slot < 0 || slot >= LYMPH_NUM_SLOTS || (n > 0 && (!xyz || !tet || !force)))
+    return fail(LYMPH_ERR_ARG, "bad point-source arguments");
+  if (n == 0) return 0;
+  std::vector<int> sp(n);
+  for (int i = 0; i < n; ++i) {
+    if (tet[i] < 0 || tet[i] >= c->n_tet) return fail(LYMPH_ERR_ARG, "source tet out of range");
+    sp[i] = c->h_tet_poly[tet[i]];
+  }
+  double *dx = nullptr, *df = nullptr;
+  int* dp = nullptr;
+  int rc = upload(&dx, xyz, 3 * (size_t)n);
+  if (!rc) rc = upload(&df, force, 3 * (size_t)n);
+  if (!rc) rc = upload(&dp, sp.data(), sp.size());
+  if (!rc)
+    rc = dispatch_p(c->p, [&](auto pc) -> int {
+      constexpr int P = decltype(pc)::value;
+      k_point_sources<P><<<1, 128, 0, c->stream>>>(dx, dp, df, n, c->md.bbox, c->md.blist,
+                                                  c->slot[slot]);
+      return check_launch("k_point_sources");
+    });
+  if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(LYMPH_ERR_HIP, "sync failed");
+  if (dx) (void)hipFree(dx);
+  if (df) (void)hipFree(df);
+  if (dp) (void)hipFree(dp);
+  return rc;
 }
 
 int lymph_set_receivers(LymphCtx* c, int n, const double* xyz, const int32_t* tet,

@@ -834,4 +834,33 @@ __global__ void k_sample(const double* __restrict__ u, const int* __restrict__ r
   out[id] = s;
 }
 
+// Point forces: rhs[P_k][i*Np+m] += f_k,i phi_m(x_k), sources applied in order
+// (one workgroup, a barrier between sources so two sources in one
+// polyhedron accumulate in order).
+template <int P>
+__global__ __launch_bounds__(128) void k_point_sources(const double* __restrict__ sxyz,
+                                                       const int* __restrict__ spoly,
+                                                       const double* __restrict__ sf, int n,
+                                                       const double* __restrict__ bbox,
+                                                       const int* __restrict__ blist,
+                                                       double* __restrict__ rhs) {
+  using D = Dims<P>;
+  constexpr int NP = D::NP, N3 = D::N3;
+  for (int k = 0; k < n; ++k) {
+    const int Pp = spoly[k];
+    const double* bb = bbox + (size_t)Pp * 6;
+    for (int e = threadIdx.x; e < N3; e += blockDim.x) {
+      const int i = e / NP, m = e % NP;
+      double v[3];
+      for (int d = 0; d < 3; ++d) {
+        const double hb = (bb[2 * d + 1] - bb[2 * d]) / 2.0, mb = (bb[2 * d + 1] + bb[2 * d]) / 2.0;
+        v[d] = leg(blist[m * 3 + d], (sxyz[k * 3 + d] - mb) / hb) / sqrt(hb);
+      }
+      const size_t o = (size_t)Pp * N3 + e;
+      rhs[o] = rhs[o] + sf[k * 3 + i] * v[0] * v[1] * v[2];
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace lymph

@@ -220,7 +220,7 @@ LYMPH_HD void face_point_rhs(const double* Fk, int face, const double* tri4, dou
 }
 
 // ---- problem data (src/problem_data_and_properties.f90) ----
-enum { kProbNull = 0, kProbRefMms = 1, kProbCubeWave = 2 };
+enum { kProbNull = 0, kProbRefMms = 1, kProbCubeWave = 2, kProbRicker = 3 };
 enum { kFnZero = 0, kFnRefVel = 1, kFnRefUex = 2, kFnCubeU = 3 };
 
 // f_null / f_time (:48-81) / cube standing wave.
@@ -309,6 +309,13 @@ LYMPH_HD void field(int fn, const double* p, double* r) {
     const double S = sin(kPi * p[0]) * sin(kPi * p[1]) * sin(kPi * p[2]);
     r[0] = r[1] = r[2] = S;
   }
+}
+
+// Ricker wavelet (1 - 2a) exp(-a), a = (pi f0 (t - t0))^2.
+inline double ricker(double f0, double t0, double t) {
+  double a = kPi * f0 * (t - t0);
+  a = a * a;
+  return (1.0 - 2.0 * a) * std::exp(-a);
 }
 
 inline double time_factor(int prob, double t) {

@@ -163,6 +163,38 @@ class Context:
         return (bits // 8) * self.n_blk * n3 * n3 + 8 * (self.n_poly * Np * (Np + 1) // 2
                                                           + 5 * self.n_poly * n3)
 
+    # ---- point sources (Ricker) -----------------------------------------
+    def add_point_sources(self, points, forces, slot=_lib.SLOT_RHS_DYN, tets=None):
+        pts = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+        f = np.ascontiguousarray(forces, np.float64).reshape(-1, 3)
+        tets = self.mesh.locate(pts) if tets is None else np.ascontiguousarray(tets, np.int32)
+        check(lib().lymph_add_point_sources(self.h, pts.shape[0], ptr(pts), ptr(tets), ptr(f), slot),
+              "lymph_add_point_sources")
+        return tets
+
+    def set_ricker(self, f0, t0=None):
+        self.ricker = (f0, 1.2 / f0 if t0 is None else t0)
+        check(lib().lymph_set_ricker(self.h, *self.ricker), "lymph_set_ricker")
+
+    def start_from_rest(self, dt, sources=None, forces=None, f0=5.0, t0=None):
+        """Operator (if needed), zero static RHS, point-force dynamic RHS, zero
+        initial state, first step at t=0; returns t after the first step."""
+        if not self.built:
+            self.make_matrices_free()
+        self.set_ricker(f0, t0)
+        self.make_rhs_free(_lib.PROB_NULL, _lib.SLOT_RHS_STAT)
+        self.make_rhs_free(_lib.PROB_RICKER, _lib.SLOT_RHS_DYN)
+        if sources is not None:
+            self.add_point_sources(sources, forces)
+        self.compute_modal_coefficients_free(_lib.FN_ZERO, _lib.SLOT_U_PREV)
+        self.compute_modal_coefficients_free(_lib.FN_ZERO, _lib.SLOT_V0)
+        g0 = lib().lymph_ricker(self.ricker[0], self.ricker[1], 0.0)
+        check(lib().lymph_first_step_dev(self.h, g0, dt, lib().lymph_slot_ptr(self.h, _lib.SLOT_U_PREV),
+                                         lib().lymph_slot_ptr(self.h, _lib.SLOT_V0),
+                                         lib().lymph_slot_ptr(self.h, _lib.SLOT_U_CUR)),
+              "lymph_first_step_dev")
+        return 0.0 + dt
+
     # ---- receivers ------------------------------------------------------
     def set_receivers(self, points, max_records=0, tets=None):
         """Receivers at `points` (n, 3), located like FIND_ELEM_FROM_POINT

@@ -1466,3 +1466,32 @@ void o_eval_points(const OCtx* c, int n, const double* pts, const int* tet, cons
     }
   }
 }
+
+/* Point force F(P_s)[i*Np+m] += f_i phi_m(x_s) for every source (the exact
+ * load of f delta(x - x_s)), phi on the bounding box of the polyhedron of the
+ * source's tet; rhs is accumulated in source order. */
+void o_point_sources(const OCtx* c, int n, const double* pts, const int* tet, const double* f,
+                     double* rhs) {
+  int Np = c->Np;
+  for (int k = 0; k < n; ++k) {
+    int P = c->tet_poly[tet[k]];
+    const double* bb = &c->bbox[P * 6];
+    for (int i = 0; i < 3; ++i)
+      for (int m = 0; m < Np; ++m) {
+        double v[3];
+        for (int d = 0; d < 3; ++d) {
+          double hb = (bb[2 * d + 1] - bb[2 * d]) / 2.0, mb = (bb[2 * d + 1] + bb[2 * d]) / 2.0;
+          v[d] = leg(c->blist[m * 3 + d], (pts[k * 3 + d] - mb) / hb) / sqrt(hb);
+        }
+        size_t o = (size_t)P * c->n3 + i * Np + m;
+        rhs[o] = rhs[o] + f[k * 3 + i] * v[0] * v[1] * v[2];
+      }
+  }
+}
+
+/* Ricker wavelet (1 - 2 a) exp(-a), a = (pi f0 (t - t0))^2 */
+double o_ricker(double f0, double t0, double t) {
+  double a = o_pi() * f0 * (t - t0);
+  a = a * a;
+  return (1.0 - 2.0 * a) * exp(-a);
+}

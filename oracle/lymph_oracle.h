@@ -113,6 +113,11 @@ void o_locate_points(const OCtx* c, int n, const double* pts, int* tet);
 void o_eval_points(const OCtx* c, int n, const double* pts, const int* tet, const double* u,
                    double* out);
 
+/* point force sources and the Ricker wavelet */
+void o_point_sources(const OCtx* c, int n, const double* pts, const int* tet, const double* f,
+                     double* rhs);
+double o_ricker(double f0, double t0, double t);
+
 /* ---- element-level restatements (checked against the reference's own
  *      Fortran routines by the tests) ---- */
 int o_np(int p);

@@ -59,6 +59,9 @@ def lib():
         L.o_time_function.restype = C.c_double
         L.o_locate_points.argtypes = [C.c_void_p, C.c_int, _dp, _ip]
         L.o_eval_points.argtypes = [C.c_void_p, C.c_int, _dp, _ip, _dp, _dp]
+        L.o_point_sources.argtypes = [C.c_void_p, C.c_int, _dp, _ip, _dp, _dp]
+        L.o_ricker.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.o_ricker.restype = C.c_double
         L.o_np.argtypes = [C.c_int]
         L.o_np.restype = C.c_int
         L.o_basis_list.argtypes = [C.c_int, _ip]
@@ -201,6 +204,13 @@ class Oracle:
                             np.ascontiguousarray(u, np.float64), out)
         return out
 
+    def point_sources(self, pts, tets, forces):
+        pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+        out = np.zeros(self.n_poly * self.n3)
+        lib().o_point_sources(self.h, pts.shape[0], pts, np.ascontiguousarray(tets, np.int32),
+                              np.ascontiguousarray(forces, np.float64).reshape(-1, 3), out)
+        return out
+
     def run(self, problem, dt, stop_time, max_steps=-1, want_u=False):
         out = np.zeros(4)
         u = np.zeros(self.n_poly * self.n3) if want_u else None
@@ -220,3 +230,7 @@ def time_function(problem, t):
 def from_deck(deck, p=None, **kw):
     return Oracle(deck["xyz"], deck["tet_vert"], deck["tria_vert"], deck["tria_tag"],
                   deck["tag_kind"], deck["tag_space"], deck["degree"] if p is None else p, **kw)
+
+
+def ricker(f0, t0, t):
+    return lib().o_ricker(f0, t0, t)

@@ -285,3 +285,46 @@ def test_vertex_export_matches_oracle(tmp_path):
     ref = o.eval_points(pts, np.repeat(np.arange(m.n_tet, dtype=np.int32), 4), r["u"])
     assert rel(vals.reshape(-1, 3), ref) < 1e-13
     export.write_vtu(str(tmp_path / "u.vtu"), m, vals)
+
+
+@pytest.mark.parametrize("p,steps", [(3, 60), (4, 40)])
+def test_ricker_point_source_heterogeneous(p, steps):
+    """Config-3 shaped case at small size: uneven region-grown polyhedra,
+    lognormal lambda/mu, rho in [0.8,1.2], traction-free z=1 face, zero
+    initial state, Ricker point force (f0=5) at an interior point.  GPU run vs
+    the oracle's step loop; receiver traces vs the oracle's evaluation."""
+    say(f"[ricker] p={p}")
+    m = M.cube_mesh(3, dirichlet_faces={"x0", "x1", "y0", "y1", "z0"}, poly_size=(4, 12), seed=9)
+    rng = np.random.default_rng(9)
+    m.lam = np.exp(rng.normal(0, 0.5, m.n_tet))
+    m.mu = np.exp(rng.normal(0, 0.5, m.n_tet))
+    m.rho = rng.uniform(0.8, 1.2, m.n_tet)
+    src = np.array([[0.41, 0.53, 0.47]])
+    force = np.array([[0.0, 0.0, 1.0]])
+    rec = rng.uniform(0.1, 0.9, (16, 3))
+    dt = 0.2 * (1 / 3) / (np.sqrt(3.0) * p * p) / 2
+    f0 = 5.0
+    o = oracle_for(m, p)
+    o.build_operator()
+    stet = o.locate(src)
+    rs = o.rhs(oracle.PROB_NULL)
+    rd = o.rhs(oracle.PROB_NULL) + o.point_sources(src, stet, force)
+    z = np.zeros(o.n_poly * o.n3)
+    t = 0.0
+    un = o.first_step(oracle.ricker(f0, 1.2 / f0, t), dt, z, z, rs, rd)
+    up = z
+    t = t + dt
+    for _ in range(steps):
+        up, un = un, o.step(oracle.ricker(f0, 1.2 / f0, t), dt, up, un, rs, rd)
+        t = t + dt
+    with L.Context(m, p) as ctx:
+        t_gpu = ctx.start_from_rest(dt, src, force, f0=f0)
+        assert rel(ctx.get_slot(_lib.SLOT_RHS_DYN), rd) < 1e-13
+        ctx.set_receivers(rec, max_records=steps)
+        t_gpu = ctx.step_n_record(L.PROB_RICKER, steps, t_gpu, dt, every=1)
+        u = ctx.get_slot(_lib.SLOT_U_CUR)
+        tr = ctx.traces()
+    assert t_gpu == t
+    assert np.abs(un).max() > 0
+    assert rel(u, un) < 1e-11
+    assert rel(tr[-1], o.eval_points(rec, o.locate(rec), un)) < 1e-11
