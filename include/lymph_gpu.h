@@ -222,6 +222,18 @@ int lymph_step_n_record(LymphCtx* ctx, int problem, int nsteps, double t0, doubl
 /* out[n_records][n][3] (may be NULL to query the count). */
 int lymph_get_traces(LymphCtx* ctx, double* out, int* n_records);
 
+/* Capture one whole solve from rest into a HIP graph on the context stream:
+ * zero initial displacement and velocity (projection kernels), first step at
+ * t=0, then `nsteps` steps (recording receivers every `every` steps when
+ * receivers are set).  Uses the RHS slots and Ricker/time factors as they are
+ * at capture time.  No reference counterpart (ensembles are new); the
+ * captured work is the same kernels the uncaptured calls launch. */
+int lymph_capture_solve(LymphCtx* ctx, int problem, double dt, int nsteps, int every);
+/* Launch the captured solve (asynchronous; traces overwritten in place). */
+int lymph_replay(LymphCtx* ctx);
+/* Wait for all work of the context's stream. */
+int lymph_sync(LymphCtx* ctx);
+
 /* COMPUTE_ERROR_L2_MATRIX_FREE / _DG_ (src/matrix_free.f90:1388-1504):
  * squared norms of uh - uex; host pointers or slots. */
 int lymph_error_l2(LymphCtx* ctx, const double* uh, const double* uex,

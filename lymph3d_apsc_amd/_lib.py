@@ -39,6 +39,7 @@ EXPORTS = [
     "lymph_time_function", "lymph_autotune", "lymph_set_receivers",
     "lymph_sample_receivers", "lymph_step_n_record", "lymph_get_traces",
     "lymph_set_apply_precision", "lymph_add_point_sources", "lymph_set_ricker", "lymph_ricker",
+    "lymph_capture_solve", "lymph_replay", "lymph_sync",
 ]
 
 
@@ -110,6 +111,9 @@ def lib():
         "lymph_add_point_sources": (_i, [_vp, _i, _vp, _vp, _vp, _i]),
         "lymph_set_ricker": (_i, [_vp, _d, _d]),
         "lymph_ricker": (_d, [_d, _d, _d]),
+        "lymph_capture_solve": (_i, [_vp, _i, _d, _i, _i]),
+        "lymph_replay": (_i, [_vp]),
+        "lymph_sync": (_i, [_vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -61,6 +61,9 @@ struct LymphCtx {
   int apply_bits = 64;
   double ricker_f0 = 5.0, ricker_t0 = 1.2 / 5.0;
   float* K32 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_records = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
   bool built = false;
@@ -279,6 +282,8 @@ int lymph_ctx_destroy(LymphCtx* c) {
   if (c->rxyz) (void)hipFree(c->rxyz);
   if (c->trace) (void)hipFree(c->trace);
   if (c->K32) (void)hipFree(c->K32);
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
   for (void* p : c->owned) (void)hipFree(p);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -849,6 +854,65 @@ int lymph_get_traces(LymphCtx* c, double* out, int* n_records) {
   if (out && c->n_rec > 0)
     HIP_TRY(hipMemcpyAsync(out, c->trace, (size_t)c->n_rec * 3 * c->n_recv * sizeof(double),
                            hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int lymph_capture_solve(LymphCtx* c, int problem, double dt, int nsteps, int every) {
+  if (int rc = need_built(c)) return rc;
+  if (nsteps < 0 || every < 1) return fail(LYMPH_ERR_ARG, "bad nsteps/every");
+  if (c->n_recv > 0 && nsteps / every > c->max_rec)
+    return fail(LYMPH_ERR_ARG, "receiver trace buffer too small for the captured solve");
+  if (!c->own_stream) return fail(LYMPH_ERR_STATE, "capture needs the context's own stream");
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->n_rec = 0;
+  HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  // zero initial state (projection kernels, no memset nodes), first step at
+  // t = 0, then the loop; every launch goes to the capturing stream
+  int rc = dispatch_p(c->p, [&](auto pc) -> int {
+    constexpr int P = decltype(pc)::value;
+    k_project<P><<<c->n_poly, 128, 0, c->stream>>>(c->md, LYMPH_FN_ZERO, c->RMm,
+                                                   c->slot[LYMPH_SLOT_U_PREV]);
+    k_project<P><<<c->n_poly, 128, 0, c->stream>>>(c->md, LYMPH_FN_ZERO, c->RMm,
+                                                   c->slot[LYMPH_SLOT_V0]);
+    return check_launch("k_project");
+  });
+  double t = 0.0;
+  if (!rc)
+    rc = launch_step(c, 1, g_of(c, problem, t), 0.5 * (dt * dt), dt, c->slot[LYMPH_SLOT_U_PREV],
+                     c->slot[LYMPH_SLOT_V0], c->slot[LYMPH_SLOT_U_CUR]);
+  t = t + dt;
+  if (!rc) {
+    if (c->n_recv > 0) rc = lymph_step_n_record(c, problem, nsteps, t, dt, every, &t);
+    else rc = lymph_step_n(c, problem, nsteps, t, dt, &t);
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (ec != hipSuccess) return fail(LYMPH_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+  c->graph = g;
+  HIP_TRY(hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+  c->graph_records = c->n_rec;
+  return 0;
+}
+
+int lymph_replay(LymphCtx* c) {
+  if (int rc = need_ctx(c)) return rc;
+  if (!c->graph_exec) return fail(LYMPH_ERR_STATE, "no captured solve: call lymph_capture_solve");
+  HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
+  c->n_rec = c->graph_records;
+  return 0;
+}
+
+int lymph_sync(LymphCtx* c) {
+  if (int rc = need_ctx(c)) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
   return 0;
 }

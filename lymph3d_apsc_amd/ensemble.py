@@ -1,0 +1,120 @@
+"""Batched ensembles: many independent elastodynamics instances (own mesh
+size, own lognormal material field, own Ricker point-source location, own
+receivers) advanced together.
+
+Design: instances of the same polynomial degree are joined into one mesh with
+disjoint components (`union`); the block-row operator of the union is block
+diagonal by instance, so one launch of the step kernel per time step advances
+every instance of the group, and each instance performs exactly the
+arithmetic of a standalone run.  The time factor (Ricker wavelet, common f0)
+is shared; the spatial source term of each instance lives in its own rows of
+the dynamic RHS.  A whole solve (zero state, first step, N steps with
+receiver sampling) is captured once into a HIP graph and replayed.
+
+No reference counterpart: the reference runs one instance per MPI job.
+"""
+import numpy as np
+
+from . import _lib
+from .matrix_free import Context
+from .mesh import Mesh, cube_mesh
+
+
+def union(meshes):
+    """One Mesh whose connected components are `meshes` (node, tet and
+    polyhedron ids offset; materials and tags carried over)."""
+    xyz, tv, tp, tri, tag, rho, lam, mu = [], [], [], [], [], [], [], []
+    no = po = 0
+    tag_space = {}
+    for m in meshes:
+        xyz.append(m.xyz)
+        tv.append(m.tet_vert + no)
+        tp.append(m.tet_poly + po)
+        tri.append(m.tria_vert + no)
+        tag.append(m.tria_tag)
+        rho.append(m.rho)
+        lam.append(m.lam)
+        mu.append(m.mu)
+        tag_space.update(m.tag_space)
+        no += m.xyz.shape[0]
+        po += m.n_poly
+    return Mesh(np.concatenate(xyz), np.concatenate(tv), np.concatenate(tp), np.concatenate(tri),
+                np.concatenate(tag), tag_space, np.concatenate(rho), np.concatenate(lam),
+                np.concatenate(mu))
+
+
+def make_instance(seed, n_range=(10, 20), p_choices=(2, 3), n_receivers=64, sigma=0.5,
+                  agglomerate=True):
+    """One seeded instance: n^3 cubes (n uniform in n_range), degree from
+    p_choices, lognormal(0, sigma) lambda and mu per tet, rho ~ U[0.8, 1.2],
+    Dirichlet on all faces, a z-directed Ricker point force at a seeded
+    interior point and seeded receivers."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(n_range[0], n_range[1] + 1))
+    p = int(rng.choice(p_choices))
+    m = cube_mesh(n, agglomerate=agglomerate)
+    m.lam = np.exp(rng.normal(0.0, sigma, m.n_tet))
+    m.mu = np.exp(rng.normal(0.0, sigma, m.n_tet))
+    m.rho = rng.uniform(0.8, 1.2, m.n_tet)
+    src = rng.uniform(0.25, 0.75, (1, 3))
+    rec = rng.uniform(0.05, 0.95, (n_receivers, 3))
+    return dict(seed=seed, n=n, p=p, mesh=m, source=src, force=np.array([[0.0, 0.0, 1.0]]),
+                receivers=rec)
+
+
+class Ensemble:
+    """Instances grouped by degree; one context per group on the GPU."""
+
+    def __init__(self, instances, dt, nsteps, every=1, f0=5.0, apply_bits=32, autotune=True):
+        self.instances = list(instances)
+        self.dt, self.nsteps, self.every, self.f0 = dt, nsteps, every, f0
+        self.groups = []
+        for p in sorted({ins["p"] for ins in self.instances}):
+            members = [i for i, ins in enumerate(self.instances) if ins["p"] == p]
+            meshes = [self.instances[i]["mesh"] for i in members]
+            um = union(meshes)
+            ctx = Context(um, p)
+            ctx.make_matrices_free()
+            if apply_bits == 32:
+                ctx.set_apply_precision(32)
+            # sources and receivers located in each instance, ids offset into the union
+            tet_off = np.cumsum([0] + [m.n_tet for m in meshes])
+            src = np.concatenate([self.instances[i]["source"] for i in members])
+            frc = np.concatenate([self.instances[i]["force"] for i in members])
+            stet = np.concatenate([m.locate(self.instances[i]["source"]) + tet_off[k]
+                                   for k, (i, m) in enumerate(zip(members, meshes))])
+            rec = np.concatenate([self.instances[i]["receivers"] for i in members])
+            rtet = np.concatenate([m.locate(self.instances[i]["receivers"]) + tet_off[k]
+                                   for k, (i, m) in enumerate(zip(members, meshes))])
+            ctx.set_ricker(f0)
+            ctx.make_rhs_free(_lib.PROB_NULL, _lib.SLOT_RHS_STAT)
+            ctx.make_rhs_free(_lib.PROB_RICKER, _lib.SLOT_RHS_DYN)
+            ctx.add_point_sources(src, frc, tets=stet.astype(np.int32))
+            ctx.set_receivers(rec, max_records=nsteps // every, tets=rtet.astype(np.int32))
+            if autotune:
+                ctx.autotune(reps=5)
+            ctx.capture_solve(_lib.PROB_RICKER, dt, nsteps, every)
+            nrec = [len(self.instances[i]["receivers"]) for i in members]
+            self.groups.append(dict(p=p, members=members, ctx=ctx, rec_split=np.cumsum([0] + nrec)))
+        self.n_dofs = sum(g["ctx"].n_poly * g["ctx"].n3 for g in self.groups)
+
+    def run(self):
+        """One ensemble solve (every instance, all steps); returns traces per
+        instance: list of (records, receivers, 3) arrays in instance order."""
+        for g in self.groups:
+            g["ctx"].replay()
+        for g in self.groups:
+            g["ctx"].sync()
+        return self.traces()
+
+    def traces(self):
+        out = [None] * len(self.instances)
+        for g in self.groups:
+            tr = g["ctx"].traces()
+            for k, i in enumerate(g["members"]):
+                out[i] = tr[:, g["rec_split"][k]:g["rec_split"][k + 1], :]
+        return out
+
+    def close(self):
+        for g in self.groups:
+            g["ctx"].close()

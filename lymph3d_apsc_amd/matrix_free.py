@@ -195,6 +195,16 @@ class Context:
               "lymph_first_step_dev")
         return 0.0 + dt
 
+    # ---- captured solves (HIP graphs) -------------------------------------
+    def capture_solve(self, problem, dt, nsteps, every=1):
+        check(lib().lymph_capture_solve(self.h, problem, dt, nsteps, every), "lymph_capture_solve")
+
+    def replay(self):
+        check(lib().lymph_replay(self.h), "lymph_replay")
+
+    def sync(self):
+        check(lib().lymph_sync(self.h), "lymph_sync")
+
     # ---- receivers ------------------------------------------------------
     def set_receivers(self, points, max_records=0, tets=None):
         """Receivers at `points` (n, 3), located like FIND_ELEM_FROM_POINT
