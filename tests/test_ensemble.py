@@ -1,0 +1,75 @@
+"""Host logic of ensembles (lymph3d_apsc_amd.ensemble) on the CPU: the union
+of instance meshes is block diagonal by instance, and the operator and mass
+factors the oracle builds for the union are, bit for bit, those of each
+instance built alone (so a batched solve performs each instance's own
+arithmetic)."""
+import numpy as np
+
+from lymph3d_apsc_amd import ensemble as E
+from lymph3d_apsc_amd import mesh as M
+from oracle import oracle
+
+
+def _oracle_for(m, p):
+    kinds = np.zeros(8, np.int32)
+    kinds[[2, 3, 4]] = -1
+    kinds[[5, 6, 7]] = -2
+    space = np.zeros(8, np.int32)
+    for t, s in m.tag_space.items():
+        space[t] = s
+    return oracle.Oracle(m.xyz, m.tet_vert, m.tria_vert, m.tria_tag, kinds, space, p,
+                         tet_poly=m.tet_poly, rho=m.rho, lam=m.lam, mu=m.mu)
+
+
+def test_make_instance_is_seeded():
+    a, b = E.make_instance(7, n_range=(2, 3)), E.make_instance(7, n_range=(2, 3))
+    assert a["n"] == b["n"] and a["p"] == b["p"]
+    assert np.array_equal(a["mesh"].lam, b["mesh"].lam)
+    assert np.array_equal(a["receivers"], b["receivers"])
+    c = E.make_instance(8, n_range=(2, 3))
+    assert not np.array_equal(a["mesh"].lam, c["mesh"].lam)
+    assert (a["mesh"].lam > 0).all() and (a["mesh"].mu > 0).all()
+
+
+def test_union_is_block_diagonal_and_bitwise_per_instance():
+    inst = [E.make_instance(s, n_range=(2, 3), p_choices=(1,)) for s in (1, 2, 3)]
+    meshes = [i["mesh"] for i in inst]
+    u = E.union(meshes)
+    assert u.n_poly == sum(m.n_poly for m in meshes)
+    assert u.n_tet == sum(m.n_tet for m in meshes)
+    # connectivity: neighbours offset, never across instances
+    to = np.cumsum([0] + [m.n_tet for m in meshes])
+    for k, m in enumerate(meshes):
+        nb = u.neigh[to[k]:to[k + 1]]
+        ref = np.where(m.neigh >= 0, m.neigh + to[k], m.neigh)
+        assert np.array_equal(nb, ref)
+        assert np.array_equal(u.normal[to[k]:to[k + 1]], m.normal)
+    ou = _oracle_for(u, 1)
+    ou.build_operator()
+    Ku = ou.K()
+    rpu, colu = ou.pattern()
+    po = 0
+    blk = 0
+    for m in meshes:
+        om = _oracle_for(m, 1)
+        om.build_operator()
+        rp, col = om.pattern()
+        assert np.array_equal(rpu[po:po + m.n_poly + 1] - rpu[po], rp)
+        assert np.array_equal(colu[blk:blk + len(col)], col + po)
+        assert np.array_equal(Ku[blk:blk + len(col)], om.K())
+        Mu = ou.mass()[2][po:po + m.n_poly]
+        assert np.array_equal(Mu, om.mass()[2])
+        po += m.n_poly
+        blk += len(col)
+    assert blk == ou.n_blk
+
+
+def test_union_point_location_offsets():
+    inst = [E.make_instance(s, n_range=(2, 2), n_receivers=5) for s in (4, 5)]
+    meshes = [i["mesh"] for i in inst]
+    u = E.union(meshes)
+    # instances overlap in space, so location is per instance then offset
+    t0 = meshes[0].locate(inst[0]["receivers"])
+    t1 = meshes[1].locate(inst[1]["receivers"]) + meshes[0].n_tet
+    assert t0.max() < meshes[0].n_tet <= t1.min()
+    assert isinstance(u, M.Mesh)
