@@ -43,16 +43,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def stream_copy_gbps():
+def stream_probe_gbps():
+    """Achievable HBM bandwidth of this box: (copy GB/s, read GB/s) of 2 GiB
+    STREAM-style kernels (liblymph_bwprobe.so), or (None, None)."""
     path = os.path.join(ROOT, "lymph3d_apsc_amd", "liblymph_bwprobe.so")
     if not os.path.exists(path):
-        return None
+        return None, None
     L = C.CDLL(path)
-    L.lymph_bw_copy.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
-    g, ms = C.c_double(), C.c_double()
-    if L.lymph_bw_copy(2 << 30, 20, C.byref(g), C.byref(ms)) != 0:
-        return None
-    return g.value
+    out = []
+    for fn in (L.lymph_bw_copy, L.lymph_bw_read):
+        fn.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        g, ms = C.c_double(), C.c_double()
+        out.append(g.value if fn(2 << 30, 20, C.byref(g), C.byref(ms)) == 0 else None)
+    return tuple(out)
 
 
 def cpu_baseline(p, n_sample, target_s):
@@ -152,7 +155,7 @@ def main():
     value = dofs / (ms_step * 1e-3)
     B = ctx.step_bytes
     achieved = B / (ms_step * 1e-3) / 1e9
-    copy = stream_copy_gbps()
+    copy, read = stream_probe_gbps()
     res = {
         "metric": METRIC, "value": value, "unit": "DOF-updates/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True,
@@ -170,6 +173,8 @@ def main():
             "bytes_per_step": B, "kernel": "k_leapfrog<3,0>",
             "stream_copy_GBps": copy,
             "frac_of_stream_copy": (achieved / copy) if copy else None,
+            "stream_read_GBps": read,
+            "frac_of_stream_read": (achieved / read) if read else None,
             "fp64_tflops": ctx.step_flops / (ms_step * 1e-3) / 1e12,
             "fp64_frac": ctx.step_flops / (ms_step * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
         },

@@ -194,6 +194,17 @@ int lymph_autotune(LymphCtx* ctx, int reps, int* n_variants, int* unr, int* xmod
  * lymph_build_operator; resets the autotuned variant. */
 int lymph_set_apply_precision(LymphCtx* ctx, int bits);
 
+/* Device-memory options for large batched runs (no reference counterpart:
+ * the reference keeps K_loc and A_dg_loc in host memory per MPI rank).
+ * lymph_ctx_set_keep_adg(ctx, 0) before lymph_build_operator skips the
+ * A_dg blocks (needed only by lymph_error_dg, which then returns
+ * LYMPH_ERR_STATE); after a build it frees them.
+ * lymph_release_fp64_operator frees the FP64 K blocks once the 32-bit apply
+ * is selected (lymph_set_apply_precision(ctx, 32)); 64-bit apply and K export
+ * then return LYMPH_ERR_STATE until lymph_build_operator runs again. */
+int lymph_ctx_set_keep_adg(LymphCtx* ctx, int keep);
+int lymph_release_fp64_operator(LymphCtx* ctx);
+
 /* Point forces f_k delta(x - x_k) added to a RHS slot (normally
  * LYMPH_SLOT_RHS_DYN): rhs[P][i*Np+m] += f_k,i phi_m(x_k) in the polyhedron of
  * tet[k].  No reference counterpart: the reference's point source is the

@@ -1,38 +1,103 @@
-// bw_probe.hip -- STREAM-style copy kernel used by bench.py to measure the
-// achievable HBM bandwidth of the box (reported beside the 8 TB/s spec).
+// bw_probe.hip -- STREAM-style copy and read kernels used by bench.py to
+// measure the achievable HBM bandwidth of the box (reported beside the 8 TB/s
+// spec) and to calibrate the FETCH_SIZE / WRITE_SIZE counters.
 // Not part of the PolyDG path; built into liblymph_bwprobe.so.
 #include <hip/hip_runtime.h>
 
-__global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ a,
-                                              double2* __restrict__ b, size_t n) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) b[i] = a[i];
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kVec = 4;  // double2 per thread, all loads issued before any use
+
+__global__ __launch_bounds__(kThreads) void k_copy(const double2* __restrict__ a,
+                                                   double2* __restrict__ b, size_t n) {
+  const size_t base = (size_t)blockIdx.x * kThreads * kVec + threadIdx.x;
+  double2 r[kVec];
+#pragma unroll
+  for (int k = 0; k < kVec; ++k) {
+    const size_t i = base + (size_t)k * kThreads;
+    if (i < n) r[k] = a[i];
+  }
+#pragma unroll
+  for (int k = 0; k < kVec; ++k) {
+    const size_t i = base + (size_t)k * kThreads;
+    if (i < n) b[i] = r[k];
+  }
 }
 
-extern "C" int lymph_bw_copy(size_t bytes, int reps, double* gbps, double* ms_per) {
+__global__ __launch_bounds__(kThreads) void k_read(const double2* __restrict__ a,
+                                                   double* __restrict__ out, size_t n) {
+  const size_t base = (size_t)blockIdx.x * kThreads * kVec + threadIdx.x;
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < kVec; ++k) {
+    const size_t i = base + (size_t)k * kThreads;
+    if (i < n) {
+      const double2 v = a[i];
+      s += v.x + v.y;
+    }
+  }
+  // one value per block keeps the loads live; lane 0 of each wave stores
+  __shared__ double red[kThreads / 64];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kThreads / 64; ++w) t += red[w];
+    out[blockIdx.x] = t;
+  }
+}
+
+int run(bool copy, size_t bytes, int reps, double* gbps, double* ms_per) {
   const size_t n = bytes / sizeof(double2);
-  double2 *a = nullptr, *b = nullptr;
+  const size_t grid = (n + (size_t)kThreads * kVec - 1) / ((size_t)kThreads * kVec);
+  if (n == 0 || reps <= 0 || grid > 0x7fffffffu) return 1;
+  double2* a = nullptr;
+  double2* b = nullptr;
+  double* out = nullptr;
   if (hipMalloc(&a, n * sizeof(double2)) != hipSuccess) return 1;
-  if (hipMalloc(&b, n * sizeof(double2)) != hipSuccess) { (void)hipFree(a); return 1; }
+  hipError_t e = copy ? hipMalloc(&b, n * sizeof(double2)) : hipMalloc(&out, grid * sizeof(double));
+  if (e != hipSuccess) {
+    (void)hipFree(a);
+    return 1;
+  }
   (void)hipMemset(a, 1, n * sizeof(double2));
-  (void)hipMemset(b, 0, n * sizeof(double2));
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
-  const unsigned grid = 256 * 8 * 4;
-  k_copy<<<grid, 256>>>(a, b, n);
+  auto launch = [&]() {
+    if (copy)
+      k_copy<<<(unsigned)grid, kThreads>>>(a, b, n);
+    else
+      k_read<<<(unsigned)grid, kThreads>>>(a, out, n);
+  };
+  launch();
   (void)hipEventRecord(e0, 0);
-  for (int r = 0; r < reps; ++r) k_copy<<<grid, 256>>>(a, b, n);
+  for (int r = 0; r < reps; ++r) launch();
   (void)hipEventRecord(e1, 0);
   (void)hipEventSynchronize(e1);
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, e0, e1);
   *ms_per = ms / reps;
-  *gbps = 2.0 * n * sizeof(double2) / (*ms_per * 1e-3) / 1e9;  // read + write
+  const double moved = (copy ? 2.0 : 1.0) * n * sizeof(double2);
+  *gbps = moved / (*ms_per * 1e-3) / 1e9;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipFree(a);
-  (void)hipFree(b);
+  if (b) (void)hipFree(b);
+  if (out) (void)hipFree(out);
   return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+}  // namespace
+
+// Copy `bytes` (read + write counted) `reps` times; GB/s and ms per launch.
+extern "C" int lymph_bw_copy(size_t bytes, int reps, double* gbps, double* ms_per) {
+  return run(true, bytes, reps, gbps, ms_per);
+}
+
+// Read `bytes` `reps` times (one double written per 256-thread block).
+extern "C" int lymph_bw_read(size_t bytes, int reps, double* gbps, double* ms_per) {
+  return run(false, bytes, reps, gbps, ms_per);
 }

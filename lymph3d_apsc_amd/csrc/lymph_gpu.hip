@@ -59,6 +59,7 @@ struct LymphCtx {
   int layout = LYMPH_LAYOUT_POLY;
   int step_variant = 0;
   int apply_bits = 64;
+  bool keep_adg = true;
   double ricker_f0 = 5.0, ricker_t0 = 1.2 / 5.0;
   float* K32 = nullptr;
   hipGraph_t graph = nullptr;
@@ -227,6 +228,7 @@ int launch_step(LymphCtx* c, int mode, double g, double c1, double c2, const dou
 int errors_dev(LymphCtx* c, const double* uh, const double* ue, double* l2sq, double* dgsq) {
   const size_t n = vec_n(c);
   double res[2];
+  if (dgsq && !c->A) return fail(LYMPH_ERR_STATE, "A_dg blocks were not kept (lymph_ctx_set_keep_adg)");
   int rc = dispatch_p(c->p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
     if (l2sq) {
@@ -282,6 +284,8 @@ int lymph_ctx_destroy(LymphCtx* c) {
   if (c->rxyz) (void)hipFree(c->rxyz);
   if (c->trace) (void)hipFree(c->trace);
   if (c->K32) (void)hipFree(c->K32);
+  if (c->K) (void)hipFree(c->K);
+  if (c->A) (void)hipFree(c->A);
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   for (void* p : c->owned) (void)hipFree(p);
@@ -422,8 +426,6 @@ int lymph_ctx_create(const LymphMesh* mesh, int p, double alpha, double theta, L
   up(&d_wtri, wtri.data(), wtri.size());
   up(&d_bl, bl.data(), bl.size());
   const size_t bs = blk_sz(c), npk = (size_t)c->Np * (c->Np + 1) / 2;
-  al(&c->K, (size_t)c->n_blk * bs);
-  al(&c->A, (size_t)c->n_blk * bs);
   al(&c->M, (size_t)np * c->Np * c->Np);
   al(&c->Mm, (size_t)np * c->Np * c->Np);
   al(&c->RM, (size_t)np * npk);
@@ -506,9 +508,20 @@ int lymph_get_pattern(const LymphCtx* c, int32_t* row_ptr, int32_t* col) {
 
 int lymph_build_operator(LymphCtx* c) {
   if (int rc = need_ctx(c)) return rc;
-  const size_t bs = blk_sz(c);
-  HIP_TRY(hipMemsetAsync(c->K, 0, (size_t)c->n_blk * bs * sizeof(double), c->stream));
-  HIP_TRY(hipMemsetAsync(c->A, 0, (size_t)c->n_blk * bs * sizeof(double), c->stream));
+  const size_t bs = blk_sz(c), nk = (size_t)c->n_blk * bs;
+  // K (and A_dg unless dropped) live outside the mesh allocations so that
+  // they can be released for batched runs
+  if (!c->K) {
+    if (int rc = dalloc(&c->K, nk)) return rc;
+    c->bytes += nk * sizeof(double);
+  }
+  if (c->keep_adg && !c->A) {
+    if (int rc = dalloc(&c->A, nk)) return rc;
+    c->bytes += nk * sizeof(double);
+  }
+  c->built = false;
+  HIP_TRY(hipMemsetAsync(c->K, 0, nk * sizeof(double), c->stream));
+  if (c->A) HIP_TRY(hipMemsetAsync(c->A, 0, nk * sizeof(double), c->stream));
   HIP_TRY(hipMemsetAsync(c->M, 0, (size_t)c->n_poly * c->Np * c->Np * sizeof(double), c->stream));
   HIP_TRY(hipMemsetAsync(c->Mm, 0, (size_t)c->n_poly * c->Np * c->Np * sizeof(double), c->stream));
   HIP_TRY(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
@@ -523,7 +536,13 @@ int lymph_build_operator(LymphCtx* c) {
     }
     k_cholesky<P><<<grid1(c->n_poly, 64), 64, 0, c->stream>>>(c->M, c->Mm, c->RM, c->RMm,
                                                               c->n_poly, c->status);
-    return check_launch("k_cholesky");
+    if (int r = check_launch("k_cholesky")) return r;
+    if (c->K32) {  // keep an existing 32-bit copy in step with the new blocks
+      k_to_fp32<P><<<grid1((size_t)c->n_blk * Dims32<P>::BS, 256), 256, 0, c->stream>>>(
+          c->K, c->K32, (size_t)c->n_blk);
+      if (int r = check_launch("k_to_fp32")) return r;
+    }
+    return 0;
   });
   if (rc) return rc;
   int st = 0;
@@ -534,11 +553,38 @@ int lymph_build_operator(LymphCtx* c) {
   return 0;
 }
 
+int lymph_ctx_set_keep_adg(LymphCtx* c, int keep) {
+  if (int rc = need_ctx(c)) return rc;
+  c->keep_adg = keep != 0;
+  if (!c->keep_adg && c->A) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipFree(c->A));
+    c->A = nullptr;
+    c->bytes -= (size_t)c->n_blk * blk_sz(c) * sizeof(double);
+  }
+  return 0;
+}
+
+int lymph_release_fp64_operator(LymphCtx* c) {
+  if (int rc = need_built(c)) return rc;
+  if (c->apply_bits != 32 || !c->K32)
+    return fail(LYMPH_ERR_STATE, "select the 32-bit apply (lymph_set_apply_precision) first");
+  if (c->K) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipFree(c->K));
+    c->K = nullptr;
+    c->bytes -= (size_t)c->n_blk * blk_sz(c) * sizeof(double);
+  }
+  return 0;
+}
+
 int lymph_export_operator(const LymphCtx* cc, double* K, double* A, double* M, double* Mm) {
   LymphCtx* c = const_cast<LymphCtx*>(cc);
   if (int rc = need_built(c)) return rc;
   const size_t bs = blk_sz(c), n3 = c->n3;
   std::vector<double> tmp((size_t)c->n_blk * bs);
+  if (K && !c->K) return fail(LYMPH_ERR_STATE, "FP64 K blocks were released");
+  if (A && !c->A) return fail(LYMPH_ERR_STATE, "A_dg blocks were not kept");
   for (int which = 0; which < 2; ++which) {
     double* dst = which ? A : K;
     if (!dst) continue;
@@ -728,11 +774,14 @@ int lymph_autotune(LymphCtx* c, int reps, int* n_variants, int* unr, int* xmode,
 int lymph_set_apply_precision(LymphCtx* c, int bits) {
   if (int rc = need_built(c)) return rc;
   if (bits != 32 && bits != 64) return fail(LYMPH_ERR_ARG, "precision must be 32 or 64");
+  if (bits == 64 && !c->K)
+    return fail(LYMPH_ERR_STATE, "FP64 K blocks were released: rebuild the operator");
   if (bits == 32 && !c->K32) {
     int rc = dispatch_p(c->p, [&](auto pc) -> int {
       constexpr int P = decltype(pc)::value;
       const size_t n = (size_t)c->n_blk * Dims32<P>::BS;
       if (int r = dalloc(&c->K32, n)) return r;
+      c->bytes += n * sizeof(float);
       k_to_fp32<P><<<grid1(n, 256), 256, 0, c->stream>>>(c->K, c->K32, (size_t)c->n_blk);
       return check_launch("k_to_fp32");
     });

@@ -83,7 +83,8 @@ __global__ __launch_bounds__(256) void k_build_operator(MeshDev m, double alpha,
   for (int d = 0; d < 6; ++d) bbP[d] = m.bbox[Pp * 6 + d];
   const double hkP = m.hk[Pp];
   double* Ks = K + (size_t)m.row_ptr[Pp] * D::BS;
-  double* As = A + (size_t)m.row_ptr[Pp] * D::BS;
+  const bool keepA = A != nullptr;  // A_dg may be dropped for batched runs
+  double* As = keepA ? A + (size_t)m.row_ptr[Pp] * D::BS : nullptr;
   double* MP = M + (size_t)Pp * NP * NP;
   double* MmP = Mm + (size_t)Pp * NP * NP;
   // volume-phase views
@@ -146,7 +147,7 @@ __global__ __launch_bounds__(256) void k_build_operator(MeshDev m, double alpha,
       }
       const size_t o = kpos(N3, i * NP + mm, j * NP + nn);
       Ks[o] = Ks[o] + s;
-      As[o] = As[o] + s;
+      if (keepA) As[o] = As[o] + s;
     }
     // faces (MAKE_STIFFNESS_FACE, assemble_element :404-661; insertion
     // src/matrix_free.f90:371-441)
@@ -193,7 +194,7 @@ __global__ __launch_bounds__(256) void k_build_operator(MeshDev m, double alpha,
       const double dd[3] = {d11, d22, d33};
       const bool interior = Q >= 0;
       double* Kb = interior ? K + (size_t)m.face_blk[t * 4 + f] * D::BS : nullptr;
-      double* Ab = interior ? A + (size_t)m.face_blk[t * 4 + f] * D::BS : nullptr;
+      double* Ab = interior && keepA ? A + (size_t)m.face_blk[t * 4 + f] * D::BS : nullptr;
       constexpr int SG = NP * NQ2;
       for (int e = tid; e < 9 * NP * NP; e += blockDim.x) {
         const int i = e / (3 * NP * NP), j = (e / (NP * NP)) % 3;
@@ -241,11 +242,11 @@ __global__ __launch_bounds__(256) void k_build_operator(MeshDev m, double alpha,
         }
         const size_t o = kpos(N3, i * NP + mm, j * NP + nn);
         Ks[o] = Ks[o] + s1;
-        As[o] = As[o] + s1;
+        if (keepA) As[o] = As[o] + s1;
         Ks[o] = Ks[o] + theta * i1 - i1t;
         if (interior) {
           Kb[o] = Kb[o] + theta * i2 + s2;
-          Ab[o] = Ab[o] + s2;
+          if (keepA) Ab[o] = Ab[o] + s2;
           Kb[o] = Kb[o] - it2;
         }
       }

@@ -13,6 +13,8 @@ receiver sampling) is captured once into a HIP graph and replayed.
 
 No reference counterpart: the reference runs one instance per MPI job.
 """
+import copy
+
 import numpy as np
 
 from . import _lib
@@ -21,26 +23,44 @@ from .mesh import Mesh, cube_mesh
 
 
 def union(meshes):
-    """One Mesh whose connected components are `meshes` (node, tet and
-    polyhedron ids offset; materials and tags carried over)."""
-    xyz, tv, tp, tri, tag, rho, lam, mu = [], [], [], [], [], [], [], []
-    no = po = 0
-    tag_space = {}
+    """One Mesh whose connected components are `meshes`: node, tet and
+    polyhedron ids offset, per-tet and per-polyhedron data concatenated (no
+    face matching is redone: no face is shared between components)."""
+    no = np.cumsum([0] + [m.xyz.shape[0] for m in meshes])
+    to = np.cumsum([0] + [m.n_tet for m in meshes])
+    po = np.cumsum([0] + [m.n_poly for m in meshes])
+    u = Mesh.__new__(Mesh)
+    cat = np.concatenate
+    u.xyz = cat([m.xyz for m in meshes])
+    u.tet_vert = cat([m.tet_vert + no[k] for k, m in enumerate(meshes)]).astype(np.int32)
+    u.tet_poly = cat([m.tet_poly + po[k] for k, m in enumerate(meshes)]).astype(np.int32)
+    u.n_tet, u.n_poly = int(to[-1]), int(po[-1])
+    u.tria_vert = cat([m.tria_vert + no[k] for k, m in enumerate(meshes)]).astype(np.int32)
+    u.tria_tag = cat([m.tria_tag for m in meshes])
+    u.tag_space = {}
     for m in meshes:
-        xyz.append(m.xyz)
-        tv.append(m.tet_vert + no)
-        tp.append(m.tet_poly + po)
-        tri.append(m.tria_vert + no)
-        tag.append(m.tria_tag)
-        rho.append(m.rho)
-        lam.append(m.lam)
-        mu.append(m.mu)
-        tag_space.update(m.tag_space)
-        no += m.xyz.shape[0]
-        po += m.n_poly
-    return Mesh(np.concatenate(xyz), np.concatenate(tv), np.concatenate(tp), np.concatenate(tri),
-                np.concatenate(tag), tag_space, np.concatenate(rho), np.concatenate(lam),
-                np.concatenate(mu))
+        u.tag_space.update(m.tag_space)
+    for f in ("rho", "lam", "mu", "neigh_face", "bc_space", "normal", "area", "bbox", "hk"):
+        setattr(u, f, np.ascontiguousarray(cat([getattr(m, f) for m in meshes])))
+    u.neigh = cat([np.where(m.neigh >= 0, m.neigh + to[k], m.neigh)
+                   for k, m in enumerate(meshes)]).astype(np.int32)
+    u.poly_tets = cat([m.poly_tets + to[k] for k, m in enumerate(meshes)]).astype(np.int32)
+    u.poly_ptr = cat([[0]] + [m.poly_ptr[1:] + to[k] for k, m in enumerate(meshes)]).astype(np.int32)
+    u.degree = u.dt = u.stop_time = None
+    return u
+
+
+_CUBES = {}
+
+
+def _cube(n, agglomerate):
+    """Cube meshes are reused across instances of the same size; each
+    instance gets its own copy of the material arrays."""
+    if (n, agglomerate) not in _CUBES:
+        m = cube_mesh(n, agglomerate=agglomerate)
+        m.locate(np.full((1, 3), 0.5))  # builds the point locator shared by the copies
+        _CUBES[(n, agglomerate)] = m
+    return copy.copy(_CUBES[(n, agglomerate)])
 
 
 def make_instance(seed, n_range=(10, 20), p_choices=(2, 3), n_receivers=64, sigma=0.5,
@@ -52,7 +72,7 @@ def make_instance(seed, n_range=(10, 20), p_choices=(2, 3), n_receivers=64, sigm
     rng = np.random.default_rng(seed)
     n = int(rng.integers(n_range[0], n_range[1] + 1))
     p = int(rng.choice(p_choices))
-    m = cube_mesh(n, agglomerate=agglomerate)
+    m = _cube(n, agglomerate)
     m.lam = np.exp(rng.normal(0.0, sigma, m.n_tet))
     m.mu = np.exp(rng.normal(0.0, sigma, m.n_tet))
     m.rho = rng.uniform(0.8, 1.2, m.n_tet)
@@ -74,9 +94,11 @@ class Ensemble:
             meshes = [self.instances[i]["mesh"] for i in members]
             um = union(meshes)
             ctx = Context(um, p)
+            ctx.set_keep_adg(False)  # traces only: no DG-norm error, half the build memory
             ctx.make_matrices_free()
             if apply_bits == 32:
                 ctx.set_apply_precision(32)
+                ctx.release_fp64_operator()
             # sources and receivers located in each instance, ids offset into the union
             tet_off = np.cumsum([0] + [m.n_tet for m in meshes])
             src = np.concatenate([self.instances[i]["source"] for i in members])

@@ -43,7 +43,7 @@ class Context:
             check(L.lymph_ctx_set_layout(h, _lib.LAYOUT_FORTRAN), "lymph_ctx_set_layout")
         info = np.zeros(8, np.int64)
         check(L.lymph_ctx_info(h, ptr(info)), "lymph_ctx_info")
-        (self.n_tet, _, _, self.n_blk, self.operator_bytes, self.device_bytes,
+        (self.n_tet, _, _, self.n_blk, self.operator_bytes, _,
          self.step_bytes, self.step_flops) = (int(v) for v in info)
 
     def close(self):
@@ -156,6 +156,21 @@ class Context:
         """64 (default) or 32: precision of the step's operator apply."""
         check(lib().lymph_set_apply_precision(self.h, bits), "lymph_set_apply_precision")
         self.apply_bits = bits
+
+    def set_keep_adg(self, keep):
+        """False: do not build (or free) the A_dg blocks; the DG-norm error
+        is then unavailable.  Saves one operator's worth of device memory."""
+        check(lib().lymph_ctx_set_keep_adg(self.h, int(bool(keep))), "lymph_ctx_set_keep_adg")
+
+    def release_fp64_operator(self):
+        """Free the FP64 K blocks once the 32-bit apply is selected."""
+        check(lib().lymph_release_fp64_operator(self.h), "lymph_release_fp64_operator")
+
+    @property
+    def device_bytes(self):
+        info = np.zeros(8, np.int64)
+        check(lib().lymph_ctx_info(self.h, ptr(info)), "lymph_ctx_info")
+        return int(info[5])
 
     def step_bytes_for(self, bits):
         """Algorithmic bytes of one step with the operator in `bits` precision."""

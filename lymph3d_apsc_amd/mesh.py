@@ -143,7 +143,16 @@ class Mesh:
             pts = v[tets].reshape(len(polys), 4 * size, 3)  # vertices with duplicates
             bbox[polys, 0::2] = pts.min(axis=1)
             bbox[polys, 1::2] = pts.max(axis=1)
-            nv = 4 * size
+            # diameter over distinct vertices only: a repeated vertex repeats
+            # distances (same squares, same bits), so the maximum is unchanged
+            vid = np.sort(self.tet_vert[tets].reshape(len(polys), 4 * size), axis=1)
+            first = np.ones(vid.shape, bool)
+            first[:, 1:] = vid[:, 1:] != vid[:, :-1]
+            nv = int(first.sum(axis=1).max())
+            sel = np.argsort(~first, axis=1, kind="stable")[:, :nv]
+            cols = np.take_along_axis(vid, sel, axis=1)
+            cols = np.where(np.take_along_axis(first, sel, axis=1), cols, vid[:, :1])
+            pts = self.xyz[cols]
             ii, jj = np.triu_indices(nv, 1)
             h = np.zeros(len(polys))
             step = max(1, int(4e6 // max(1, len(ii))))
@@ -157,15 +166,43 @@ class Mesh:
 
     def locate(self, points):
         """Tet of each point as FIND_ELEM_FROM_POINT picks it (src/find_tet.f90:
-        82-124): nearest barycentre ((x1+x2+x3+x4)/4), first minimum on ties."""
+        82-124): nearest barycentre ((x1+x2+x3+x4)/4), first minimum on ties.
+        Large queries take 32 candidates from a k-d tree and decide among them
+        with the same squared-distance expression; a point whose candidates
+        are too close to call is searched exhaustively."""
         pts = np.asarray(points, np.float64).reshape(-1, 3)
-        v = self.xyz[self.tet_vert]
-        bc = (((v[:, 0] + v[:, 1]) + v[:, 2]) + v[:, 3]) / 4
+        loc = getattr(self, "_locator", None)
+        if loc is None or loc[0] is not self.tet_vert:
+            v = self.xyz[self.tet_vert]
+            loc = [self.tet_vert, (((v[:, 0] + v[:, 1]) + v[:, 2]) + v[:, 3]) / 4, None]
+            self._locator = loc  # shared by shallow copies of this mesh
+        bc = loc[1]
+
+        def brute(q):
+            d = q[:, None, :] - bc[None, :, :]
+            d2 = d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1] + d[..., 2] * d[..., 2]
+            return np.argmin(d2, axis=1)  # first minimum on ties
+
         out = np.zeros(pts.shape[0], np.int32)
-        for k in range(pts.shape[0]):
-            d = pts[k] - bc
-            d2 = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]
-            out[k] = int(np.argmin(d2))
+        if pts.shape[0] * self.n_tet <= 200_000 or self.n_tet <= 64:
+            step = max(1, int(2e7 // max(1, self.n_tet)))
+            for k in range(0, pts.shape[0], step):
+                out[k:k + step] = brute(pts[k:k + step])
+            return out
+        if loc[2] is None:
+            from scipy.spatial import cKDTree
+            loc[2] = cKDTree(bc)
+        tree = loc[2]
+        dist, cand = tree.query(pts, k=32)
+        d = pts[:, None, :] - bc[cand]
+        d2 = d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1] + d[..., 2] * d[..., 2]
+        m = d2.min(axis=1)
+        out[:] = np.where(d2 == m[:, None], cand, self.n_tet).min(axis=1)
+        # the exhaustive rule could pick a tet outside the candidates only if
+        # the 32nd candidate ties the nearest up to rounding
+        unsure = np.nonzero(dist[:, -1] * dist[:, -1] <= m * (1 + 1e-9) + 1e-300)[0]
+        if unsure.size:
+            out[unsure] = brute(pts[unsure])
         return out
 
     def hmax(self):

@@ -64,6 +64,29 @@ def test_union_is_block_diagonal_and_bitwise_per_instance():
     assert blk == ou.n_blk
 
 
+def test_union_equals_mesh_built_from_scratch():
+    """The concatenating union gives the same arrays as face matching and
+    geometry redone on the joined mesh."""
+    inst = [E.make_instance(s, n_range=(2, 3)) for s in (1, 2, 3)]
+    meshes = [i["mesh"] for i in inst]
+    u = E.union(meshes)
+    no = np.cumsum([0] + [m.xyz.shape[0] for m in meshes])
+    po = np.cumsum([0] + [m.n_poly for m in meshes])
+    cat = np.concatenate
+    full = M.Mesh(cat([m.xyz for m in meshes]), cat([m.tet_vert + no[k] for k, m in enumerate(meshes)]),
+                  cat([m.tet_poly + po[k] for k, m in enumerate(meshes)]),
+                  cat([m.tria_vert + no[k] for k, m in enumerate(meshes)]),
+                  cat([m.tria_tag for m in meshes]), meshes[0].tag_space,
+                  cat([m.rho for m in meshes]), cat([m.lam for m in meshes]), cat([m.mu for m in meshes]))
+    assert set(vars(full)) == set(vars(u))
+    for k, v in vars(full).items():
+        if isinstance(v, np.ndarray):
+            assert np.array_equal(v, getattr(u, k)), k
+            assert v.dtype == getattr(u, k).dtype, k
+        else:
+            assert v == getattr(u, k), k
+
+
 def test_union_point_location_offsets():
     inst = [E.make_instance(s, n_range=(2, 2), n_receivers=5) for s in (4, 5)]
     meshes = [i["mesh"] for i in inst]

@@ -270,6 +270,47 @@ def test_fp32_apply(name, p):
         assert rel(a64, b) < 1e-13
 
 
+def test_memory_options_keep_results():
+    """Dropping A_dg and releasing the FP64 blocks changes no result: the
+    operator built without A_dg, the 32-bit apply after the release and a
+    rebuild all give bit for bit the steps of a context that keeps everything;
+    what was dropped reports a state error."""
+    say("[memory options]")
+    m = make_mesh("hetero")
+    p = 2
+    rng = np.random.default_rng(4)
+    with L.Context(m, p) as full, L.Context(m, p) as lean:
+        n = m.n_poly * full.n3
+        u0, u1 = rng.standard_normal(n), rng.standard_normal(n)
+        dt, g = 1e-3, 0.37
+        full.make_matrices_free()
+        lean.set_keep_adg(False)
+        lean.make_matrices_free()
+        assert lean.device_bytes < full.device_bytes
+        for c in (full, lean):
+            c.make_rhs_free(L.PROB_NULL, _lib.SLOT_RHS_STAT)
+            c.make_rhs_free(L.PROB_REF_MMS, _lib.SLOT_RHS_DYN)
+        a64 = full.time_step_matrix_free(g, dt, u0, u1)
+        assert np.array_equal(lean.time_step_matrix_free(g, dt, u0, u1), a64)
+        with pytest.raises(L.LymphError):
+            lean.compute_error_dg_matrix_free(u0, u1)
+        full.set_apply_precision(32)
+        lean.set_apply_precision(32)
+        before = lean.device_bytes
+        lean.release_fp64_operator()
+        assert lean.device_bytes < before
+        a32 = full.time_step_matrix_free(g, dt, u0, u1)
+        assert np.array_equal(lean.time_step_matrix_free(g, dt, u0, u1), a32)
+        with pytest.raises(L.LymphError):
+            lean.set_apply_precision(64)
+        with pytest.raises(L.LymphError):
+            lean.export_operator()
+        lean.make_matrices_free()  # rebuild: FP64 blocks back, 32-bit copy refreshed
+        assert np.array_equal(lean.time_step_matrix_free(g, dt, u0, u1), a32)
+        lean.set_apply_precision(64)
+        assert np.array_equal(lean.time_step_matrix_free(g, dt, u0, u1), a64)
+
+
 def test_vertex_export_matches_oracle(tmp_path):
     """Vertex values for the VTK export (receiver kernel at tet vertices) vs
     the oracle's point evaluation of the same vector."""
