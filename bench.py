@@ -97,6 +97,72 @@ def cpu_baseline(p, n_sample, target_s):
                 _unused=float(u2[0]))
 
 
+def ensemble_leg(n_inst, steps, plain_sample, f0=5.0, seed0=1000):
+    """Batched ensemble: n_inst seeded instances (n^3 cubes, n uniform in
+    10..20, p in {2, 3}, lognormal lambda/mu, Ricker point force, 64
+    receivers recorded every step), one context per degree with the 32-bit
+    apply and the whole solve captured in a HIP graph, against the plain path
+    (FP64, one instance at a time, no graph) on a sample of the same
+    instances; accuracy = relative L2 difference of the receiver traces."""
+    from lymph3d_apsc_amd import ensemble as E
+    from lymph3d_apsc_amd import _lib
+    import lymph3d_apsc_amd as L
+    t0 = time.perf_counter()
+    inst = [E.make_instance(seed0 + s) for s in range(n_inst)]
+    # one dt for every instance: CFL 0.2 with the fastest P-wave speed
+    dt = min(0.2 * (1.0 / i["n"]) / (np.sqrt(((i["mesh"].lam + 2 * i["mesh"].mu) / i["mesh"].rho).max())
+                                     * i["p"] ** 2) for i in inst)
+    t_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ens = E.Ensemble(inst, dt, steps, every=1, f0=f0, apply_bits=32, autotune=True)
+    t_setup = time.perf_counter() - t0
+    ens.run()  # first replay (untimed)
+    t0 = time.perf_counter()
+    tr = ens.run()
+    t_ens = time.perf_counter() - t0
+    groups = [{"p": g["p"], "instances": len(g["members"]), "n_poly": g["ctx"].n_poly,
+               "n_blk": g["ctx"].n_blk, "device_GB": g["ctx"].device_bytes / 1e9,
+               "step_bytes": g["ctx"].step_bytes_for(32)} for g in ens.groups]
+    ens.close()
+    del ens
+    # plain path on a sample: FP64, one instance at a time, uncaptured
+    sample = list(range(0, n_inst, max(1, n_inst // plain_sample)))[:plain_sample]
+    t_plain, t_plain_setup, errs = 0.0, 0.0, []
+    for k in sample:
+        ins = inst[k]
+        s0 = time.perf_counter()
+        with L.Context(ins["mesh"], ins["p"]) as ctx:
+            ctx.set_keep_adg(False)
+            t = ctx.start_from_rest(dt, ins["source"], ins["force"], f0=f0)
+            ctx.set_receivers(ins["receivers"], max_records=steps)
+            ctx.sync()
+            s1 = time.perf_counter()
+            ctx.step_n_record(L.PROB_RICKER, steps, t, dt, every=1)
+            ctx.sync()
+            s2 = time.perf_counter()
+            ref = ctx.traces()
+        t_plain += s2 - s1
+        t_plain_setup += s1 - s0
+        errs.append(float(np.linalg.norm(tr[k] - ref) / max(np.linalg.norm(ref), 1e-300)))
+    plain_rate = len(sample) / t_plain
+    ens_rate = n_inst / t_ens
+    return {
+        "value": ens_rate, "unit": "ensemble solves/s",
+        "instances": n_inst, "steps": steps, "dt": dt, "receivers_per_instance": 64,
+        "dofs_total": int(sum(g["n_poly"] * 3 * (g["p"] + 1) * (g["p"] + 2) * (g["p"] + 3) // 6
+                              for g in groups)),
+        "groups": groups, "seconds_per_ensemble_solve": t_ens,
+        "step_GBps": sum(g["step_bytes"] for g in groups) * (steps + 1) / t_ens / 1e9,
+        "setup_s": t_setup, "instance_generation_s": t_gen,
+        "plain_fp64": {"value": plain_rate, "unit": "solves/s", "sample_instances": len(sample),
+                       "seconds": t_plain, "setup_seconds": t_plain_setup},
+        "speedup_vs_plain": ens_rate / plain_rate,
+        "trace_rel_l2_vs_fp64_max": max(errs), "trace_rel_l2_vs_fp64_mean": float(np.mean(errs)),
+        "note": "solve = zero state, first step, then `steps` leapfrog steps recording 64 "
+                "receivers per step; setup (operator build, RHS) timed separately",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +175,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-apply measurement")
     ap.add_argument("--no-autotune", action="store_true", help="use the default step variant")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--ens-instances", type=int, default=256)
+    ap.add_argument("--ens-steps", type=int, default=1000)
+    ap.add_argument("--ens-plain-sample", type=int, default=8)
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -208,6 +278,14 @@ def main():
         }
         log(f"fp32 apply: {ms32_step:.3f} ms/step, rel diff vs fp64 {res['fp32_apply']['rel_l2_diff_vs_fp64']:.2e}")
     del ctx
+    if not a.no_ensemble:
+        try:
+            res["ensemble"] = ensemble_leg(a.ens_instances, a.ens_steps, a.ens_plain_sample)
+            e = res["ensemble"]
+            log(f"ensemble: {e['value']:.2f} solves/s ({e['instances']} instances), plain FP64 "
+                f"{e['plain_fp64']['value']:.2f} solves/s, trace rel L2 {e['trace_rel_l2_vs_fp64_max']:.2e}")
+        except Exception as ex:  # reported, never fatal
+            res["ensemble"] = {"error": repr(ex)}
     if not a.no_cpu_baseline:
         try:
             cb = cpu_baseline(p, a.cpu_sample, a.cpu_seconds)

@@ -217,6 +217,14 @@ int lymph_add_point_sources(LymphCtx* ctx, int n, const double* xyz, const int32
 int lymph_set_ricker(LymphCtx* ctx, double f0, double t0);
 /* (1 - 2a) exp(-a), a = (pi f0 (t - t0))^2 */
 double lymph_ricker(double f0, double t0, double t);
+/* Per-group wavelets for batched instances: polyhedron i takes the Ricker
+ * factor of group poly_group[i] (f0[g], t0[g]) in the stepping entry points
+ * that evaluate the time factor themselves (lymph_step_n, lymph_time_steps,
+ * lymph_step_n_record, lymph_capture_solve) with LYMPH_PROB_RICKER; the
+ * factors are computed on the device per step.  n_groups = 0 restores the
+ * single lymph_set_ricker wavelet.  No reference counterpart. */
+int lymph_set_source_groups(LymphCtx* ctx, int n_groups, const int32_t* poly_group,
+                            const double* f0, const double* t0);
 
 /* Receivers: point values of the displacement, evaluated like EXPORT_SOLUTION
  * (src/solution_processing.f90:150-170) in the tet given for each point

@@ -61,6 +61,11 @@ struct LymphCtx {
   int apply_bits = 64;
   bool keep_adg = true;
   double ricker_f0 = 5.0, ricker_t0 = 1.2 / 5.0;
+  // per-group Ricker wavelets (ensembles): group of each polyhedron, f0/t0
+  // per group, and the table of this step's factors
+  int n_groups = 0;
+  int* gsel = nullptr;
+  double *gf0 = nullptr, *gt0 = nullptr, *gtab = nullptr;
   float* K32 = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -128,14 +133,15 @@ template <int P, int MODE, int I = 0>
 void launch_leapfrog(int variant, int grid, hipStream_t st, const double* K, const int* rp,
                      const int* col, const double* Rp, const double* rs, const double* rd,
                      const double* u, const double* wa, double* out, double g, double c1,
-                     double c2) {
+                     double c2, const int* gsel = nullptr, const double* gtab = nullptr) {
   if constexpr (I < n_var<P>()) {
     if (variant == I) {
       k_leapfrog<P, MODE, StepVar<P>::unr[I], StepVar<P>::xs[I]><<<grid, 64, 0, st>>>(
-          K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2);
+          K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab);
       return;
     }
-    launch_leapfrog<P, MODE, I + 1>(variant, grid, st, K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2);
+    launch_leapfrog<P, MODE, I + 1>(variant, grid, st, K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2,
+                                    gsel, gtab);
   }
 }
 
@@ -150,15 +156,16 @@ template <int P, int MODE, int I = 0>
 void launch_leapfrog32(int variant, int grid, hipStream_t st, const float* K, const int* rp,
                        const int* col, const double* Rp, const double* rs, const double* rd,
                        const double* u, const double* wa, double* out, double g, double c1,
-                       double c2) {
+                       double c2, const int* gsel = nullptr, const double* gtab = nullptr) {
   if constexpr (I < n_var32<P>()) {
     if (variant == I) {
       k_leapfrog32<P, MODE, StepVar32<P>::unr[I]><<<grid, 64, 0, st>>>(K, rp, col, Rp, rs, rd, u,
-                                                                       wa, out, g, c1, c2);
+                                                                       wa, out, g, c1, c2, gsel,
+                                                                       gtab);
       return;
     }
     launch_leapfrog32<P, MODE, I + 1>(variant, grid, st, K, rp, col, Rp, rs, rd, u, wa, out, g, c1,
-                                      c2);
+                                      c2, gsel, gtab);
   }
 }
 
@@ -202,7 +209,9 @@ int d2h_vec(LymphCtx* c, const double* d, double* h) {
 }
 
 int launch_step(LymphCtx* c, int mode, double g, double c1, double c2, const double* u,
-                const double* wa, double* out) {
+                const double* wa, double* out, bool grouped = false) {
+  const int* gsel = grouped ? c->gsel : nullptr;
+  const double* gtab = grouped ? c->gtab : nullptr;
   return dispatch_p(c->p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
     const double* rs = c->slot[LYMPH_SLOT_RHS_STAT];
@@ -210,19 +219,33 @@ int launch_step(LymphCtx* c, int mode, double g, double c1, double c2, const dou
     if (c->apply_bits == 32) {
       if (mode == 0)
         launch_leapfrog32<P, 0>(c->step_variant, c->n_poly, c->stream, c->K32, c->md.row_ptr,
-                                c->md.col, c->RM, rs, rd, u, wa, out, g, c1, c2);
+                                c->md.col, c->RM, rs, rd, u, wa, out, g, c1, c2, gsel, gtab);
       else
         launch_leapfrog32<P, 1>(c->step_variant, c->n_poly, c->stream, c->K32, c->md.row_ptr,
-                                c->md.col, c->RM, rs, rd, u, wa, out, g, c1, c2);
+                                c->md.col, c->RM, rs, rd, u, wa, out, g, c1, c2, gsel, gtab);
     } else if (mode == 0) {
       launch_leapfrog<P, 0>(c->step_variant, c->n_poly, c->stream, c->K, c->md.row_ptr, c->md.col,
-                            c->RM, rs, rd, u, wa, out, g, c1, c2);
+                            c->RM, rs, rd, u, wa, out, g, c1, c2, gsel, gtab);
     } else {
       launch_leapfrog<P, 1>(c->step_variant, c->n_poly, c->stream, c->K, c->md.row_ptr, c->md.col,
-                            c->RM, rs, rd, u, wa, out, g, c1, c2);
+                            c->RM, rs, rd, u, wa, out, g, c1, c2, gsel, gtab);
     }
     return check_launch("k_leapfrog");
   });
+}
+
+// Time factor of `problem` at t for a step launched next: the scalar g, and
+// with per-group Ricker wavelets the device table for this step (filled on
+// the context's stream, so each step reads its own t).
+int prepare_g(LymphCtx* c, int problem, double t, double* g, bool* grouped) {
+  *g = g_of(c, problem, t);
+  *grouped = problem == LYMPH_PROB_RICKER && c->n_groups > 0;
+  if (*grouped) {
+    k_ricker_table<<<grid1(c->n_groups, 64), 64, 0, c->stream>>>(c->gf0, c->gt0, c->n_groups, t,
+                                                                c->gtab);
+    return check_launch("k_ricker_table");
+  }
+  return 0;
 }
 
 int errors_dev(LymphCtx* c, const double* uh, const double* ue, double* l2sq, double* dgsq) {
@@ -284,6 +307,8 @@ int lymph_ctx_destroy(LymphCtx* c) {
   if (c->rxyz) (void)hipFree(c->rxyz);
   if (c->trace) (void)hipFree(c->trace);
   if (c->K32) (void)hipFree(c->K32);
+  for (void* q : {(void*)c->gsel, (void*)c->gf0, (void*)c->gt0, (void*)c->gtab})
+    if (q) (void)hipFree(q);
   if (c->K) (void)hipFree(c->K);
   if (c->A) (void)hipFree(c->A);
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
@@ -691,8 +716,11 @@ int lymph_step_n(LymphCtx* c, int problem, int nsteps, double t0, double dt, dou
   double t = t0;
   const double dt2 = dt * dt;
   for (int s = 0; s < nsteps; ++s) {
-    if (int rc = launch_step(c, 0, g_of(c, problem, t), dt2, 0.0, c->slot[LYMPH_SLOT_U_CUR],
-                             c->slot[LYMPH_SLOT_U_PREV], c->slot[LYMPH_SLOT_U_NEXT]))
+    double g;
+    bool grouped;
+    if (int rc = prepare_g(c, problem, t, &g, &grouped)) return rc;
+    if (int rc = launch_step(c, 0, g, dt2, 0.0, c->slot[LYMPH_SLOT_U_CUR],
+                             c->slot[LYMPH_SLOT_U_PREV], c->slot[LYMPH_SLOT_U_NEXT], grouped))
       return rc;
     double* prev = c->slot[LYMPH_SLOT_U_PREV];
     c->slot[LYMPH_SLOT_U_PREV] = c->slot[LYMPH_SLOT_U_CUR];
@@ -790,6 +818,34 @@ int lymph_set_apply_precision(LymphCtx* c, int bits) {
   }
   c->apply_bits = bits;
   c->step_variant = 0;
+  return 0;
+}
+
+int lymph_set_source_groups(LymphCtx* c, int n_groups, const int32_t* poly_group,
+                            const double* f0, const double* t0) {
+  if (int rc = need_ctx(c)) return rc;
+  if (n_groups < 0) return fail(LYMPH_ERR_ARG, "n_groups < 0");
+  if (n_groups > 0) {
+    if (!poly_group || !f0 || !t0) return fail(LYMPH_ERR_ARG, "null group array");
+    for (int i = 0; i < c->n_poly; ++i)
+      if (poly_group[i] < 0 || poly_group[i] >= n_groups)
+        return fail(LYMPH_ERR_ARG, "polyhedron group out of range");
+    for (int i = 0; i < n_groups; ++i)
+      if (!(f0[i] > 0)) return fail(LYMPH_ERR_ARG, "f0 must be positive");
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (void** q : {(void**)&c->gsel, (void**)&c->gf0, (void**)&c->gt0, (void**)&c->gtab})
+    if (*q) {
+      (void)hipFree(*q);
+      *q = nullptr;
+    }
+  c->n_groups = 0;
+  if (n_groups == 0) return 0;
+  if (int rc = upload(&c->gsel, poly_group, (size_t)c->n_poly)) return rc;
+  if (int rc = upload(&c->gf0, f0, (size_t)n_groups)) return rc;
+  if (int rc = upload(&c->gt0, t0, (size_t)n_groups)) return rc;
+  if (int rc = dalloc(&c->gtab, (size_t)n_groups)) return rc;
+  c->n_groups = n_groups;
   return 0;
 }
 
@@ -931,9 +987,12 @@ int lymph_capture_solve(LymphCtx* c, int problem, double dt, int nsteps, int eve
     return check_launch("k_project");
   });
   double t = 0.0;
+  double g0 = 0.0;
+  bool grouped = false;
+  if (!rc) rc = prepare_g(c, problem, t, &g0, &grouped);
   if (!rc)
-    rc = launch_step(c, 1, g_of(c, problem, t), 0.5 * (dt * dt), dt, c->slot[LYMPH_SLOT_U_PREV],
-                     c->slot[LYMPH_SLOT_V0], c->slot[LYMPH_SLOT_U_CUR]);
+    rc = launch_step(c, 1, g0, 0.5 * (dt * dt), dt, c->slot[LYMPH_SLOT_U_PREV],
+                     c->slot[LYMPH_SLOT_V0], c->slot[LYMPH_SLOT_U_CUR], grouped);
   t = t + dt;
   if (!rc) {
     if (c->n_recv > 0) rc = lymph_step_n_record(c, problem, nsteps, t, dt, every, &t);

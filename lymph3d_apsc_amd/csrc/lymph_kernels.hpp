@@ -508,7 +508,8 @@ __global__ __launch_bounds__(64) void k_leapfrog(const double* __restrict__ K,
                                                  const double* __restrict__ u,
                                                  const double* __restrict__ wa,
                                                  double* __restrict__ out, double g, double c1,
-                                                 double c2) {
+                                                 double c2, const int* __restrict__ gsel,
+                                                 const double* __restrict__ gtab) {
   using D = Dims<P>;
   constexpr int NP = D::NP, N3 = D::N3, NPAIR = D::NPAIR;
   constexpr int RPL = (N3 + 63) / 64;
@@ -595,10 +596,13 @@ __global__ __launch_bounds__(64) void k_leapfrog(const double* __restrict__ K,
     }
     return;
   }
+  // time factor: one scalar, or per source group (ensembles whose instances
+  // carry their own wavelet; the table is filled for this step)
+  const double gp = gsel ? gtab[gsel[Pp]] : g;
 #pragma unroll
   for (int k = 0; k < RPL; ++k) {
     const int r = lane + 64 * k;
-    if (r < N3) sw[r] = -acc[k] + rs[o + r] + rd[o + r] * g;
+    if (r < N3) sw[r] = -acc[k] + rs[o + r] + rd[o + r] * gp;
   }
   __syncthreads();
   if (lane < 3) solve_packed<NP>(sR, sw + lane * NP);
@@ -655,7 +659,8 @@ __global__ __launch_bounds__(64) void k_leapfrog32(const float* __restrict__ K,
                                                    const double* __restrict__ u,
                                                    const double* __restrict__ wa,
                                                    double* __restrict__ out, double g, double c1,
-                                                   double c2) {
+                                                   double c2, const int* __restrict__ gsel,
+                                                   const double* __restrict__ gtab) {
   using D = Dims<P>;
   constexpr int NP = D::NP, N3 = D::N3, NQUAD = Dims32<P>::NQUAD;
   constexpr int RPL = (N3 + 63) / 64;
@@ -723,10 +728,13 @@ __global__ __launch_bounds__(64) void k_leapfrog32(const float* __restrict__ K,
     for (int k = 0; k < RPL; ++k) acc[k] = (b == b0) ? (double)s[k] : acc[k] + (double)s[k];
   }
   const size_t o = (size_t)Pp * N3;
+  // time factor: one scalar, or per source group (ensembles whose instances
+  // carry their own wavelet; the table is filled for this step)
+  const double gp = gsel ? gtab[gsel[Pp]] : g;
 #pragma unroll
   for (int k = 0; k < RPL; ++k) {
     const int r = lane + 64 * k;
-    if (r < N3) sw[r] = -acc[k] + rs[o + r] + rd[o + r] * g;
+    if (r < N3) sw[r] = -acc[k] + rs[o + r] + rd[o + r] * gp;
   }
   __syncthreads();
   if (lane < 3) solve_packed<NP>(sR, sw + lane * NP);
@@ -862,6 +870,14 @@ __global__ __launch_bounds__(128) void k_point_sources(const double* __restrict_
     }
     __syncthreads();
   }
+}
+
+
+// Per-group Ricker time factors of one step: gtab[i] = ricker(f0[i], t0[i], t).
+__global__ void k_ricker_table(const double* __restrict__ f0, const double* __restrict__ t0,
+                               int n, double t, double* __restrict__ gtab) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) gtab[i] = ricker(f0[i], t0[i], t);
 }
 
 }  // namespace lymph

@@ -64,7 +64,7 @@ def _cube(n, agglomerate):
 
 
 def make_instance(seed, n_range=(10, 20), p_choices=(2, 3), n_receivers=64, sigma=0.5,
-                  agglomerate=True):
+                  agglomerate=True, f0=5.0):
     """One seeded instance: n^3 cubes (n uniform in n_range), degree from
     p_choices, lognormal(0, sigma) lambda and mu per tet, rho ~ U[0.8, 1.2],
     Dirichlet on all faces, a z-directed Ricker point force at a seeded
@@ -79,11 +79,13 @@ def make_instance(seed, n_range=(10, 20), p_choices=(2, 3), n_receivers=64, sigm
     src = rng.uniform(0.25, 0.75, (1, 3))
     rec = rng.uniform(0.05, 0.95, (n_receivers, 3))
     return dict(seed=seed, n=n, p=p, mesh=m, source=src, force=np.array([[0.0, 0.0, 1.0]]),
-                receivers=rec)
+                receivers=rec, f0=f0)
 
 
 class Ensemble:
-    """Instances grouped by degree; one context per group on the GPU."""
+    """Instances grouped by degree; one context per group on the GPU.  Each
+    instance's Ricker wavelet uses its own f0 (instance["f0"], default `f0`)
+    through per-group time factors computed on the device each step."""
 
     def __init__(self, instances, dt, nsteps, every=1, f0=5.0, apply_bits=32, autotune=True):
         self.instances = list(instances)
@@ -109,6 +111,9 @@ class Ensemble:
             rtet = np.concatenate([m.locate(self.instances[i]["receivers"]) + tet_off[k]
                                    for k, (i, m) in enumerate(zip(members, meshes))])
             ctx.set_ricker(f0)
+            # each instance keeps its own wavelet (f0, t0 = 1.2/f0)
+            f0s = np.array([self.instances[i].get("f0", f0) for i in members], np.float64)
+            ctx.set_source_groups(np.repeat(np.arange(len(members)), [m.n_poly for m in meshes]), f0s)
             ctx.make_rhs_free(_lib.PROB_NULL, _lib.SLOT_RHS_STAT)
             ctx.make_rhs_free(_lib.PROB_RICKER, _lib.SLOT_RHS_DYN)
             ctx.add_point_sources(src, frc, tets=stet.astype(np.int32))

@@ -191,6 +191,19 @@ class Context:
         self.ricker = (f0, 1.2 / f0 if t0 is None else t0)
         check(lib().lymph_set_ricker(self.h, *self.ricker), "lymph_set_ricker")
 
+    def set_source_groups(self, poly_group, f0, t0=None):
+        """Per-group Ricker wavelets: polyhedron i uses (f0[g], t0[g]) with
+        g = poly_group[i]; t0 defaults to 1.2/f0.  None clears."""
+        if poly_group is None:
+            check(lib().lymph_set_source_groups(self.h, 0, None, None, None), "lymph_set_source_groups")
+            return
+        pg = np.ascontiguousarray(poly_group, np.int32)
+        f0 = np.ascontiguousarray(f0, np.float64)
+        t0 = 1.2 / f0 if t0 is None else np.ascontiguousarray(t0, np.float64)
+        self._groups = (pg, f0, t0)
+        check(lib().lymph_set_source_groups(self.h, f0.size, ptr(pg), ptr(f0), ptr(t0)),
+              "lymph_set_source_groups")
+
     def start_from_rest(self, dt, sources=None, forces=None, f0=5.0, t0=None):
         """Operator (if needed), zero static RHS, point-force dynamic RHS, zero
         initial state, first step at t=0; returns t after the first step."""
