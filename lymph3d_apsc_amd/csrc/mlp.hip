@@ -1,0 +1,562 @@
+// mlp.hip -- liblymph_mlp.so: fully connected network (GELU) trained and
+// evaluated with hand-written BF16 MFMA kernels for gfx950 (MI355X).
+//
+// GEMM: C[M][N] = sum_k A(m,k) B(k,n), bf16 operands, FP32 accumulation on
+// v_mfma_f32_16x16x32_bf16.  64x64 output tile per 256-thread workgroup, each
+// of the 4 waves owning a 32x32 quarter (2x2 MFMA tiles); k advances 32 at a
+// time through LDS tiles stored k-contiguous for both operands, so every
+// lane's MFMA fragment is one 16-byte LDS read.  Operands may be k-contiguous
+// in memory (row-major A, "W[n][k]" B: 16-byte loads copied as they are) or
+// m/n-contiguous (16-byte loads along m/n, transposed into LDS), which covers
+// the forward pass (X W^T), the input gradient (dZ W) and the weight
+// gradient (dZ^T X) without transposed copies.  Epilogues fuse bias + GELU
+// (forward, keeping the FP32 pre-activation) and the GELU derivative
+// (backward).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lymph_mlp.h"
+
+namespace {
+
+typedef uint16_t bf16_t;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+thread_local std::string g_err;
+int fail(int code, std::string m) {
+  g_err = std::move(m);
+  return code;
+}
+#define HIP_TRY(x)                                                                          \
+  do {                                                                                      \
+    hipError_t e_ = (x);                                                                    \
+    if (e_ != hipSuccess) return fail(LYMPH_MLP_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// float -> bfloat16, round to nearest even (NaN kept quiet)
+__host__ __device__ inline bf16_t f2bf(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (bf16_t)(u >> 16);
+}
+__device__ inline float bf2f(bf16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ inline float gelu_grad(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
+         z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 8;  // LDS row: 32 k + 8 pad (80 B)
+enum { EPI_F32 = 0, EPI_GELU_FWD = 1, EPI_GELU_BWD = 2 };
+
+struct GemmArgs {
+  int M, N, K;
+  const bf16_t* A;  // A(m,k) = A_KC ? A[m*lda + k] : A[k*lda + m]
+  int lda;
+  const bf16_t* B;  // B(k,n) = B_KC ? B[n*ldb + k] : B[k*ldb + n]
+  int ldb;
+  float* C32;  // EPI_F32: result (+ bias);   EPI_GELU_FWD: pre-activation z
+  bf16_t* C16; // EPI_GELU_FWD: gelu(z);      EPI_GELU_BWD: acc * gelu'(aux)
+  int ldc;
+  const float* bias;  // [N] or null
+  const float* aux;   // EPI_GELU_BWD: pre-activation, ldc layout
+};
+
+// one 64 x 32 operand tile into LDS, rows = m (or n), k contiguous
+template <bool KC>
+__device__ inline void load_tile(bf16_t* __restrict__ s, const bf16_t* __restrict__ g, int ld,
+                                 int r0, int R, int k0, int K, int tid) {
+  if (KC) {
+    const int r = tid >> 2, kc = (tid & 3) * 8;
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (r0 + r < R && k0 + kc < K)
+      v = *reinterpret_cast<const uint4*>(g + (size_t)(r0 + r) * ld + k0 + kc);
+    *reinterpret_cast<uint4*>(s + r * LDK + kc) = v;
+  } else {
+    const int kk = tid >> 3, rc = (tid & 7) * 8;
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (k0 + kk < K && r0 + rc < R)
+      v = *reinterpret_cast<const uint4*>(g + (size_t)(k0 + kk) * ld + r0 + rc);
+    const bf16_t* e = reinterpret_cast<const bf16_t*>(&v);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[(rc + j) * LDK + kk] = e[j];
+  }
+}
+
+template <bool A_KC, bool B_KC, int EPI>
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) bf16_t sA[BM * LDK];
+  __shared__ __attribute__((aligned(16))) bf16_t sB[BN * LDK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  for (int k0 = 0; k0 < g.K; k0 += BK) {
+    load_tile<A_KC>(sA, g.A, g.lda, m0, g.M, k0, g.K, tid);
+    load_tile<B_KC>(sB, g.B, g.ldb, n0, g.N, k0, g.K, tid);
+    __syncthreads();
+    bf16x8 af[2], bfg[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+      af[i] = *reinterpret_cast<const bf16x8*>(sA + (wm + i * 16 + fr) * LDK + fk);
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+      bfg[j] = *reinterpret_cast<const bf16x8*>(sB + (wn + j * 16 + fr) * LDK + fk);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfg[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+  // D fragment: lane holds rows 4*(lane/16) + r, column lane % 16 of each 16x16 tile
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
+        const int n = n0 + wn + j * 16 + (lane & 15);
+        if (m >= g.M || n >= g.N) continue;
+        const size_t o = (size_t)m * g.ldc + n;
+        float v = acc[i][j][r];
+        if (EPI == EPI_F32) {
+          g.C32[o] = g.bias ? v + g.bias[n] : v;
+        } else if (EPI == EPI_GELU_FWD) {
+          const float z = v + g.bias[n];
+          g.C32[o] = z;
+          g.C16[o] = f2bf(gelu(z));
+        } else {
+          g.C16[o] = f2bf(v * gelu_grad(g.aux[o]));
+        }
+      }
+}
+
+__global__ void k_gather(const float* __restrict__ X, int w, const int* __restrict__ idx, int B,
+                         bf16_t* __restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)B * w) return;
+  const int b = (int)(i / w), k = (int)(i % w);
+  out[i] = f2bf(X[(size_t)idx[b] * w + k]);
+}
+
+// dY = scale * (y - t) in bf16; per-block partial sums of (y - t)^2
+__global__ __launch_bounds__(256) void k_loss(const float* __restrict__ Y, int N,
+                                              const float* __restrict__ T,
+                                              const int* __restrict__ idx, int B, float scale,
+                                              bf16_t* __restrict__ dY, float* __restrict__ part) {
+  __shared__ float red[256];
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float d2 = 0.f;
+  if (i < (size_t)B * N) {
+    const int b = (int)(i / N), n = (int)(i % N);
+    const float d = Y[i] - T[(size_t)idx[b] * N + n];
+    dY[i] = f2bf(scale * d);
+    d2 = d * d;
+  }
+  red[threadIdx.x] = d2;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// ordered single-lane sum (deterministic)
+__global__ void k_sum(const float* __restrict__ part, int n, float scale, float* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s += part[i];
+  out[0] = (float)(s * scale);
+}
+
+// out[n] = sum_b D[b][n]
+__global__ void k_colsum(const bf16_t* __restrict__ D, int B, int N, float* __restrict__ out) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  float s = 0.f;
+  for (int b = 0; b < B; ++b) s += bf2f(D[(size_t)b * N + n]);
+  out[n] = s;
+}
+
+__global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
+                       const float* __restrict__ gr, bf16_t* __restrict__ wb, size_t n, float lr,
+                       float b1, float b2, float eps, float c1, float c2) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float g = gr[i];
+  const float mi = b1 * m[i] + (1.f - b1) * g;
+  const float vi = b2 * v[i] + (1.f - b2) * g * g;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+  w[i] = wi;
+  if (wb) wb[i] = f2bf(wi);
+}
+
+__global__ void k_to_bf16(const float* __restrict__ a, bf16_t* __restrict__ b, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = f2bf(a[i]);
+}
+
+unsigned blocks(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+struct LymphMlp {
+  int L = 0, max_batch = 0;
+  std::vector<int> w;                  // widths
+  std::vector<size_t> woff, boff;      // offsets of W_l, b_l in the parameter vector
+  size_t n_params = 0;
+  float *P = nullptr, *G = nullptr, *Mo = nullptr, *Vo = nullptr;  // master, grad, Adam
+  bf16_t* Wb = nullptr;                // bf16 copy (W parts used; b parts unused)
+  std::vector<bf16_t*> H;              // H[0] input .. H[L-1] (bf16 activations)
+  std::vector<float*> Z;               // Z[0..L-2] pre-activations
+  float* Y = nullptr;                  // [B][w_L]
+  bf16_t *dY = nullptr, *dZa = nullptr, *dZb = nullptr;
+  float *part = nullptr, *loss = nullptr;
+  int* idx = nullptr;
+  int n_data = 0;
+  float *X = nullptr, *T = nullptr;
+  int adam_t = 0;
+  std::vector<void*> owned;
+};
+
+namespace {
+
+template <class T>
+int dalloc(LymphMlp* m, T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  if (hipMalloc((void**)p, n * sizeof(T)) != hipSuccess)
+    return fail(LYMPH_MLP_ERR_HIP, "hipMalloc(" + std::to_string(n * sizeof(T)) + " B) failed");
+  m->owned.push_back((void*)*p);
+  return 0;
+}
+
+template <bool A_KC, bool B_KC, int EPI>
+int gemm(const GemmArgs& a) {
+  if (a.M <= 0 || a.N <= 0 || a.K <= 0) return 0;
+  dim3 grid(blocks(a.N, BN), blocks(a.M, BM));
+  k_gemm<A_KC, B_KC, EPI><<<grid, 256>>>(a);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : fail(LYMPH_MLP_ERR_HIP, std::string("k_gemm: ") + hipGetErrorString(e));
+}
+
+// forward of B rows of H[0]; last layer into Y
+int forward(LymphMlp* m, int B) {
+  for (int l = 0; l < m->L; ++l) {
+    GemmArgs a{};
+    a.M = B; a.N = m->w[l + 1]; a.K = m->w[l];
+    a.A = m->H[l]; a.lda = m->w[l];
+    a.B = m->Wb + m->woff[l]; a.ldb = m->w[l];
+    a.ldc = m->w[l + 1];
+    a.bias = m->P + m->boff[l];
+    if (l + 1 < m->L) {
+      a.C32 = m->Z[l];
+      a.C16 = m->H[l + 1];
+      if (int rc = gemm<true, true, EPI_GELU_FWD>(a)) return rc;
+    } else {
+      a.C32 = m->Y;
+      if (int rc = gemm<true, true, EPI_F32>(a)) return rc;
+    }
+  }
+  return 0;
+}
+
+// loss of Y against T rows idx (device), gradients into G
+int backward(LymphMlp* m, int B, const float* T, const int* idx) {
+  const int NL = m->w[m->L];
+  const size_t ny = (size_t)B * NL;
+  const unsigned nb = blocks(ny, 256);
+  k_loss<<<nb, 256>>>(m->Y, NL, T, idx, B, 2.0f / (float)ny, m->dY, m->part);
+  k_sum<<<1, 64>>>(m->part, (int)nb, 1.0f / (float)ny, m->loss);
+  bf16_t* dZ = m->dY;
+  for (int l = m->L - 1; l >= 0; --l) {
+    const int N = m->w[l + 1], K = m->w[l];
+    GemmArgs a{};  // dW_l = dZ_l^T H_l
+    a.M = N; a.N = K; a.K = B;
+    a.A = dZ; a.lda = N;
+    a.B = m->H[l]; a.ldb = K;
+    a.C32 = m->G + m->woff[l]; a.ldc = K;
+    if (int rc = gemm<false, false, EPI_F32>(a)) return rc;
+    k_colsum<<<blocks(N, 256), 256>>>(dZ, B, N, m->G + m->boff[l]);
+    if (l > 0) {  // dZ_{l-1} = (dZ_l W_l) * gelu'(Z_{l-1})
+      bf16_t* nxt = (dZ == m->dZa) ? m->dZb : m->dZa;
+      GemmArgs d{};
+      d.M = B; d.N = K; d.K = N;
+      d.A = dZ; d.lda = N;
+      d.B = m->Wb + m->woff[l]; d.ldb = K;
+      d.C16 = nxt; d.ldc = K;
+      d.aux = m->Z[l - 1];
+      if (int rc = gemm<true, false, EPI_GELU_BWD>(d)) return rc;
+      dZ = nxt;
+    }
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : fail(LYMPH_MLP_ERR_HIP, std::string("backward: ") + hipGetErrorString(e));
+}
+
+int need(const LymphMlp* m) { return m ? 0 : fail(LYMPH_MLP_ERR_ARG, "null network"); }
+
+}  // namespace
+
+extern "C" {
+
+const char* lymph_mlp_last_error(void) { return g_err.c_str(); }
+
+int lymph_mlp_destroy(LymphMlp* m) {
+  if (!m) return 0;
+  (void)hipDeviceSynchronize();
+  for (void* p : m->owned) (void)hipFree(p);
+  delete m;
+  return 0;
+}
+
+int lymph_mlp_create(int n_layers, const int32_t* widths, int max_batch, LymphMlp** out) {
+  if (!out) return fail(LYMPH_MLP_ERR_ARG, "out is null");
+  *out = nullptr;
+  if (n_layers < 1 || !widths) return fail(LYMPH_MLP_ERR_ARG, "need at least one layer");
+  if (max_batch < 8 || max_batch % 8) return fail(LYMPH_MLP_ERR_ARG, "max_batch must be a positive multiple of 8");
+  for (int l = 0; l <= n_layers; ++l)
+    if (widths[l] < 8 || widths[l] % 8) return fail(LYMPH_MLP_ERR_ARG, "widths must be positive multiples of 8");
+  int dev = 0;
+  if (hipGetDeviceCount(&dev) != hipSuccess || dev == 0)
+    return fail(LYMPH_MLP_ERR_HIP, "no HIP device: liblymph_mlp needs an MI355X (gfx950)");
+  auto* m = new LymphMlp;
+  m->L = n_layers;
+  m->max_batch = max_batch;
+  m->w.assign(widths, widths + n_layers + 1);
+  size_t off = 0;
+  int maxh = 8;
+  for (int l = 0; l < n_layers; ++l) {
+    m->woff.push_back(off);
+    off += (size_t)m->w[l + 1] * m->w[l];
+    m->boff.push_back(off);
+    off += (size_t)m->w[l + 1];
+    if (l + 1 < n_layers) maxh = std::max(maxh, m->w[l + 1]);
+  }
+  m->n_params = off;
+  int rc = 0;
+  auto al = [&](auto** p, size_t n) {
+    if (!rc) rc = dalloc(m, p, n);
+  };
+  al(&m->P, off);
+  al(&m->G, off);
+  al(&m->Mo, off);
+  al(&m->Vo, off);
+  al(&m->Wb, off);
+  m->H.assign(n_layers, nullptr);
+  m->Z.assign(n_layers, nullptr);
+  for (int l = 0; l < n_layers; ++l) al(&m->H[l], (size_t)max_batch * m->w[l]);
+  for (int l = 0; l + 1 < n_layers; ++l) al(&m->Z[l], (size_t)max_batch * m->w[l + 1]);
+  const int NL = m->w[n_layers];
+  al(&m->Y, (size_t)max_batch * NL);
+  al(&m->dY, (size_t)max_batch * NL);
+  al(&m->dZa, (size_t)max_batch * maxh);
+  al(&m->dZb, (size_t)max_batch * maxh);
+  al(&m->part, blocks((size_t)max_batch * NL, 256));
+  al(&m->loss, 1);
+  al(&m->idx, (size_t)max_batch);
+  if (rc) {
+    lymph_mlp_destroy(m);
+    return rc;
+  }
+  if (hipMemset(m->P, 0, off * 4) != hipSuccess || hipMemset(m->Mo, 0, off * 4) != hipSuccess ||
+      hipMemset(m->Vo, 0, off * 4) != hipSuccess || hipMemset(m->Wb, 0, off * 2) != hipSuccess) {
+    lymph_mlp_destroy(m);
+    return fail(LYMPH_MLP_ERR_HIP, "hipMemset failed");
+  }
+  *out = m;
+  return 0;
+}
+
+int lymph_mlp_n_params(const LymphMlp* m, int64_t* n) {
+  if (int rc = need(m)) return rc;
+  if (n) *n = (int64_t)m->n_params;
+  return 0;
+}
+
+int lymph_mlp_set_params(LymphMlp* m, const float* p) {
+  if (int rc = need(m)) return rc;
+  if (!p) return fail(LYMPH_MLP_ERR_ARG, "null params");
+  HIP_TRY(hipMemcpy(m->P, p, m->n_params * 4, hipMemcpyHostToDevice));
+  k_to_bf16<<<blocks(m->n_params, 256), 256>>>(m->P, m->Wb, m->n_params);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemset(m->Mo, 0, m->n_params * 4));
+  HIP_TRY(hipMemset(m->Vo, 0, m->n_params * 4));
+  HIP_TRY(hipDeviceSynchronize());
+  m->adam_t = 0;
+  return 0;
+}
+
+int lymph_mlp_get_params(const LymphMlp* m, float* p) {
+  if (int rc = need(m)) return rc;
+  if (!p) return fail(LYMPH_MLP_ERR_ARG, "null params");
+  HIP_TRY(hipMemcpy(p, m->P, m->n_params * 4, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int lymph_mlp_set_data(LymphMlp* m, int n, const float* x, const float* y) {
+  if (int rc = need(m)) return rc;
+  if (n < 1 || !x || !y) return fail(LYMPH_MLP_ERR_ARG, "empty data set");
+  const size_t nx = (size_t)n * m->w[0], ny = (size_t)n * m->w[m->L];
+  (void)hipDeviceSynchronize();
+  for (float** q : {&m->X, &m->T})
+    if (*q) {
+      for (auto& o : m->owned)
+        if (o == (void*)*q) o = nullptr;
+      (void)hipFree(*q);
+      *q = nullptr;
+    }
+  if (int rc = dalloc(m, &m->X, nx)) return rc;
+  if (int rc = dalloc(m, &m->T, ny)) return rc;
+  HIP_TRY(hipMemcpy(m->X, x, nx * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(m->T, y, ny * 4, hipMemcpyHostToDevice));
+  m->n_data = n;
+  return 0;
+}
+
+int lymph_mlp_train(LymphMlp* m, int n_steps, int batch, const int32_t* order, float lr,
+                    float beta1, float beta2, float eps, float* losses) {
+  if (int rc = need(m)) return rc;
+  if (!m->X) return fail(LYMPH_MLP_ERR_STATE, "no training data: call lymph_mlp_set_data");
+  if (batch < 8 || batch > m->max_batch || batch % 8)
+    return fail(LYMPH_MLP_ERR_ARG, "batch must be a multiple of 8 and at most max_batch");
+  if (n_steps < 0 || !order) return fail(LYMPH_MLP_ERR_ARG, "bad steps/order");
+  for (size_t i = 0; i < (size_t)n_steps * batch; ++i)
+    if (order[i] < 0 || order[i] >= m->n_data) return fail(LYMPH_MLP_ERR_ARG, "sample index out of range");
+  int* dorder = nullptr;
+  HIP_TRY(hipMalloc(&dorder, sizeof(int) * std::max<size_t>(1, (size_t)n_steps * batch)));
+  HIP_TRY(hipMemcpy(dorder, order, sizeof(int) * (size_t)n_steps * batch, hipMemcpyHostToDevice));
+  int rc = 0;
+  for (int s = 0; s < n_steps && !rc; ++s) {
+    const int* idx = dorder + (size_t)s * batch;
+    k_gather<<<blocks((size_t)batch * m->w[0], 256), 256>>>(m->X, m->w[0], idx, batch, m->H[0]);
+    rc = forward(m, batch);
+    if (!rc) rc = backward(m, batch, m->T, idx);
+    if (!rc && losses) {
+      hipError_t e = hipMemcpy(&losses[s], m->loss, 4, hipMemcpyDeviceToHost);
+      if (e != hipSuccess) rc = fail(LYMPH_MLP_ERR_HIP, "loss copy failed");
+    }
+    if (!rc) {
+      m->adam_t++;
+      const float c1 = 1.0f / (1.0f - std::pow(beta1, (float)m->adam_t));
+      const float c2 = 1.0f / (1.0f - std::pow(beta2, (float)m->adam_t));
+      k_adam<<<blocks(m->n_params, 256), 256>>>(m->P, m->Mo, m->Vo, m->G, m->Wb, m->n_params, lr,
+                                                beta1, beta2, eps, c1, c2);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) rc = fail(LYMPH_MLP_ERR_HIP, std::string("k_adam: ") + hipGetErrorString(e));
+    }
+  }
+  (void)hipDeviceSynchronize();
+  (void)hipFree(dorder);
+  return rc;
+}
+
+int lymph_mlp_gradients(LymphMlp* m, int batch, const float* x, const float* y, float* grads,
+                        float* loss) {
+  if (int rc = need(m)) return rc;
+  if (batch < 8 || batch > m->max_batch || batch % 8)
+    return fail(LYMPH_MLP_ERR_ARG, "batch must be a multiple of 8 and at most max_batch");
+  if (!x || !y) return fail(LYMPH_MLP_ERR_ARG, "null batch");
+  const size_t nx = (size_t)batch * m->w[0], ny = (size_t)batch * m->w[m->L];
+  float *dx = nullptr, *dy = nullptr;
+  HIP_TRY(hipMalloc(&dx, nx * 4));
+  if (hipMalloc(&dy, ny * 4) != hipSuccess) {
+    (void)hipFree(dx);
+    return fail(LYMPH_MLP_ERR_HIP, "hipMalloc failed");
+  }
+  std::vector<int> id(batch);
+  for (int i = 0; i < batch; ++i) id[i] = i;
+  int rc = 0;
+  if (hipMemcpy(dx, x, nx * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dy, y, ny * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(m->idx, id.data(), 4 * batch, hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(LYMPH_MLP_ERR_HIP, "upload failed");
+  if (!rc) {
+    k_gather<<<blocks(nx, 256), 256>>>(dx, m->w[0], m->idx, batch, m->H[0]);
+    rc = forward(m, batch);
+  }
+  if (!rc) rc = backward(m, batch, dy, m->idx);
+  if (!rc && grads && hipMemcpy(grads, m->G, m->n_params * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(LYMPH_MLP_ERR_HIP, "gradient copy failed");
+  if (!rc && loss && hipMemcpy(loss, m->loss, 4, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(LYMPH_MLP_ERR_HIP, "loss copy failed");
+  (void)hipDeviceSynchronize();
+  (void)hipFree(dx);
+  (void)hipFree(dy);
+  return rc;
+}
+
+int lymph_mlp_predict(LymphMlp* m, int n, const float* x, float* y) {
+  if (int rc = need(m)) return rc;
+  if (n < 0 || (n > 0 && (!x || !y))) return fail(LYMPH_MLP_ERR_ARG, "bad inputs");
+  const int w0 = m->w[0], NL = m->w[m->L];
+  float* dx = nullptr;
+  HIP_TRY(hipMalloc(&dx, (size_t)std::max(1, n) * w0 * 4));
+  int rc = 0;
+  if (n > 0 && hipMemcpy(dx, x, (size_t)n * w0 * 4, hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(LYMPH_MLP_ERR_HIP, "upload failed");
+  std::vector<int> id(m->max_batch);
+  for (int i = 0; i < m->max_batch; ++i) id[i] = i;
+  if (!rc && hipMemcpy(m->idx, id.data(), 4 * m->max_batch, hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(LYMPH_MLP_ERR_HIP, "upload failed");
+  for (int b0 = 0; b0 < n && !rc; b0 += m->max_batch) {
+    const int B = std::min(m->max_batch, n - b0);
+    k_gather<<<blocks((size_t)B * w0, 256), 256>>>(dx + (size_t)b0 * w0, w0, m->idx, B, m->H[0]);
+    rc = forward(m, B);
+    if (!rc && hipMemcpy(y + (size_t)b0 * NL, m->Y, (size_t)B * NL * 4, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = fail(LYMPH_MLP_ERR_HIP, "download failed");
+  }
+  (void)hipDeviceSynchronize();
+  (void)hipFree(dx);
+  return rc;
+}
+
+int lymph_mlp_time_predict(LymphMlp* m, int n, int reps, double* ms) {
+  if (int rc = need(m)) return rc;
+  if (!m->X || n < 1 || n > m->n_data || reps < 1) return fail(LYMPH_MLP_ERR_ARG, "bad n/reps or no data");
+  std::vector<int> id(m->max_batch);
+  for (int i = 0; i < m->max_batch; ++i) id[i] = i;
+  HIP_TRY(hipMemcpy(m->idx, id.data(), 4 * m->max_batch, hipMemcpyHostToDevice));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  int rc = 0;
+  HIP_TRY(hipEventRecord(e0, 0));
+  for (int r = 0; r < reps && !rc; ++r)
+    for (int b0 = 0; b0 < n && !rc; b0 += m->max_batch) {
+      const int B = std::min(m->max_batch, n - b0);
+      k_gather<<<blocks((size_t)B * m->w[0], 256), 256>>>(m->X + (size_t)b0 * m->w[0], m->w[0],
+                                                          m->idx, B, m->H[0]);
+      rc = forward(m, B);
+    }
+  HIP_TRY(hipEventRecord(e1, 0));
+  HIP_TRY(hipEventSynchronize(e1));
+  float f = 0.f;
+  HIP_TRY(hipEventElapsedTime(&f, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms) *ms = f / reps;
+  return rc;
+}
+
+}  // extern "C"
