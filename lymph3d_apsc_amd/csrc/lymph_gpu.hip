@@ -88,6 +88,7 @@ struct LymphCtx {
   int n_recv = 0, max_rec = 0, n_rec = 0;
   int* rpoly = nullptr;
   double* rxyz = nullptr;
+  double* rnow = nullptr;  // 3 n_recv values of one lymph_sample_receivers call
   double* trace = nullptr;
 };
 
@@ -305,6 +306,7 @@ int lymph_ctx_destroy(LymphCtx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->rpoly) (void)hipFree(c->rpoly);
   if (c->rxyz) (void)hipFree(c->rxyz);
+  if (c->rnow) (void)hipFree(c->rnow);
   if (c->trace) (void)hipFree(c->trace);
   if (c->K32) (void)hipFree(c->K32);
   for (void* q : {(void*)c->gsel, (void*)c->gf0, (void*)c->gt0, (void*)c->gtab})
@@ -899,15 +901,21 @@ int lymph_set_receivers(LymphCtx* c, int n, const double* xyz, const int32_t* te
     if (tet[i] < 0 || tet[i] >= c->n_tet) return fail(LYMPH_ERR_ARG, "receiver tet out of range");
     rp[i] = c->h_tet_poly[tet[i]];
   }
+  HIP_TRY(hipStreamSynchronize(c->stream));
   if (c->rpoly) (void)hipFree(c->rpoly);
   if (c->rxyz) (void)hipFree(c->rxyz);
+  if (c->rnow) (void)hipFree(c->rnow);
   if (c->trace) (void)hipFree(c->trace);
-  c->rpoly = nullptr; c->rxyz = nullptr; c->trace = nullptr;
-  c->n_recv = n; c->max_rec = max_records; c->n_rec = 0;
+  c->rpoly = nullptr; c->rxyz = nullptr; c->rnow = nullptr; c->trace = nullptr;
+  c->n_recv = 0; c->max_rec = 0; c->n_rec = 0;
   if (n == 0) return 0;
+  // receivers count only once every buffer they use exists
   if (int rc = upload(&c->rpoly, rp.data(), rp.size())) return rc;
   if (int rc = upload(&c->rxyz, xyz, 3 * (size_t)n)) return rc;
+  if (int rc = dalloc(&c->rnow, 3 * (size_t)n)) return rc;
   if (int rc = dalloc(&c->trace, 3 * (size_t)n * (size_t)(max_records > 0 ? max_records : 1))) return rc;
+  c->n_recv = n;
+  c->max_rec = max_records;
   return 0;
 }
 
@@ -926,8 +934,8 @@ int lymph_sample_receivers(LymphCtx* c, int slot, double* out) {
   if (int rc = need_ctx(c)) return rc;
   if (slot < 0 || slot >= LYMPH_NUM_SLOTS || !out) return fail(LYMPH_ERR_ARG, "bad slot/pointer");
   if (c->n_recv == 0) return fail(LYMPH_ERR_STATE, "no receivers set");
-  if (int rc = sample_into(c, c->slot[slot], c->stage[3])) return rc;
-  HIP_TRY(hipMemcpyAsync(out, c->stage[3], 3 * (size_t)c->n_recv * sizeof(double),
+  if (int rc = sample_into(c, c->slot[slot], c->rnow)) return rc;
+  HIP_TRY(hipMemcpyAsync(out, c->rnow, 3 * (size_t)c->n_recv * sizeof(double),
                          hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return 0;
