@@ -28,6 +28,20 @@ struct Dims {
   static constexpr size_t BS = (size_t)N3 * N3C;
 };
 
+// Operator blocks are read exactly once per step and never again before the
+// next step has streamed ~all of HBM past the caches: non-temporal loads keep
+// them from displacing the coefficient vectors (which neighbours re-read).
+typedef double lymph_dv2 __attribute__((ext_vector_type(2)));
+typedef float lymph_fv4 __attribute__((ext_vector_type(4)));
+__device__ inline double2 load_stream(const double* p) {
+  const lymph_dv2 t = __builtin_nontemporal_load(reinterpret_cast<const lymph_dv2*>(p));
+  return make_double2(t.x, t.y);
+}
+__device__ inline float4 load_stream(const float* p) {
+  const lymph_fv4 t = __builtin_nontemporal_load(reinterpret_cast<const lymph_fv4*>(p));
+  return make_float4(t.x, t.y, t.z, t.w);
+}
+
 LYMPH_HD size_t kpos(int N3, int r, int c) {
   return ((size_t)(c >> 1) * N3 + r) * 2 + (c & 1);
 }
@@ -548,7 +562,7 @@ __global__ __launch_bounds__(64) void k_leapfrog(const double* __restrict__ K,
         for (int k = 0; k < RPL; ++k) {
           const int r = lane + 64 * k;
           if ((NPAIR % UNR == 0 || c2i < NPAIR) && r < N3)
-            kv[uu][k] = *reinterpret_cast<const double2*>(Kb + ((size_t)c2i * N3 + r) * 2);
+            kv[uu][k] = load_stream(Kb + ((size_t)c2i * N3 + r) * 2);
         }
       }
 #pragma unroll
@@ -696,7 +710,7 @@ __global__ __launch_bounds__(64) void k_leapfrog32(const float* __restrict__ K,
         for (int k = 0; k < RPL; ++k) {
           const int r = lane + 64 * k;
           if ((NQUAD % UNR == 0 || q < NQUAD) && r < N3)
-            kv[uu][k] = *reinterpret_cast<const float4*>(Kb + ((size_t)q * N3 + r) * 4);
+            kv[uu][k] = load_stream(Kb + ((size_t)q * N3 + r) * 4);
         }
       }
 #pragma unroll
