@@ -282,12 +282,12 @@ int forward(LymphMlp* m, int B) {
 }
 
 // loss of Y against T rows idx (device), gradients into G
-int backward(LymphMlp* m, int B, const float* T, const int* idx) {
+int backward(LymphMlp* m, int B, const float* T, const int* idx, float* loss_out) {
   const int NL = m->w[m->L];
   const size_t ny = (size_t)B * NL;
   const unsigned nb = blocks(ny, 256);
   k_loss<<<nb, 256>>>(m->Y, NL, T, idx, B, 2.0f / (float)ny, m->dY, m->part);
-  k_sum<<<1, 64>>>(m->part, (int)nb, 1.0f / (float)ny, m->loss);
+  k_sum<<<1, 64>>>(m->part, (int)nb, 1.0f / (float)ny, loss_out);
   bf16_t* dZ = m->dY;
   for (int l = m->L - 1; l >= 0; --l) {
     const int N = m->w[l + 1], K = m->w[l];
@@ -444,18 +444,19 @@ int lymph_mlp_train(LymphMlp* m, int n_steps, int batch, const int32_t* order, f
   for (size_t i = 0; i < (size_t)n_steps * batch; ++i)
     if (order[i] < 0 || order[i] >= m->n_data) return fail(LYMPH_MLP_ERR_ARG, "sample index out of range");
   int* dorder = nullptr;
+  float* dloss = nullptr;
   HIP_TRY(hipMalloc(&dorder, sizeof(int) * std::max<size_t>(1, (size_t)n_steps * batch)));
+  if (hipMalloc(&dloss, sizeof(float) * std::max(1, n_steps)) != hipSuccess) {
+    (void)hipFree(dorder);
+    return fail(LYMPH_MLP_ERR_HIP, "hipMalloc failed");
+  }
   HIP_TRY(hipMemcpy(dorder, order, sizeof(int) * (size_t)n_steps * batch, hipMemcpyHostToDevice));
   int rc = 0;
   for (int s = 0; s < n_steps && !rc; ++s) {
     const int* idx = dorder + (size_t)s * batch;
     k_gather<<<blocks((size_t)batch * m->w[0], 256), 256>>>(m->X, m->w[0], idx, batch, m->H[0]);
     rc = forward(m, batch);
-    if (!rc) rc = backward(m, batch, m->T, idx);
-    if (!rc && losses) {
-      hipError_t e = hipMemcpy(&losses[s], m->loss, 4, hipMemcpyDeviceToHost);
-      if (e != hipSuccess) rc = fail(LYMPH_MLP_ERR_HIP, "loss copy failed");
-    }
+    if (!rc) rc = backward(m, batch, m->T, idx, dloss + s);
     if (!rc) {
       m->adam_t++;
       const float c1 = 1.0f / (1.0f - std::pow(beta1, (float)m->adam_t));
@@ -466,8 +467,12 @@ int lymph_mlp_train(LymphMlp* m, int n_steps, int batch, const int32_t* order, f
       if (e != hipSuccess) rc = fail(LYMPH_MLP_ERR_HIP, std::string("k_adam: ") + hipGetErrorString(e));
     }
   }
+  if (!rc && losses && n_steps > 0 &&
+      hipMemcpy(losses, dloss, sizeof(float) * n_steps, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(LYMPH_MLP_ERR_HIP, "loss copy failed");
   (void)hipDeviceSynchronize();
   (void)hipFree(dorder);
+  (void)hipFree(dloss);
   return rc;
 }
 
@@ -495,7 +500,7 @@ int lymph_mlp_gradients(LymphMlp* m, int batch, const float* x, const float* y, 
     k_gather<<<blocks(nx, 256), 256>>>(dx, m->w[0], m->idx, batch, m->H[0]);
     rc = forward(m, batch);
   }
-  if (!rc) rc = backward(m, batch, dy, m->idx);
+  if (!rc) rc = backward(m, batch, dy, m->idx, m->loss);
   if (!rc && grads && hipMemcpy(grads, m->G, m->n_params * 4, hipMemcpyDeviceToHost) != hipSuccess)
     rc = fail(LYMPH_MLP_ERR_HIP, "gradient copy failed");
   if (!rc && loss && hipMemcpy(loss, m->loss, 4, hipMemcpyDeviceToHost) != hipSuccess)

@@ -14,8 +14,8 @@ Pipeline: the training and held-out sets are solved by the batched ensemble
 path (ensemble.Ensemble: 32-bit apply, HIP-graph captured); the held-out
 truth by the same path with the FP64 apply.  An ensemble of networks (widths
 8-512-512-512-out, GELU) is trained on the GPU by liblymph_mlp.so (BF16 MFMA,
-FP32 master weights, Adam) on standardised inputs and per-output
-standardised traces.  At evaluation the spread of the networks' predictions
+FP32 master weights, Adam) on standardised inputs and centred traces
+scaled by one global factor.  At evaluation the spread of the networks' predictions
 is the error estimate: a prediction whose relative spread exceeds the
 threshold is replaced by a solver run.
 
@@ -99,9 +99,10 @@ class Surrogate:
     def fit(self, thetas, traces, epochs=200, batch=256, lr=1e-3, log=None):
         x = ((thetas - IN_MEAN) / IN_STD).astype(np.float32)
         y = traces.reshape(len(traces), -1).astype(np.float64)
+        # per-output mean, one global scale: the loss then weighs every
+        # output as the relative L2 error of a trace does
         self.y_mean = y.mean(axis=0)
-        s = y.std(axis=0)
-        self.y_std = np.maximum(s, 1e-3 * s.max() + 1e-300)
+        self.y_std = max(float(np.sqrt(np.mean((y - self.y_mean) ** 2))), 1e-300)
         yn = ((y - self.y_mean) / self.y_std).astype(np.float32)
         n = len(x)
         steps_per_epoch = n // batch
