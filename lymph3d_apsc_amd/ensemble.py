@@ -63,18 +63,42 @@ def _cube(n, agglomerate):
     return copy.copy(_CUBES[(n, agglomerate)])
 
 
+def lognormal_field(rng, points, sigma=0.5, modes=4):
+    """Smooth lognormal random field at `points` (n, 3): exp(sigma s(x)) with
+    s = sqrt(2/modes) sum_k a_k cos(pi k.x + phase_k), a_k ~ N(0, 1), wave
+    vectors k in {1, 2}^3 -- pointwise median 1 and log-standard deviation
+    sigma on average."""
+    kv = rng.integers(1, 3, (modes, 3))
+    ph = rng.uniform(0.0, 2.0 * np.pi, modes)
+    a = rng.standard_normal(modes)
+    s = np.zeros(points.shape[0])
+    for k in range(modes):
+        s += a[k] * np.cos(np.pi * (points @ kv[k]) + ph[k])
+    return np.exp(sigma * np.sqrt(2.0 / modes) * s)
+
+
 def make_instance(seed, n_range=(10, 20), p_choices=(2, 3), n_receivers=64, sigma=0.5,
                   agglomerate=True, f0=5.0):
     """One seeded instance: n^3 cubes (n uniform in n_range), degree from
-    p_choices, lognormal(0, sigma) lambda and mu per tet, rho ~ U[0.8, 1.2],
-    Dirichlet on all faces, a z-directed Ricker point force at a seeded
-    interior point and seeded receivers."""
+    p_choices, smooth lognormal lambda and mu fields (lognormal_field at the
+    tet barycentres), rho ~ U[0.8, 1.2] per tet, Dirichlet on all faces, a
+    z-directed Ricker point force at a seeded interior point and seeded
+    receivers.
+
+    The fields are smooth on purpose: the face terms take the material of the
+    element that owns the face on both sides (src/assemble_element.f90:446),
+    so with independent per-tet lambda/mu the operator is far from symmetric
+    and the leapfrog iteration grows without bound (a 6^3 mesh at p=2 grows
+    tenfold per unit time whatever dt); smooth fields keep the jumps small and
+    the iteration stable."""
     rng = np.random.default_rng(seed)
     n = int(rng.integers(n_range[0], n_range[1] + 1))
     p = int(rng.choice(p_choices))
     m = _cube(n, agglomerate)
-    m.lam = np.exp(rng.normal(0.0, sigma, m.n_tet))
-    m.mu = np.exp(rng.normal(0.0, sigma, m.n_tet))
+    v = m.xyz[m.tet_vert]
+    bc = (((v[:, 0] + v[:, 1]) + v[:, 2]) + v[:, 3]) / 4
+    m.lam = lognormal_field(rng, bc, sigma)
+    m.mu = lognormal_field(rng, bc, sigma)
     m.rho = rng.uniform(0.8, 1.2, m.n_tet)
     src = rng.uniform(0.25, 0.75, (1, 3))
     rec = rng.uniform(0.05, 0.95, (n_receivers, 3))

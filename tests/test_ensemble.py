@@ -96,3 +96,32 @@ def test_union_point_location_offsets():
     t1 = meshes[1].locate(inst[1]["receivers"]) + meshes[0].n_tet
     assert t0.max() < meshes[0].n_tet <= t1.min()
     assert isinstance(u, M.Mesh)
+
+
+def _asymmetry(m, p=2):
+    o = _oracle_for(m, p)
+    o.build_operator()
+    K = o.K()
+    rp, col = o.pattern()
+    blk = {}
+    for P in range(o.n_poly):
+        for b in range(rp[P], rp[P + 1]):
+            blk[(P, col[b])] = K[b]
+    return max(np.abs(A - blk[(Q, P)].T).max() for (P, Q), A in blk.items()) / np.abs(K).max()
+
+
+def test_face_material_convention_and_field_smoothness():
+    """The face terms take the owning element's material on both sides
+    (src/assemble_element.f90:446), so the operator is symmetric (SIPG) only
+    for uniform material; independent per-tet lognormal fields make it far
+    from symmetric, smooth fields (the ensembles' instances) only slightly."""
+    m = M.cube_mesh(3, agglomerate=True)
+    assert _asymmetry(m) < 1e-12
+    rng = np.random.default_rng(0)
+    rough = M.cube_mesh(8, agglomerate=True)
+    rough.lam = np.exp(rng.normal(0, 0.5, rough.n_tet))
+    rough.mu = np.exp(rng.normal(0, 0.5, rough.n_tet))
+    smooth = E.make_instance(0, n_range=(8, 8), p_choices=(2,))["mesh"]
+    a_rough, a_smooth = _asymmetry(rough), _asymmetry(smooth)
+    assert a_rough > 0.1
+    assert a_smooth < 0.4 * a_rough
