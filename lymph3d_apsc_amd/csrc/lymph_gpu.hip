@@ -177,6 +177,9 @@ int check_launch(const char* what) {
 }
 
 unsigned grid1(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+// grid of a grid-stride kernel: at most 2^20 workgroups, so blocks x threads
+// stays far below the 2^32 work-items one dispatch can address
+unsigned grid_stride(size_t n, unsigned b) { return (unsigned)std::min<size_t>((n + b - 1) / b, 1u << 20); }
 
 double g_of(const LymphCtx* c, int problem, double t) {
   if (problem == LYMPH_PROB_RICKER) return ricker(c->ricker_f0, c->ricker_t0, t);
@@ -565,7 +568,7 @@ int lymph_build_operator(LymphCtx* c) {
                                                               c->n_poly, c->status);
     if (int r = check_launch("k_cholesky")) return r;
     if (c->K32) {  // keep an existing 32-bit copy in step with the new blocks
-      k_to_fp32<P><<<grid1((size_t)c->n_blk * Dims32<P>::BS, 256), 256, 0, c->stream>>>(
+      k_to_fp32<P><<<grid_stride((size_t)c->n_blk * Dims32<P>::BS, 256), 256, 0, c->stream>>>(
           c->K, c->K32, (size_t)c->n_blk);
       if (int r = check_launch("k_to_fp32")) return r;
     }
@@ -812,7 +815,7 @@ int lymph_set_apply_precision(LymphCtx* c, int bits) {
       const size_t n = (size_t)c->n_blk * Dims32<P>::BS;
       if (int r = dalloc(&c->K32, n)) return r;
       c->bytes += n * sizeof(float);
-      k_to_fp32<P><<<grid1(n, 256), 256, 0, c->stream>>>(c->K, c->K32, (size_t)c->n_blk);
+      k_to_fp32<P><<<grid_stride(n, 256), 256, 0, c->stream>>>(c->K, c->K32, (size_t)c->n_blk);
       return check_launch("k_to_fp32");
     });
     if (rc) return rc;

@@ -654,13 +654,17 @@ template <int P>
 __global__ void k_to_fp32(const double* __restrict__ K, float* __restrict__ K32, size_t n_blk) {
   using D = Dims<P>;
   constexpr int N3 = D::N3;
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_blk * Dims32<P>::BS) return;
-  const size_t b = i / Dims32<P>::BS;
-  const size_t w = i % Dims32<P>::BS;  // within block, quad layout
-  const int c = (int)(w / (N3 * 4)) * 4 + (int)(w % 4);
-  const int r = (int)((w / 4) % N3);
-  K32[i] = c < N3 ? (float)K[b * D::BS + kpos(N3, r, c)] : 0.0f;
+  // grid-stride: an ensemble's operator can hold more than 2^32 entries, more
+  // work-items than one dispatch can address
+  const size_t n = n_blk * Dims32<P>::BS;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t b = i / Dims32<P>::BS;
+    const size_t w = i % Dims32<P>::BS;  // within block, quad layout
+    const int c = (int)(w / (N3 * 4)) * 4 + (int)(w % 4);
+    const int r = (int)((w / 4) % N3);
+    K32[i] = c < N3 ? (float)K[b * D::BS + kpos(N3, r, c)] : 0.0f;
+  }
 }
 
 template <int P, int MODE, int UNR>

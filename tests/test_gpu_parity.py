@@ -270,6 +270,31 @@ def test_fp32_apply(name, p):
         assert rel(a64, b) < 1e-13
 
 
+def test_fp32_operator_beyond_2_32_entries():
+    """An operator of more than 2^32 entries (an ensemble-sized context) is
+    converted to FP32 in full: every polyhedron's FP32 step stays within
+    1e-4 of its FP64 step, the last ones included."""
+    say("[fp32 > 2^32 entries]")
+    m = M.cube_mesh(56, agglomerate=True)
+    p = 3
+    rng = np.random.default_rng(8)
+    with L.Context(m, p) as ctx:
+        assert ctx.n_blk * ctx.n3 * ctx.n3 > 2 ** 32
+        ctx.set_keep_adg(False)
+        ctx.make_matrices_free()
+        ctx.make_rhs_free(L.PROB_NULL, _lib.SLOT_RHS_STAT)
+        ctx.make_rhs_free(L.PROB_NULL, _lib.SLOT_RHS_DYN)
+        n = m.n_poly * ctx.n3
+        u0, u1 = rng.standard_normal(n), rng.standard_normal(n)
+        dt, g = 1e-3, 0.0
+        a64 = ctx.time_step_matrix_free(g, dt, u0, u1) - (2 * u1 - u0)
+        ctx.set_apply_precision(32)
+        a32 = ctx.time_step_matrix_free(g, dt, u0, u1) - (2 * u1 - u0)
+    d = np.abs(a32 - a64).reshape(m.n_poly, -1).max(axis=1)
+    s = np.abs(a64).reshape(m.n_poly, -1).max(axis=1)
+    assert (d <= 1e-4 * s).all(), int(np.argmax(d / s))
+
+
 def test_memory_options_keep_results():
     """Dropping A_dg and releasing the FP64 blocks changes no result: the
     operator built without A_dg, the 32-bit apply after the release and a
