@@ -58,6 +58,19 @@ def stream_probe_gbps():
     return tuple(out)
 
 
+def pmc_traffic(n, p):
+    """HBM bytes per step-kernel launch measured by the FETCH_SIZE/WRITE_SIZE
+    counter runs summarised in profiles/ (calibrated against the copy and
+    read probes), for the workload they were taken on; else None."""
+    path = os.path.join(ROOT, "profiles", f"k_leapfrog_cube{n}_p{p}_pmc_traffic.md")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("- traffic_bytes_per_launch:"):
+            return float(line.split(":")[1]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def cpu_baseline(p, n_sample, target_s):
     """Time the oracle's leapfrog step on an n_sample^3-cube sample of the
     same structure, using the host's cores (OpenMP)."""
@@ -226,6 +239,7 @@ def main():
     B = ctx.step_bytes
     achieved = B / (ms_step * 1e-3) / 1e9
     copy, read = stream_probe_gbps()
+    traffic, traffic_src = pmc_traffic(a.n, p)
     res = {
         "metric": METRIC, "value": value, "unit": "DOF-updates/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True,
@@ -239,7 +253,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+            "traffic_over_algorithmic": (traffic / B) if traffic else None,
             "bytes_per_step": B, "kernel": "k_leapfrog<3,0>",
             "stream_copy_GBps": copy,
             "frac_of_stream_copy": (achieved / copy) if copy else None,
