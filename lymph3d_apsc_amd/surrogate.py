@@ -65,7 +65,7 @@ def cfl_dt(n, p):
     return 0.2 * (1.0 / n) / (np.sqrt(3 * lam_max) * p * p)
 
 
-def solve(thetas, n, p, steps, every, rec, apply_bits=32, chunk=512):
+def solve(thetas, n, p, steps, every, rec, apply_bits=32, chunk=512, log=None):
     """Receiver traces (len(thetas), steps//every, n_receivers, 3) of every
     parameter set, solved as batched ensembles of `chunk` instances; also
     returns (seconds in replays, seconds in set-up)."""
@@ -85,6 +85,9 @@ def solve(thetas, n, p, steps, every, rec, apply_bits=32, chunk=512):
         ens.close()
         t_setup += t1 - t0
         t_run += t2 - t1
+        if log:
+            log(f"  solved {min(c0 + chunk, len(thetas))}/{len(thetas)} ({apply_bits}-bit apply): "
+                f"set-up {t1 - t0:.1f} s, solve {t2 - t1:.1f} s")
     return out, t_run, t_setup
 
 
@@ -166,10 +169,10 @@ def run(n_train=4096, n_eval=512, n=16, p=2, steps=1000, every=5, n_receivers=64
     th_train = np.stack([sample_params(seed0 + i) for i in range(n_train)])
     th_eval = np.stack([sample_params(seed0 + n_train + i) for i in range(n_eval)])
     t0 = time.perf_counter()
-    y_train, run32, set32 = solve(th_train, n, p, steps, every, rec, 32, chunk)
+    y_train, run32, set32 = solve(th_train, n, p, steps, every, rec, 32, chunk, log)
     log(f"training set: {n_train} FP32 ensemble solves in {run32:.1f} s (+{set32:.1f} s set-up)")
-    y_eval32, run32e, _ = solve(th_eval, n, p, steps, every, rec, 32, chunk)
-    y_eval64, run64e, set64 = solve(th_eval, n, p, steps, every, rec, 64, chunk)
+    y_eval32, run32e, _ = solve(th_eval, n, p, steps, every, rec, 32, chunk, log)
+    y_eval64, run64e, set64 = solve(th_eval, n, p, steps, every, rec, 64, chunk, log)
     log(f"held-out set: FP32 {run32e:.1f} s, FP64 {run64e:.1f} s")
     t_data = time.perf_counter() - t0
     # plain FP64 path, one instance at a time, uncaptured
