@@ -56,7 +56,7 @@ __device__ inline float gelu_grad(float z) {
          z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
 
-constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 8;  // LDS row: 32 k + 8 pad (80 B)
+constexpr int BM = 64, BN = 64, BK = 64, LDK = BK + 8;  // LDS row: 64 k + 8 pad (144 B)
 enum { EPI_F32 = 0, EPI_GELU_FWD = 1, EPI_GELU_BWD = 2 };
 
 struct GemmArgs {
@@ -70,26 +70,58 @@ struct GemmArgs {
   int ldc;
   const float* bias;  // [N] or null
   const float* aux;   // EPI_GELU_BWD: pre-activation, ldc layout
+  int kchunk;         // k range per blockIdx.z (multiple of BK)
+  float* part;        // split-k partial sums [z][M][N], or null for a direct epilogue
 };
 
-// one 64 x 32 operand tile into LDS, rows = m (or n), k contiguous
+// One 64 x 64 operand tile, two 16-byte pieces per thread: global -> registers
+// (issued before the previous tile's MFMAs) and registers -> LDS (k contiguous).
 template <bool KC>
-__device__ inline void load_tile(bf16_t* __restrict__ s, const bf16_t* __restrict__ g, int ld,
-                                 int r0, int R, int k0, int K, int tid) {
-  if (KC) {
-    const int r = tid >> 2, kc = (tid & 3) * 8;
-    uint4 v = make_uint4(0u, 0u, 0u, 0u);
-    if (r0 + r < R && k0 + kc < K)
-      v = *reinterpret_cast<const uint4*>(g + (size_t)(r0 + r) * ld + k0 + kc);
-    *reinterpret_cast<uint4*>(s + r * LDK + kc) = v;
-  } else {
-    const int kk = tid >> 3, rc = (tid & 7) * 8;
-    uint4 v = make_uint4(0u, 0u, 0u, 0u);
-    if (k0 + kk < K && r0 + rc < R)
-      v = *reinterpret_cast<const uint4*>(g + (size_t)(k0 + kk) * ld + r0 + rc);
-    const bf16_t* e = reinterpret_cast<const bf16_t*>(&v);
+__device__ inline void fetch(uint4 (&v)[2], const bf16_t* __restrict__ g, int ld, int r0, int R,
+                             int k0, int ke, int tid) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) s[(rc + j) * LDK + kk] = e[j];
+  for (int j = 0; j < 2; ++j) {
+    const int c = tid + 256 * j;
+    v[j] = make_uint4(0u, 0u, 0u, 0u);
+    if (KC) {
+      const int r = c >> 3, kc = (c & 7) * 8;
+      if (r0 + r < R && k0 + kc < ke)
+        v[j] = *reinterpret_cast<const uint4*>(g + (size_t)(r0 + r) * ld + k0 + kc);
+    } else {
+      const int kk = c >> 3, rc = (c & 7) * 8;
+      if (k0 + kk < ke && r0 + rc < R)
+        v[j] = *reinterpret_cast<const uint4*>(g + (size_t)(k0 + kk) * ld + r0 + rc);
+    }
+  }
+}
+
+template <bool KC>
+__device__ inline void put(bf16_t* __restrict__ s, const uint4 (&v)[2], int tid) {
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int c = tid + 256 * j;
+    if (KC) {
+      *reinterpret_cast<uint4*>(s + (c >> 3) * LDK + (c & 7) * 8) = v[j];
+    } else {
+      const int kk = c >> 3, rc = (c & 7) * 8;
+      const bf16_t* e = reinterpret_cast<const bf16_t*>(&v[j]);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) s[(rc + q) * LDK + kk] = e[q];
+    }
+  }
+}
+
+template <int EPI>
+__device__ inline void epilogue(const GemmArgs& g, int m, int n, float v) {
+  const size_t o = (size_t)m * g.ldc + n;
+  if (EPI == EPI_F32) {
+    g.C32[o] = g.bias ? v + g.bias[n] : v;
+  } else if (EPI == EPI_GELU_FWD) {
+    const float z = v + g.bias[n];
+    g.C32[o] = z;
+    g.C16[o] = f2bf(gelu(z));
+  } else {
+    g.C16[o] = f2bf(v * gelu_grad(g.aux[o]));
   }
 }
 
@@ -100,28 +132,40 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int kb = blockIdx.z * g.kchunk;
+  const int ke = min(g.K, kb + g.kchunk);
   f32x4 acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
   const int fr = lane & 15, fk = (lane >> 4) * 8;
-  for (int k0 = 0; k0 < g.K; k0 += BK) {
-    load_tile<A_KC>(sA, g.A, g.lda, m0, g.M, k0, g.K, tid);
-    load_tile<B_KC>(sB, g.B, g.ldb, n0, g.N, k0, g.K, tid);
+  uint4 ra[2], rb[2];
+  fetch<A_KC>(ra, g.A, g.lda, m0, g.M, kb, ke, tid);
+  fetch<B_KC>(rb, g.B, g.ldb, n0, g.N, kb, ke, tid);
+  for (int k0 = kb; k0 < ke; k0 += BK) {
+    put<A_KC>(sA, ra, tid);
+    put<B_KC>(sB, rb, tid);
     __syncthreads();
-    bf16x8 af[2], bfg[2];
+    if (k0 + BK < ke) {  // next tile in flight while this one is multiplied
+      fetch<A_KC>(ra, g.A, g.lda, m0, g.M, k0 + BK, ke, tid);
+      fetch<B_KC>(rb, g.B, g.ldb, n0, g.N, k0 + BK, ke, tid);
+    }
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-      af[i] = *reinterpret_cast<const bf16x8*>(sA + (wm + i * 16 + fr) * LDK + fk);
+    for (int ks = 0; ks < BK; ks += 32) {
+      bf16x8 af[2], bfg[2];
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
-      bfg[j] = *reinterpret_cast<const bf16x8*>(sB + (wn + j * 16 + fr) * LDK + fk);
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < 2; ++i)
+        af[i] = *reinterpret_cast<const bf16x8*>(sA + (wm + i * 16 + fr) * LDK + ks + fk);
 #pragma unroll
       for (int j = 0; j < 2; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfg[j], acc[i][j], 0, 0, 0);
+        bfg[j] = *reinterpret_cast<const bf16x8*>(sB + (wn + j * 16 + fr) * LDK + ks + fk);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfg[j], acc[i][j], 0, 0, 0);
+    }
     __syncthreads();
   }
   // D fragment: lane holds rows 4*(lane/16) + r, column lane % 16 of each 16x16 tile
@@ -134,18 +178,22 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
         const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
         const int n = n0 + wn + j * 16 + (lane & 15);
         if (m >= g.M || n >= g.N) continue;
-        const size_t o = (size_t)m * g.ldc + n;
-        float v = acc[i][j][r];
-        if (EPI == EPI_F32) {
-          g.C32[o] = g.bias ? v + g.bias[n] : v;
-        } else if (EPI == EPI_GELU_FWD) {
-          const float z = v + g.bias[n];
-          g.C32[o] = z;
-          g.C16[o] = f2bf(gelu(z));
-        } else {
-          g.C16[o] = f2bf(v * gelu_grad(g.aux[o]));
-        }
+        if (g.part)
+          g.part[((size_t)blockIdx.z * g.M + m) * g.N + n] = acc[i][j][r];
+        else
+          epilogue<EPI>(g, m, n, acc[i][j][r]);
       }
+}
+
+// split-k: sum the S partial tiles in order, then the epilogue
+template <int EPI>
+__global__ void k_splitk(GemmArgs g, int S) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)g.M * g.N) return;
+  const int m = (int)(i / g.N), n = (int)(i % g.N);
+  float v = 0.f;
+  for (int z = 0; z < S; ++z) v += g.part[((size_t)z * g.M + m) * g.N + n];
+  epilogue<EPI>(g, m, n, v);
 }
 
 __global__ void k_gather(const float* __restrict__ X, int w, const int* __restrict__ idx, int B,
@@ -236,6 +284,8 @@ struct LymphMlp {
   int n_data = 0;
   float *X = nullptr, *T = nullptr;
   int adam_t = 0;
+  float* ws = nullptr;  // split-k partial sums
+  size_t ws_floats = 0;
   std::vector<void*> owned;
 };
 
@@ -251,11 +301,27 @@ int dalloc(LymphMlp* m, T** p, size_t n) {
   return 0;
 }
 
+// Launch: 64x64 tiles; a GEMM with few tiles and a long k range (the input
+// gradient of the output layer: 4 x 8 tiles, k = 38,400) is split along k
+// over up to kMaxSplit workgroups per tile and reduced in order.
+constexpr int kMaxSplit = 32;
 template <bool A_KC, bool B_KC, int EPI>
-int gemm(const GemmArgs& a) {
+int gemm(LymphMlp* m, GemmArgs a) {
   if (a.M <= 0 || a.N <= 0 || a.K <= 0) return 0;
-  dim3 grid(blocks(a.N, BN), blocks(a.M, BM));
+  const unsigned tiles = blocks(a.N, BN) * blocks(a.M, BM);
+  int S = 1;
+  if (tiles < 256 && a.K > 1024) S = std::min<int>(kMaxSplit, (a.K + 1023) / 1024);
+  a.kchunk = (int)(blocks(blocks(a.K, S), BK) * BK);
+  S = (int)blocks(a.K, a.kchunk);
+  a.part = nullptr;
+  if (S > 1) {
+    if ((size_t)S * a.M * a.N > m->ws_floats)
+      return fail(LYMPH_MLP_ERR_STATE, "split-k workspace too small");
+    a.part = m->ws;
+  }
+  dim3 grid(blocks(a.N, BN), blocks(a.M, BM), S);
   k_gemm<A_KC, B_KC, EPI><<<grid, 256>>>(a);
+  if (S > 1) k_splitk<EPI><<<blocks((size_t)a.M * a.N, 256), 256>>>(a, S);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : fail(LYMPH_MLP_ERR_HIP, std::string("k_gemm: ") + hipGetErrorString(e));
 }
@@ -272,10 +338,10 @@ int forward(LymphMlp* m, int B) {
     if (l + 1 < m->L) {
       a.C32 = m->Z[l];
       a.C16 = m->H[l + 1];
-      if (int rc = gemm<true, true, EPI_GELU_FWD>(a)) return rc;
+      if (int rc = gemm<true, true, EPI_GELU_FWD>(m, a)) return rc;
     } else {
       a.C32 = m->Y;
-      if (int rc = gemm<true, true, EPI_F32>(a)) return rc;
+      if (int rc = gemm<true, true, EPI_F32>(m, a)) return rc;
     }
   }
   return 0;
@@ -296,7 +362,7 @@ int backward(LymphMlp* m, int B, const float* T, const int* idx, float* loss_out
     a.A = dZ; a.lda = N;
     a.B = m->H[l]; a.ldb = K;
     a.C32 = m->G + m->woff[l]; a.ldc = K;
-    if (int rc = gemm<false, false, EPI_F32>(a)) return rc;
+    if (int rc = gemm<false, false, EPI_F32>(m, a)) return rc;
     k_colsum<<<blocks(N, 256), 256>>>(dZ, B, N, m->G + m->boff[l]);
     if (l > 0) {  // dZ_{l-1} = (dZ_l W_l) * gelu'(Z_{l-1})
       bf16_t* nxt = (dZ == m->dZa) ? m->dZb : m->dZa;
@@ -306,7 +372,7 @@ int backward(LymphMlp* m, int B, const float* T, const int* idx, float* loss_out
       d.B = m->Wb + m->woff[l]; d.ldb = K;
       d.C16 = nxt; d.ldc = K;
       d.aux = m->Z[l - 1];
-      if (int rc = gemm<true, false, EPI_GELU_BWD>(d)) return rc;
+      if (int rc = gemm<true, false, EPI_GELU_BWD>(m, d)) return rc;
       dZ = nxt;
     }
   }
@@ -375,6 +441,9 @@ int lymph_mlp_create(int n_layers, const int32_t* widths, int max_batch, LymphMl
   al(&m->part, blocks((size_t)max_batch * NL, 256));
   al(&m->loss, 1);
   al(&m->idx, (size_t)max_batch);
+  // split-k is used only when M x N spans few tiles (< 256 of 64 x 64)
+  m->ws_floats = (size_t)kMaxSplit * 256 * 64 * 64;
+  al(&m->ws, m->ws_floats);
   if (rc) {
     lymph_mlp_destroy(m);
     return rc;

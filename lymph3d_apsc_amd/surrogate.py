@@ -159,8 +159,8 @@ def rel_err(a, b):
 
 
 def run(n_train=4096, n_eval=512, n=16, p=2, steps=1000, every=5, n_receivers=64, n_nets=4,
-        width=512, epochs=200, batch=256, threshold=1e-2, chunk=512, plain_sample=8, seed0=10_000,
-        log=None):
+        width=512, epochs=200, batch=256, lr=1e-3, threshold=1e-2, chunk=512, plain_sample=8,
+        seed0=10_000, log=None):
     """Whole pipeline; returns a dict of results (errors, throughputs)."""
     from . import _lib
     from .matrix_free import Context
@@ -195,8 +195,13 @@ def run(n_train=4096, n_eval=512, n=16, p=2, steps=1000, every=5, n_receivers=64
     # networks
     t0 = time.perf_counter()
     sur = Surrogate(y_train[0].size, n_nets=n_nets, width=width, max_batch=max(batch, 64))
-    hist = sur.fit(th_train, y_train, epochs=epochs, batch=batch, log=log)
+    hist = sur.fit(th_train, y_train, epochs=epochs, batch=batch, lr=lr, log=log)
     t_train = time.perf_counter() - t0
+    w = sur.widths
+    # GEMM flops of one training step: forward, weight gradient, input gradient
+    # (none for the first layer's input)
+    step_flops = 2.0 * batch * sum(k * n for k, n in zip(w[:-1], w[1:])) * 3 - 2.0 * batch * w[0] * w[1]
+    n_steps = n_nets * epochs * (n_train // batch)
     mean, spread = sur.predict(th_eval)
     t_pred = sur.time_predict(th_eval)
     err_sur = rel_err(mean, y_eval64)
@@ -214,7 +219,7 @@ def run(n_train=4096, n_eval=512, n=16, p=2, steps=1000, every=5, n_receivers=64
         "config": {"n_train": n_train, "n_eval": n_eval, "mesh": f"{n}^3 cubes x 6 tets, agglomerated",
                    "p": p, "steps": steps, "every": every, "receivers": n_receivers,
                    "outputs": int(y_train[0].size), "widths": sur.widths, "nets": n_nets,
-                   "epochs": epochs, "batch": batch, "threshold": threshold, "dt": dt},
+                   "epochs": epochs, "batch": batch, "lr": lr, "threshold": threshold, "dt": dt},
         "throughput_solves_per_s": {
             "plain_fp64": plain_rate,
             "ensemble_fp32": (n_train + n_eval) / (run32 + run32e),
@@ -231,6 +236,9 @@ def run(n_train=4096, n_eval=512, n=16, p=2, steps=1000, every=5, n_receivers=64
         "spread_vs_error_correlation": corr,
         "final_train_loss": [h[-1] for h in hist],
         "seconds": {"data": t_data, "train": t_train, "predict_all": t_pred},
+        "training": {"steps": n_steps, "ms_per_step": 1e3 * t_train / n_steps,
+                     "gemm_tflops": step_flops * n_steps / t_train / 1e12,
+                     "bf16_mfma_peak_tflops": 2500.0},
     }
 
 
@@ -244,7 +252,8 @@ if __name__ == "__main__":
     ap.add_argument("--n", type=int, default=16)
     ap.add_argument("--epochs", type=int, default=200)
     ap.add_argument("--nets", type=int, default=4)
+    ap.add_argument("--lr", type=float, default=1e-3)
     a = ap.parse_args()
-    res = run(n_train=a.n_train, n_eval=a.n_eval, n=a.n, epochs=a.epochs, n_nets=a.nets,
+    res = run(n_train=a.n_train, n_eval=a.n_eval, n=a.n, epochs=a.epochs, n_nets=a.nets, lr=a.lr,
               log=lambda *m: print(*m, file=sys.stderr, flush=True))
     print(json.dumps(res))

@@ -30,8 +30,10 @@ def rel(a, b):
     return float(np.linalg.norm(np.asarray(a, float) - b) / max(np.linalg.norm(b), 1e-300))
 
 
-# shapes that exercise full and partial 64x64 tiles, k tails and width 8
-SHAPES = [([8, 64, 64, 128], 64), ([8, 72, 136, 40], 24), ([8, 512, 512, 512, 1536], 128)]
+# shapes that exercise full and partial 64x64 tiles, k tails, width 8 and the
+# split-k path (output layers of 1536 and 38,400 outputs)
+SHAPES = [([8, 64, 64, 128], 64), ([8, 72, 136, 40], 24), ([8, 512, 512, 512, 1536], 128),
+          ([8, 64, 64, 38400], 32)]
 
 
 @pytest.mark.parametrize("widths,batch", SHAPES)
