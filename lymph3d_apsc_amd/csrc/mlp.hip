@@ -56,7 +56,7 @@ __device__ inline float gelu_grad(float z) {
          z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
 
-constexpr int BM = 64, BN = 64, BK = 64, LDK = BK + 8;  // LDS row: 64 k + 8 pad (144 B)
+constexpr int BK = 64, LDK = BK + 8;  // LDS row: 64 k + 8 pad (144 B)
 enum { EPI_F32 = 0, EPI_GELU_FWD = 1, EPI_GELU_BWD = 2 };
 
 struct GemmArgs {
@@ -74,13 +74,14 @@ struct GemmArgs {
   float* part;        // split-k partial sums [z][M][N], or null for a direct epilogue
 };
 
-// One 64 x 64 operand tile, two 16-byte pieces per thread: global -> registers
-// (issued before the previous tile's MFMAs) and registers -> LDS (k contiguous).
-template <bool KC>
-__device__ inline void fetch(uint4 (&v)[2], const bf16_t* __restrict__ g, int ld, int r0, int R,
-                             int k0, int ke, int tid) {
+// One TM x 64 operand tile in TM/32 16-byte pieces per thread: global ->
+// registers (issued before the previous tile's MFMAs), registers -> LDS with
+// k contiguous.
+template <bool KC, int TM>
+__device__ inline void fetch(uint4 (&v)[TM / 32], const bf16_t* __restrict__ g, int ld, int r0,
+                             int R, int k0, int ke, int tid) {
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
+  for (int j = 0; j < TM / 32; ++j) {
     const int c = tid + 256 * j;
     v[j] = make_uint4(0u, 0u, 0u, 0u);
     if (KC) {
@@ -88,22 +89,22 @@ __device__ inline void fetch(uint4 (&v)[2], const bf16_t* __restrict__ g, int ld
       if (r0 + r < R && k0 + kc < ke)
         v[j] = *reinterpret_cast<const uint4*>(g + (size_t)(r0 + r) * ld + k0 + kc);
     } else {
-      const int kk = c >> 3, rc = (c & 7) * 8;
+      const int kk = c / (TM / 8), rc = (c % (TM / 8)) * 8;
       if (k0 + kk < ke && r0 + rc < R)
         v[j] = *reinterpret_cast<const uint4*>(g + (size_t)(k0 + kk) * ld + r0 + rc);
     }
   }
 }
 
-template <bool KC>
-__device__ inline void put(bf16_t* __restrict__ s, const uint4 (&v)[2], int tid) {
+template <bool KC, int TM>
+__device__ inline void put(bf16_t* __restrict__ s, const uint4 (&v)[TM / 32], int tid) {
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
+  for (int j = 0; j < TM / 32; ++j) {
     const int c = tid + 256 * j;
     if (KC) {
       *reinterpret_cast<uint4*>(s + (c >> 3) * LDK + (c & 7) * 8) = v[j];
     } else {
-      const int kk = c >> 3, rc = (c & 7) * 8;
+      const int kk = c / (TM / 8), rc = (c % (TM / 8)) * 8;
       const bf16_t* e = reinterpret_cast<const bf16_t*>(&v[j]);
 #pragma unroll
       for (int q = 0; q < 8; ++q) s[(rc + q) * LDK + kk] = e[q];
@@ -125,54 +126,57 @@ __device__ inline void epilogue(const GemmArgs& g, int m, int n, float v) {
   }
 }
 
-template <bool A_KC, bool B_KC, int EPI>
+// TM x TM output tile per 256-thread workgroup; 2 x 2 waves, each owning a
+// (TM/2) x (TM/2) quarter as (TM/32) x (TM/32) MFMA tiles of 16 x 16.
+template <bool A_KC, bool B_KC, int EPI, int TM>
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
-  __shared__ __attribute__((aligned(16))) bf16_t sA[BM * LDK];
-  __shared__ __attribute__((aligned(16))) bf16_t sB[BN * LDK];
+  constexpr int T = TM / 32;  // MFMA tiles per wave per dimension
+  __shared__ __attribute__((aligned(16))) bf16_t sA[TM * LDK];
+  __shared__ __attribute__((aligned(16))) bf16_t sB[TM * LDK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TM;
+  const int wm = (wave >> 1) * (TM / 2), wn = (wave & 1) * (TM / 2);
   const int kb = blockIdx.z * g.kchunk;
   const int ke = min(g.K, kb + g.kchunk);
-  f32x4 acc[2][2];
+  f32x4 acc[T][T];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < T; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < T; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
   const int fr = lane & 15, fk = (lane >> 4) * 8;
-  uint4 ra[2], rb[2];
-  fetch<A_KC>(ra, g.A, g.lda, m0, g.M, kb, ke, tid);
-  fetch<B_KC>(rb, g.B, g.ldb, n0, g.N, kb, ke, tid);
+  uint4 ra[T], rb[T];
+  fetch<A_KC, TM>(ra, g.A, g.lda, m0, g.M, kb, ke, tid);
+  fetch<B_KC, TM>(rb, g.B, g.ldb, n0, g.N, kb, ke, tid);
   for (int k0 = kb; k0 < ke; k0 += BK) {
-    put<A_KC>(sA, ra, tid);
-    put<B_KC>(sB, rb, tid);
+    put<A_KC, TM>(sA, ra, tid);
+    put<B_KC, TM>(sB, rb, tid);
     __syncthreads();
     if (k0 + BK < ke) {  // next tile in flight while this one is multiplied
-      fetch<A_KC>(ra, g.A, g.lda, m0, g.M, k0 + BK, ke, tid);
-      fetch<B_KC>(rb, g.B, g.ldb, n0, g.N, k0 + BK, ke, tid);
+      fetch<A_KC, TM>(ra, g.A, g.lda, m0, g.M, k0 + BK, ke, tid);
+      fetch<B_KC, TM>(rb, g.B, g.ldb, n0, g.N, k0 + BK, ke, tid);
     }
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 32) {
-      bf16x8 af[2], bfg[2];
+      bf16x8 af[T], bfg[T];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < T; ++i)
         af[i] = *reinterpret_cast<const bf16x8*>(sA + (wm + i * 16 + fr) * LDK + ks + fk);
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < T; ++j)
         bfg[j] = *reinterpret_cast<const bf16x8*>(sB + (wn + j * 16 + fr) * LDK + ks + fk);
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < T; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < T; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfg[j], acc[i][j], 0, 0, 0);
     }
     __syncthreads();
   }
   // D fragment: lane holds rows 4*(lane/16) + r, column lane % 16 of each 16x16 tile
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < T; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < T; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
@@ -227,21 +231,35 @@ __global__ __launch_bounds__(256) void k_loss(const float* __restrict__ Y, int N
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// ordered single-lane sum (deterministic)
-__global__ void k_sum(const float* __restrict__ part, int n, float scale, float* __restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// deterministic sum of n partials by one 256-thread workgroup: thread t adds
+// partials t, t+256, ... in order, then a fixed tree over the threads
+__global__ __launch_bounds__(256) void k_sum(const float* __restrict__ part, int n, float scale,
+                                             float* __restrict__ out) {
+  __shared__ double red[256];
   double s = 0.0;
-  for (int i = 0; i < n; ++i) s += part[i];
-  out[0] = (float)(s * scale);
+  for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = (float)(red[0] * scale);
 }
 
-// out[n] = sum_b D[b][n]
-__global__ void k_colsum(const bf16_t* __restrict__ D, int B, int N, float* __restrict__ out) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+// out[n] = sum_b D[b][n]: 64 columns per workgroup, its 4 waves take every
+// 4th row, then wave sums combine in order (deterministic)
+__global__ __launch_bounds__(256) void k_colsum(const bf16_t* __restrict__ D, int B, int N,
+                                                float* __restrict__ out) {
+  __shared__ float red[4][64];
+  const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int n = blockIdx.x * 64 + c;
   float s = 0.f;
-  for (int b = 0; b < B; ++b) s += bf2f(D[(size_t)b * N + n]);
-  out[n] = s;
+  if (n < N)
+    for (int b = w; b < B; b += 4) s += bf2f(D[(size_t)b * N + n]);
+  red[w][c] = s;
+  __syncthreads();
+  if (w == 0 && n < N) out[n] = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
 }
 
 __global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
@@ -284,7 +302,7 @@ struct LymphMlp {
   int n_data = 0;
   float *X = nullptr, *T = nullptr;
   int adam_t = 0;
-  float* ws = nullptr;  // split-k partial sums
+  float* ws = nullptr;  // split-k partial sums, sized by the largest split GEMM seen
   size_t ws_floats = 0;
   std::vector<void*> owned;
 };
@@ -301,26 +319,40 @@ int dalloc(LymphMlp* m, T** p, size_t n) {
   return 0;
 }
 
-// Launch: 64x64 tiles; a GEMM with few tiles and a long k range (the input
-// gradient of the output layer: 4 x 8 tiles, k = 38,400) is split along k
-// over up to kMaxSplit workgroups per tile and reduced in order.
+// Launch: 128 x 128 tiles when they still give >= 256 workgroups, else
+// 64 x 64; a GEMM with few tiles and a long k range (the input gradient of the
+// output layer: k = 38,400) is split along k over up to kMaxSplit workgroups
+// per tile and reduced in order.
 constexpr int kMaxSplit = 32;
 template <bool A_KC, bool B_KC, int EPI>
 int gemm(LymphMlp* m, GemmArgs a) {
   if (a.M <= 0 || a.N <= 0 || a.K <= 0) return 0;
-  const unsigned tiles = blocks(a.N, BN) * blocks(a.M, BM);
+  const bool big = blocks(a.N, 128) * blocks(a.M, 128) >= 256 || a.K > 1024;
+  const int TM = big ? 128 : 64;
+  const unsigned tiles = blocks(a.N, TM) * blocks(a.M, TM);
   int S = 1;
   if (tiles < 256 && a.K > 1024) S = std::min<int>(kMaxSplit, (a.K + 1023) / 1024);
   a.kchunk = (int)(blocks(blocks(a.K, S), BK) * BK);
   S = (int)blocks(a.K, a.kchunk);
   a.part = nullptr;
   if (S > 1) {
-    if ((size_t)S * a.M * a.N > m->ws_floats)
-      return fail(LYMPH_MLP_ERR_STATE, "split-k workspace too small");
+    const size_t need = (size_t)S * a.M * a.N;
+    if (need > m->ws_floats) {  // grown once, on the first call of a larger shape
+      (void)hipDeviceSynchronize();
+      if (m->ws) (void)hipFree(m->ws);
+      m->ws = nullptr;
+      m->ws_floats = 0;
+      if (hipMalloc(&m->ws, need * sizeof(float)) != hipSuccess)
+        return fail(LYMPH_MLP_ERR_HIP, "hipMalloc of the split-k workspace failed");
+      m->ws_floats = need;
+    }
     a.part = m->ws;
   }
-  dim3 grid(blocks(a.N, BN), blocks(a.M, BM), S);
-  k_gemm<A_KC, B_KC, EPI><<<grid, 256>>>(a);
+  dim3 grid(blocks(a.N, TM), blocks(a.M, TM), S);
+  if (big)
+    k_gemm<A_KC, B_KC, EPI, 128><<<grid, 256>>>(a);
+  else
+    k_gemm<A_KC, B_KC, EPI, 64><<<grid, 256>>>(a);
   if (S > 1) k_splitk<EPI><<<blocks((size_t)a.M * a.N, 256), 256>>>(a, S);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : fail(LYMPH_MLP_ERR_HIP, std::string("k_gemm: ") + hipGetErrorString(e));
@@ -353,7 +385,7 @@ int backward(LymphMlp* m, int B, const float* T, const int* idx, float* loss_out
   const size_t ny = (size_t)B * NL;
   const unsigned nb = blocks(ny, 256);
   k_loss<<<nb, 256>>>(m->Y, NL, T, idx, B, 2.0f / (float)ny, m->dY, m->part);
-  k_sum<<<1, 64>>>(m->part, (int)nb, 1.0f / (float)ny, loss_out);
+  k_sum<<<1, 256>>>(m->part, (int)nb, 1.0f / (float)ny, loss_out);
   bf16_t* dZ = m->dY;
   for (int l = m->L - 1; l >= 0; --l) {
     const int N = m->w[l + 1], K = m->w[l];
@@ -363,7 +395,7 @@ int backward(LymphMlp* m, int B, const float* T, const int* idx, float* loss_out
     a.B = m->H[l]; a.ldb = K;
     a.C32 = m->G + m->woff[l]; a.ldc = K;
     if (int rc = gemm<false, false, EPI_F32>(m, a)) return rc;
-    k_colsum<<<blocks(N, 256), 256>>>(dZ, B, N, m->G + m->boff[l]);
+    k_colsum<<<blocks(N, 64), 256>>>(dZ, B, N, m->G + m->boff[l]);
     if (l > 0) {  // dZ_{l-1} = (dZ_l W_l) * gelu'(Z_{l-1})
       bf16_t* nxt = (dZ == m->dZa) ? m->dZb : m->dZa;
       GemmArgs d{};
@@ -392,6 +424,7 @@ int lymph_mlp_destroy(LymphMlp* m) {
   if (!m) return 0;
   (void)hipDeviceSynchronize();
   for (void* p : m->owned) (void)hipFree(p);
+  if (m->ws) (void)hipFree(m->ws);
   delete m;
   return 0;
 }
@@ -441,9 +474,6 @@ int lymph_mlp_create(int n_layers, const int32_t* widths, int max_batch, LymphMl
   al(&m->part, blocks((size_t)max_batch * NL, 256));
   al(&m->loss, 1);
   al(&m->idx, (size_t)max_batch);
-  // split-k is used only when M x N spans few tiles (< 256 of 64 x 64)
-  m->ws_floats = (size_t)kMaxSplit * 256 * 64 * 64;
-  al(&m->ws, m->ws_floats);
   if (rc) {
     lymph_mlp_destroy(m);
     return rc;

@@ -212,6 +212,16 @@ def run(n_train=4096, n_eval=512, n=16, p=2, steps=1000, every=5, n_receivers=64
     n_fb = int(fallback.sum())
     hybrid_time = t_pred + n_fb / plain_rate
     corr = float(np.corrcoef(spread, err_sur)[0, 1]) if n_eval > 2 else None
+    # what each spread threshold buys: share answered by the networks, their
+    # error, and the hybrid's throughput and error (fallbacks exact)
+    tradeoff = []
+    for tau in (threshold, 0.02, 0.05, 0.1, 0.2, 0.3, 0.5, 1.0):
+        acc = spread <= tau
+        t_h = t_pred + int((~acc).sum()) / plain_rate
+        tradeoff.append({"threshold": tau, "accepted": float(acc.mean()),
+                         "accepted_mean_err": float(err_sur[acc].mean()) if acc.any() else None,
+                         "hybrid_mean_err": float(np.where(acc, err_sur, 0.0).mean()),
+                         "hybrid_solves_per_s": n_eval / t_h})
     sur.close()
     q = lambda a: {"mean": float(np.mean(a)), "median": float(np.median(a)),  # noqa: E731
                    "p90": float(np.quantile(a, 0.9)), "max": float(np.max(a))}
@@ -233,6 +243,7 @@ def run(n_train=4096, n_eval=512, n=16, p=2, steps=1000, every=5, n_receivers=64
             "hybrid_with_fallback": q(hybrid_err),
         },
         "fallback_fraction": n_fb / n_eval,
+        "threshold_tradeoff": tradeoff,
         "spread_vs_error_correlation": corr,
         "final_train_loss": [h[-1] for h in hist],
         "seconds": {"data": t_data, "train": t_train, "predict_all": t_pred},
