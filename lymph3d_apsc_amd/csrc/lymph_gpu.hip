@@ -137,8 +137,8 @@ void launch_leapfrog(int variant, int grid, hipStream_t st, const double* K, con
                      double c2, const int* gsel = nullptr, const double* gtab = nullptr) {
   if constexpr (I < n_var<P>()) {
     if (variant == I) {
-      k_leapfrog<P, MODE, StepVar<P>::unr[I], StepVar<P>::xs[I]><<<grid, 64, 0, st>>>(
-          K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab);
+      k_leapfrog<P, MODE, StepVar<P>::unr[I], StepVar<P>::xs[I]><<<step_grid(grid), 64, 0, st>>>(
+          K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid);
       return;
     }
     launch_leapfrog<P, MODE, I + 1>(variant, grid, st, K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2,
@@ -160,9 +160,8 @@ void launch_leapfrog32(int variant, int grid, hipStream_t st, const float* K, co
                        double c2, const int* gsel = nullptr, const double* gtab = nullptr) {
   if constexpr (I < n_var32<P>()) {
     if (variant == I) {
-      k_leapfrog32<P, MODE, StepVar32<P>::unr[I]><<<grid, 64, 0, st>>>(K, rp, col, Rp, rs, rd, u,
-                                                                       wa, out, g, c1, c2, gsel,
-                                                                       gtab);
+      k_leapfrog32<P, MODE, StepVar32<P>::unr[I]><<<step_grid(grid), 64, 0, st>>>(
+          K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid);
       return;
     }
     launch_leapfrog32<P, MODE, I + 1>(variant, grid, st, K, rp, col, Rp, rs, rd, u, wa, out, g, c1,

@@ -507,6 +507,17 @@ __device__ __forceinline__ double lane_bcast(double v, int l) {
   return __hiloint2double(hi, lo);
 }
 
+// Polyhedron of this workgroup.  Workgroups are dealt to the 8 XCDs round
+// robin (workgroup b runs on XCD b % 8), each XCD with its own L2; the grid is
+// padded to a multiple of 8 and XCD x is given the x-th eighth of the
+// polyhedra, so neighbours -- whose coefficient blocks a row re-reads -- mostly
+// share an L2.  A bijection of [0, grid); ids >= n_poly are padding.
+__device__ inline int step_poly() {
+  const int per = (int)(gridDim.x >> 3);
+  return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+}
+inline unsigned step_grid(int n_poly) { return 8u * (unsigned)((n_poly + 7) / 8); }
+
 // UNR = column pairs whose loads are issued together (register footprint vs
 // loads in flight).  XS = how the neighbour's coefficients reach the lanes:
 // 0 = wave-uniform scalar loads per pair, 1 = one coalesced vector load per
@@ -523,14 +534,15 @@ __global__ __launch_bounds__(64) void k_leapfrog(const double* __restrict__ K,
                                                  const double* __restrict__ wa,
                                                  double* __restrict__ out, double g, double c1,
                                                  double c2, const int* __restrict__ gsel,
-                                                 const double* __restrict__ gtab) {
+                                                 const double* __restrict__ gtab, int n_poly) {
   using D = Dims<P>;
   constexpr int NP = D::NP, N3 = D::N3, NPAIR = D::NPAIR;
   constexpr int RPL = (N3 + 63) / 64;
   __shared__ double sw[N3];
   __shared__ double sR[D::NPK];
   const int lane = threadIdx.x;
-  const int Pp = blockIdx.x;
+  const int Pp = step_poly();
+  if (Pp >= n_poly) return;  // padding workgroups of the last XCD share
   if (MODE != 2)
     for (int k = lane; k < D::NPK; k += 64) sR[k] = Rp[(size_t)Pp * D::NPK + k];
   double acc[RPL];
@@ -678,14 +690,15 @@ __global__ __launch_bounds__(64) void k_leapfrog32(const float* __restrict__ K,
                                                    const double* __restrict__ wa,
                                                    double* __restrict__ out, double g, double c1,
                                                    double c2, const int* __restrict__ gsel,
-                                                   const double* __restrict__ gtab) {
+                                                   const double* __restrict__ gtab, int n_poly) {
   using D = Dims<P>;
   constexpr int NP = D::NP, N3 = D::N3, NQUAD = Dims32<P>::NQUAD;
   constexpr int RPL = (N3 + 63) / 64;
   __shared__ double sw[N3];
   __shared__ double sR[D::NPK];
   const int lane = threadIdx.x;
-  const int Pp = blockIdx.x;
+  const int Pp = step_poly();
+  if (Pp >= n_poly) return;  // padding workgroups of the last XCD share
   for (int k = lane; k < D::NPK; k += 64) sR[k] = Rp[(size_t)Pp * D::NPK + k];
   double acc[RPL];
 #pragma unroll
