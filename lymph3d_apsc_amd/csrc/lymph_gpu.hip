@@ -454,7 +454,7 @@ int lymph_ctx_create(const LymphMesh* mesh, int p, double alpha, double theta, L
   up(&d_qtri, qtri.data(), qtri.size());
   up(&d_wtri, wtri.data(), wtri.size());
   up(&d_bl, bl.data(), bl.size());
-  const size_t bs = blk_sz(c), npk = (size_t)c->Np * (c->Np + 1) / 2;
+  const size_t npk = (size_t)c->Np * (c->Np + 1) / 2;
   al(&c->M, (size_t)np * c->Np * c->Np);
   al(&c->Mm, (size_t)np * c->Np * c->Np);
   al(&c->RM, (size_t)np * npk);
