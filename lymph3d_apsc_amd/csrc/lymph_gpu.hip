@@ -66,7 +66,9 @@ struct LymphCtx {
   int n_groups = 0;
   int* gsel = nullptr;
   double *gf0 = nullptr, *gt0 = nullptr, *gtab = nullptr;
-  float* K32 = nullptr;
+  float* K32 = nullptr;     // 32-bit apply: ~K = R^-T K R^-1 (mass-orthonormal basis)
+  double* tv[5] = {};       // 32-bit stepping: ~u rotation (0..2), ~F_s (3), ~F_d (4)
+  int* blk_row = nullptr;   // row polyhedron of each block
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int graph_records = 0;
@@ -89,6 +91,8 @@ struct LymphCtx {
   int* rpoly = nullptr;
   double* rxyz = nullptr;
   double* rnow = nullptr;  // 3 n_recv values of one lymph_sample_receivers call
+  int* rpoly_u = nullptr;  // distinct receiver polyhedra
+  int n_rpoly_u = 0;
   double* trace = nullptr;
 };
 
@@ -176,9 +180,11 @@ int check_launch(const char* what) {
 }
 
 unsigned grid1(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
-// grid of a grid-stride kernel: at most 2^20 workgroups, so blocks x threads
-// stays far below the 2^32 work-items one dispatch can address
-unsigned grid_stride(size_t n, unsigned b) { return (unsigned)std::min<size_t>((n + b - 1) / b, 1u << 20); }
+size_t Dims32Any(int p) {
+  const int n3 = (p + 1) * (p + 2) * (p + 3) / 2;
+  return (size_t)n3 * (size_t)((n3 + 3) / 4) * 4;
+}
+
 
 double g_of(const LymphCtx* c, int problem, double t) {
   if (problem == LYMPH_PROB_RICKER) return ricker(c->ricker_f0, c->ricker_t0, t);
@@ -217,8 +223,9 @@ int launch_step(LymphCtx* c, int mode, double g, double c1, double c2, const dou
   const double* gtab = grouped ? c->gtab : nullptr;
   return dispatch_p(c->p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
-    const double* rs = c->slot[LYMPH_SLOT_RHS_STAT];
-    const double* rd = c->slot[LYMPH_SLOT_RHS_DYN];
+    // the 32-bit apply works on mass-orthonormal vectors: ~F from tilde_rhs
+    const double* rs = c->apply_bits == 32 ? c->tv[3] : c->slot[LYMPH_SLOT_RHS_STAT];
+    const double* rd = c->apply_bits == 32 ? c->tv[4] : c->slot[LYMPH_SLOT_RHS_DYN];
     if (c->apply_bits == 32) {
       if (mode == 0)
         launch_leapfrog32<P, 0>(c->step_variant, c->n_poly, c->stream, c->K32, c->md.row_ptr,
@@ -235,6 +242,50 @@ int launch_step(LymphCtx* c, int mode, double g, double c1, double c2, const dou
     }
     return check_launch("k_leapfrog");
   });
+}
+
+// Mass-orthonormal transforms (32-bit apply): mode 0 ~u = R u, 1 u = R^-1 ~u,
+// 2 ~F = R^-T F; `list` restricts mode 1 to n listed polyhedra.
+int tilde(LymphCtx* c, int mode, const double* in, double* out, const int* list = nullptr,
+          int n = -1) {
+  if (n < 0) n = c->n_poly;
+  if (n == 0) return 0;
+  return dispatch_p(c->p, [&](auto pc) -> int {
+    constexpr int P = decltype(pc)::value;
+    k_tilde<P><<<grid1(3 * (size_t)n, 128), 128, 0, c->stream>>>(mode, c->RM, in, out, list, n);
+    return check_launch("k_tilde");
+  });
+}
+
+// ~F_s, ~F_d of the current RHS slots (every 32-bit stepping call starts here)
+int tilde_rhs(LymphCtx* c) {
+  if (int rc = tilde(c, 2, c->slot[LYMPH_SLOT_RHS_STAT], c->tv[3])) return rc;
+  return tilde(c, 2, c->slot[LYMPH_SLOT_RHS_DYN], c->tv[4]);
+}
+
+// ~K from the FP64 blocks and the mass factors (K32 and the tilde vectors
+// allocated by the caller)
+int build_ortho(LymphCtx* c) {
+  double* S = nullptr;
+  const size_t npk = (size_t)c->Np * (c->Np + 1) / 2;
+  if (int rc = dalloc(&S, (size_t)c->n_poly * npk)) return rc;
+  int rc = dispatch_p(c->p, [&](auto pc) -> int {
+    constexpr int P = decltype(pc)::value;
+    k_inv_upper<P><<<grid1((size_t)c->n_poly * c->Np, 64), 64, 0, c->stream>>>(c->RM, S, c->n_poly);
+    if (int r = check_launch("k_inv_upper")) return r;
+    const size_t chunk = 1u << 20;  // workgroups per launch (one per block)
+    for (size_t b0 = 0; b0 < (size_t)c->n_blk; b0 += chunk) {
+      const unsigned nb = (unsigned)std::min(chunk, (size_t)c->n_blk - b0);
+      k_ortho_operator<P><<<nb, 256, 0, c->stream>>>(c->K, S, c->blk_row, c->md.col, c->K32, b0);
+      if (int r = check_launch("k_ortho_operator")) return r;
+    }
+    return 0;
+  });
+  const hipError_t e = hipStreamSynchronize(c->stream);
+  (void)hipFree(S);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(LYMPH_ERR_HIP, "k_ortho_operator failed");
+  return 0;
 }
 
 // Time factor of `problem` at t for a step launched next: the scalar g, and
@@ -311,6 +362,9 @@ int lymph_ctx_destroy(LymphCtx* c) {
   if (c->rnow) (void)hipFree(c->rnow);
   if (c->trace) (void)hipFree(c->trace);
   if (c->K32) (void)hipFree(c->K32);
+  for (double* q : c->tv)
+    if (q) (void)hipFree(q);
+  if (c->rpoly_u) (void)hipFree(c->rpoly_u);
   for (void* q : {(void*)c->gsel, (void*)c->gf0, (void*)c->gt0, (void*)c->gtab})
     if (q) (void)hipFree(q);
   if (c->K) (void)hipFree(c->K);
@@ -448,6 +502,12 @@ int lymph_ctx_create(const LymphMesh* mesh, int p, double alpha, double theta, L
   up(&d_ptet, ptet.data(), ptet.size());
   up(&d_rowp, row_ptr.data(), row_ptr.size());
   up(&d_col, col.data(), col.size());
+  {
+    std::vector<int> br(col.size());
+    for (int P = 0; P < np; ++P)
+      for (int b = row_ptr[P]; b < row_ptr[P + 1]; ++b) br[b] = P;
+    up(&c->blk_row, br.data(), br.size());
+  }
   up(&d_fblk, face_blk.data(), face_blk.size());
   up(&d_qtet, qtet.data(), qtet.size());
   up(&d_wtet, wtet.data(), wtet.size());
@@ -566,11 +626,6 @@ int lymph_build_operator(LymphCtx* c) {
     k_cholesky<P><<<grid1(c->n_poly, 64), 64, 0, c->stream>>>(c->M, c->Mm, c->RM, c->RMm,
                                                               c->n_poly, c->status);
     if (int r = check_launch("k_cholesky")) return r;
-    if (c->K32) {  // keep an existing 32-bit copy in step with the new blocks
-      k_to_fp32<P><<<grid_stride((size_t)c->n_blk * Dims32<P>::BS, 256), 256, 0, c->stream>>>(
-          c->K, c->K32, (size_t)c->n_blk);
-      if (int r = check_launch("k_to_fp32")) return r;
-    }
     return 0;
   });
   if (rc) return rc;
@@ -578,6 +633,8 @@ int lymph_build_operator(LymphCtx* c) {
   HIP_TRY(hipMemcpyAsync(&st, c->status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (st) return fail(LYMPH_EXIT_SINGULARMTX, "mass matrix is not positive definite");
+  if (c->K32)  // keep an existing 32-bit operator in step with the new blocks
+    if (int r = build_ortho(c)) return r;
   c->built = true;
   return 0;
 }
@@ -682,17 +739,93 @@ double* lymph_slot_ptr(LymphCtx* c, int slot) {
   return c->slot[slot];
 }
 
+namespace {
+// One step from device vectors in the original basis, time factor prepared:
+// mode 1 (first step) a = u0, b = v0; mode 0 a = u^n, b = u^{n-1}.  The
+// 32-bit apply steps the mass-orthonormal vectors and transforms back.
+int step_dev(LymphCtx* c, int mode, double g, double c1, double c2, const double* a,
+             const double* b, double* out, bool grouped) {
+  if (c->apply_bits != 32) return launch_step(c, mode, g, c1, c2, a, b, out, grouped);
+  if (int rc = tilde_rhs(c)) return rc;
+  if (int rc = tilde(c, 0, a, c->tv[0])) return rc;
+  if (int rc = tilde(c, 0, b, c->tv[1])) return rc;
+  if (int rc = launch_step(c, mode, g, c1, c2, c->tv[0], c->tv[1], c->tv[2], grouped)) return rc;
+  return tilde(c, 1, c->tv[2], out);
+}
+
+int sample_into(LymphCtx* c, const double* u, double* d_out);
+
+// nsteps leapfrog steps on the slots U_PREV, U_CUR (t0 = time of the first
+// of them); with `record`, receivers sampled every `every` steps.  The 64-bit
+// apply rotates the slot pointers; the 32-bit apply steps mass-orthonormal
+// copies and writes the last two states back into U_PREV, U_CUR.
+int run_loop(LymphCtx* c, int problem, int nsteps, double t0, double dt, int every, bool record,
+             double* t_out) {
+  const bool b32 = c->apply_bits == 32;
+  double *prev, *cur, *next;
+  if (b32) {
+    if (int rc = tilde_rhs(c)) return rc;
+    if (int rc = tilde(c, 0, c->slot[LYMPH_SLOT_U_PREV], c->tv[0])) return rc;
+    if (int rc = tilde(c, 0, c->slot[LYMPH_SLOT_U_CUR], c->tv[1])) return rc;
+    prev = c->tv[0];
+    cur = c->tv[1];
+    next = c->tv[2];
+  } else {
+    prev = c->slot[LYMPH_SLOT_U_PREV];
+    cur = c->slot[LYMPH_SLOT_U_CUR];
+    next = c->slot[LYMPH_SLOT_U_NEXT];
+  }
+  double t = t0;
+  const double dt2 = dt * dt;
+  int rc = 0;
+  for (int s = 1; s <= nsteps && !rc; ++s) {
+    double g;
+    bool grouped;
+    rc = prepare_g(c, problem, t, &g, &grouped);
+    if (!rc) rc = launch_step(c, 0, g, dt2, 0.0, cur, prev, next, grouped);
+    double* tmp = prev;
+    prev = cur;
+    cur = next;
+    next = tmp;
+    t = t + dt;
+    if (!rc && record && s % every == 0) {
+      if (c->n_rec >= c->max_rec) {
+        rc = fail(LYMPH_ERR_ARG, "receiver trace buffer full");
+        break;
+      }
+      const double* src = cur;
+      if (b32) {  // back to the original basis, receiver polyhedra only
+        rc = tilde(c, 1, cur, c->stage[3], c->rpoly_u, c->n_rpoly_u);
+        src = c->stage[3];
+      }
+      if (!rc) rc = sample_into(c, src, c->trace + (size_t)c->n_rec * 3 * c->n_recv);
+      if (!rc) c->n_rec++;
+    }
+  }
+  if (b32) {
+    if (!rc) rc = tilde(c, 1, prev, c->slot[LYMPH_SLOT_U_PREV]);
+    if (!rc) rc = tilde(c, 1, cur, c->slot[LYMPH_SLOT_U_CUR]);
+  } else {
+    c->slot[LYMPH_SLOT_U_PREV] = prev;
+    c->slot[LYMPH_SLOT_U_CUR] = cur;
+    c->slot[LYMPH_SLOT_U_NEXT] = next;
+  }
+  if (t_out) *t_out = t;
+  return rc;
+}
+}  // namespace
+
 int lymph_first_step_dev(LymphCtx* c, double g0, double dt, const double* u0, const double* v0,
                          double* u1) {
   if (int rc = need_built(c)) return rc;
   const double dt2 = dt * dt, half_dt2 = 0.5 * dt2;
-  return launch_step(c, 1, g0, half_dt2, dt, u0, v0, u1);
+  return step_dev(c, 1, g0, half_dt2, dt, u0, v0, u1, false);
 }
 
 int lymph_step_dev(LymphCtx* c, double g, double dt, const double* um1, const double* u,
                    double* up1) {
   if (int rc = need_built(c)) return rc;
-  return launch_step(c, 0, g, dt * dt, 0.0, u, um1, up1);
+  return step_dev(c, 0, g, dt * dt, 0.0, u, um1, up1, false);
 }
 
 int lymph_first_step(LymphCtx* c, double g0, double dt, const double* u0, const double* v0,
@@ -717,23 +850,7 @@ int lymph_step(LymphCtx* c, double g, double dt, const double* um1, const double
 int lymph_step_n(LymphCtx* c, int problem, int nsteps, double t0, double dt, double* t_out) {
   if (int rc = need_built(c)) return rc;
   if (nsteps < 0) return fail(LYMPH_ERR_ARG, "nsteps < 0");
-  double t = t0;
-  const double dt2 = dt * dt;
-  for (int s = 0; s < nsteps; ++s) {
-    double g;
-    bool grouped;
-    if (int rc = prepare_g(c, problem, t, &g, &grouped)) return rc;
-    if (int rc = launch_step(c, 0, g, dt2, 0.0, c->slot[LYMPH_SLOT_U_CUR],
-                             c->slot[LYMPH_SLOT_U_PREV], c->slot[LYMPH_SLOT_U_NEXT], grouped))
-      return rc;
-    double* prev = c->slot[LYMPH_SLOT_U_PREV];
-    c->slot[LYMPH_SLOT_U_PREV] = c->slot[LYMPH_SLOT_U_CUR];
-    c->slot[LYMPH_SLOT_U_CUR] = c->slot[LYMPH_SLOT_U_NEXT];
-    c->slot[LYMPH_SLOT_U_NEXT] = prev;
-    t = t + dt;
-  }
-  if (t_out) *t_out = t;
-  return 0;
+  return run_loop(c, problem, nsteps, t0, dt, 1, false, t_out);
 }
 
 int lymph_time_steps(LymphCtx* c, int problem, int nsteps, double t0, double dt, double* ms) {
@@ -809,16 +926,16 @@ int lymph_set_apply_precision(LymphCtx* c, int bits) {
   if (bits == 64 && !c->K)
     return fail(LYMPH_ERR_STATE, "FP64 K blocks were released: rebuild the operator");
   if (bits == 32 && !c->K32) {
-    int rc = dispatch_p(c->p, [&](auto pc) -> int {
-      constexpr int P = decltype(pc)::value;
-      const size_t n = (size_t)c->n_blk * Dims32<P>::BS;
-      if (int r = dalloc(&c->K32, n)) return r;
-      c->bytes += n * sizeof(float);
-      k_to_fp32<P><<<grid_stride(n, 256), 256, 0, c->stream>>>(c->K, c->K32, (size_t)c->n_blk);
-      return check_launch("k_to_fp32");
-    });
-    if (rc) return rc;
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    const size_t n = (size_t)c->n_blk * Dims32Any(c->p);
+    if (int r = dalloc(&c->K32, n)) return r;
+    c->bytes += n * sizeof(float);
+    for (double*& q : c->tv) {
+      if (!q) {
+        if (int r = dalloc(&q, vec_n(c))) return r;
+        c->bytes += vec_n(c) * sizeof(double);
+      }
+    }
+    if (int r = build_ortho(c)) return r;
   }
   c->apply_bits = bits;
   c->step_variant = 0;
@@ -908,13 +1025,22 @@ int lymph_set_receivers(LymphCtx* c, int n, const double* xyz, const int32_t* te
   if (c->rxyz) (void)hipFree(c->rxyz);
   if (c->rnow) (void)hipFree(c->rnow);
   if (c->trace) (void)hipFree(c->trace);
+  if (c->rpoly_u) (void)hipFree(c->rpoly_u);
   c->rpoly = nullptr; c->rxyz = nullptr; c->rnow = nullptr; c->trace = nullptr;
+  c->rpoly_u = nullptr; c->n_rpoly_u = 0;
   c->n_recv = 0; c->max_rec = 0; c->n_rec = 0;
   if (n == 0) return 0;
   // receivers count only once every buffer they use exists
   if (int rc = upload(&c->rpoly, rp.data(), rp.size())) return rc;
   if (int rc = upload(&c->rxyz, xyz, 3 * (size_t)n)) return rc;
   if (int rc = dalloc(&c->rnow, 3 * (size_t)n)) return rc;
+  {
+    std::vector<int> u(rp);
+    std::sort(u.begin(), u.end());
+    u.erase(std::unique(u.begin(), u.end()), u.end());
+    if (int rc = upload(&c->rpoly_u, u.data(), u.size())) return rc;
+    c->n_rpoly_u = (int)u.size();
+  }
   if (int rc = dalloc(&c->trace, 3 * (size_t)n * (size_t)(max_records > 0 ? max_records : 1))) return rc;
   c->n_recv = n;
   c->max_rec = max_records;
@@ -948,19 +1074,7 @@ int lymph_step_n_record(LymphCtx* c, int problem, int nsteps, double t0, double 
   if (int rc = need_built(c)) return rc;
   if (c->n_recv == 0) return fail(LYMPH_ERR_STATE, "no receivers set");
   if (every < 1 || nsteps < 0) return fail(LYMPH_ERR_ARG, "bad nsteps/every");
-  double t = t0;
-  for (int s = 1; s <= nsteps; ++s) {
-    if (int rc = lymph_step_n(c, problem, 1, t, dt, &t)) return rc;
-    if (s % every == 0) {
-      if (c->n_rec >= c->max_rec) return fail(LYMPH_ERR_ARG, "receiver trace buffer full");
-      if (int rc = sample_into(c, c->slot[LYMPH_SLOT_U_CUR],
-                               c->trace + (size_t)c->n_rec * 3 * c->n_recv))
-        return rc;
-      c->n_rec++;
-    }
-  }
-  if (t_out) *t_out = t;
-  return 0;
+  return run_loop(c, problem, nsteps, t0, dt, every, true, t_out);
 }
 
 int lymph_get_traces(LymphCtx* c, double* out, int* n_records) {
@@ -1001,8 +1115,8 @@ int lymph_capture_solve(LymphCtx* c, int problem, double dt, int nsteps, int eve
   bool grouped = false;
   if (!rc) rc = prepare_g(c, problem, t, &g0, &grouped);
   if (!rc)
-    rc = launch_step(c, 1, g0, 0.5 * (dt * dt), dt, c->slot[LYMPH_SLOT_U_PREV],
-                     c->slot[LYMPH_SLOT_V0], c->slot[LYMPH_SLOT_U_CUR], grouped);
+    rc = step_dev(c, 1, g0, 0.5 * (dt * dt), dt, c->slot[LYMPH_SLOT_U_PREV],
+                  c->slot[LYMPH_SLOT_V0], c->slot[LYMPH_SLOT_U_CUR], grouped);
   t = t + dt;
   if (!rc) {
     if (c->n_recv > 0) rc = lymph_step_n_record(c, problem, nsteps, t, dt, every, &t);

@@ -662,20 +662,126 @@ LYMPH_HD size_t kpos32(int N3, int r, int c) {
   return ((size_t)(c >> 2) * N3 + r) * 4 + (c & 3);
 }
 
+// ---------------------------------------------------------------------------
+// 32-bit apply in the mass-orthonormal basis.  With M_P = R_P^T R_P per
+// polyhedron, the variables ~u = R u turn the leapfrog step into
+//   ~u^{n+1} = 2 ~u^n - ~u^{n-1} + dt^2 (~F - ~K ~u^n),
+//   ~K_PQ = R_P^-T K_PQ R_Q^-1,   ~F = R^-T F,
+// the same iteration with M^-1 turned into the identity.  ~K is scaled like
+// M^-1 K whatever the conditioning of the bounding-box Legendre basis on the
+// polyhedron, so rounding it to FP32 perturbs the iteration by FP32 rounding
+// relative to itself.  Rounding K in the original basis perturbs M^-1 K by
+// that rounding times cond(M_P) -- up to 2e8 at p = 4 on agglomerates of
+// 4-40 tets, enough to make the iteration diverge.  The 32-bit step also
+// needs no triangular solves.
+// ---------------------------------------------------------------------------
+
+// S = R^-1 (upper triangular, packed like R), one thread per (polyhedron, column)
 template <int P>
-__global__ void k_to_fp32(const double* __restrict__ K, float* __restrict__ K32, size_t n_blk) {
+__global__ void k_inv_upper(const double* __restrict__ RM, double* __restrict__ S, int n_poly) {
   using D = Dims<P>;
-  constexpr int N3 = D::N3;
-  // grid-stride: an ensemble's operator can hold more than 2^32 entries, more
-  // work-items than one dispatch can address
-  const size_t n = n_blk * Dims32<P>::BS;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const size_t b = i / Dims32<P>::BS;
-    const size_t w = i % Dims32<P>::BS;  // within block, quad layout
-    const int c = (int)(w / (N3 * 4)) * 4 + (int)(w % 4);
-    const int r = (int)((w / 4) % N3);
-    K32[i] = c < N3 ? (float)K[b * D::BS + kpos(N3, r, c)] : 0.0f;
+  constexpr int NP = D::NP;
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n_poly * NP) return;
+  const int Pp = id / NP, j = id % NP;
+  const double* R = RM + (size_t)Pp * D::NPK;
+  double* Sp = S + (size_t)Pp * D::NPK;
+  double x[NP];  // column j of R^-1: R x = e_j, x[i] = 0 for i > j
+#pragma unroll
+  for (int i = 0; i < NP; ++i) x[i] = 0.0;
+  for (int i = j; i >= 0; --i) {
+    double s = (i == j) ? 1.0 : 0.0;
+    for (int k = i + 1; k <= j; ++k) s = s - R[pk_off(NP, i) + (k - i)] * x[k];
+    x[i] = s / R[pk_off(NP, i)];
+  }
+  for (int i = 0; i <= j; ++i) Sp[pk_off(NP, i) + (j - i)] = x[i];
+}
+
+// ~K_PQ = S_P^T K_PQ S_Q rounded to FP32 in the column-quad layout; one
+// workgroup per block, one component strip (Np rows) at a time:
+// T = X S_Q (per column component), Z = S_P^T T.
+template <int P>
+__global__ __launch_bounds__(256) void k_ortho_operator(const double* __restrict__ K,
+                                                        const double* __restrict__ S,
+                                                        const int* __restrict__ blk_row,
+                                                        const int* __restrict__ col,
+                                                        float* __restrict__ K32, size_t b0) {
+  using D = Dims<P>;
+  constexpr int NP = D::NP, N3 = D::N3, NPK = D::NPK, NQUAD = Dims32<P>::NQUAD;
+  __shared__ double SP[NPK], SQ[NPK];
+  __shared__ double X[NP * N3], T[NP * N3];
+  const int tid = threadIdx.x;
+  const size_t b = b0 + blockIdx.x;
+  const int Pp = blk_row[b], Q = col[b];
+  for (int k = tid; k < NPK; k += blockDim.x) {
+    SP[k] = S[(size_t)Pp * NPK + k];
+    SQ[k] = S[(size_t)Q * NPK + k];
+  }
+  const double* Kb = K + b * D::BS;
+  float* Ob = K32 + b * Dims32<P>::BS;
+  for (int i = 0; i < 3; ++i) {
+    __syncthreads();
+    for (int e = tid; e < NP * N3; e += blockDim.x) {
+      const int r = e / N3, c = e % N3;
+      X[e] = Kb[kpos(N3, i * NP + r, c)];
+    }
+    __syncthreads();
+    for (int e = tid; e < NP * N3; e += blockDim.x) {  // T = X S_Q, S_Q upper
+      const int r = e / N3, c = e % N3, j = c / NP, cc = c % NP;
+      double s = 0.0;
+      for (int k = 0; k <= cc; ++k) s = s + X[r * N3 + j * NP + k] * SQ[pk_off(NP, k) + (cc - k)];
+      T[e] = s;
+    }
+    __syncthreads();
+    for (int e = tid; e < NP * N3; e += blockDim.x) {  // Z = S_P^T T
+      const int r = e / N3, c = e % N3;
+      double s = 0.0;
+      for (int k = 0; k <= r; ++k) s = s + SP[pk_off(NP, k) + (r - k)] * T[k * N3 + c];
+      Ob[kpos32(N3, i * NP + r, c)] = (float)s;
+    }
+  }
+  // zero padding columns of the last quad
+  constexpr int PAD = 4 * NQUAD - N3;
+  if constexpr (PAD > 0) {
+    for (int e = tid; e < N3 * PAD; e += blockDim.x) {
+      const int r = e / PAD, c = N3 + e % PAD;
+      Ob[kpos32(N3, r, c)] = 0.0f;
+    }
+  }
+}
+
+// Vector transforms per (polyhedron, component): mode 0 ~u = R u, mode 1
+// u = R^-1 ~u, mode 2 ~F = R^-T F.  `list` (n entries) restricts mode 1 to
+// the listed polyhedra (receivers); null = all n polyhedra.
+template <int P>
+__global__ void k_tilde(int mode, const double* __restrict__ RM, const double* __restrict__ in,
+                        double* __restrict__ out, const int* __restrict__ list, int n) {
+  using D = Dims<P>;
+  constexpr int NP = D::NP;
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n * 3) return;
+  const int Pp = list ? list[id / 3] : id / 3, comp = id % 3;
+  const double* R = RM + (size_t)Pp * D::NPK;
+  const double* x = in + (size_t)Pp * D::N3 + comp * NP;
+  double* y = out + (size_t)Pp * D::N3 + comp * NP;
+  if (mode == 0) {
+    for (int i = 0; i < NP; ++i) {
+      double s = 0.0;
+      for (int k = i; k < NP; ++k) s = s + R[pk_off(NP, i) + (k - i)] * x[k];
+      y[i] = s;
+    }
+  } else if (mode == 1) {
+    for (int i = NP - 1; i >= 0; --i) {
+      double s = x[i];
+      for (int k = i + 1; k < NP; ++k) s = s - R[pk_off(NP, i) + (k - i)] * y[k];
+      y[i] = s / R[pk_off(NP, i)];
+    }
+  } else {
+    for (int i = 0; i < NP; ++i) {
+      double s = x[i];
+      for (int k = 0; k < i; ++k) s = s - R[pk_off(NP, k) + (i - k)] * y[k];
+      y[i] = s / R[pk_off(NP, i)];
+    }
   }
 }
 
@@ -691,15 +797,13 @@ __global__ __launch_bounds__(64) void k_leapfrog32(const float* __restrict__ K,
                                                    double* __restrict__ out, double g, double c1,
                                                    double c2, const int* __restrict__ gsel,
                                                    const double* __restrict__ gtab, int n_poly) {
+  // all vectors in the mass-orthonormal basis (K is ~K); Rp is unused
   using D = Dims<P>;
-  constexpr int NP = D::NP, N3 = D::N3, NQUAD = Dims32<P>::NQUAD;
+  constexpr int N3 = D::N3, NQUAD = Dims32<P>::NQUAD;
   constexpr int RPL = (N3 + 63) / 64;
-  __shared__ double sw[N3];
-  __shared__ double sR[D::NPK];
   const int lane = threadIdx.x;
   const int Pp = step_poly();
   if (Pp >= n_poly) return;  // padding workgroups of the last XCD share
-  for (int k = lane; k < D::NPK; k += 64) sR[k] = Rp[(size_t)Pp * D::NPK + k];
   double acc[RPL];
 #pragma unroll
   for (int k = 0; k < RPL; ++k) acc[k] = 0.0;
@@ -765,19 +869,12 @@ __global__ __launch_bounds__(64) void k_leapfrog32(const float* __restrict__ K,
 #pragma unroll
   for (int k = 0; k < RPL; ++k) {
     const int r = lane + 64 * k;
-    if (r < N3) sw[r] = -acc[k] + rs[o + r] + rd[o + r] * gp;
-  }
-  __syncthreads();
-  if (lane < 3) solve_packed<NP>(sR, sw + lane * NP);
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < RPL; ++k) {
-    const int r = lane + 64 * k;
     if (r < N3) {
+      const double w = -acc[k] + rs[o + r] + rd[o + r] * gp;
       if (MODE == 0)
-        out[o + r] = c1 * sw[r] + 2.0 * u[o + r] - wa[o + r];
+        out[o + r] = c1 * w + 2.0 * u[o + r] - wa[o + r];
       else
-        out[o + r] = c1 * sw[r] + u[o + r] + c2 * wa[o + r];
+        out[o + r] = c1 * w + u[o + r] + c2 * wa[o + r];
     }
   }
 }

@@ -312,7 +312,7 @@ LYMPH_HD void field(int fn, const double* p, double* r) {
 }
 
 // Ricker wavelet (1 - 2a) exp(-a), a = (pi f0 (t - t0))^2.
-LYMPH_HD inline double ricker(double f0, double t0, double t) {
+LYMPH_HD double ricker(double f0, double t0, double t) {
   double a = kPi * f0 * (t - t0);
   a = a * a;
   return (1.0 - 2.0 * a) * exp(-a);

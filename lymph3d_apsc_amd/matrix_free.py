@@ -173,10 +173,13 @@ class Context:
         return int(info[5])
 
     def step_bytes_for(self, bits):
-        """Algorithmic bytes of one step with the operator in `bits` precision."""
+        """Algorithmic bytes of one step with the operator in `bits`
+        precision: the blocks, five vectors, and for the 64-bit apply the
+        packed mass factors (the 32-bit apply works in the mass-orthonormal
+        basis and solves nothing)."""
         n3, Np = self.n3, self.Np
-        return (bits // 8) * self.n_blk * n3 * n3 + 8 * (self.n_poly * Np * (Np + 1) // 2
-                                                          + 5 * self.n_poly * n3)
+        factors = 8 * self.n_poly * Np * (Np + 1) // 2 if bits == 64 else 0
+        return (bits // 8) * self.n_blk * n3 * n3 + factors + 8 * 5 * self.n_poly * n3
 
     # ---- point sources (Ricker) -----------------------------------------
     def add_point_sources(self, points, forces, slot=_lib.SLOT_RHS_DYN, tets=None):

@@ -295,6 +295,37 @@ def test_fp32_operator_beyond_2_32_entries():
     assert (d <= 1e-4 * s).all(), int(np.argmax(d / s))
 
 
+def test_fp32_apply_on_ill_conditioned_agglomerates():
+    """Config 3's shape -- 30^3 cubes x 6 tets grown into polyhedra of 4-40
+    tets, p=4, Neumann top, Ricker source, 2000 steps: the bounding-box
+    Legendre basis makes the mass blocks ill conditioned (cond up to ~1e8),
+    yet the 32-bit apply (in the mass-orthonormal basis) stays within 1e-3 of
+    the FP64 receiver traces instead of diverging."""
+    say("[fp32 ill-conditioned p=4]")
+    from lymph3d_apsc_amd import ensemble as E
+    m = M.cube_mesh(30, dirichlet_faces={"x0", "x1", "y0", "y1", "z0"}, poly_size=(4, 40), seed=3)
+    rng = np.random.default_rng(3)
+    bc = m.xyz[m.tet_vert].mean(axis=1)
+    m.lam, m.mu = E.lognormal_field(rng, bc), E.lognormal_field(rng, bc)
+    m.rho = rng.uniform(0.8, 1.2, m.n_tet)
+    p, steps = 4, 2000
+    dt = 0.2 * (1 / 30) / (np.sqrt(((m.lam + 2 * m.mu) / m.rho).max()) * p * p)
+    rec = rng.uniform(0.05, 0.95, (32, 3))
+    traces = []
+    with L.Context(m, p) as ctx:
+        ctx.set_keep_adg(False)
+        ctx.make_matrices_free()
+        for bits in (64, 32):
+            ctx.set_apply_precision(bits)
+            t = ctx.start_from_rest(dt, [[0.5, 0.45, 0.55]], [[0.0, 0.0, 1.0]], f0=5.0)
+            ctx.set_receivers(rec, max_records=steps // 10)
+            ctx.step_n_record(L.PROB_RICKER, steps, t, dt, every=10)
+            traces.append(ctx.traces())
+    a64, a32 = traces
+    assert np.isfinite(a32).all() and np.abs(a64).max() > 0
+    assert np.linalg.norm(a32 - a64) < 1e-3 * np.linalg.norm(a64)
+
+
 def test_memory_options_keep_results():
     """Dropping A_dg and releasing the FP64 blocks changes no result: the
     operator built without A_dg, the 32-bit apply after the release and a
