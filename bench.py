@@ -270,7 +270,7 @@ def main():
         "setup_s": t_build,
     }
     if not a.no_fp32:
-        # FP32 operator apply (FP64 RHS, mass solves, update): same start state
+        # 32-bit operator apply (mass-orthonormal basis; FP64 RHS, transforms, update): same start state
         # and the same number of steps as the FP64 timed run
         ctx.run(prob, dt, 1e30, max_steps=1 + a.warmup)
         ctx.set_apply_precision(32)

@@ -187,11 +187,15 @@ int lymph_autotune(LymphCtx* ctx, int reps, int* n_variants, int* unr, int* xmod
                    double* ms, int* chosen);
 
 /* Operator precision of the step's block apply: 64 (default) or 32.  With 32
- * the assembled FP64 blocks are rounded once to FP32 (a second copy, half
- * the bytes per step); products and per-block sums are FP32, the sum over
- * blocks, RHS, mass solves (FP64 Cholesky factors) and update stay FP64.
- * No reference counterpart (the reference is FP64 only).  Call after
- * lymph_build_operator; resets the autotuned variant. */
+ * the steps run in the mass-orthonormal basis u~ = R u (M = R^T R per
+ * polyhedron): the blocks R_P^-T K_PQ R_Q^-1 are formed in FP64 and rounded
+ * once to FP32 (a second copy, half the bytes per step), products and
+ * per-block sums are FP32, the sum over blocks, the RHS, the basis transforms
+ * at the start and end of every stepping call and the update stay FP64, and
+ * the step needs no triangular solves.  Rounding K itself would perturb
+ * M^-1 K by the rounding times cond(M_P), which diverges on ill-conditioned
+ * agglomerates.  No reference counterpart (the reference is FP64 only).
+ * Call after lymph_build_operator; resets the autotuned variant. */
 int lymph_set_apply_precision(LymphCtx* ctx, int bits);
 
 /* Device-memory options for large batched runs (no reference counterpart:

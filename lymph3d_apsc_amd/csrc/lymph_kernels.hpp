@@ -646,11 +646,12 @@ __global__ __launch_bounds__(64) void k_leapfrog(const double* __restrict__ K,
 }
 
 // ---------------------------------------------------------------------------
-// FP32-apply variant of the step: the operator blocks are held in FP32 in
+// FP32-apply variant of the step: the operator blocks, taken to the
+// mass-orthonormal basis in FP64 (k_ortho_operator below), are held in FP32 in
 // "column-quad" order (entry (r, c) at ((c>>2)*N3 + r)*4 + (c&3); a lane
 // reads four columns of its row as one 16-byte load, half the bytes of the
-// FP64 stream), products and the per-block sum are FP32; the sum over blocks,
-// the RHS, the mass solves and the update stay FP64.
+// FP64 stream); products and the per-block sum are FP32; the sum over blocks,
+// the RHS, the basis transforms and the update stay FP64.
 // ---------------------------------------------------------------------------
 template <int P>
 struct Dims32 {
