@@ -219,8 +219,8 @@ def main():
     t_build = time.perf_counter() - t0
     log(f"setup + {1 + a.warmup} steps: {t_build:.2f}s; device bytes {ctx.device_bytes / 1e9:.2f} GB")
     t_start = r0["t_final"]
-    chosen, tune = ((None, None), {}) if a.no_autotune else ctx.autotune(reps=20)
-    log(f"autotune: chosen (group size, broadcast mode) {chosen}; ms/step {tune}")
+    chosen, tune = ((None, None, None), {}) if a.no_autotune else ctx.autotune(reps=20)
+    log(f"autotune: chosen (group size, broadcast mode, waves per polyhedron) {chosen}; ms/step {tune}")
     # timed region: K device-resident steps, HIP events on the context stream
     w0 = time.perf_counter()
     ms = ctx.time_steps(prob, a.steps, t_start, dt)
@@ -265,7 +265,7 @@ def main():
         },
         "wall_ms_per_step": wall * 1e3 / a.steps,
         "autotune": {"chosen": list(chosen),
-                     "ms_per_step": {f"unr{k[0]}_x{k[1]}": v for k, v in tune.items()}},
+                     "ms_per_step": {f"unr{k[0]}_x{k[1]}_w{k[2]}": v for k, v in tune.items()}},
         "err_l2": e_l2, "err_dg": e_dg,
         "setup_s": t_build,
     }
@@ -274,7 +274,7 @@ def main():
         # and the same number of steps as the FP64 timed run
         ctx.run(prob, dt, 1e30, max_steps=1 + a.warmup)
         ctx.set_apply_precision(32)
-        ch32, tune32 = ((None, None), {}) if a.no_autotune else ctx.autotune(reps=20)
+        ch32, tune32 = ((None, None, None), {}) if a.no_autotune else ctx.autotune(reps=20)
         ms32 = ctx.time_steps(prob, a.steps, t_start, dt)
         u32 = ctx.get_slot(_lib.SLOT_U_CUR)
         ms32_step = ms32 / a.steps
@@ -289,7 +289,7 @@ def main():
             "rel_l2_diff_vs_fp64": float(np.linalg.norm(u32 - u) / np.linalg.norm(u)),
             "err_l2": float(np.sqrt(ctx.compute_error_l2_matrix_free(u32, uex))),
             "autotune": {"chosen": list(ch32),
-                         "ms_per_step": {f"unr{k[0]}": v for k, v in tune32.items()}},
+                         "ms_per_step": {f"unr{k[0]}_w{k[2]}": v for k, v in tune32.items()}},
         }
         log(f"fp32 apply: {ms32_step:.3f} ms/step, rel diff vs fp64 {res['fp32_apply']['rel_l2_diff_vs_fp64']:.2e}")
     del ctx

@@ -178,13 +178,21 @@ int lymph_time_steps(LymphCtx* ctx, int problem, int nsteps, double t0,
 
 /* Time every variant of the step kernel for `reps` steps on copies of the
  * state slots, keep the fastest for this context and report them: for
- * variant i, unr[i] = column pairs loaded per unrolled group, xmode[i] = how
- * the neighbour's coefficients reach the lanes (0 scalar loads, 1 vector load
- * + lane broadcast), ms[i] = milliseconds per step; arrays of at least 16
- * entries; *chosen = index of the variant kept.  No reference counterpart
- * (the reference has one CPU loop); results do not depend on the variant. */
-int lymph_autotune(LymphCtx* ctx, int reps, int* n_variants, int* unr, int* xmode,
+ * variant i, unr[i] = column pairs (quads for the 32-bit apply) loaded per
+ * unrolled group, xmode[i] = how the neighbour's coefficients reach the lanes
+ * (0 scalar loads, 1 vector load + lane broadcast), wpp[i] = waves sharing one
+ * polyhedron's blocks (their block products are added in block order, so
+ * results do not depend on it), ms[i] = milliseconds per step (-1: not run,
+ * the longest block row does not fit the variant's LDS); arrays of at least
+ * 16 entries, any may be NULL; *chosen = index of the variant kept.  No
+ * reference counterpart (the reference has one CPU loop); results do not
+ * depend on the variant. */
+int lymph_autotune(LymphCtx* ctx, int reps, int* n_variants, int* unr, int* xmode, int* wpp,
                    double* ms, int* chosen);
+/* Select step-kernel variant `index` (as numbered by lymph_autotune for the
+ * current apply precision) without timing; *n_variants (may be NULL) = count.
+ * No reference counterpart. */
+int lymph_set_step_variant(LymphCtx* ctx, int index, int* n_variants);
 
 /* Operator precision of the step's block apply: 64 (default) or 32.  With 32
  * the steps run in the mass-orthonormal basis u~ = R u (M = R^T R per

@@ -36,7 +36,7 @@ EXPORTS = [
     "lymph_get_slot", "lymph_slot_ptr", "lymph_first_step", "lymph_step",
     "lymph_first_step_dev", "lymph_step_dev", "lymph_step_n", "lymph_time_steps",
     "lymph_error_l2", "lymph_error_dg", "lymph_error_slots", "lymph_run",
-    "lymph_time_function", "lymph_autotune", "lymph_set_receivers",
+    "lymph_time_function", "lymph_autotune", "lymph_set_step_variant", "lymph_set_receivers",
     "lymph_sample_receivers", "lymph_step_n_record", "lymph_get_traces",
     "lymph_set_apply_precision", "lymph_set_source_groups", "lymph_ctx_set_keep_adg", "lymph_release_fp64_operator",
     "lymph_add_point_sources", "lymph_set_ricker", "lymph_ricker",
@@ -103,7 +103,8 @@ def lib():
         "lymph_error_slots": (_i, [_vp, _i, _i, _vp, _vp]),
         "lymph_run": (_i, [_vp, _i, _d, _d, _i, _vp, _vp]),
         "lymph_time_function": (_d, [_i, _d]),
-        "lymph_autotune": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
+        "lymph_autotune": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "lymph_set_step_variant": (_i, [_vp, _i, _vp]),
         "lymph_set_receivers": (_i, [_vp, _i, _vp, _vp, _i]),
         "lymph_sample_receivers": (_i, [_vp, _i, _vp]),
         "lymph_step_n_record": (_i, [_vp, _i, _i, _d, _d, _i, _vp]),

@@ -69,6 +69,7 @@ struct LymphCtx {
   float* K32 = nullptr;     // 32-bit apply: ~K = R^-T K R^-1 (mass-orthonormal basis)
   double* tv[5] = {};       // 32-bit stepping: ~u rotation (0..2), ~F_s (3), ~F_d (4)
   int* blk_row = nullptr;   // row polyhedron of each block
+  int max_row = 0;          // most blocks in one block row
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int graph_records = 0;
@@ -115,61 +116,98 @@ int dispatch_p(int p, F&& f) {
 // Step-kernel variants per degree: (column pairs per unrolled load group,
 // neighbour-coefficient broadcast mode); index 0 is the default until
 // lymph_autotune picks one.
+// Step-kernel variants per degree: column pairs per unrolled load group,
+// neighbour-coefficient broadcast mode, waves per polyhedron; index 0 is the
+// default until lymph_autotune picks one.
 template <int P> struct StepVar;
 template <> struct StepVar<1> {
-  static constexpr int unr[] = {6, 6, 3, 2};
-  static constexpr int xs[] = {1, 0, 1, 1};
+  static constexpr int unr[] = {6, 6, 3, 2, 6};
+  static constexpr int xs[] = {1, 0, 1, 1, 1};
+  static constexpr int wpp[] = {1, 1, 1, 1, 2};
 };
 template <> struct StepVar<2> {
-  static constexpr int unr[] = {5, 5, 15, 3, 15};
-  static constexpr int xs[] = {1, 0, 1, 1, 0};
+  static constexpr int unr[] = {5, 5, 15, 3, 15, 5, 5};
+  static constexpr int xs[] = {1, 0, 1, 1, 0, 1, 1};
+  static constexpr int wpp[] = {1, 1, 1, 1, 1, 2, 4};
 };
 template <> struct StepVar<3> {
-  static constexpr int unr[] = {10, 10, 5, 6, 15, 30, 30, 3};
-  static constexpr int xs[] = {1, 0, 1, 1, 1, 1, 0, 1};
+  static constexpr int unr[] = {10, 10, 5, 6, 15, 30, 30, 3, 30, 10};
+  static constexpr int xs[] = {1, 0, 1, 1, 1, 1, 0, 1, 1, 1};
+  static constexpr int wpp[] = {1, 1, 1, 1, 1, 1, 1, 1, 2, 4};
 };
 template <> struct StepVar<4> {
-  static constexpr int unr[] = {4, 4, 2, 8, 1};
-  static constexpr int xs[] = {1, 0, 1, 1, 1};
+  static constexpr int unr[] = {4, 4, 2, 8, 1, 4, 4, 4, 2};
+  static constexpr int xs[] = {1, 0, 1, 1, 1, 0, 1, 0, 1};
+  static constexpr int wpp[] = {1, 1, 1, 1, 1, 2, 2, 4, 4};
 };
 template <int P> constexpr int n_var() { return (int)(sizeof(StepVar<P>::unr) / sizeof(int)); }
+
+// dynamic LDS of a multi-wave variant (per-block products of the longest
+// block row) must stay within this; otherwise variant 0 runs instead
+constexpr size_t kMaxDynLds = 64 * 1024;
 
 template <int P, int MODE, int I = 0>
 void launch_leapfrog(int variant, int grid, hipStream_t st, const double* K, const int* rp,
                      const int* col, const double* Rp, const double* rs, const double* rd,
                      const double* u, const double* wa, double* out, double g, double c1,
-                     double c2, const int* gsel = nullptr, const double* gtab = nullptr) {
+                     double c2, const int* gsel, const double* gtab, size_t dyn) {
   if constexpr (I < n_var<P>()) {
     if (variant == I) {
-      k_leapfrog<P, MODE, StepVar<P>::unr[I], StepVar<P>::xs[I]><<<step_grid(grid), 64, 0, st>>>(
-          K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid);
+      constexpr int W = StepVar<P>::wpp[I];
+      if (W > 1 && dyn > kMaxDynLds) {
+        launch_leapfrog<P, MODE, 0>(0, grid, st, K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2,
+                                    gsel, gtab, dyn);
+        return;
+      }
+      k_leapfrog<P, MODE, StepVar<P>::unr[I], StepVar<P>::xs[I], W>
+          <<<step_grid(grid), 64 * W, W > 1 ? dyn : 0, st>>>(K, rp, col, Rp, rs, rd, u, wa, out, g,
+                                                             c1, c2, gsel, gtab, grid);
       return;
     }
     launch_leapfrog<P, MODE, I + 1>(variant, grid, st, K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2,
-                                    gsel, gtab);
+                                    gsel, gtab, dyn);
   }
 }
 
 template <int P> struct StepVar32;
-template <> struct StepVar32<1> { static constexpr int unr[] = {3, 1}; };
-template <> struct StepVar32<2> { static constexpr int unr[] = {4, 2, 8}; };
-template <> struct StepVar32<3> { static constexpr int unr[] = {5, 3, 15}; };
-template <> struct StepVar32<4> { static constexpr int unr[] = {3, 1, 9}; };
+template <> struct StepVar32<1> {
+  static constexpr int unr[] = {3, 1, 3};
+  static constexpr int wpp[] = {1, 1, 2};
+};
+template <> struct StepVar32<2> {
+  static constexpr int unr[] = {4, 2, 8, 4, 4};
+  static constexpr int wpp[] = {1, 1, 1, 2, 4};
+};
+template <> struct StepVar32<3> {
+  static constexpr int unr[] = {5, 3, 15, 5, 5};
+  static constexpr int wpp[] = {1, 1, 1, 2, 4};
+};
+template <> struct StepVar32<4> {
+  static constexpr int unr[] = {3, 1, 9, 3, 3};
+  static constexpr int wpp[] = {1, 1, 1, 2, 4};
+};
 template <int P> constexpr int n_var32() { return (int)(sizeof(StepVar32<P>::unr) / sizeof(int)); }
 
 template <int P, int MODE, int I = 0>
 void launch_leapfrog32(int variant, int grid, hipStream_t st, const float* K, const int* rp,
                        const int* col, const double* Rp, const double* rs, const double* rd,
                        const double* u, const double* wa, double* out, double g, double c1,
-                       double c2, const int* gsel = nullptr, const double* gtab = nullptr) {
+                       double c2, const int* gsel, const double* gtab, size_t dyn) {
   if constexpr (I < n_var32<P>()) {
     if (variant == I) {
-      k_leapfrog32<P, MODE, StepVar32<P>::unr[I]><<<step_grid(grid), 64, 0, st>>>(
-          K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid);
+      constexpr int W = StepVar32<P>::wpp[I];
+      if (W > 1 && dyn > kMaxDynLds) {
+        launch_leapfrog32<P, MODE, 0>(0, grid, st, K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2,
+                                      gsel, gtab, dyn);
+        return;
+      }
+      k_leapfrog32<P, MODE, StepVar32<P>::unr[I], W>
+          <<<step_grid(grid), 64 * W, W > 1 ? dyn : 0, st>>>(K, rp, col, Rp, rs, rd, u, wa, out, g,
+                                                             c1, c2, gsel, gtab, grid);
       return;
     }
     launch_leapfrog32<P, MODE, I + 1>(variant, grid, st, K, rp, col, Rp, rs, rd, u, wa, out, g, c1,
-                                      c2, gsel, gtab);
+                                      c2, gsel, gtab, dyn);
   }
 }
 
@@ -221,6 +259,7 @@ int launch_step(LymphCtx* c, int mode, double g, double c1, double c2, const dou
                 const double* wa, double* out, bool grouped = false) {
   const int* gsel = grouped ? c->gsel : nullptr;
   const double* gtab = grouped ? c->gtab : nullptr;
+  const size_t dyn = (size_t)c->max_row * c->n3 * sizeof(double);
   return dispatch_p(c->p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
     // the 32-bit apply works on mass-orthonormal vectors: ~F from tilde_rhs
@@ -229,16 +268,16 @@ int launch_step(LymphCtx* c, int mode, double g, double c1, double c2, const dou
     if (c->apply_bits == 32) {
       if (mode == 0)
         launch_leapfrog32<P, 0>(c->step_variant, c->n_poly, c->stream, c->K32, c->md.row_ptr,
-                                c->md.col, c->RM, rs, rd, u, wa, out, g, c1, c2, gsel, gtab);
+                                c->md.col, c->RM, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, dyn);
       else
         launch_leapfrog32<P, 1>(c->step_variant, c->n_poly, c->stream, c->K32, c->md.row_ptr,
-                                c->md.col, c->RM, rs, rd, u, wa, out, g, c1, c2, gsel, gtab);
+                                c->md.col, c->RM, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, dyn);
     } else if (mode == 0) {
       launch_leapfrog<P, 0>(c->step_variant, c->n_poly, c->stream, c->K, c->md.row_ptr, c->md.col,
-                            c->RM, rs, rd, u, wa, out, g, c1, c2, gsel, gtab);
+                            c->RM, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, dyn);
     } else {
       launch_leapfrog<P, 1>(c->step_variant, c->n_poly, c->stream, c->K, c->md.row_ptr, c->md.col,
-                            c->RM, rs, rd, u, wa, out, g, c1, c2, gsel, gtab);
+                            c->RM, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, dyn);
     }
     return check_launch("k_leapfrog");
   });
@@ -316,7 +355,8 @@ int errors_dev(LymphCtx* c, const double* uh, const double* ue, double* l2sq, do
     if (dgsq) {
       k_sub<<<grid1(n, 256), 256, 0, c->stream>>>(uh, ue, c->stage[2], n);
       launch_leapfrog<P, 2>(0, c->n_poly, c->stream, c->A, c->md.row_ptr, c->md.col, nullptr,
-                            nullptr, nullptr, c->stage[2], nullptr, c->part, 0.0, 0.0, 0.0);
+                            nullptr, nullptr, c->stage[2], nullptr, c->part, 0.0, 0.0, 0.0,
+                            nullptr, nullptr, 0);
       k_seq_sum<<<1, 64, 0, c->stream>>>(c->part, c->n_poly, c->dsum + 1);
     }
     return check_launch("error kernels");
@@ -504,8 +544,10 @@ int lymph_ctx_create(const LymphMesh* mesh, int p, double alpha, double theta, L
   up(&d_col, col.data(), col.size());
   {
     std::vector<int> br(col.size());
-    for (int P = 0; P < np; ++P)
+    for (int P = 0; P < np; ++P) {
+      c->max_row = std::max(c->max_row, row_ptr[P + 1] - row_ptr[P]);
       for (int b = row_ptr[P]; b < row_ptr[P + 1]; ++b) br[b] = P;
+    }
     up(&c->blk_row, br.data(), br.size());
   }
   up(&d_fblk, face_blk.data(), face_blk.size());
@@ -871,14 +913,16 @@ int lymph_time_steps(LymphCtx* c, int problem, int nsteps, double t0, double dt,
   return 0;
 }
 
-int lymph_autotune(LymphCtx* c, int reps, int* n_variants, int* unr, int* xmode, double* ms,
-                   int* chosen) {
+int lymph_autotune(LymphCtx* c, int reps, int* n_variants, int* unr, int* xmode, int* wpp,
+                   double* ms, int* chosen) {
   if (int rc = need_built(c)) return rc;
   if (reps < 1) return fail(LYMPH_ERR_ARG, "reps < 1");
   return dispatch_p(c->p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
-    const int NV = c->apply_bits == 32 ? n_var32<P>() : n_var<P>();
+    const bool b32 = c->apply_bits == 32;
+    const int NV = b32 ? n_var32<P>() : n_var<P>();
     if (n_variants) *n_variants = NV;
+    const bool fits = (size_t)c->max_row * c->n3 * sizeof(double) <= kMaxDynLds;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -891,6 +935,14 @@ int lymph_autotune(LymphCtx* c, int reps, int* n_variants, int* unr, int* xmode,
     double best = 1e300;
     int bi = 0;
     for (int v = 0; v < NV; ++v) {
+      const int w = b32 ? StepVar32<P>::wpp[v] : StepVar<P>::wpp[v];
+      if (unr) unr[v] = b32 ? StepVar32<P>::unr[v] : StepVar<P>::unr[v];
+      if (xmode) xmode[v] = b32 ? 1 : StepVar<P>::xs[v];
+      if (wpp) wpp[v] = w;
+      if (w > 1 && !fits) {  // would run variant 0: not a candidate
+        if (ms) ms[v] = -1.0;
+        continue;
+      }
       c->step_variant = v;
       if (int rc = launch_step(c, 0, 0.5, 1e-8, 0.0, c->stage[1], c->stage[0], c->stage[2])) {
         c->step_variant = saved;
@@ -907,8 +959,6 @@ int lymph_autotune(LymphCtx* c, int reps, int* n_variants, int* unr, int* xmode,
       float f = 0.f;
       HIP_TRY(hipEventElapsedTime(&f, e0, e1));
       const double t = (double)f / reps;
-      if (unr) unr[v] = c->apply_bits == 32 ? StepVar32<P>::unr[v] : StepVar<P>::unr[v];
-      if (xmode) xmode[v] = c->apply_bits == 32 ? 1 : StepVar<P>::xs[v];
       if (ms) ms[v] = t;
       if (t < best) { best = t; bi = v; }
     }
@@ -916,6 +966,18 @@ int lymph_autotune(LymphCtx* c, int reps, int* n_variants, int* unr, int* xmode,
     (void)hipEventDestroy(e1);
     c->step_variant = bi;
     if (chosen) *chosen = bi;
+    return 0;
+  });
+}
+
+int lymph_set_step_variant(LymphCtx* c, int index, int* n_variants) {
+  if (int rc = need_ctx(c)) return rc;
+  return dispatch_p(c->p, [&](auto pc) -> int {
+    constexpr int P = decltype(pc)::value;
+    const int NV = c->apply_bits == 32 ? n_var32<P>() : n_var<P>();
+    if (n_variants) *n_variants = NV;
+    if (index < 0 || index >= NV) return fail(LYMPH_ERR_ARG, "no such step variant");
+    c->step_variant = index;
     return 0;
   });
 }

@@ -523,8 +523,8 @@ inline unsigned step_grid(int n_poly) { return 8u * (unsigned)((n_poly + 7) / 8)
 // 0 = wave-uniform scalar loads per pair, 1 = one coalesced vector load per
 // block (lane c holds x[c]) broadcast with v_readlane, so the inner loop waits
 // only on the block's own HBM loads.  Chosen per degree by lymph_autotune.
-template <int P, int MODE, int UNR, int XS>
-__global__ __launch_bounds__(64) void k_leapfrog(const double* __restrict__ K,
+template <int P, int MODE, int UNR, int XS, int WPP = 1>
+__global__ __launch_bounds__(64 * WPP) void k_leapfrog(const double* __restrict__ K,
                                                  const int* __restrict__ row_ptr,
                                                  const int* __restrict__ col,
                                                  const double* __restrict__ Rp,
@@ -540,16 +540,19 @@ __global__ __launch_bounds__(64) void k_leapfrog(const double* __restrict__ K,
   constexpr int RPL = (N3 + 63) / 64;
   __shared__ double sw[N3];
   __shared__ double sR[D::NPK];
-  const int lane = threadIdx.x;
+  extern __shared__ double sblk[];  // WPP > 1: per-block products [block][row]
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int Pp = step_poly();
   if (Pp >= n_poly) return;  // padding workgroups of the last XCD share
   if (MODE != 2)
-    for (int k = lane; k < D::NPK; k += 64) sR[k] = Rp[(size_t)Pp * D::NPK + k];
+    for (int k = threadIdx.x; k < D::NPK; k += 64 * WPP) sR[k] = Rp[(size_t)Pp * D::NPK + k];
   double acc[RPL];
 #pragma unroll
   for (int k = 0; k < RPL; ++k) acc[k] = 0.0;
   const int b0 = row_ptr[Pp], b1 = row_ptr[Pp + 1];
-  for (int b = b0; b < b1; ++b) {
+  // WPP waves share a polyhedron's blocks round robin; each block's product
+  // goes to LDS and wave 0 adds them in block order, the reference's order
+  for (int b = b0 + wv; b < b1; b += WPP) {
     const int cb = __builtin_amdgcn_readfirstlane(col[b]);
     const double* __restrict__ xv = u + (size_t)cb * N3;
     const double* __restrict__ Kb = K + (size_t)b * D::BS;
@@ -604,18 +607,38 @@ __global__ __launch_bounds__(64) void k_leapfrog(const double* __restrict__ K,
         }
       }
     }
+    if constexpr (WPP == 1) {
 #pragma unroll
-    for (int k = 0; k < RPL; ++k) acc[k] = (b == b0) ? s[k] : acc[k] + s[k];
+      for (int k = 0; k < RPL; ++k) acc[k] = (b == b0) ? s[k] : acc[k] + s[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) {
+        const int r = lane + 64 * k;
+        if (r < N3) sblk[(size_t)(b - b0) * N3 + r] = s[k];
+      }
+    }
+  }
+  if constexpr (WPP > 1) {
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) {
+      const int r = lane + 64 * k;
+      if (r < N3 && wv == 0) {
+        double a = sblk[r];
+        for (int j = 1; j < b1 - b0; ++j) a = a + sblk[(size_t)j * N3 + r];
+        acc[k] = a;
+      }
+    }
   }
   const size_t o = (size_t)Pp * N3;
   if (MODE == 2) {
 #pragma unroll
     for (int k = 0; k < RPL; ++k) {
       const int r = lane + 64 * k;
-      if (r < N3) sw[r] = acc[k];
+      if (r < N3 && wv == 0) sw[r] = acc[k];
     }
     __syncthreads();
-    if (lane == 0) {
+    if (threadIdx.x == 0) {
       double el = 0.0;
       for (int r = 0; r < N3; ++r) el = el + u[o + r] * sw[r];
       out[Pp] = el;
@@ -628,15 +651,15 @@ __global__ __launch_bounds__(64) void k_leapfrog(const double* __restrict__ K,
 #pragma unroll
   for (int k = 0; k < RPL; ++k) {
     const int r = lane + 64 * k;
-    if (r < N3) sw[r] = -acc[k] + rs[o + r] + rd[o + r] * gp;
+    if (r < N3 && wv == 0) sw[r] = -acc[k] + rs[o + r] + rd[o + r] * gp;
   }
   __syncthreads();
-  if (lane < 3) solve_packed<NP>(sR, sw + lane * NP);
+  if (threadIdx.x < 3) solve_packed<NP>(sR, sw + threadIdx.x * NP);
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < RPL; ++k) {
     const int r = lane + 64 * k;
-    if (r < N3) {
+    if (r < N3 && wv == 0) {
       if (MODE == 0)
         out[o + r] = c1 * sw[r] + 2.0 * u[o + r] - wa[o + r];
       else
@@ -786,8 +809,8 @@ __global__ void k_tilde(int mode, const double* __restrict__ RM, const double* _
   }
 }
 
-template <int P, int MODE, int UNR>
-__global__ __launch_bounds__(64) void k_leapfrog32(const float* __restrict__ K,
+template <int P, int MODE, int UNR, int WPP = 1>
+__global__ __launch_bounds__(64 * WPP) void k_leapfrog32(const float* __restrict__ K,
                                                    const int* __restrict__ row_ptr,
                                                    const int* __restrict__ col,
                                                    const double* __restrict__ Rp,
@@ -802,14 +825,15 @@ __global__ __launch_bounds__(64) void k_leapfrog32(const float* __restrict__ K,
   using D = Dims<P>;
   constexpr int N3 = D::N3, NQUAD = Dims32<P>::NQUAD;
   constexpr int RPL = (N3 + 63) / 64;
-  const int lane = threadIdx.x;
+  extern __shared__ double sblk[];  // WPP > 1: per-block products [block][row]
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int Pp = step_poly();
   if (Pp >= n_poly) return;  // padding workgroups of the last XCD share
   double acc[RPL];
 #pragma unroll
   for (int k = 0; k < RPL; ++k) acc[k] = 0.0;
   const int b0 = row_ptr[Pp], b1 = row_ptr[Pp + 1];
-  for (int b = b0; b < b1; ++b) {
+  for (int b = b0 + wv; b < b1; b += WPP) {
     const int cb = __builtin_amdgcn_readfirstlane(col[b]);
     const double* __restrict__ xv = u + (size_t)cb * N3;
     const float* __restrict__ Kb = K + (size_t)b * Dims32<P>::BS;
@@ -860,8 +884,29 @@ __global__ __launch_bounds__(64) void k_leapfrog32(const float* __restrict__ K,
         }
       }
     }
+    if constexpr (WPP == 1) {
 #pragma unroll
-    for (int k = 0; k < RPL; ++k) acc[k] = (b == b0) ? (double)s[k] : acc[k] + (double)s[k];
+      for (int k = 0; k < RPL; ++k) acc[k] = (b == b0) ? (double)s[k] : acc[k] + (double)s[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) {
+        const int r = lane + 64 * k;
+        if (r < N3) sblk[(size_t)(b - b0) * N3 + r] = (double)s[k];
+      }
+    }
+  }
+  if constexpr (WPP > 1) {
+    __syncthreads();
+    if (wv != 0) return;  // no barrier follows in this kernel
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) {
+      const int r = lane + 64 * k;
+      if (r < N3) {
+        double a = sblk[r];
+        for (int j = 1; j < b1 - b0; ++j) a = a + sblk[(size_t)j * N3 + r];
+        acc[k] = a;
+      }
+    }
   }
   const size_t o = (size_t)Pp * N3;
   // time factor: one scalar, or per source group (ensembles whose instances

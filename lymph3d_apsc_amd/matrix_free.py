@@ -141,16 +141,31 @@ class Context:
 
     def autotune(self, reps=20):
         """Time the step-kernel variants and keep the fastest; returns
-        (chosen (group size, broadcast mode), {(group size, mode): ms/step})."""
+        (chosen (group size, broadcast mode, waves per polyhedron),
+        {(group size, mode, waves): ms/step}) -- ms -1 for variants not run."""
         nv = C.c_int()
         unr = np.zeros(16, np.int32)
         xm = np.zeros(16, np.int32)
+        wp = np.zeros(16, np.int32)
         ms = np.zeros(16)
         ch = C.c_int()
-        check(lib().lymph_autotune(self.h, reps, C.byref(nv), ptr(unr), ptr(xm), ptr(ms),
+        check(lib().lymph_autotune(self.h, reps, C.byref(nv), ptr(unr), ptr(xm), ptr(wp), ptr(ms),
                                    C.byref(ch)), "lymph_autotune")
-        table = {(int(unr[i]), int(xm[i])): float(ms[i]) for i in range(nv.value)}
-        return (int(unr[ch.value]), int(xm[ch.value])), table
+        table = {(int(unr[i]), int(xm[i]), int(wp[i])): float(ms[i]) for i in range(nv.value)}
+        c = ch.value
+        return (int(unr[c]), int(xm[c]), int(wp[c])), table
+
+    def set_step_variant(self, index):
+        """Select step-kernel variant `index` (numbered as in autotune);
+        returns the number of variants."""
+        nv = C.c_int()
+        check(lib().lymph_set_step_variant(self.h, int(index), C.byref(nv)), "lymph_set_step_variant")
+        return nv.value
+
+    def n_step_variants(self):
+        nv = C.c_int()
+        check(lib().lymph_set_step_variant(self.h, 0, C.byref(nv)), "lymph_set_step_variant")
+        return nv.value
 
     def set_apply_precision(self, bits):
         """64 (default) or 32: precision of the step's operator apply."""

@@ -326,6 +326,39 @@ def test_fp32_apply_on_ill_conditioned_agglomerates():
     assert np.linalg.norm(a32 - a64) < 1e-3 * np.linalg.norm(a64)
 
 
+@pytest.mark.parametrize("name,p", [("hetero", 2), ("grown4", 4)])
+def test_step_variants_agree_bitwise(name, p):
+    """Every step-kernel variant -- load group size, neighbour broadcast mode,
+    waves per polyhedron -- gives bit for bit the same step, for the 64-bit
+    apply (which matches the oracle) and for the 32-bit apply."""
+    say(f"[variants] {name} p={p}")
+    if name == "grown4":
+        m = M.cube_mesh(4, dirichlet_faces={"x0", "x1", "y0", "y1", "z0"}, poly_size=(4, 40), seed=3)
+    else:
+        m = make_mesh(name)
+    o = oracle_for(m, p)
+    o.build_operator()
+    rng = np.random.default_rng(6)
+    n = m.n_poly * 3 * o.Np
+    u0, u1 = rng.standard_normal(n), rng.standard_normal(n)
+    dt, g = 1e-3, 0.37
+    ref = o.step(g, dt, u0, u1, o.rhs(L.PROB_NULL), o.rhs(L.PROB_REF_MMS))
+    with L.Context(m, p) as ctx:
+        ctx.make_matrices_free()
+        ctx.make_rhs_free(L.PROB_NULL, _lib.SLOT_RHS_STAT)
+        ctx.make_rhs_free(L.PROB_REF_MMS, _lib.SLOT_RHS_DYN)
+        for bits in (64, 32):
+            ctx.set_apply_precision(bits)
+            outs = []
+            for v in range(ctx.n_step_variants()):
+                ctx.set_step_variant(v)
+                outs.append(ctx.time_step_matrix_free(g, dt, u0, u1))
+            for v, a in enumerate(outs[1:], 1):
+                assert np.array_equal(a, outs[0]), (bits, v)
+            if bits == 64:
+                assert rel(outs[0], ref) < 1e-13
+
+
 def test_memory_options_keep_results():
     """Dropping A_dg and releasing the FP64 blocks changes no result: the
     operator built without A_dg, the 32-bit apply after the release and a
