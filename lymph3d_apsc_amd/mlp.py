@@ -144,3 +144,23 @@ class Mlp:
         ms = C.c_double()
         _check(lib().lymph_mlp_time_predict(self.h, n, reps, C.byref(ms)), "lymph_mlp_time_predict")
         return ms.value
+
+
+if __name__ == "__main__":
+    # Training-step timing of the surrogate's network (widths 8-512-512-512-
+    # 38400, batch 256) on random data: python -m lymph3d_apsc_amd.mlp
+    import time
+    widths, batch = [8, 512, 512, 512, 38400], 256
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((1024, widths[0])).astype(np.float32)
+    y = rng.standard_normal((1024, widths[-1])).astype(np.float32)
+    with Mlp(widths, max_batch=batch, seed=0) as net:
+        net.set_data(x, y)
+        order = np.concatenate([rng.permutation(1024) for _ in range(10)]).astype(np.int32)[:batch * 40]
+        net.train(order[:batch * 4], batch, 1e-3)
+        t = time.perf_counter()
+        net.train(order, batch, 1e-3)
+        dt = time.perf_counter() - t
+        flops = 2.0 * batch * sum(k * n for k, n in zip(widths[:-1], widths[1:])) * 3 - 2.0 * batch * widths[0] * widths[1]
+        print(f"training step {1e3 * dt / 40:.3f} ms ({flops * 40 / dt / 1e12:.1f} TFLOP/s); "
+              f"forward of 1024 samples {net.time_predict(1024, 10):.3f} ms", flush=True)
