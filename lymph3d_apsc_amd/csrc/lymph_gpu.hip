@@ -988,16 +988,24 @@ int lymph_set_apply_precision(LymphCtx* c, int bits) {
   if (bits == 64 && !c->K)
     return fail(LYMPH_ERR_STATE, "FP64 K blocks were released: rebuild the operator");
   if (bits == 32 && !c->K32) {
-    const size_t n = (size_t)c->n_blk * Dims32Any(c->p);
-    if (int r = dalloc(&c->K32, n)) return r;
-    c->bytes += n * sizeof(float);
+    // vectors first: K32 exists only once everything the 32-bit path uses does
     for (double*& q : c->tv) {
       if (!q) {
         if (int r = dalloc(&q, vec_n(c))) return r;
         c->bytes += vec_n(c) * sizeof(double);
       }
     }
-    if (int r = build_ortho(c)) return r;
+    const size_t n = (size_t)c->n_blk * Dims32Any(c->p);
+    float* k32 = nullptr;
+    if (int r = dalloc(&k32, n)) return r;
+    c->K32 = k32;
+    c->bytes += n * sizeof(float);
+    if (int r = build_ortho(c)) {
+      (void)hipFree(c->K32);
+      c->K32 = nullptr;
+      c->bytes -= n * sizeof(float);
+      return r;
+    }
   }
   c->apply_bits = bits;
   c->step_variant = 0;
