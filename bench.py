@@ -146,6 +146,7 @@ def ensemble_leg(n_inst, steps, plain_sample, f0=5.0, seed0=1000):
         s0 = time.perf_counter()
         with L.Context(ins["mesh"], ins["p"]) as ctx:
             ctx.set_keep_adg(False)
+            ctx.set_build("factored")
             t = ctx.start_from_rest(dt, ins["source"], ins["force"], f0=f0)
             ctx.set_receivers(ins["receivers"], max_records=steps)
             ctx.sync()
@@ -213,6 +214,7 @@ def main():
     dt = 0.2 * h / (np.sqrt(3.0) * p * p)
     prob = L.PROB_CUBE_WAVE
     ctx = L.Context(m, p)
+    ctx.set_build("factored")  # same operator to rounding, ~5x faster set-up at p=3
     t0 = time.perf_counter()
     # operator, RHS, IC projections, first step and the warm-up steps
     r0 = ctx.run(prob, dt, 1e30, max_steps=1 + a.warmup)
@@ -267,7 +269,7 @@ def main():
         "autotune": {"chosen": list(chosen),
                      "ms_per_step": {f"unr{k[0]}_x{k[1]}_w{k[2]}": v for k, v in tune.items()}},
         "err_l2": e_l2, "err_dg": e_dg,
-        "setup_s": t_build,
+        "setup_s": t_build, "operator_build": "factored",
     }
     if not a.no_fp32:
         # 32-bit operator apply (mass-orthonormal basis; FP64 RHS, transforms, update): same start state

@@ -215,6 +215,17 @@ int lymph_set_apply_precision(LymphCtx* ctx, int bits);
  * is selected (lymph_set_apply_precision(ctx, 32)); 64-bit apply and K export
  * then return LYMPH_ERR_STATE until lymph_build_operator runs again. */
 int lymph_ctx_set_keep_adg(LymphCtx* ctx, int keep);
+
+/* How lymph_build_operator evaluates MAKE_MATRICES_FREE's sums:
+ * LYMPH_BUILD_REFERENCE (default) term by term in the reference's operand
+ * order (entries agree with it to the last ulps); LYMPH_BUILD_FACTORED as
+ * combinations of per-basis-pair quadrature sums (D_ab = sum w|J| d_a phi_m
+ * d_b phi_n, face sums of phi phi and d_a phi phi), about 10x fewer flops,
+ * entries equal to rounding (~1e-15 relative); mass blocks are identical in
+ * both.  No reference counterpart (a device-side evaluation choice). */
+#define LYMPH_BUILD_REFERENCE 0
+#define LYMPH_BUILD_FACTORED 1
+int lymph_ctx_set_build(LymphCtx* ctx, int mode);
 int lymph_release_fp64_operator(LymphCtx* ctx);
 
 /* Point forces f_k delta(x - x_k) added to a RHS slot (normally

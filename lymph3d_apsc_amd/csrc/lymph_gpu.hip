@@ -60,6 +60,7 @@ struct LymphCtx {
   int step_variant = 0;
   int apply_bits = 64;
   bool keep_adg = true;
+  int build_mode = LYMPH_BUILD_REFERENCE;
   double ricker_f0 = 5.0, ricker_t0 = 1.2 / 5.0;
   // per-group Ricker wavelets (ensembles): group of each polyhedron, f0/t0
   // per group, and the table of this step's factors
@@ -661,8 +662,12 @@ int lymph_build_operator(LymphCtx* c) {
     const int chunk = 1 << 16;
     for (int p0 = 0; p0 < c->n_poly; p0 += chunk) {
       const int n = std::min(chunk, c->n_poly - p0);
-      k_build_operator<P><<<n, 256, 0, c->stream>>>(c->md, c->alpha, c->theta, c->K, c->A, c->M,
-                                                   c->Mm, p0);
+      if (c->build_mode == LYMPH_BUILD_FACTORED)
+        k_build_factored<P><<<n, 256, 0, c->stream>>>(c->md, c->alpha, c->theta, c->K, c->A, c->M,
+                                                      c->Mm, p0);
+      else
+        k_build_operator<P><<<n, 256, 0, c->stream>>>(c->md, c->alpha, c->theta, c->K, c->A, c->M,
+                                                     c->Mm, p0);
       if (int r = check_launch("k_build_operator")) return r;
     }
     k_cholesky<P><<<grid1(c->n_poly, 64), 64, 0, c->stream>>>(c->M, c->Mm, c->RM, c->RMm,
@@ -678,6 +683,14 @@ int lymph_build_operator(LymphCtx* c) {
   if (c->K32)  // keep an existing 32-bit operator in step with the new blocks
     if (int r = build_ortho(c)) return r;
   c->built = true;
+  return 0;
+}
+
+int lymph_ctx_set_build(LymphCtx* c, int mode) {
+  if (int rc = need_ctx(c)) return rc;
+  if (mode != LYMPH_BUILD_REFERENCE && mode != LYMPH_BUILD_FACTORED)
+    return fail(LYMPH_ERR_ARG, "unknown build mode");
+  c->build_mode = mode;
   return 0;
 }
 

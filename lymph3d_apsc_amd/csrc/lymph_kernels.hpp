@@ -268,6 +268,229 @@ __global__ __launch_bounds__(256) void k_build_operator(MeshDev m, double alpha,
   }
 }
 
+// Operator build, factored form (same operator, fewer flops).  The face and
+// volume terms of MAKE_STIFFNESS_VOLUME / MAKE_STIFFNESS_FACE are bilinear in
+// the basis values, so each is a combination of a few quadrature sums per
+// basis pair (m, n):
+//   volume:  D_ab = sum_q w|J| d_a phi_m d_b phi_n  (a, b = x, y, z)
+//     V_ii = (l+2m) D_ii + m sum_{a != i} D_aa,  V_ij = l D_ij + m D_ji;
+//   face:    PP = sum w phiP_m phiP_n,  PQ = sum w phiP_m phiQ_n,
+//            BP_a = sum w d_a phiP_m phiP_n,  BPt_a = sum w d_a phiP_n phiP_m,
+//            CP_a = sum w d_a phiP_m phiQ_n,  DQ_a = sum w d_a phiQ_n phiP_m,
+//     with T(i,j;X) = i==j ? (l+2m) n_i X_i + m sum_{a != i} n_a X_a
+//                          : l n_j X_i + m n_i X_j,
+//     S+ = s ar PP c_ij, I+ = k ar T(i,j;BP), I+^T = k ar T(j,i;BPt),
+//     I- = -ar/2 T(i,j;CP), I-^T = ar/2 T(i,j;DQ), S- = -s ar PQ c_ij,
+//     c_ii = dd_i, c_ij = n_i n_j / 2, k = 1 (Dirichlet) or 1/2 (interior);
+// inserted like k_build_operator (per tet: volume, then faces 1..4).  Each
+// thread owns basis pairs (m, n) for the whole polyhedron and keeps its
+// self-block entries in registers, so every entry accumulates in one fixed
+// order; sums are regrouped relative to the reference, so entries agree with
+// it to rounding (1e-15 relative), not bit for bit.  The mass blocks are
+// summed exactly as k_build_operator does.
+template <int P>
+__global__ __launch_bounds__(256) void k_build_factored(MeshDev m, double alpha, double theta,
+                                                         double* __restrict__ K,
+                                                         double* __restrict__ A,
+                                                         double* __restrict__ M,
+                                                         double* __restrict__ Mm, int poly0) {
+  using D = Dims<P>;
+  constexpr int NP = D::NP, N3 = D::N3, NQ3 = D::NQ3, NQ2 = D::NQ2;
+  constexpr int VOLSZ = 4 * NP * NQ3, FACESZ = 8 * NP * NQ2;
+  constexpr int SHSZ = VOLSZ > FACESZ ? VOLSZ : FACESZ;
+  constexpr int PAIRS = NP * NP;
+  constexpr int PPT = (PAIRS + 255) / 256;  // pairs per thread
+  __shared__ double sh[SHSZ];
+  __shared__ double spt[NQ3 * 3];
+  __shared__ double swq[NQ3];
+  __shared__ int sbl[NP * 3];
+  const int tid = threadIdx.x;
+  const int Pp = poly0 + blockIdx.x;
+  const bool keepA = A != nullptr;
+  for (int i = tid; i < NP * 3; i += blockDim.x) sbl[i] = m.blist[i];
+  double bbP[6];
+  for (int d = 0; d < 6; ++d) bbP[d] = m.bbox[Pp * 6 + d];
+  const double hkP = m.hk[Pp];
+  double* Ks = K + (size_t)m.row_ptr[Pp] * D::BS;
+  double* As = keepA ? A + (size_t)m.row_ptr[Pp] * D::BS : nullptr;
+  double* phi = sh;               // [NP][NQ3]
+  double* dphi = sh + NP * NQ3;   // [3][NP][NQ3]
+  double* p1 = sh;                // [NP][NQ2]
+  double* p2 = sh + NP * NQ2;
+  double* g1 = sh + 2 * NP * NQ2; // [3][NP][NQ2]
+  double* g2 = sh + 5 * NP * NQ2;
+  double* MP = M + (size_t)Pp * NP * NP;
+  double* MmP = Mm + (size_t)Pp * NP * NP;
+  for (int a = m.ptet_ptr[Pp]; a < m.ptet_ptr[Pp + 1]; ++a) {
+    const int t = m.ptet[a];
+    double v[12], Fk[12], Jdet;
+    for (int i = 0; i < 12; ++i) v[i] = m.tet_xyz[(size_t)t * 12 + i];
+    affine_map(v, Fk, Jdet);
+    const double aJ = fabs(Jdet);
+    const double rho = m.rho[t], lam = m.lam[t], mu = m.mu[t];
+    const double l2m = lam + 2 * mu;
+    __syncthreads();
+    for (int q = tid; q < NQ3; q += blockDim.x) {
+      map_point(Fk, &m.qtet[q * 4], &spt[q * 3]);
+      swq[q] = m.wtet[q] * aJ;
+    }
+    __syncthreads();
+    for (int e = tid; e < NP * NQ3; e += blockDim.x) {
+      const int f = e / NQ3, q = e % NQ3;
+      double ph, d0, d1, d2;
+      basis_at(&sbl[f * 3], bbP, &spt[q * 3], ph, d0, d1, d2);
+      phi[e] = ph;
+      dphi[(0 * NP + f) * NQ3 + q] = d0;
+      dphi[(1 * NP + f) * NQ3 + q] = d1;
+      dphi[(2 * NP + f) * NQ3 + q] = d2;
+    }
+    __syncthreads();
+    constexpr int SD = NP * NQ3;
+#pragma unroll 1
+    for (int k = 0; k < PPT; ++k) {
+      const int pr = tid + 256 * k;
+      if (pr >= PAIRS) continue;
+      const int mm = pr / NP, nn = pr % NP;
+      double sm = 0.0, Dv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll 2
+      for (int q = 0; q < NQ3; ++q) {
+        // mass exactly as k_build_operator sums it
+        sm = sm + 1.0 * m.wtet[q] * aJ * phi[mm * NQ3 + q] * phi[nn * NQ3 + q];
+        const double wq = swq[q];
+        const double gm0 = dphi[mm * NQ3 + q], gm1 = dphi[SD + mm * NQ3 + q],
+                     gm2 = dphi[2 * SD + mm * NQ3 + q];
+        const double gn0 = wq * dphi[nn * NQ3 + q], gn1 = wq * dphi[SD + nn * NQ3 + q],
+                     gn2 = wq * dphi[2 * SD + nn * NQ3 + q];
+        Dv[0][0] += gm0 * gn0; Dv[0][1] += gm0 * gn1; Dv[0][2] += gm0 * gn2;
+        Dv[1][0] += gm1 * gn0; Dv[1][1] += gm1 * gn1; Dv[1][2] += gm1 * gn2;
+        Dv[2][0] += gm2 * gn0; Dv[2][1] += gm2 * gn1; Dv[2][2] += gm2 * gn2;
+      }
+      MmP[pr] = MmP[pr] + sm;
+      MP[pr] = MP[pr] + sm * rho;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          double val;
+          if (i == j)
+            val = l2m * Dv[i][i] + mu * Dv[(i + 1) % 3][(i + 1) % 3] + mu * Dv[(i + 2) % 3][(i + 2) % 3];
+          else
+            val = lam * Dv[i][j] + mu * Dv[j][i];
+          const size_t o = kpos(N3, i * NP + mm, j * NP + nn);
+          Ks[o] = Ks[o] + val;
+          if (keepA) As[o] = As[o] + val;
+        }
+    }
+#pragma unroll 1
+    for (int f = 0; f < 4; ++f) {
+      const int e2 = m.neigh[t * 4 + f];
+      int Q = -1;
+      if (e2 >= 0) {
+        Q = m.tet_poly[e2];
+        if (Q == Pp) continue;  // intra-polyhedron face
+      }
+      if (e2 == -2) continue;  // Neumann: no stiffness contribution
+      __syncthreads();
+      for (int q = tid; q < NQ2; q += blockDim.x) face_point(Fk, f, &m.qtri[q * 4], &spt[q * 3]);
+      __syncthreads();
+      double bbQ[6];
+      if (Q >= 0)
+        for (int d = 0; d < 6; ++d) bbQ[d] = m.bbox[Q * 6 + d];
+      for (int e = tid; e < NP * NQ2; e += blockDim.x) {
+        const int fb = e / NQ2, q = e % NQ2;
+        double ph, d0, d1, d2;
+        basis_at(&sbl[fb * 3], bbP, &spt[q * 3], ph, d0, d1, d2);
+        p1[e] = ph;
+        g1[(0 * NP + fb) * NQ2 + q] = d0;
+        g1[(1 * NP + fb) * NQ2 + q] = d1;
+        g1[(2 * NP + fb) * NQ2 + q] = d2;
+        if (Q >= 0) {
+          basis_at(&sbl[fb * 3], bbQ, &spt[q * 3], ph, d0, d1, d2);
+          p2[e] = ph;
+          g2[(0 * NP + fb) * NQ2 + q] = d0;
+          g2[(1 * NP + fb) * NQ2 + q] = d1;
+          g2[(2 * NP + fb) * NQ2 + q] = d2;
+        }
+      }
+      __syncthreads();
+      const double* nrm = &m.normal[(t * 4 + f) * 3];
+      const double n1 = nrm[0], n2 = nrm[1], n3 = nrm[2];
+      const double nv[3] = {n1, n2, n3};
+      const double ar = m.area[t * 4 + f];
+      const double Dbar = lam + 2 * mu;
+      const double hmin = (Q >= 0) ? (hkP < m.hk[Q] ? hkP : m.hk[Q]) : hkP;
+      const double sigma = alpha * (P * P) / hmin * Dbar;
+      const double d11 = fabs(n1) * fabs(n1) + 0.5 * (fabs(n2) * fabs(n2)) + 0.5 * (fabs(n3) * fabs(n3));
+      const double d22 = fabs(n2) * fabs(n2) + 0.5 * (fabs(n1) * fabs(n1)) + 0.5 * (fabs(n3) * fabs(n3));
+      const double d33 = fabs(n3) * fabs(n3) + 0.5 * (fabs(n1) * fabs(n1)) + 0.5 * (fabs(n2) * fabs(n2));
+      const double dd[3] = {d11, d22, d33};
+      const bool interior = Q >= 0;
+      const double kk = interior ? 0.5 : 1.0;
+      double* Kb = interior ? K + (size_t)m.face_blk[t * 4 + f] * D::BS : nullptr;
+      double* Ab = interior && keepA ? A + (size_t)m.face_blk[t * 4 + f] * D::BS : nullptr;
+      constexpr int SG = NP * NQ2;
+      // T(i,j;X): the traction pattern of itemp
+      auto Tij = [&](int i, int j, const double* X) -> double {
+        if (i == j) return l2m * nv[i] * X[i] + mu * nv[(i + 1) % 3] * X[(i + 1) % 3] +
+                           mu * nv[(i + 2) % 3] * X[(i + 2) % 3];
+        return lam * nv[j] * X[i] + mu * nv[i] * X[j];
+      };
+#pragma unroll 1
+      for (int k = 0; k < PPT; ++k) {
+        const int pr = tid + 256 * k;
+        if (pr >= PAIRS) continue;
+        const int mm = pr / NP, nn = pr % NP;
+        double PPs = 0.0, PQs = 0.0, BP[3] = {0, 0, 0}, BPt[3] = {0, 0, 0}, CP[3] = {0, 0, 0},
+               DQ[3] = {0, 0, 0};
+#pragma unroll 2
+        for (int q = 0; q < NQ2; ++q) {
+          const double w = m.wtri[q];
+          const double pm = p1[mm * NQ2 + q], pn = p1[nn * NQ2 + q];
+          const double wpm = w * pm, wpn = w * pn;
+          PPs += wpm * pn;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            BP[d] += g1[d * SG + mm * NQ2 + q] * wpn;
+            BPt[d] += g1[d * SG + nn * NQ2 + q] * wpm;
+          }
+          if (interior) {
+            const double qn = p2[nn * NQ2 + q];
+            const double wqn = w * qn;
+            PQs += wpm * qn;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              CP[d] += g1[d * SG + mm * NQ2 + q] * wqn;
+              DQ[d] += g2[d * SG + nn * NQ2 + q] * wpm;
+            }
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            const double cij = (i == j) ? dd[i] : 0.5 * nv[i] * nv[j];
+            const double s1 = sigma * ar * PPs * cij;
+            const double i1 = kk * ar * Tij(i, j, BP);
+            const double i1t = kk * ar * Tij(j, i, BPt);
+            const size_t o = kpos(N3, i * NP + mm, j * NP + nn);
+            Ks[o] = Ks[o] + s1;
+            if (keepA) As[o] = As[o] + s1;
+            Ks[o] = Ks[o] + theta * i1 - i1t;
+            if (interior) {
+              const double i2 = -0.5 * ar * Tij(i, j, CP);
+              const double it2 = 0.5 * ar * Tij(i, j, DQ);
+              const double s2 = -(sigma * ar * PQs * cij);
+              Kb[o] = Kb[o] + theta * i2 + s2;
+              if (keepA) Ab[o] = Ab[o] + s2;
+              Kb[o] = Kb[o] - it2;
+            }
+          }
+      }
+    }
+  }
+
+}
+
 // Cholesky of both mass blocks of a polyhedron (cholesky,
 // src/vet_mat_operations.f90:84-120), one thread per polyhedron; writes the
 // packed upper factors.  status[0] set to 1 if a block is not SPD.

@@ -121,6 +121,7 @@ class Ensemble:
             um = union(meshes)
             ctx = Context(um, p)
             ctx.set_keep_adg(False)  # traces only: no DG-norm error, half the build memory
+            ctx.set_build("factored")
             ctx.make_matrices_free()
             if apply_bits == 32:
                 ctx.set_apply_precision(32)

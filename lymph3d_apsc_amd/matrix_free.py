@@ -177,6 +177,13 @@ class Context:
         is then unavailable.  Saves one operator's worth of device memory."""
         check(lib().lymph_ctx_set_keep_adg(self.h, int(bool(keep))), "lymph_ctx_set_keep_adg")
 
+    def set_build(self, mode):
+        """"reference" (default: the reference's operand order) or "factored"
+        (per-basis-pair quadrature sums, ~10x fewer flops, same operator to
+        rounding) for the next make_matrices_free."""
+        code = {"reference": 0, "factored": 1}[mode]
+        check(lib().lymph_ctx_set_build(self.h, code), "lymph_ctx_set_build")
+
     def release_fp64_operator(self):
         """Free the FP64 K blocks once the 32-bit apply is selected."""
         check(lib().lymph_release_fp64_operator(self.h), "lymph_release_fp64_operator")

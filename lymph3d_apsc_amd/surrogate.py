@@ -182,6 +182,7 @@ def run(n_train=4096, n_eval=512, n=16, p=2, steps=1000, every=5, n_receivers=64
         ins = instance(th_eval[k], n, p, rec)
         with Context(ins["mesh"], p) as ctx:
             ctx.set_keep_adg(False)
+            ctx.set_build("factored")
             t = ctx.start_from_rest(dt, ins["source"], ins["force"], f0=ins["f0"])
             ctx.set_receivers(rec, max_records=steps // every)
             ctx.sync()

@@ -120,6 +120,29 @@ def test_operator_rhs_projection(name, p):
             assert (den == 0 and num == 0) or np.sqrt(num / den) < 1e-13, (fn, num, den)
 
 
+@pytest.mark.parametrize("name,p", [("tet_25", 2), ("cube3agg", 3), ("hetero", 2), ("cube2agg", 4),
+                                    ("tet_5", 1)])
+def test_factored_build_matches_oracle(name, p):
+    """The factored operator build (per-basis-pair quadrature sums) gives the
+    oracle's K and A_dg to 1e-13 of each block's largest entry and the
+    reference-order build's mass blocks bit for bit."""
+    say(f"[factored build] {name} p={p}")
+    m = make_mesh(name)
+    o = oracle_for(m, p)
+    o.build_operator()
+    oK, oA = o.K(), o.Adg()
+    with L.Context(m, p) as ref, L.Context(m, p) as fac:
+        ref.make_matrices_free()
+        fac.set_build("factored")
+        fac.make_matrices_free()
+        _, _, Mr, Mmr = ref.export_operator()
+        K, A, Mf, Mmf = fac.export_operator()
+        for b in range(K.shape[0]):
+            assert rel(K[b], oK[b]) < 1e-13, (b, rel(K[b], oK[b]))
+            assert rel(A[b], oA[b]) < 1e-13
+        assert np.array_equal(Mf, Mr) and np.array_equal(Mmf, Mmr)
+
+
 @pytest.mark.parametrize("name,p", [("tet_25", 2), ("cube3agg", 3), ("hetero", 2), ("cube2agg", 4)])
 def test_steps_match_oracle(name, p):
     say(f"[steps] {name} p={p}")
@@ -170,13 +193,16 @@ def test_steps_match_oracle(name, p):
 
 
 @pytest.mark.parametrize("name", ["tet_5", "tet_25", "tet_125", "tet_100"])
-def test_reference_known_answers(name):
+@pytest.mark.parametrize("build", ["reference", "factored"])
+def test_reference_known_answers(name, build):
     """Full runs of the reference's decks on the GPU: printed errors vs the
-    reference's known answers and the final vector vs the oracle's."""
-    say(f"[known answer] {name}")
+    reference's known answers and the final vector vs the oracle's, with
+    either operator build."""
+    say(f"[known answer] {name} {build}")
     ka = KA[name]
     m = M.read_deck(os.path.join(DECKS, name))
     with L.Context(m, m.degree) as ctx:
+        ctx.set_build(build)
         r = ctx.run(L.PROB_REF_MMS, m.dt, m.stop_time, want_u=True)
     assert r["nsteps"] == ka["steps"]
     tol = 5e-12 if name == "tet_100" else 5e-13
