@@ -196,8 +196,9 @@ def test_steps_match_oracle(name, p):
 @pytest.mark.parametrize("build", ["reference", "factored"])
 def test_reference_known_answers(name, build):
     """Full runs of the reference's decks on the GPU: printed errors vs the
-    reference's known answers and the final vector vs the oracle's, with
-    either operator build."""
+    reference's known answers (5e-13, 5e-12 for the 3183-step p=3 deck; four
+    times that with the factored build) and the final vector vs the oracle's
+    (1e-11), with either operator build."""
     say(f"[known answer] {name} {build}")
     ka = KA[name]
     m = M.read_deck(os.path.join(DECKS, name))
@@ -205,7 +206,11 @@ def test_reference_known_answers(name, build):
         ctx.set_build(build)
         r = ctx.run(L.PROB_REF_MMS, m.dt, m.stop_time, want_u=True)
     assert r["nsteps"] == ka["steps"]
+    # the printed errors are small differences (u - Pi_h u_ex); the factored
+    # build regroups the operator's sums, which moves them by a few 1e-12
     tol = 5e-12 if name == "tet_100" else 5e-13
+    if build == "factored":
+        tol *= 4
     assert abs(r["err_l2"] / ka["err_l2"] - 1) < tol, r
     assert abs(r["err_dg"] / ka["err_dg"] - 1) < tol, r
     o = oracle_for(m, m.degree)
