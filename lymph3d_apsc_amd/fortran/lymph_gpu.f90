@@ -22,6 +22,8 @@ module lymph_gpu
   integer(c_int), parameter :: LYMPH_SLOT_RHS_STAT = 0, LYMPH_SLOT_RHS_DYN = 1
   integer(c_int), parameter :: LYMPH_SLOT_U_PREV = 2, LYMPH_SLOT_U_CUR = 3, LYMPH_SLOT_U_NEXT = 4
   integer(c_int), parameter :: LYMPH_SLOT_V0 = 5, LYMPH_SLOT_UEX = 6
+  integer(c_int), parameter :: LYMPH_PROB_RICKER = 3
+  integer(c_int), parameter :: LYMPH_BUILD_REFERENCE = 0, LYMPH_BUILD_FACTORED = 1
 
   ! struct LymphMesh
   type, bind(C) :: lymph_mesh
@@ -115,6 +117,206 @@ module lymph_gpu
       real(c_double), value :: dt, stop_time
       real(c_double) :: u_final(*)
       real(c_double), intent(out) :: out(4)
+      integer(c_int) :: rc
+    end function
+    function lymph_version() bind(C, name='lymph_version') result(s)
+      import :: c_ptr
+      type(c_ptr) :: s
+    end function
+    function lymph_ctx_info(ctx, info) bind(C, name='lymph_ctx_info') result(rc)
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), intent(out) :: info(8)
+      integer(c_int) :: rc
+    end function
+    function lymph_ctx_set_stream(ctx, stream) bind(C, name='lymph_ctx_set_stream') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, stream
+      integer(c_int) :: rc
+    end function
+    function lymph_ctx_get_stream(ctx) bind(C, name='lymph_ctx_get_stream') result(s)
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr) :: s
+    end function
+    function lymph_get_pattern(ctx, row_ptr, col) bind(C, name='lymph_get_pattern') result(rc)
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: row_ptr(*), col(*)
+      integer(c_int) :: rc
+    end function
+    function lymph_export_operator(ctx, K, A_dg, M, M_modal) bind(C, name='lymph_export_operator') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, K, A_dg, M, M_modal   ! c_loc of the host arrays, or c_null_ptr
+      integer(c_int) :: rc
+    end function
+    function lymph_set_slot(ctx, slot, host) bind(C, name='lymph_set_slot') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: slot
+      real(c_double), intent(in) :: host(*)
+      integer(c_int) :: rc
+    end function
+    function lymph_get_slot(ctx, slot, host) bind(C, name='lymph_get_slot') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: slot
+      real(c_double), intent(out) :: host(*)
+      integer(c_int) :: rc
+    end function
+    function lymph_slot_ptr(ctx, slot) bind(C, name='lymph_slot_ptr') result(d)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: slot
+      type(c_ptr) :: d
+    end function
+    function lymph_first_step_dev(ctx, g0, dt, u0, v0, u1) bind(C, name='lymph_first_step_dev') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, u0, v0, u1   ! device pointers (lymph_slot_ptr)
+      real(c_double), value :: g0, dt
+      integer(c_int) :: rc
+    end function
+    function lymph_step_dev(ctx, g, dt, um1, u, up1) bind(C, name='lymph_step_dev') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, um1, u, up1   ! device pointers (lymph_slot_ptr)
+      real(c_double), value :: g, dt
+      integer(c_int) :: rc
+    end function
+    function lymph_step_n(ctx, problem, nsteps, t0, dt, t_out) bind(C, name='lymph_step_n') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: problem, nsteps
+      real(c_double), value :: t0, dt
+      real(c_double), intent(out) :: t_out
+      integer(c_int) :: rc
+    end function
+    function lymph_time_steps(ctx, problem, nsteps, t0, dt, ms) bind(C, name='lymph_time_steps') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: problem, nsteps
+      real(c_double), value :: t0, dt
+      real(c_double), intent(out) :: ms
+      integer(c_int) :: rc
+    end function
+    function lymph_autotune(ctx, reps, n_variants, unr, xmode, wpp, ms, chosen) &
+        bind(C, name='lymph_autotune') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: reps
+      integer(c_int), intent(out) :: n_variants, unr(16), xmode(16), wpp(16), chosen
+      real(c_double), intent(out) :: ms(16)
+      integer(c_int) :: rc
+    end function
+    function lymph_set_step_variant(ctx, idx, n_variants) bind(C, name='lymph_set_step_variant') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: idx
+      integer(c_int), intent(out) :: n_variants
+      integer(c_int) :: rc
+    end function
+    function lymph_set_apply_precision(ctx, bits) bind(C, name='lymph_set_apply_precision') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: bits
+      integer(c_int) :: rc
+    end function
+    function lymph_ctx_set_keep_adg(ctx, keep) bind(C, name='lymph_ctx_set_keep_adg') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: keep
+      integer(c_int) :: rc
+    end function
+    function lymph_ctx_set_build(ctx, mode) bind(C, name='lymph_ctx_set_build') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: mode
+      integer(c_int) :: rc
+    end function
+    function lymph_release_fp64_operator(ctx) bind(C, name='lymph_release_fp64_operator') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    function lymph_add_point_sources(ctx, n, xyz, tet, force, slot) &
+        bind(C, name='lymph_add_point_sources') result(rc)
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n, slot
+      real(c_double), intent(in) :: xyz(*), force(*)
+      integer(c_int32_t), intent(in) :: tet(*)
+      integer(c_int) :: rc
+    end function
+    function lymph_set_ricker(ctx, f0, t0) bind(C, name='lymph_set_ricker') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: f0, t0
+      integer(c_int) :: rc
+    end function
+    function lymph_ricker(f0, t0, t) bind(C, name='lymph_ricker') result(g)
+      import :: c_double
+      real(c_double), value :: f0, t0, t
+      real(c_double) :: g
+    end function
+    function lymph_set_source_groups(ctx, n_groups, poly_group, f0, t0) &
+        bind(C, name='lymph_set_source_groups') result(rc)
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n_groups
+      integer(c_int32_t), intent(in) :: poly_group(*)
+      real(c_double), intent(in) :: f0(*), t0(*)
+      integer(c_int) :: rc
+    end function
+    function lymph_set_receivers(ctx, n, xyz, tet, max_records) bind(C, name='lymph_set_receivers') result(rc)
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n, max_records
+      real(c_double), intent(in) :: xyz(*)
+      integer(c_int32_t), intent(in) :: tet(*)
+      integer(c_int) :: rc
+    end function
+    function lymph_sample_receivers(ctx, slot, out) bind(C, name='lymph_sample_receivers') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: slot
+      real(c_double), intent(out) :: out(*)
+      integer(c_int) :: rc
+    end function
+    function lymph_step_n_record(ctx, problem, nsteps, t0, dt, every, t_out) &
+        bind(C, name='lymph_step_n_record') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: problem, nsteps, every
+      real(c_double), value :: t0, dt
+      real(c_double), intent(out) :: t_out
+      integer(c_int) :: rc
+    end function
+    function lymph_get_traces(ctx, out, n_records) bind(C, name='lymph_get_traces') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, out   ! c_loc of the host array, or c_null_ptr for the count
+      integer(c_int), intent(out) :: n_records
+      integer(c_int) :: rc
+    end function
+    function lymph_capture_solve(ctx, problem, dt, nsteps, every) bind(C, name='lymph_capture_solve') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: problem, nsteps, every
+      real(c_double), value :: dt
+      integer(c_int) :: rc
+    end function
+    function lymph_replay(ctx) bind(C, name='lymph_replay') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    function lymph_sync(ctx) bind(C, name='lymph_sync') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    function lymph_error_slots(ctx, slot_uh, slot_uex, l2sq, dgsq) bind(C, name='lymph_error_slots') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, l2sq, dgsq   ! c_loc of reals, or c_null_ptr
+      integer(c_int), value :: slot_uh, slot_uex
       integer(c_int) :: rc
     end function
     function lymph_time_function(problem, t) bind(C, name='lymph_time_function') result(g)

@@ -84,3 +84,14 @@ def test_fortran_dropin_links_and_fails_loudly_without_gpu(tmp_path):
     r = subprocess.run([exe, str(f)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 3, r.stdout + r.stderr
     assert "no HIP device" in r.stdout
+
+
+def test_fortran_module_binds_every_entry_point():
+    """lymph3d_apsc_amd/fortran/lymph_gpu.f90 declares an interface, bound by
+    name, for every function of include/lymph_gpu.h."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    hdr = open(os.path.join(root, "include", "lymph_gpu.h")).read()
+    declared = set(re.findall(r"^[A-Za-z_][A-Za-z0-9_ ]*\*?\s*\b(lymph_\w+)\s*\(", hdr, re.M))
+    f90 = open(os.path.join(root, "lymph3d_apsc_amd", "fortran", "lymph_gpu.f90")).read()
+    bound = set(re.findall(r"bind\(C, name='(lymph_\w+)'\)", f90))
+    assert declared and declared <= bound, sorted(declared - bound)
