@@ -118,18 +118,19 @@ int dispatch_p(int p, F&& f) {
 // neighbour-coefficient broadcast mode); index 0 is the default until
 // lymph_autotune picks one.
 // Step-kernel variants per degree: column pairs per unrolled load group,
-// neighbour-coefficient broadcast mode, waves per polyhedron; index 0 is the
-// default until lymph_autotune picks one.
+// neighbour-coefficient broadcast mode (0 scalar loads, 1 lane broadcast,
+// 2 packed: 64/3Np polyhedra per wave, coefficients through LDS), waves per
+// polyhedron; index 0 is the default until lymph_autotune picks one.
 template <int P> struct StepVar;
 template <> struct StepVar<1> {
-  static constexpr int unr[] = {6, 6, 3, 2, 6};
-  static constexpr int xs[] = {1, 0, 1, 1, 1};
-  static constexpr int wpp[] = {1, 1, 1, 1, 2};
+  static constexpr int unr[] = {6, 6, 3, 2, 6, 1};
+  static constexpr int xs[] = {1, 0, 1, 1, 1, 2};
+  static constexpr int wpp[] = {1, 1, 1, 1, 2, 1};
 };
 template <> struct StepVar<2> {
-  static constexpr int unr[] = {5, 5, 15, 3, 15, 5, 5};
-  static constexpr int xs[] = {1, 0, 1, 1, 0, 1, 1};
-  static constexpr int wpp[] = {1, 1, 1, 1, 1, 2, 4};
+  static constexpr int unr[] = {5, 5, 15, 3, 15, 5, 5, 1};
+  static constexpr int xs[] = {1, 0, 1, 1, 0, 1, 1, 2};
+  static constexpr int wpp[] = {1, 1, 1, 1, 1, 2, 4, 1};
 };
 template <> struct StepVar<3> {
   static constexpr int unr[] = {10, 10, 5, 6, 15, 30, 30, 3, 30, 10};
@@ -155,6 +156,12 @@ void launch_leapfrog(int variant, int grid, hipStream_t st, const double* K, con
   if constexpr (I < n_var<P>()) {
     if (variant == I) {
       constexpr int W = StepVar<P>::wpp[I];
+      if constexpr (StepVar<P>::xs[I] == 2) {
+        constexpr int G = 64 / (3 * np_of(P));
+        k_leapfrog_packed<P, MODE, 64><<<step_grid((grid + G - 1) / G), 64, 0, st>>>(
+            K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid);
+        return;
+      } else {
       if (W > 1 && dyn > kMaxDynLds) {
         launch_leapfrog<P, MODE, 0>(0, grid, st, K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2,
                                     gsel, gtab, dyn);
@@ -164,6 +171,7 @@ void launch_leapfrog(int variant, int grid, hipStream_t st, const double* K, con
           <<<step_grid(grid), 64 * W, W > 1 ? dyn : 0, st>>>(K, rp, col, Rp, rs, rd, u, wa, out, g,
                                                              c1, c2, gsel, gtab, grid);
       return;
+      }
     }
     launch_leapfrog<P, MODE, I + 1>(variant, grid, st, K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2,
                                     gsel, gtab, dyn);
@@ -172,20 +180,24 @@ void launch_leapfrog(int variant, int grid, hipStream_t st, const double* K, con
 
 template <int P> struct StepVar32;
 template <> struct StepVar32<1> {
-  static constexpr int unr[] = {3, 1, 3};
-  static constexpr int wpp[] = {1, 1, 2};
+  static constexpr int unr[] = {3, 1, 3, 1};
+  static constexpr int wpp[] = {1, 1, 2, 1};
+  static constexpr int pk[] = {0, 0, 0, 1};
 };
 template <> struct StepVar32<2> {
-  static constexpr int unr[] = {4, 2, 8, 4, 4};
-  static constexpr int wpp[] = {1, 1, 1, 2, 4};
+  static constexpr int unr[] = {4, 2, 8, 4, 4, 1};
+  static constexpr int wpp[] = {1, 1, 1, 2, 4, 1};
+  static constexpr int pk[] = {0, 0, 0, 0, 0, 1};
 };
 template <> struct StepVar32<3> {
   static constexpr int unr[] = {5, 3, 15, 5, 5};
   static constexpr int wpp[] = {1, 1, 1, 2, 4};
+  static constexpr int pk[] = {0, 0, 0, 0, 0};
 };
 template <> struct StepVar32<4> {
   static constexpr int unr[] = {3, 1, 9, 3, 3};
   static constexpr int wpp[] = {1, 1, 1, 2, 4};
+  static constexpr int pk[] = {0, 0, 0, 0, 0};
 };
 template <int P> constexpr int n_var32() { return (int)(sizeof(StepVar32<P>::unr) / sizeof(int)); }
 
@@ -197,6 +209,12 @@ void launch_leapfrog32(int variant, int grid, hipStream_t st, const float* K, co
   if constexpr (I < n_var32<P>()) {
     if (variant == I) {
       constexpr int W = StepVar32<P>::wpp[I];
+      if constexpr (StepVar32<P>::pk[I] == 1) {
+        constexpr int G = 64 / (3 * np_of(P));
+        k_leapfrog_packed<P, MODE, 32><<<step_grid((grid + G - 1) / G), 64, 0, st>>>(
+            K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid);
+        return;
+      } else {
       if (W > 1 && dyn > kMaxDynLds) {
         launch_leapfrog32<P, MODE, 0>(0, grid, st, K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2,
                                       gsel, gtab, dyn);
@@ -206,6 +224,7 @@ void launch_leapfrog32(int variant, int grid, hipStream_t st, const float* K, co
           <<<step_grid(grid), 64 * W, W > 1 ? dyn : 0, st>>>(K, rp, col, Rp, rs, rd, u, wa, out, g,
                                                              c1, c2, gsel, gtab, grid);
       return;
+      }
     }
     launch_leapfrog32<P, MODE, I + 1>(variant, grid, st, K, rp, col, Rp, rs, rd, u, wa, out, g, c1,
                                       c2, gsel, gtab, dyn);
@@ -950,7 +969,7 @@ int lymph_autotune(LymphCtx* c, int reps, int* n_variants, int* unr, int* xmode,
     for (int v = 0; v < NV; ++v) {
       const int w = b32 ? StepVar32<P>::wpp[v] : StepVar<P>::wpp[v];
       if (unr) unr[v] = b32 ? StepVar32<P>::unr[v] : StepVar<P>::unr[v];
-      if (xmode) xmode[v] = b32 ? 1 : StepVar<P>::xs[v];
+      if (xmode) xmode[v] = b32 ? (StepVar32<P>::pk[v] ? 2 : 1) : StepVar<P>::xs[v];
       if (wpp) wpp[v] = w;
       if (w > 1 && !fits) {  // would run variant 0: not a candidate
         if (ms) ms[v] = -1.0;

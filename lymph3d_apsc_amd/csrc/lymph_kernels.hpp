@@ -891,6 +891,115 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog(const double* __restrict_
   }
 }
 
+// Packed variant of the step for small blocks (3Np <= 32: p = 1, 2): one
+// wave carries G = 64 / 3Np polyhedra, lane = (polyhedron, row), so lanes are
+// not left idle by short rows.  Each polyhedron walks its own block row; the
+// neighbour's coefficients go through LDS (one segment per polyhedron); the
+// block products are added in block order and the solves and update are the
+// ones of k_leapfrog, so results are bit-identical to it.  TB = 64 (FP64
+// operator in column pairs) or 32 (mass-orthonormal FP32 operator in column
+// quads, no solves).
+template <int P, int MODE, int TB>
+__global__ __launch_bounds__(64) void k_leapfrog_packed(const void* __restrict__ Kv,
+                                                        const int* __restrict__ row_ptr,
+                                                        const int* __restrict__ col,
+                                                        const double* __restrict__ Rp,
+                                                        const double* __restrict__ rs,
+                                                        const double* __restrict__ rd,
+                                                        const double* __restrict__ u,
+                                                        const double* __restrict__ wa,
+                                                        double* __restrict__ out, double g,
+                                                        double c1, double c2,
+                                                        const int* __restrict__ gsel,
+                                                        const double* __restrict__ gtab,
+                                                        int n_poly) {
+  using D = Dims<P>;
+  constexpr int NP = D::NP, N3 = D::N3, NPAIR = D::NPAIR;
+  static_assert(N3 <= 32, "packed step needs 3Np <= 32");
+  constexpr int G = 64 / N3;
+  __shared__ double sx[G][N3 + 1];
+  __shared__ double sw[G][N3];
+  __shared__ double sR[G][D::NPK];
+  const int lane = threadIdx.x;
+  const int sub = lane / N3, r = lane % N3;
+  const int Pp = step_poly() * G + sub;
+  const bool act = sub < G && Pp < n_poly;
+  if (MODE != 2 && TB == 64)
+    for (int k = lane; k < G * D::NPK; k += 64) {
+      const int sp = k / D::NPK, e = k % D::NPK, Q = step_poly() * G + sp;
+      if (Q < n_poly) sR[sp][e] = Rp[(size_t)Q * D::NPK + e];
+    }
+  const int b0 = act ? row_ptr[Pp] : 0, cnt = act ? row_ptr[Pp + 1] - b0 : 0;
+  double acc = 0.0;
+  for (int j = 0; __any(j < cnt); ++j) {
+    const bool on = j < cnt;
+    const int b = b0 + j;
+    if (on) sx[sub][r] = u[(size_t)col[b] * N3 + r];
+    __syncthreads();
+    double s = 0.0;
+    if (on) {
+      if constexpr (TB == 64) {
+        const double* Kb = static_cast<const double*>(Kv) + (size_t)b * D::BS;
+        for (int c2i = 0; c2i < NPAIR; ++c2i) {
+          const double2 kv = load_stream(Kb + ((size_t)c2i * N3 + r) * 2);
+          const bool odd_tail = (N3 % 2 == 1) && (c2i == NPAIR - 1);
+          s = s + kv.x * sx[sub][2 * c2i];
+          if (!odd_tail) s = s + kv.y * sx[sub][2 * c2i + 1];
+        }
+      } else {
+        constexpr int NQUAD = (N3 + 3) / 4;
+        const float* Kb = static_cast<const float*>(Kv) + (size_t)b * ((size_t)N3 * NQUAD * 4);
+        float sf = 0.0f;
+        for (int q = 0; q < NQUAD; ++q) {
+          const float4 kv = load_stream(Kb + ((size_t)q * N3 + r) * 4);
+          float xq[4];
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj) xq[jj] = (4 * q + jj < N3) ? (float)sx[sub][4 * q + jj] : 0.0f;
+          sf = sf + kv.x * xq[0];
+          sf = sf + kv.y * xq[1];
+          sf = sf + kv.z * xq[2];
+          sf = sf + kv.w * xq[3];
+        }
+        s = (double)sf;
+      }
+      acc = (j == 0) ? s : acc + s;
+    }
+    __syncthreads();
+  }
+  const size_t o = (size_t)Pp * N3 + r;
+  if (MODE == 2) {
+    if (act) sw[sub][r] = acc;
+    __syncthreads();
+    if (act && r == 0) {
+      double el = 0.0;
+      for (int k = 0; k < N3; ++k) el = el + u[(size_t)Pp * N3 + k] * sw[sub][k];
+      out[Pp] = el;
+    }
+    return;
+  }
+  double w = 0.0;
+  if (act) {
+    const double gp = gsel ? gtab[gsel[Pp]] : g;
+    w = -acc + rs[o] + rd[o] * gp;
+  }
+  if constexpr (TB == 64) {
+    if (act) sw[sub][r] = w;
+    __syncthreads();
+    if (lane < 3 * G) {
+      const int sp = lane / 3, comp = lane % 3;
+      if (step_poly() * G + sp < n_poly) solve_packed<NP>(sR[sp], sw[sp] + comp * NP);
+    }
+    __syncthreads();
+    if (act) w = sw[sub][r];
+  }
+  if (act) {
+    if (MODE == 0)
+      out[o] = c1 * w + 2.0 * u[o] - wa[o];
+    else
+      out[o] = c1 * w + u[o] + c2 * wa[o];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // FP32-apply variant of the step: the operator blocks, taken to the
 // mass-orthonormal basis in FP64 (k_ortho_operator below), are held in FP32 in

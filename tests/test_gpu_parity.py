@@ -357,11 +357,12 @@ def test_fp32_apply_on_ill_conditioned_agglomerates():
     assert np.linalg.norm(a32 - a64) < 1e-3 * np.linalg.norm(a64)
 
 
-@pytest.mark.parametrize("name,p", [("hetero", 2), ("grown4", 4)])
+@pytest.mark.parametrize("name,p", [("hetero", 1), ("hetero", 2), ("grown4", 4)])
 def test_step_variants_agree_bitwise(name, p):
     """Every step-kernel variant -- load group size, neighbour broadcast mode,
-    waves per polyhedron -- gives bit for bit the same step, for the 64-bit
-    apply (which matches the oracle) and for the 32-bit apply."""
+    waves per polyhedron, several polyhedra per wave at p <= 2 -- gives bit
+    for bit the same step and first step, for the 64-bit apply (which matches
+    the oracle) and for the 32-bit apply."""
     say(f"[variants] {name} p={p}")
     if name == "grown4":
         m = M.cube_mesh(4, dirichlet_faces={"x0", "x1", "y0", "y1", "z0"}, poly_size=(4, 40), seed=3)
@@ -380,12 +381,14 @@ def test_step_variants_agree_bitwise(name, p):
         ctx.make_rhs_free(L.PROB_REF_MMS, _lib.SLOT_RHS_DYN)
         for bits in (64, 32):
             ctx.set_apply_precision(bits)
-            outs = []
+            outs, firsts = [], []
             for v in range(ctx.n_step_variants()):
                 ctx.set_step_variant(v)
                 outs.append(ctx.time_step_matrix_free(g, dt, u0, u1))
-            for v, a in enumerate(outs[1:], 1):
-                assert np.array_equal(a, outs[0]), (bits, v)
+                firsts.append(ctx.first_time_step_matrix_free(g, dt, u0, u1))
+            for v in range(1, len(outs)):
+                assert np.array_equal(outs[v], outs[0]), (bits, v)
+                assert np.array_equal(firsts[v], firsts[0]), (bits, v)
             if bits == 64:
                 assert rel(outs[0], ref) < 1e-13
 
