@@ -21,6 +21,7 @@ algorithm, OpenMP over polyhedra) timed on this host's cores on a bounded
 sample of the same workload (a smaller cube of the same structure).
 
 Usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--n 40] [--p 3]
+       python bench.py --degree-sweep     (p = 1..4 step table instead)
 """
 import argparse
 import ctypes as C
@@ -177,6 +178,35 @@ def ensemble_leg(n_inst, steps, plain_sample, f0=5.0, seed0=1000):
     }
 
 
+def degree_sweep(steps=200):
+    """Step throughput at p = 1..4 on agglomerated cubes of about 4 M DOFs
+    (70^3, 52^3, 40^3, 33^3 cubes), FP64 and 32-bit apply, each autotuned."""
+    import lymph3d_apsc_amd as L
+    from lymph3d_apsc_amd import _lib, mesh as M
+    rows = []
+    for p, n in ((1, 70), (2, 52), (3, 40), (4, 33)):
+        m = M.cube_mesh(n, agglomerate=True)
+        dt = 0.2 * (1 / n) / (np.sqrt(3.0) * p * p)
+        with L.Context(m, p) as ctx:
+            ctx.set_keep_adg(False)
+            ctx.set_build("factored")
+            ctx.make_matrices_free()
+            ctx.make_rhs_free(L.PROB_NULL, _lib.SLOT_RHS_STAT)
+            ctx.make_rhs_free(L.PROB_CUBE_WAVE, _lib.SLOT_RHS_DYN)
+            row = dict(p=p, n=n, n_poly=m.n_poly, dofs=m.n_poly * ctx.n3, n_blk=ctx.n_blk)
+            for bits in (64, 32):
+                ctx.set_apply_precision(bits)
+                chosen, _ = ctx.autotune(reps=10)
+                ms = ctx.time_steps(L.PROB_CUBE_WAVE, steps, dt, dt) / steps
+                B = ctx.step_bytes_for(bits)
+                row[f"fp{bits}"] = dict(ms_per_step=ms, dof_updates_per_s=row["dofs"] / (ms * 1e-3),
+                                        variant=list(chosen), GBps=B / ms / 1e6,
+                                        frac=B / ms / 1e6 / HBM_PEAK_GBS)
+            log(json.dumps(row))
+            rows.append(row)
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,6 +223,8 @@ def main():
     ap.add_argument("--ens-instances", type=int, default=256)
     ap.add_argument("--ens-steps", type=int, default=1000)
     ap.add_argument("--ens-plain-sample", type=int, default=8)
+    ap.add_argument("--degree-sweep", action="store_true",
+                    help="instead of the bench line: step throughput at p = 1..4 (one JSON line)")
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -206,6 +238,9 @@ def main():
     if L.device_count() == 0:
         print(json.dumps({"error": "no HIP device visible"}))
         return 1
+    if a.degree_sweep:
+        print(json.dumps({"degree_sweep": degree_sweep()}))
+        return 0
     t_setup = time.perf_counter()
     m = M.cube_mesh(a.n, agglomerate=True)
     log(f"mesh: {m.n_tet} tets, {m.n_poly} polyhedra ({time.perf_counter() - t_setup:.1f}s)")

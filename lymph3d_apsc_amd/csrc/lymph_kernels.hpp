@@ -917,7 +917,9 @@ __global__ __launch_bounds__(64) void k_leapfrog_packed(const void* __restrict__
   constexpr int NP = D::NP, N3 = D::N3, NPAIR = D::NPAIR;
   static_assert(N3 <= 32, "packed step needs 3Np <= 32");
   constexpr int G = 64 / N3;
-  __shared__ double sx[G][N3 + 1];
+  constexpr int NQUAD = (N3 + 3) / 4;
+  // neighbour coefficients, double-buffered by block parity: one barrier per block
+  __shared__ double sx[2][G][N3 + 1];
   __shared__ double sw[G][N3];
   __shared__ double sR[G][D::NPK];
   const int lane = threadIdx.x;
@@ -930,41 +932,56 @@ __global__ __launch_bounds__(64) void k_leapfrog_packed(const void* __restrict__
       if (Q < n_poly) sR[sp][e] = Rp[(size_t)Q * D::NPK + e];
     }
   const int b0 = act ? row_ptr[Pp] : 0, cnt = act ? row_ptr[Pp + 1] - b0 : 0;
+  double xn = cnt > 0 ? u[(size_t)col[b0] * N3 + r] : 0.0;  // neighbour value of the next block
   double acc = 0.0;
   for (int j = 0; __any(j < cnt); ++j) {
     const bool on = j < cnt;
     const int b = b0 + j;
-    if (on) sx[sub][r] = u[(size_t)col[b] * N3 + r];
-    __syncthreads();
-    double s = 0.0;
+    const int buf = j & 1;
+    // the block's loads and the next neighbour value are issued before the
+    // barrier, so they overlap the exchange of this block's coefficients
+    double2 kd[TB == 64 ? NPAIR : 1];
+    float4 kf[TB == 32 ? NQUAD : 1];
     if (on) {
       if constexpr (TB == 64) {
         const double* Kb = static_cast<const double*>(Kv) + (size_t)b * D::BS;
+#pragma unroll
+        for (int c2i = 0; c2i < NPAIR; ++c2i) kd[c2i] = load_stream(Kb + ((size_t)c2i * N3 + r) * 2);
+      } else {
+        const float* Kb = static_cast<const float*>(Kv) + (size_t)b * ((size_t)N3 * NQUAD * 4);
+#pragma unroll
+        for (int q = 0; q < NQUAD; ++q) kf[q] = load_stream(Kb + ((size_t)q * N3 + r) * 4);
+      }
+      sx[buf][sub][r] = xn;
+      if (j + 1 < cnt) xn = u[(size_t)col[b + 1] * N3 + r];
+    }
+    __syncthreads();
+    if (on) {
+      double s = 0.0;
+      if constexpr (TB == 64) {
+#pragma unroll
         for (int c2i = 0; c2i < NPAIR; ++c2i) {
-          const double2 kv = load_stream(Kb + ((size_t)c2i * N3 + r) * 2);
           const bool odd_tail = (N3 % 2 == 1) && (c2i == NPAIR - 1);
-          s = s + kv.x * sx[sub][2 * c2i];
-          if (!odd_tail) s = s + kv.y * sx[sub][2 * c2i + 1];
+          s = s + kd[c2i].x * sx[buf][sub][2 * c2i];
+          if (!odd_tail) s = s + kd[c2i].y * sx[buf][sub][2 * c2i + 1];
         }
       } else {
-        constexpr int NQUAD = (N3 + 3) / 4;
-        const float* Kb = static_cast<const float*>(Kv) + (size_t)b * ((size_t)N3 * NQUAD * 4);
         float sf = 0.0f;
+#pragma unroll
         for (int q = 0; q < NQUAD; ++q) {
-          const float4 kv = load_stream(Kb + ((size_t)q * N3 + r) * 4);
           float xq[4];
 #pragma unroll
-          for (int jj = 0; jj < 4; ++jj) xq[jj] = (4 * q + jj < N3) ? (float)sx[sub][4 * q + jj] : 0.0f;
-          sf = sf + kv.x * xq[0];
-          sf = sf + kv.y * xq[1];
-          sf = sf + kv.z * xq[2];
-          sf = sf + kv.w * xq[3];
+          for (int jj = 0; jj < 4; ++jj)
+            xq[jj] = (4 * q + jj < N3) ? (float)sx[buf][sub][4 * q + jj] : 0.0f;
+          sf = sf + kf[q].x * xq[0];
+          sf = sf + kf[q].y * xq[1];
+          sf = sf + kf[q].z * xq[2];
+          sf = sf + kf[q].w * xq[3];
         }
         s = (double)sf;
       }
       acc = (j == 0) ? s : acc + s;
     }
-    __syncthreads();
   }
   const size_t o = (size_t)Pp * N3 + r;
   if (MODE == 2) {
