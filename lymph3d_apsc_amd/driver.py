@@ -7,7 +7,11 @@ matrix-free path, on the GPU:
   `do while (t <= stop_time)` loop -> exact-solution projection scaled by
   time_function(t) -> L2 / DG errors (printed as :515-517).
 
+With monitors (the deck's SAVEOUT path, :400-406 and :441) the loop also
+writes the EnSight monitor files every TMONITOR steps.
+
 Usage: python -m lymph3d_apsc_amd.driver <deck folder> [--max-steps N]
+           [--monitor DIR [--every N]]
 """
 import argparse
 import sys
@@ -25,7 +29,64 @@ def compute_cfl(hmax, dt, speed=1.0):
     return not (dt > hmax / speed)
 
 
-def run_deck(folder, max_steps=-1, p=None, verbose=True, problem=_lib.PROB_REF_MMS):
+def _fields(problem):
+    """Initial displacement, initial velocity and exact solution of a data set
+    (the choice lymph_run makes)."""
+    u0 = _lib.FN_CUBE_U if problem == _lib.PROB_CUBE_WAVE else _lib.FN_ZERO
+    v0 = _lib.FN_REF_VEL if problem == _lib.PROB_REF_MMS else _lib.FN_ZERO
+    ex = {_lib.PROB_REF_MMS: _lib.FN_REF_UEX, _lib.PROB_CUBE_WAVE: _lib.FN_CUBE_U}.get(problem, _lib.FN_ZERO)
+    return u0, v0, ex
+
+
+def run_monitored(ctx, problem, dt, stop_time, every, out, max_steps=-1, strict=False):
+    """The loop of src/Lymph3D.f90:325-441 with SAVEOUT monitors: after loop
+    iteration num_dt (0-based) the state is exported when num_dt % every == 0
+    (EXPORT_SOLUTION: vertex values of every tet; the mesh, polyhedron and
+    boundary files with the first export), and solution.case is written at
+    the end.  The steps run on the device in chunks between exports, with
+    t accumulated as the reference does, so the final state and errors are
+    those of Context.run bit for bit."""
+    from . import export as X
+    u0f, v0f, exf = _fields(problem)
+    if not ctx.built:
+        ctx.make_matrices_free()
+    ctx.make_rhs_free(_lib.PROB_NULL, _lib.SLOT_RHS_STAT)
+    ctx.make_rhs_free(problem, _lib.SLOT_RHS_DYN)
+    u0 = ctx.compute_modal_coefficients_free(u0f, _lib.SLOT_U_PREV)
+    v0 = ctx.compute_modal_coefficients_free(v0f, _lib.SLOT_V0)
+    ctx.set_slot(_lib.SLOT_U_CUR, ctx.first_time_step_matrix_free(
+        _lib.time_function(problem, 0.0), dt, u0, v0))
+    t = 0.0 + dt
+    nsteps, num_dt, files = 1, 0, []
+    while t <= stop_time and (max_steps < 0 or nsteps < max_steps):
+        # steps up to and including the next export, stopping where the loop would
+        k, tt = 0, t
+        while True:
+            k += 1
+            tt = tt + dt
+            if (num_dt + k - 1) % every == 0 or not tt <= stop_time or \
+                    (max_steps >= 0 and nsteps + k >= max_steps):
+                break
+        t = ctx.step_n(problem, k, t, dt)
+        nsteps += k
+        num_dt += k
+        if (num_dt - 1) % every == 0:
+            vals = X.vertex_values(ctx)
+            if not files:
+                X.write_ensight_mesh(out, ctx.mesh, strict)
+                X.write_ensight_boundary(out, ctx.mesh, strict=strict)
+            files.append(X.write_ensight_solution(out, vals, num_dt - 1))
+    X.write_ensight_case(out, dt, stop_time, every)
+    uex = ctx.compute_modal_coefficients_free(exf) * _lib.time_function(problem, t)
+    l2 = ctx.compute_error_l2_matrix_free(ctx.get_slot(_lib.SLOT_U_CUR), uex)
+    dg = ctx.compute_error_dg_matrix_free(ctx.get_slot(_lib.SLOT_U_CUR), uex)
+    return dict(t_final=t, nsteps=nsteps, err_l2=l2 ** 0.5, err_dg=dg ** 0.5, files=files)
+
+
+def run_deck(folder, max_steps=-1, p=None, verbose=True, problem=_lib.PROB_REF_MMS, monitor=None,
+             every=None):
+    """Run a deck; with `monitor` a folder, also write the EnSight monitor
+    files there every `every` steps (default: the deck's TMONITOR)."""
     mesh = read_deck(folder)
     hmax = mesh.hmax()
     if not compute_cfl(hmax, mesh.dt):
@@ -33,7 +94,11 @@ def run_deck(folder, max_steps=-1, p=None, verbose=True, problem=_lib.PROB_REF_M
     deg = mesh.degree if p is None else p
     t0 = time.perf_counter()
     with Context(mesh, deg) as ctx:
-        res = ctx.run(problem, mesh.dt, mesh.stop_time, max_steps)
+        if monitor:
+            res = run_monitored(ctx, problem, mesh.dt, mesh.stop_time,
+                                int(every or mesh.monitor_every or 1), monitor, max_steps)
+        else:
+            res = ctx.run(problem, mesh.dt, mesh.stop_time, max_steps)
     res["wall_s"] = time.perf_counter() - t0
     res["hmax"] = hmax
     res["dofs"] = mesh.n_dof(deg)
@@ -51,8 +116,10 @@ def main(argv=None):
     ap.add_argument("deck")
     ap.add_argument("--max-steps", type=int, default=-1)
     ap.add_argument("--degree", type=int, default=None)
+    ap.add_argument("--monitor", default=None, help="folder for the EnSight monitor files")
+    ap.add_argument("--every", type=int, default=None, help="steps between exports (default TMONITOR)")
     a = ap.parse_args(argv)
-    run_deck(a.deck, a.max_steps, a.degree)
+    run_deck(a.deck, a.max_steps, a.degree, monitor=a.monitor, every=a.every)
     return 0
 
 

@@ -310,6 +310,7 @@ def read_deck(folder, force_unit_material=True):
              np.asarray(ttag), tag_space, rho, lam, mu, degree=deg,
              dt=float(hdr.get("TIMESTEP", "nan")), stop_time=float(hdr.get("STOPTIME", "nan")))
     m.mixed_degree = len(degs) > 1
+    m.monitor_every = int(hdr.get("TMONITOR", "0"))  # steps between SAVEOUT exports
     return m
 
 
@@ -432,3 +433,121 @@ def grow_polyhedra(tets, n, size, seed):
                     break
     _, dense = np.unique(poly, return_inverse=True)
     return dense.astype(np.int32)
+
+
+# ----------------------------------------------------------------------------
+# Writers of the reference's input and partition files: a mesh generated or
+# agglomerated here can be handed to the reference's Fortran host, and decks
+# and partitions round-trip through read_deck / read_elem_in_poly.
+
+def _mate_tags(mesh):
+    """Material tag per tet: tets with equal (rho, lambda, mu) share a tag,
+    numbered 1, 8, 9, ... (2-7 are the boundary tags of the BC mapping)."""
+    trip = np.stack([mesh.rho, mesh.lam, mesh.mu], axis=1)
+    uniq, inv = np.unique(trip, axis=0, return_inverse=True)
+    tags = np.array([1] + list(range(8, 8 + len(uniq) - 1)), np.int64)
+    return uniq, tags, tags[inv.ravel()]
+
+
+def write_deck(folder, mesh, degree, dt, stop_time, grid="grid", mat="mat", monitor_every=100,
+               save_output=0):
+    """Write `mesh` as a LYMPH3D deck: Poly.input (src/Poly_readfile.f90:9-187),
+    <mat>.mate (:327-707) and <grid>.mesh (:715-1049).
+
+    * Materials: one MATE line per distinct (rho, lambda, mu), with
+      vs = sqrt(mu/rho), vp = sqrt((lambda + 2 mu)/rho), the inverse of the
+      reader's lambda = rho (vp^2 - 2 vs^2), mu = rho vs^2 (:408-412).
+    * Boundary triangles keep their tags; one DIRI/NEUM line per tag with its
+      space-function tag.
+    * Agglomerated meshes: one `poly` element line per tet, in tet order, with
+      the 1-based polyhedron id (read as elem_in_poly, :1017-1032).
+    * Nodes are written with 17 significant digits (lines stay under the
+      reader's 100 characters), so coordinates round-trip exactly."""
+    os.makedirs(folder, exist_ok=True)
+    uniq, tags, tet_tag = _mate_tags(mesh)
+    with open(os.path.join(folder, "Poly.input"), "w") as f:
+        f.write(f"GRIDFILE  {grid}\nMATFILE   {mat}\nMPIFILE   FILES_MPI\nMONFILE   MONITORS\n\n")
+        f.write(f"TIMESTEP {float(dt)!r}\nTMONITOR {int(monitor_every)}\nSTOPTIME {float(stop_time)!r}\n"
+                f"SAVEOUT {int(save_output)}\n")
+    with open(os.path.join(folder, mat + ".mate"), "w") as f:
+        f.write("      mat_id   degree    rho    vs    vp    qs    qp\n")
+        for (rho, lam, mu), t in zip(uniq, tags):
+            vs, vp = np.sqrt(mu / rho), np.sqrt((lam + 2 * mu) / rho)
+            f.write(f"MATE {int(t):4d} {int(degree):4d} {float(rho)!r} {float(vs)!r} {float(vp)!r} 30 60\n")
+        f.write("\n")
+        for t in sorted({int(x) for x in mesh.tria_tag}):
+            key = "DIRI" if t in DIRICHLET_TAGS else "NEUM"
+            f.write(f"{key} {t:4d} {int(mesh.tag_space.get(t, 0)):4d} 0.0 0.0 0.0 0.0\n")
+    agg = mesh.n_poly != mesh.n_tet or np.any(mesh.tet_poly != np.arange(mesh.n_tet))
+    n_elem = mesh.tria_vert.shape[0] + mesh.n_tet + (mesh.n_tet if agg else 0)
+    with open(os.path.join(folder, grid + ".mesh"), "w") as f:
+        f.write(f"{mesh.xyz.shape[0]:8d} {n_elem:8d}   0   0   0\n")
+        for i, (x, y, z) in enumerate(mesh.xyz, 1):
+            f.write(f"{i}  {x:+.16e}  {y:+.16e}  {z:+.16e}\n")
+        k = 0
+        for (a, b, c), t in zip(mesh.tria_vert + 1, mesh.tria_tag):
+            k += 1
+            f.write(f"{k} {int(t)} tria {a} {b} {c}\n")
+        for (a, b, c, d), t in zip(mesh.tet_vert + 1, tet_tag):
+            k += 1
+            f.write(f"{k} {int(t)} tetra {a} {b} {c} {d}\n")
+        if agg:
+            for pid in mesh.tet_poly + 1:
+                k += 1
+                f.write(f"{k} 0 poly {int(pid)}\n")
+
+
+def partition_file_name(rank=0):
+    """elem_in_poly_NNNNNN.mpi with the rank in the last digits (the name the
+    reference gives rank `rank` < 10, src/MAKE_PARTITION_AND_MPI_FILES.f90:401-404)."""
+    return f"elem_in_poly_{rank:06d}.mpi"
+
+
+def write_elem_in_poly(folder, mesh, rank=0):
+    """Tet -> polyhedron map in the reference's partition-file format (written
+    by MESH_AGGLOMERATION / WRITE_POLY_INFO, :463-476, :551-556; read by
+    CREATE_LOCAL_MESH, :1163-1172): the tet count, then one line per tet with
+    its 1-based global id and 1-based polyhedron id."""
+    os.makedirs(folder, exist_ok=True)
+    path = os.path.join(folder, partition_file_name(rank))
+    with open(path, "w") as f:
+        f.write(f" {mesh.n_tet:12d}\n")
+        for i, pid in enumerate(mesh.tet_poly + 1, 1):
+            f.write(f" {i:12d} {int(pid):12d}\n")
+    return path
+
+
+def write_poly_global(folder, mesh):
+    """poly_global.mpi (CREATE_GLOBAL_POLY_MAP, :618-630): the tet count, then
+    the 1-based polyhedron id of every tet, one per line."""
+    os.makedirs(folder, exist_ok=True)
+    path = os.path.join(folder, "poly_global.mpi")
+    with open(path, "w") as f:
+        f.write(f" {mesh.n_tet:12d}\n")
+        for pid in mesh.tet_poly + 1:
+            f.write(f" {int(pid):12d}\n")
+    return path
+
+
+def read_elem_in_poly(path, n_poly=None):
+    """0-based polyhedron id per tet from a partition file (second column,
+    positional, as CREATE_LOCAL_MESH reads it).  With n_poly given, the
+    reference's MESH_CORRECTION (src/MESH_CORRECTION.f90, single rank) is
+    applied: every id 1..n_poly left without tets takes the first tet of a
+    polyhedron with more than one, scanning polyhedra from 1 upwards and
+    filling the missing ids from the last."""
+    with open(path) as f:
+        n = int(f.readline().split()[0])
+        ids = np.array([int(f.readline().split()[1]) for _ in range(n)], np.int64)
+    if n_poly is not None:
+        counts = np.bincount(ids, minlength=n_poly + 1)
+        missing = [q for q in range(1, n_poly + 1) if counts[q] == 0]
+        i = 1
+        while missing and i <= n_poly:
+            tets = np.nonzero(ids == i)[0]
+            if tets.size > 1:
+                ids[tets[0]] = missing.pop()
+            i += 1
+        if missing:
+            raise MeshError(4, f"{len(missing)} polyhedra left without tetrahedra")
+    return (ids - 1).astype(np.int32)
