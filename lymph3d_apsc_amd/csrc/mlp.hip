@@ -2,10 +2,11 @@
 // evaluated with hand-written BF16 MFMA kernels for gfx950 (MI355X).
 //
 // GEMM: C[M][N] = sum_k A(m,k) B(k,n), bf16 operands, FP32 accumulation on
-// v_mfma_f32_16x16x32_bf16.  64x64 output tile per 256-thread workgroup, each
-// of the 4 waves owning a 32x32 quarter (2x2 MFMA tiles); k advances 32 at a
-// time through LDS tiles stored k-contiguous for both operands, so every
-// lane's MFMA fragment is one 16-byte LDS read.  Operands may be k-contiguous
+// v_mfma_f32_16x16x32_bf16.  TM x TM output tile (TM = 128, or 64 for small
+// or split-k GEMMs) per 256-thread workgroup, each of the 4 waves owning a
+// (TM/2)^2 quarter as (TM/32)^2 MFMA tiles; k advances 64 at a time through
+// LDS tiles stored k-contiguous for both operands, so every lane's MFMA
+// fragment is one 16-byte LDS read.  Operands may be k-contiguous
 // in memory (row-major A, "W[n][k]" B: 16-byte loads copied as they are) or
 // m/n-contiguous (16-byte loads along m/n, transposed into LDS), which covers
 // the forward pass (X W^T), the input gradient (dZ W) and the weight
@@ -77,37 +78,69 @@ struct GemmArgs {
 // One TM x 64 operand tile in TM/32 16-byte pieces per thread: global ->
 // registers (issued before the previous tile's MFMAs), registers -> LDS with
 // k contiguous.
+//  KC (k contiguous in memory): piece c = tid + 256 j is row c/8, k 8(c%8)..+7,
+//    stored as it is.
+//  not KC (rows contiguous in memory): thread tid owns rows 8 mg..8 mg+7 and
+//    k rows RK kg..RK kg+RK-1 (mg = tid % (TM/8), kg = tid / (TM/8), RK = TM/32):
+//    RK loads of 8 rows each, coalesced along the rows; in registers each row's
+//    RK k values are gathered and written as one RK*2-byte LDS store.  The k
+//    chunks (8 values) of row r are XOR-swizzled by (r/16) % 8 so that the 16
+//    rows a wave writes at once fall in distinct banks; a 16-row MFMA tile has
+//    one swizzle, so the fragment reads stay conflict-free.
+template <bool KC>
+__device__ inline int lds_k(int row, int k) {
+  if (KC) return k;
+  return ((((k >> 3) ^ (row >> 4)) & 7) << 3) | (k & 7);
+}
+
 template <bool KC, int TM>
 __device__ inline void fetch(uint4 (&v)[TM / 32], const bf16_t* __restrict__ g, int ld, int r0,
                              int R, int k0, int ke, int tid) {
+  constexpr int RK = TM / 32;
 #pragma unroll
-  for (int j = 0; j < TM / 32; ++j) {
-    const int c = tid + 256 * j;
+  for (int j = 0; j < RK; ++j) {
     v[j] = make_uint4(0u, 0u, 0u, 0u);
     if (KC) {
+      const int c = tid + 256 * j;
       const int r = c >> 3, kc = (c & 7) * 8;
       if (r0 + r < R && k0 + kc < ke)
         v[j] = *reinterpret_cast<const uint4*>(g + (size_t)(r0 + r) * ld + k0 + kc);
     } else {
-      const int kk = c / (TM / 8), rc = (c % (TM / 8)) * 8;
+      const int mg = tid % (TM / 8), kk = (tid / (TM / 8)) * RK + j, rc = mg * 8;
       if (k0 + kk < ke && r0 + rc < R)
         v[j] = *reinterpret_cast<const uint4*>(g + (size_t)(k0 + kk) * ld + r0 + rc);
     }
   }
 }
 
+__device__ inline uint32_t word(const uint4& v, int w) {
+  return w == 0 ? v.x : (w == 1 ? v.y : (w == 2 ? v.z : v.w));
+}
+
 template <bool KC, int TM>
 __device__ inline void put(bf16_t* __restrict__ s, const uint4 (&v)[TM / 32], int tid) {
+  constexpr int RK = TM / 32;
+  if (KC) {
 #pragma unroll
-  for (int j = 0; j < TM / 32; ++j) {
-    const int c = tid + 256 * j;
-    if (KC) {
+    for (int j = 0; j < RK; ++j) {
+      const int c = tid + 256 * j;
       *reinterpret_cast<uint4*>(s + (c >> 3) * LDK + (c & 7) * 8) = v[j];
-    } else {
-      const int kk = c / (TM / 8), rc = (c % (TM / 8)) * 8;
-      const bf16_t* e = reinterpret_cast<const bf16_t*>(&v[j]);
+    }
+  } else {
+    static_assert(RK == 2 || RK == 4, "tile rows");
+    const int mg = tid % (TM / 8), kb = (tid / (TM / 8)) * RK;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) s[(rc + q) * LDK + kk] = e[q];
+    for (int q = 0; q < 8; ++q) {
+      const int row = mg * 8 + q;
+      // element q of k row j: half (q & 1) of word q / 2 of v[j]
+      uint32_t h[RK];
+#pragma unroll
+      for (int j = 0; j < RK; ++j) h[j] = (q & 1) ? (word(v[j], q >> 1) >> 16) : (word(v[j], q >> 1) & 0xffffu);
+      bf16_t* dst = s + row * LDK + lds_k<false>(row, kb);
+      if constexpr (RK == 4)
+        *reinterpret_cast<uint2*>(dst) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+      else
+        *reinterpret_cast<uint32_t*>(dst) = h[0] | (h[1] << 16);
     }
   }
 }
@@ -159,11 +192,15 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     for (int ks = 0; ks < BK; ks += 32) {
       bf16x8 af[T], bfg[T];
 #pragma unroll
-      for (int i = 0; i < T; ++i)
-        af[i] = *reinterpret_cast<const bf16x8*>(sA + (wm + i * 16 + fr) * LDK + ks + fk);
+      for (int i = 0; i < T; ++i) {
+        const int row = wm + i * 16 + fr;
+        af[i] = *reinterpret_cast<const bf16x8*>(sA + row * LDK + lds_k<A_KC>(row, ks + fk));
+      }
 #pragma unroll
-      for (int j = 0; j < T; ++j)
-        bfg[j] = *reinterpret_cast<const bf16x8*>(sB + (wn + j * 16 + fr) * LDK + ks + fk);
+      for (int j = 0; j < T; ++j) {
+        const int row = wn + j * 16 + fr;
+        bfg[j] = *reinterpret_cast<const bf16x8*>(sB + row * LDK + lds_k<B_KC>(row, ks + fk));
+      }
 #pragma unroll
       for (int i = 0; i < T; ++i)
 #pragma unroll
@@ -208,19 +245,31 @@ __global__ void k_gather(const float* __restrict__ X, int w, const int* __restri
   out[i] = f2bf(X[(size_t)idx[b] * w + k]);
 }
 
-// dY = scale * (y - t) in bf16; per-block partial sums of (y - t)^2
+// dY = scale * (y - t) in bf16; per-block partial sums of (y - t)^2.  Each
+// thread takes 8 consecutive outputs of one row (N is a multiple of 8): two
+// 16-byte loads of Y and of the target row, one 16-byte store of dY.
+constexpr int kLossPer = 8 * 256;  // outputs per workgroup
 __global__ __launch_bounds__(256) void k_loss(const float* __restrict__ Y, int N,
                                               const float* __restrict__ T,
                                               const int* __restrict__ idx, int B, float scale,
                                               bf16_t* __restrict__ dY, float* __restrict__ part) {
   __shared__ float red[256];
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 8;
   float d2 = 0.f;
   if (i < (size_t)B * N) {
     const int b = (int)(i / N), n = (int)(i % N);
-    const float d = Y[i] - T[(size_t)idx[b] * N + n];
-    dY[i] = f2bf(scale * d);
-    d2 = d * d;
+    const float4* y = reinterpret_cast<const float4*>(Y + i);
+    const float4* t = reinterpret_cast<const float4*>(T + (size_t)idx[b] * N + n);
+    const float4 y0 = y[0], y1 = y[1], t0 = t[0], t1 = t[1];
+    const float d[8] = {y0.x - t0.x, y0.y - t0.y, y0.z - t0.z, y0.w - t0.w,
+                        y1.x - t1.x, y1.y - t1.y, y1.z - t1.z, y1.w - t1.w};
+    uint32_t o[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      o[k] = (uint32_t)f2bf(scale * d[2 * k]) | ((uint32_t)f2bf(scale * d[2 * k + 1]) << 16);
+      d2 += d[2 * k] * d[2 * k] + d[2 * k + 1] * d[2 * k + 1];
+    }
+    *reinterpret_cast<uint4*>(dY + i) = make_uint4(o[0], o[1], o[2], o[3]);
   }
   red[threadIdx.x] = d2;
   __syncthreads();
@@ -247,19 +296,35 @@ __global__ __launch_bounds__(256) void k_sum(const float* __restrict__ part, int
   if (threadIdx.x == 0) out[0] = (float)(red[0] * scale);
 }
 
-// out[n] = sum_b D[b][n]: 64 columns per workgroup, its 4 waves take every
-// 4th row, then wave sums combine in order (deterministic)
+// out[n] = sum_b D[b][n]: 64 columns per workgroup as 8 groups of 8 (one
+// 16-byte load per row); 32 row groups per column group, row group g adding
+// rows g, g + 32, ... in order, then the 32 partials of each column in order
+// (deterministic).  N is a multiple of 8.
 __global__ __launch_bounds__(256) void k_colsum(const bf16_t* __restrict__ D, int B, int N,
                                                 float* __restrict__ out) {
-  __shared__ float red[4][64];
-  const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int n = blockIdx.x * 64 + c;
-  float s = 0.f;
-  if (n < N)
-    for (int b = w; b < B; b += 4) s += bf2f(D[(size_t)b * N + n]);
-  red[w][c] = s;
+  __shared__ float red[32][65];
+  const int cg = threadIdx.x & 7, rg = threadIdx.x >> 3;
+  const int n0 = blockIdx.x * 64 + cg * 8;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (n0 < N)
+    for (int b = rg; b < B; b += 32) {
+      const uint4 v = *reinterpret_cast<const uint4*>(D + (size_t)b * N + n0);
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        s[2 * k] += bf2f((bf16_t)(w[k] & 0xffffu));
+        s[2 * k + 1] += bf2f((bf16_t)(w[k] >> 16));
+      }
+    }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) red[rg][cg * 8 + k] = s[k];
   __syncthreads();
-  if (w == 0 && n < N) out[n] = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
+  if (threadIdx.x < 64) {
+    const int n = blockIdx.x * 64 + threadIdx.x;
+    float t = 0.f;
+    for (int g = 0; g < 32; ++g) t += red[g][threadIdx.x];
+    if (n < N) out[n] = t;
+  }
 }
 
 __global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
@@ -322,12 +387,12 @@ int dalloc(LymphMlp* m, T** p, size_t n) {
 // Launch: 128 x 128 tiles when they still give >= 256 workgroups, else
 // 64 x 64; a GEMM with few tiles and a long k range (the input gradient of the
 // output layer: k = 38,400) is split along k over up to kMaxSplit workgroups
-// per tile and reduced in order.
+// per tile (32 x 32 = 1024 workgroups there) and reduced in order.
 constexpr int kMaxSplit = 32;
 template <bool A_KC, bool B_KC, int EPI>
 int gemm(LymphMlp* m, GemmArgs a) {
   if (a.M <= 0 || a.N <= 0 || a.K <= 0) return 0;
-  const bool big = blocks(a.N, 128) * blocks(a.M, 128) >= 256 || a.K > 1024;
+  const bool big = blocks(a.N, 128) * blocks(a.M, 128) >= 256;
   const int TM = big ? 128 : 64;
   const unsigned tiles = blocks(a.N, TM) * blocks(a.M, TM);
   int S = 1;
@@ -383,7 +448,7 @@ int forward(LymphMlp* m, int B) {
 int backward(LymphMlp* m, int B, const float* T, const int* idx, float* loss_out) {
   const int NL = m->w[m->L];
   const size_t ny = (size_t)B * NL;
-  const unsigned nb = blocks(ny, 256);
+  const unsigned nb = blocks(ny, kLossPer);
   k_loss<<<nb, 256>>>(m->Y, NL, T, idx, B, 2.0f / (float)ny, m->dY, m->part);
   k_sum<<<1, 256>>>(m->part, (int)nb, 1.0f / (float)ny, loss_out);
   bf16_t* dZ = m->dY;
