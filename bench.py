@@ -178,6 +178,37 @@ def ensemble_leg(n_inst, steps, plain_sample, f0=5.0, seed0=1000):
     }
 
 
+def surrogate_leg(n_train, n_eval, epochs, threshold=1e-2):
+    """Learned surrogate on a bounded sample of its workload: n_train + n_eval
+    parameter sets on 16^3 cubes at p=2 (1000 steps, 64 receivers every 5
+    steps), solved by the 32-bit ensemble path (truth: the FP64 ensemble), four
+    8-512-512-512-38400 networks trained on the device; throughput and error
+    against the plain FP64 path one instance at a time."""
+    from lymph3d_apsc_amd import surrogate as S
+    r = S.run(n_train=n_train, n_eval=n_eval, epochs=epochs, threshold=threshold, plain_sample=4,
+              log=log)
+    thr = r["throughput_solves_per_s"]
+    err = r["rel_l2_vs_fp64"]
+    return {
+        "value": thr["surrogate"], "unit": "surrogate predictions/s (4 networks, inputs resident)",
+        "sample": f"{n_train} training / {n_eval} held-out instances, {epochs} epochs "
+                  "(the full-size run uses 4096 / 512 and 1000 epochs)",
+        "plain_fp64_solves_per_s": thr["plain_fp64"],
+        "ensemble_fp32_solves_per_s": thr["ensemble_fp32"],
+        "hybrid_solves_per_s": thr["hybrid_with_fallback"], "threshold": threshold,
+        "fallback_fraction": r["fallback_fraction"],
+        "rel_l2_vs_fp64_mean": err["surrogate"]["mean"],
+        "rel_l2_vs_fp64_median": err["surrogate"]["median"],
+        "mean_trace_predictor_rel_l2": err["training_mean_predictor"]["mean"],
+        "ensemble_fp32_rel_l2_max": err["ensemble_fp32"]["max"],
+        "spread_error_correlation": r["spread_vs_error_correlation"],
+        "threshold_tradeoff": r["threshold_tradeoff"],
+        "training_ms_per_step": r["training"]["ms_per_step"],
+        "training_gemm_tflops": r["training"]["gemm_tflops"],
+        "seconds": r["seconds"],
+    }
+
+
 def degree_sweep(steps=200):
     """Step throughput at p = 1..4 on agglomerated cubes of about 4 M DOFs
     (70^3, 52^3, 40^3, 33^3 cubes), FP64 and 32-bit apply, each autotuned."""
@@ -223,6 +254,10 @@ def main():
     ap.add_argument("--ens-instances", type=int, default=256)
     ap.add_argument("--ens-steps", type=int, default=1000)
     ap.add_argument("--ens-plain-sample", type=int, default=8)
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg")
+    ap.add_argument("--sur-train", type=int, default=1024)
+    ap.add_argument("--sur-eval", type=int, default=128)
+    ap.add_argument("--sur-epochs", type=int, default=300)
     ap.add_argument("--degree-sweep", action="store_true",
                     help="instead of the bench line: step throughput at p = 1..4 (one JSON line)")
     a = ap.parse_args()
@@ -338,6 +373,15 @@ def main():
                 f"{e['plain_fp64']['value']:.2f} solves/s, trace rel L2 {e['trace_rel_l2_vs_fp64_max']:.2e}")
         except Exception as ex:  # reported, never fatal
             res["ensemble"] = {"error": repr(ex)}
+    if not a.no_surrogate:
+        try:
+            res["surrogate"] = surrogate_leg(a.sur_train, a.sur_eval, a.sur_epochs)
+            sg = res["surrogate"]
+            log(f"surrogate: {sg['value']:.3g} predictions/s, rel L2 vs FP64 mean "
+                f"{sg['rel_l2_vs_fp64_mean']:.3f} (mean trace {sg['mean_trace_predictor_rel_l2']:.3f}); "
+                f"fallback {sg['fallback_fraction']:.2f} at {sg['threshold']}")
+        except Exception as ex:  # reported, never fatal
+            res["surrogate"] = {"error": repr(ex)}
     if not a.no_cpu_baseline:
         try:
             cb = cpu_baseline(p, a.cpu_sample, a.cpu_seconds)
