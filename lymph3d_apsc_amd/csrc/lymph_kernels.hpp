@@ -741,6 +741,16 @@ __device__ inline int step_poly() {
 }
 inline unsigned step_grid(int n_poly) { return 8u * (unsigned)((n_poly + 7) / 8); }
 
+// With a dispatch order (order[i] = polyhedron of slot i, step_grid entries,
+// padding slots holding n_poly): each XCD's share of the polyhedra is walked
+// longest block row first, so on meshes of uneven polyhedra the long rows
+// start early instead of forming the tail of the launch.  Every polyhedron is
+// still computed by one workgroup exactly as without it.
+__device__ inline int step_poly(const int* __restrict__ order) {
+  const int i = step_poly();
+  return order ? order[i] : i;
+}
+
 // UNR = column pairs whose loads are issued together (register footprint vs
 // loads in flight).  XS = how the neighbour's coefficients reach the lanes:
 // 0 = wave-uniform scalar loads per pair, 1 = one coalesced vector load per
@@ -757,7 +767,8 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog(const double* __restrict_
                                                  const double* __restrict__ wa,
                                                  double* __restrict__ out, double g, double c1,
                                                  double c2, const int* __restrict__ gsel,
-                                                 const double* __restrict__ gtab, int n_poly) {
+                                                 const double* __restrict__ gtab, int n_poly,
+    const int* __restrict__ order) {
   using D = Dims<P>;
   constexpr int NP = D::NP, N3 = D::N3, NPAIR = D::NPAIR;
   constexpr int RPL = (N3 + 63) / 64;
@@ -765,7 +776,7 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog(const double* __restrict_
   __shared__ double sR[D::NPK];
   extern __shared__ double sblk[];  // WPP > 1: per-block products [block][row]
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int Pp = step_poly();
+  const int Pp = step_poly(order);
   if (Pp >= n_poly) return;  // padding workgroups of the last XCD share
   if (MODE != 2)
     for (int k = threadIdx.x; k < D::NPK; k += 64 * WPP) sR[k] = Rp[(size_t)Pp * D::NPK + k];
@@ -1169,14 +1180,15 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog32(const float* __restrict
                                                    const double* __restrict__ wa,
                                                    double* __restrict__ out, double g, double c1,
                                                    double c2, const int* __restrict__ gsel,
-                                                   const double* __restrict__ gtab, int n_poly) {
+                                                   const double* __restrict__ gtab, int n_poly,
+    const int* __restrict__ order) {
   // all vectors in the mass-orthonormal basis (K is ~K); Rp is unused
   using D = Dims<P>;
   constexpr int N3 = D::N3, NQUAD = Dims32<P>::NQUAD;
   constexpr int RPL = (N3 + 63) / 64;
   extern __shared__ double sblk[];  // WPP > 1: per-block products [block][row]
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int Pp = step_poly();
+  const int Pp = step_poly(order);
   if (Pp >= n_poly) return;  // padding workgroups of the last XCD share
   double acc[RPL];
 #pragma unroll
