@@ -40,6 +40,7 @@
 extern "C" {
 #endif
 
+#define LYMPH_EXIT_INSTAB 2 /* src/Poly_global.f90 EXIT_INSTAB */
 #define LYMPH_EXIT_SETUP 4
 #define LYMPH_EXIT_SINGULARMTX 5
 #define LYMPH_EXIT_NEIGHBOUR 18
@@ -293,6 +294,14 @@ int lymph_error_slots(LymphCtx* ctx, int slot_uh, int slot_uex, double* l2sq,
  * square roots printed at src/Lymph3D.f90:516-517); u_final may be NULL. */
 int lymph_run(LymphCtx* ctx, int problem, double dt, double stop_time,
               int max_steps, double* u_final, double* out);
+
+/* Instability control: *finite = 1 if slot `slot` holds only finite values,
+ * 0 if it holds a NaN or an infinity; synchronous.  lymph_run applies it to
+ * u^n every 256 steps and after the last one and then returns
+ * LYMPH_EXIT_INSTAB.  The reference parses an instability-control flag
+ * (IS_instabilitycontrol, src/Poly_global.f90; exit code EXIT_INSTAB) but
+ * never tests the solution. */
+int lymph_check_finite(LymphCtx* ctx, int slot, int* finite);
 
 /* Time factor g(t) of a data set: sin(sqrt2 pi t) for LYMPH_PROB_REF_MMS
  * (time_function), cos(sqrt3 pi t) for LYMPH_PROB_CUBE_WAVE. */

@@ -14,6 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblymph_gpu.so")
 
 # exit / error codes (src/Poly_global.f90:130-173 where they exist)
+EXIT_INSTAB = 2
 EXIT_SETUP = 4
 EXIT_SINGULARMTX = 5
 EXIT_NEIGHBOUR = 18
@@ -35,7 +36,7 @@ EXPORTS = [
     "lymph_export_operator", "lymph_build_rhs", "lymph_project", "lymph_set_slot",
     "lymph_get_slot", "lymph_slot_ptr", "lymph_first_step", "lymph_step",
     "lymph_first_step_dev", "lymph_step_dev", "lymph_step_n", "lymph_time_steps",
-    "lymph_error_l2", "lymph_error_dg", "lymph_error_slots", "lymph_run",
+    "lymph_error_l2", "lymph_error_dg", "lymph_error_slots", "lymph_run", "lymph_check_finite",
     "lymph_time_function", "lymph_autotune", "lymph_set_step_variant", "lymph_set_receivers",
     "lymph_sample_receivers", "lymph_step_n_record", "lymph_get_traces",
     "lymph_set_apply_precision", "lymph_set_source_groups", "lymph_ctx_set_keep_adg", "lymph_ctx_set_build", "lymph_release_fp64_operator",
@@ -101,6 +102,7 @@ def lib():
         "lymph_error_l2": (_i, [_vp, _vp, _vp, _vp]),
         "lymph_error_dg": (_i, [_vp, _vp, _vp, _vp]),
         "lymph_error_slots": (_i, [_vp, _i, _i, _vp, _vp]),
+        "lymph_check_finite": (_i, [_vp, _i, _vp]),
         "lymph_run": (_i, [_vp, _i, _d, _d, _i, _vp, _vp]),
         "lymph_time_function": (_d, [_i, _d]),
         "lymph_autotune": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),

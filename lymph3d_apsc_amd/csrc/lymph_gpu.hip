@@ -1289,6 +1289,30 @@ int lymph_error_slots(LymphCtx* c, int su, int se, double* l2sq, double* dgsq) {
   return errors_dev(c, c->slot[su], c->slot[se], l2sq, dgsq);
 }
 
+namespace {
+// Instability control (the reference's EXIT_INSTAB): *finite = 0 if the
+// vector holds a NaN or an infinity.  Synchronises the context's stream.
+int finite_dev(LymphCtx* c, const double* v, int* finite) {
+  const size_t n = vec_n(c);
+  k_set_int<<<1, 1, 0, c->stream>>>(c->status, 0);
+  k_nonfinite<<<(unsigned)std::min<size_t>(grid1(n, 256), 2048), 256, 0, c->stream>>>(v, n, c->status);
+  if (int rc = check_launch("k_nonfinite")) return rc;
+  int h = 0;
+  HIP_TRY(hipMemcpyAsync(&h, c->status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  *finite = h ? 0 : 1;
+  return 0;
+}
+constexpr int kInstabEvery = 256;  // lymph_run checks u^n every this many steps
+}  // namespace
+
+int lymph_check_finite(LymphCtx* c, int slot, int* finite) {
+  if (int rc = need_ctx(c)) return rc;
+  if (slot < 0 || slot >= LYMPH_NUM_SLOTS) return fail(LYMPH_ERR_ARG, "bad slot");
+  if (!finite) return fail(LYMPH_ERR_ARG, "finite is null");
+  return finite_dev(c, c->slot[slot], finite);
+}
+
 int lymph_run(LymphCtx* c, int problem, double dt, double stop_time, int max_steps,
               double* u_final, double* out) {
   if (int rc = need_ctx(c)) return rc;
@@ -1316,6 +1340,13 @@ int lymph_run(LymphCtx* c, int problem, double dt, double stop_time, int max_ste
     if (int rc = lymph_step_n(c, problem, 1, t, dt, &tn)) return rc;
     nsteps++;
     t = tn;
+    if (nsteps % kInstabEvery == 0 || !(t <= stop_time) || nsteps == max_steps) {
+      int ok = 1;
+      if (int rc = finite_dev(c, c->slot[LYMPH_SLOT_U_CUR], &ok)) return rc;
+      if (!ok)
+        return fail(LYMPH_EXIT_INSTAB, "numerical instability: u holds NaN/Inf after step " +
+                                           std::to_string(nsteps) + " (t = " + std::to_string(t) + ")");
+    }
   }
   if (int rc = lymph_project(c, fn_ex, LYMPH_SLOT_UEX, nullptr)) return rc;
   k_scale<<<grid1(vec_n(c), 256), 256, 0, c->stream>>>(c->slot[LYMPH_SLOT_UEX],

@@ -1325,6 +1325,17 @@ __global__ void k_sub(const double* __restrict__ a, const double* __restrict__ b
   if (i < n) c[i] = a[i] - b[i];
 }
 
+// flag[0] = 1 if some a[i] is NaN or infinite (flag cleared by k_set_int
+// first; every writer stores the same value, so no atomics are needed)
+__global__ void k_set_int(int* __restrict__ flag, int v) { flag[0] = v; }
+__global__ void k_nonfinite(const double* __restrict__ a, size_t n, int* __restrict__ flag) {
+  bool bad = false;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    bad |= !isfinite(a[i]);
+  if (__any(bad) && bad) flag[0] = 1;
+}
+
 __global__ void k_scale(double* __restrict__ a, double g, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0.0 * 0.0 + a[i] * g;

@@ -12,6 +12,7 @@ module lymph_gpu
   use iso_c_binding
   implicit none
 
+  integer(c_int), parameter :: LYMPH_EXIT_INSTAB = 2
   integer(c_int), parameter :: LYMPH_EXIT_SETUP = 4, LYMPH_EXIT_SINGULARMTX = 5
   integer(c_int), parameter :: LYMPH_EXIT_NEIGHBOUR = 18
   integer(c_int), parameter :: LYMPH_ERR_ARG = 101, LYMPH_ERR_HIP = 102, LYMPH_ERR_STATE = 103
@@ -317,6 +318,13 @@ module lymph_gpu
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, l2sq, dgsq   ! c_loc of reals, or c_null_ptr
       integer(c_int), value :: slot_uh, slot_uex
+      integer(c_int) :: rc
+    end function
+    function lymph_check_finite(ctx, slot, finite) bind(C, name='lymph_check_finite') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: slot
+      integer(c_int), intent(out) :: finite
       integer(c_int) :: rc
     end function
     function lymph_time_function(problem, t) bind(C, name='lymph_time_function') result(g)

@@ -299,6 +299,12 @@ class Context:
         check(lib().lymph_error_dg(self.h, ptr(uh), ptr(uex), C.byref(e)), "lymph_error_dg")
         return e.value
 
+    def check_finite(self, slot=_lib.SLOT_U_CUR):
+        """True if slot `slot` holds only finite values (instability control)."""
+        ok = C.c_int()
+        check(lib().lymph_check_finite(self.h, slot, C.byref(ok)), "lymph_check_finite")
+        return bool(ok.value)
+
     def error_slots(self, slot_uh, slot_uex):
         l2, dg = C.c_double(), C.c_double()
         check(lib().lymph_error_slots(self.h, slot_uh, slot_uex, C.byref(l2), C.byref(dg)),

@@ -248,6 +248,31 @@ def test_singular_mass_reports_exit_code():
         assert e.value.code == _lib.ERR_STATE
 
 
+def test_instability_control():
+    """A time step far beyond the CFL limit (forced from the host) makes the
+    leapfrog iteration overflow; lymph_run stops with the reference's
+    EXIT_INSTAB at its next check instead of running on with NaN.  The slot
+    check itself sees a single NaN."""
+    say("[instability]")
+    m = make_mesh("tet_5")
+    with L.Context(m, 1) as ctx:
+        with pytest.raises(L.LymphError) as e:
+            ctx.run(L.PROB_REF_MMS, 5.0, 1e9, max_steps=2000)
+        assert e.value.code == _lib.EXIT_INSTAB
+        u = np.ones(ctx.n_poly * ctx.n3)
+        ctx.set_slot(_lib.SLOT_U_CUR, u)
+        assert ctx.check_finite(_lib.SLOT_U_CUR)
+        u[len(u) // 2] = np.nan
+        ctx.set_slot(_lib.SLOT_U_CUR, u)
+        assert not ctx.check_finite(_lib.SLOT_U_CUR)
+        u[len(u) // 2] = -np.inf
+        ctx.set_slot(_lib.SLOT_U_CUR, u)
+        assert not ctx.check_finite(_lib.SLOT_U_CUR)
+        # a stable run passes every check
+        r = ctx.run(L.PROB_REF_MMS, m.dt, m.stop_time, max_steps=600)
+        assert r["nsteps"] == 600 and np.isfinite(r["err_l2"])
+
+
 @pytest.mark.parametrize("p", [2, 3])
 def test_receivers_and_traces(p):
     """Receiver values (EXPORT_SOLUTION's point evaluation) of the GPU state vs
