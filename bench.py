@@ -274,7 +274,9 @@ def main():
         print(json.dumps({"error": "no HIP device visible"}))
         return 1
     if a.degree_sweep:
-        print(json.dumps({"degree_sweep": degree_sweep()}))
+        copy, read = stream_probe_gbps()
+        print(json.dumps({"degree_sweep": degree_sweep(), "stream_copy_GBps": copy,
+                          "stream_read_GBps": read}))
         return 0
     t_setup = time.perf_counter()
     m = M.cube_mesh(a.n, agglomerate=True)

@@ -239,9 +239,9 @@ int check_launch(const char* what) {
 }
 
 unsigned grid1(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
-size_t Dims32Any(int p) {
-  const int n3 = (p + 1) * (p + 2) * (p + 3) / 2;
-  return (size_t)n3 * (size_t)((n3 + 3) / 4) * 4;
+size_t Dims32Any(int p) {  // Dims32<p>::BS
+  const size_t n3 = (size_t)(p + 1) * (p + 2) * (p + 3) / 2;
+  return (n3 * n3 + 3) / 4 * 4;
 }
 
 

@@ -33,6 +33,7 @@ struct Dims {
 // them from displacing the coefficient vectors (which neighbours re-read).
 typedef double lymph_dv2 __attribute__((ext_vector_type(2)));
 typedef float lymph_fv4 __attribute__((ext_vector_type(4)));
+typedef float lymph_fv2 __attribute__((ext_vector_type(2)));
 __device__ inline double2 load_stream(const double* p) {
   const lymph_dv2 t = __builtin_nontemporal_load(reinterpret_cast<const lymph_dv2*>(p));
   return make_double2(t.x, t.y);
@@ -41,9 +42,38 @@ __device__ inline float4 load_stream(const float* p) {
   const lymph_fv4 t = __builtin_nontemporal_load(reinterpret_cast<const lymph_fv4*>(p));
   return make_float4(t.x, t.y, t.z, t.w);
 }
+// the tail columns of a 32-bit block row (N3 % 4 of them; 1 or 2 for p <= 4)
+template <int T>
+__device__ inline void load_tail(const float* p, float (&v)[T]) {
+  static_assert(T == 1 || T == 2, "tail of 1 or 2 columns");
+  if constexpr (T == 2) {
+    const lymph_fv2 t = __builtin_nontemporal_load(reinterpret_cast<const lymph_fv2*>(p));
+    v[0] = t.x;
+    v[1] = t.y;
+  } else {
+    v[0] = __builtin_nontemporal_load(p);
+  }
+}
 
 LYMPH_HD size_t kpos(int N3, int r, int c) {
   return ((size_t)(c >> 1) * N3 + r) * 2 + (c & 1);
+}
+
+// Layout of a 32-bit block: the first 4*NQ columns as column quads (entry
+// (r, c) at ((c>>2)*N3 + r)*4 + (c&3)), then the T = N3 % 4 tail columns
+// row by row (at 4*NQ*N3 + r*T + c - 4*NQ); no padding columns, so the block
+// stream is exactly N3^2 floats (the block stride is rounded up to 4 floats
+// to keep every block 16-byte aligned).
+template <int P>
+struct Dims32 {
+  static constexpr int N3 = 3 * np_of(P);
+  static constexpr int NQ = N3 / 4, T = N3 % 4;
+  static constexpr size_t BS = ((size_t)N3 * N3 + 3) / 4 * 4;
+};
+LYMPH_HD size_t kpos32(int N3, int r, int c) {
+  const int c0 = N3 / 4 * 4;
+  if (c < c0) return ((size_t)(c >> 2) * N3 + r) * 4 + (c & 3);
+  return (size_t)c0 * N3 + (size_t)r * (N3 % 4) + (c - c0);
 }
 LYMPH_HD int pk_off(int NP, int i) { return i * NP - i * (i - 1) / 2; }
 
@@ -928,7 +958,7 @@ __global__ __launch_bounds__(64) void k_leapfrog_packed(const void* __restrict__
   constexpr int NP = D::NP, N3 = D::N3, NPAIR = D::NPAIR;
   static_assert(N3 <= 32, "packed step needs 3Np <= 32");
   constexpr int G = 64 / N3;
-  constexpr int NQUAD = (N3 + 3) / 4;
+  constexpr int NQUAD = Dims32<P>::NQ, TAIL = Dims32<P>::T;
   // neighbour coefficients, double-buffered by block parity: one barrier per block
   __shared__ double sx[2][G][N3 + 1];
   __shared__ double sw[G][N3];
@@ -953,15 +983,17 @@ __global__ __launch_bounds__(64) void k_leapfrog_packed(const void* __restrict__
     // barrier, so they overlap the exchange of this block's coefficients
     double2 kd[TB == 64 ? NPAIR : 1];
     float4 kf[TB == 32 ? NQUAD : 1];
+    float kt[TB == 32 && TAIL > 0 ? TAIL : 1];
     if (on) {
       if constexpr (TB == 64) {
         const double* Kb = static_cast<const double*>(Kv) + (size_t)b * D::BS;
 #pragma unroll
         for (int c2i = 0; c2i < NPAIR; ++c2i) kd[c2i] = load_stream(Kb + ((size_t)c2i * N3 + r) * 2);
       } else {
-        const float* Kb = static_cast<const float*>(Kv) + (size_t)b * ((size_t)N3 * NQUAD * 4);
+        const float* Kb = static_cast<const float*>(Kv) + (size_t)b * Dims32<P>::BS;
 #pragma unroll
         for (int q = 0; q < NQUAD; ++q) kf[q] = load_stream(Kb + ((size_t)q * N3 + r) * 4);
+        if constexpr (TAIL > 0) load_tail<TAIL>(Kb + (size_t)4 * NQUAD * N3 + (size_t)r * TAIL, kt);
       }
       sx[buf][sub][r] = xn;
       if (j + 1 < cnt) xn = u[(size_t)col[b + 1] * N3 + r];
@@ -988,6 +1020,10 @@ __global__ __launch_bounds__(64) void k_leapfrog_packed(const void* __restrict__
           sf = sf + kf[q].y * xq[1];
           sf = sf + kf[q].z * xq[2];
           sf = sf + kf[q].w * xq[3];
+        }
+        if constexpr (TAIL > 0) {
+#pragma unroll
+          for (int j = 0; j < TAIL; ++j) sf = sf + kt[j] * (float)sx[buf][sub][4 * NQUAD + j];
         }
         s = (double)sf;
       }
@@ -1036,15 +1072,6 @@ __global__ __launch_bounds__(64) void k_leapfrog_packed(const void* __restrict__
 // FP64 stream); products and the per-block sum are FP32; the sum over blocks,
 // the RHS, the basis transforms and the update stay FP64.
 // ---------------------------------------------------------------------------
-template <int P>
-struct Dims32 {
-  static constexpr int N3 = 3 * np_of(P);
-  static constexpr int NQUAD = (N3 + 3) / 4;
-  static constexpr size_t BS = (size_t)N3 * NQUAD * 4;
-};
-LYMPH_HD size_t kpos32(int N3, int r, int c) {
-  return ((size_t)(c >> 2) * N3 + r) * 4 + (c & 3);
-}
 
 // ---------------------------------------------------------------------------
 // 32-bit apply in the mass-orthonormal basis.  With M_P = R_P^T R_P per
@@ -1091,7 +1118,7 @@ __global__ __launch_bounds__(256) void k_ortho_operator(const double* __restrict
                                                         const int* __restrict__ col,
                                                         float* __restrict__ K32, size_t b0) {
   using D = Dims<P>;
-  constexpr int NP = D::NP, N3 = D::N3, NPK = D::NPK, NQUAD = Dims32<P>::NQUAD;
+  constexpr int NP = D::NP, N3 = D::N3, NPK = D::NPK;
   __shared__ double SP[NPK], SQ[NPK];
   __shared__ double X[NP * N3], T[NP * N3];
   const int tid = threadIdx.x;
@@ -1124,14 +1151,10 @@ __global__ __launch_bounds__(256) void k_ortho_operator(const double* __restrict
       Ob[kpos32(N3, i * NP + r, c)] = (float)s;
     }
   }
-  // zero padding columns of the last quad
-  constexpr int PAD = 4 * NQUAD - N3;
-  if constexpr (PAD > 0) {
-    for (int e = tid; e < N3 * PAD; e += blockDim.x) {
-      const int r = e / PAD, c = N3 + e % PAD;
-      Ob[kpos32(N3, r, c)] = 0.0f;
-    }
-  }
+  // the floats that round the block stride up to a multiple of 4
+  constexpr int PAD = (int)(Dims32<P>::BS - (size_t)N3 * N3);
+  if constexpr (PAD > 0)
+    if (tid < PAD) Ob[(size_t)N3 * N3 + tid] = 0.0f;
 }
 
 // Vector transforms per (polyhedron, component): mode 0 ~u = R u, mode 1
@@ -1184,7 +1207,7 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog32(const float* __restrict
     const int* __restrict__ order) {
   // all vectors in the mass-orthonormal basis (K is ~K); Rp is unused
   using D = Dims<P>;
-  constexpr int N3 = D::N3, NQUAD = Dims32<P>::NQUAD;
+  constexpr int N3 = D::N3, NQUAD = Dims32<P>::NQ, TAIL = Dims32<P>::T;
   constexpr int RPL = (N3 + 63) / 64;
   extern __shared__ double sblk[];  // WPP > 1: per-block products [block][row]
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -1242,6 +1265,26 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog32(const float* __restrict
               s[k] = s[k] + kv[uu][k].w * xq[3];
             }
           }
+        }
+      }
+    }
+    if constexpr (TAIL > 0) {  // last N3 % 4 columns, stored row by row
+      constexpr int C0 = 4 * NQUAD;
+      float xt[TAIL];
+#pragma unroll
+      for (int j = 0; j < TAIL; ++j) {
+        const int c = C0 + j;
+        const float src = (RPL > 1 && (c >> 6)) ? xl[RPL - 1] : xl[0];
+        xt[j] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src), c & 63));
+      }
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) {
+        const int r = lane + 64 * k;
+        if (r < N3) {
+          float kt[TAIL];
+          load_tail<TAIL>(Kb + (size_t)C0 * N3 + (size_t)r * TAIL, kt);
+#pragma unroll
+          for (int j = 0; j < TAIL; ++j) s[k] = s[k] + kt[j] * xt[j];
         }
       }
     }
