@@ -59,11 +59,11 @@ def stream_probe_gbps():
     return tuple(out)
 
 
-def pmc_traffic(n, p):
+def pmc_traffic(n, p, kernel="k_leapfrog"):
     """HBM bytes per step-kernel launch measured by the FETCH_SIZE/WRITE_SIZE
     counter runs summarised in profiles/ (calibrated against the copy and
     read probes), for the workload they were taken on; else None."""
-    path = os.path.join(ROOT, "profiles", f"k_leapfrog_cube{n}_p{p}_pmc_traffic.md")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_cube{n}_p{p}_pmc_traffic.md")
     if not os.path.exists(path):
         return None, None
     for line in open(path):
@@ -353,13 +353,16 @@ def main():
         u32 = ctx.get_slot(_lib.SLOT_U_CUR)
         ms32_step = ms32 / a.steps
         B32 = ctx.step_bytes_for(32)
+        traffic32, traffic32_src = pmc_traffic(a.n, p, "k_leapfrog32")
         res["fp32_apply"] = {
             "value": dofs / (ms32_step * 1e-3), "unit": "DOF-updates/s", "ms_per_step": ms32_step,
             "speedup_vs_fp64": ms_step / ms32_step,
             "roofline": {"bound": "hbm", "achieved": B32 / (ms32_step * 1e-3) / 1e9,
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": B32 / (ms32_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                         "bytes_per_step": B32},
+                         "traffic": traffic32, "traffic_source": traffic32_src,
+                         "traffic_over_algorithmic": (traffic32 / B32) if traffic32 else None,
+                         "bytes_per_step": B32, "kernel": "k_leapfrog32<3,0>"},
             "rel_l2_diff_vs_fp64": float(np.linalg.norm(u32 - u) / np.linalg.norm(u)),
             "err_l2": float(np.sqrt(ctx.compute_error_l2_matrix_free(u32, uex))),
             "autotune": {"chosen": list(ch32),
