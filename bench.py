@@ -329,7 +329,7 @@ def main():
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
             "traffic_over_algorithmic": (traffic / B) if traffic else None,
-            "bytes_per_step": B, "kernel": "k_leapfrog<3,0>",
+            "bytes_per_step": B, "kernel": f"k_leapfrog<{p},0>",
             "stream_copy_GBps": copy,
             "frac_of_stream_copy": (achieved / copy) if copy else None,
             "stream_read_GBps": read,
@@ -362,7 +362,7 @@ def main():
                          "frac": B32 / (ms32_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
                          "traffic": traffic32, "traffic_source": traffic32_src,
                          "traffic_over_algorithmic": (traffic32 / B32) if traffic32 else None,
-                         "bytes_per_step": B32, "kernel": "k_leapfrog32<3,0>"},
+                         "bytes_per_step": B32, "kernel": f"k_leapfrog32<{p},0>"},
             "rel_l2_diff_vs_fp64": float(np.linalg.norm(u32 - u) / np.linalg.norm(u)),
             "err_l2": float(np.sqrt(ctx.compute_error_l2_matrix_free(u32, uex))),
             "autotune": {"chosen": list(ch32),
