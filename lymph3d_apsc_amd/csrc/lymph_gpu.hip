@@ -134,14 +134,14 @@ template <> struct StepVar<2> {
   static constexpr int wpp[] = {1, 1, 1, 1, 1, 2, 4, 1};
 };
 template <> struct StepVar<3> {
-  static constexpr int unr[] = {10, 10, 5, 6, 15, 30, 30, 3, 30, 10};
-  static constexpr int xs[] = {1, 0, 1, 1, 1, 1, 0, 1, 1, 1};
-  static constexpr int wpp[] = {1, 1, 1, 1, 1, 1, 1, 1, 2, 4};
+  static constexpr int unr[] = {10, 10, 5, 6, 15, 30, 30, 3, 30, 10, 30, 15};
+  static constexpr int xs[] = {1, 0, 1, 1, 1, 1, 0, 1, 1, 1, 1, 1};
+  static constexpr int wpp[] = {1, 1, 1, 1, 1, 1, 1, 1, 2, 4, 4, 2};
 };
 template <> struct StepVar<4> {
-  static constexpr int unr[] = {4, 4, 2, 8, 1, 4, 4, 4, 2};
-  static constexpr int xs[] = {1, 0, 1, 1, 1, 0, 1, 0, 1};
-  static constexpr int wpp[] = {1, 1, 1, 1, 1, 2, 2, 4, 4};
+  static constexpr int unr[] = {4, 4, 2, 8, 1, 4, 4, 4, 2, 8, 8};
+  static constexpr int xs[] = {1, 0, 1, 1, 1, 0, 1, 0, 1, 1, 0};
+  static constexpr int wpp[] = {1, 1, 1, 1, 1, 2, 2, 4, 4, 4, 4};
 };
 template <int P> constexpr int n_var() { return (int)(sizeof(StepVar<P>::unr) / sizeof(int)); }
 
@@ -191,16 +191,21 @@ template <> struct StepVar32<2> {
   static constexpr int pk[] = {0, 0, 0, 0, 0, 1};
 };
 template <> struct StepVar32<3> {
-  static constexpr int unr[] = {5, 3, 15, 5, 5};
-  static constexpr int wpp[] = {1, 1, 1, 2, 4};
-  static constexpr int pk[] = {0, 0, 0, 0, 0};
+  static constexpr int unr[] = {5, 3, 15, 5, 5, 15, 15, 8};
+  static constexpr int wpp[] = {1, 1, 1, 2, 4, 2, 4, 4};
+  static constexpr int pk[] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 template <> struct StepVar32<4> {
-  static constexpr int unr[] = {3, 1, 9, 3, 3};
-  static constexpr int wpp[] = {1, 1, 1, 2, 4};
-  static constexpr int pk[] = {0, 0, 0, 0, 0};
+  static constexpr int unr[] = {3, 1, 9, 3, 3, 9, 9, 13};
+  static constexpr int wpp[] = {1, 1, 1, 2, 4, 2, 4, 4};
+  static constexpr int pk[] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 template <int P> constexpr int n_var32() { return (int)(sizeof(StepVar32<P>::unr) / sizeof(int)); }
+// lymph_autotune reports into caller arrays of 16 entries
+static_assert(n_var<1>() <= 16 && n_var<2>() <= 16 && n_var<3>() <= 16 && n_var<4>() <= 16 &&
+                  n_var32<1>() <= 16 && n_var32<2>() <= 16 && n_var32<3>() <= 16 &&
+                  n_var32<4>() <= 16,
+              "at most 16 step variants per degree");
 
 template <int P, int MODE, int I = 0>
 void launch_leapfrog32(int variant, int grid, hipStream_t st, const float* K, const int* rp,

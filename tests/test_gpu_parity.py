@@ -273,6 +273,33 @@ def test_instability_control():
         assert r["nsteps"] == 600 and np.isfinite(r["err_l2"])
 
 
+def test_resume_from_slots():
+    """Checkpoint/resume: the state after k steps (U_PREV, U_CUR and t) read
+    out with get_slot and loaded into a fresh context continues the run bit
+    for bit."""
+    say("[resume]")
+    m = make_mesh("cube3agg")
+    p, dt = 3, 2e-3
+    with L.Context(m, p) as a:
+        a.make_matrices_free()
+        a.make_rhs_free(L.PROB_NULL, _lib.SLOT_RHS_STAT)
+        a.make_rhs_free(L.PROB_CUBE_WAVE, _lib.SLOT_RHS_DYN)
+        u0 = a.compute_modal_coefficients_free(L.FN_CUBE_U, _lib.SLOT_U_PREV)
+        a.set_slot(_lib.SLOT_U_CUR, a.first_time_step_matrix_free(1.0, dt, u0, np.zeros_like(u0)))
+        t = a.step_n(L.PROB_CUBE_WAVE, 40, dt, dt)
+        ck = (a.get_slot(_lib.SLOT_U_PREV), a.get_slot(_lib.SLOT_U_CUR), t)
+        t_end = a.step_n(L.PROB_CUBE_WAVE, 60, t, dt)
+        full = a.get_slot(_lib.SLOT_U_CUR)
+    with L.Context(m, p) as b:
+        b.make_matrices_free()
+        b.make_rhs_free(L.PROB_NULL, _lib.SLOT_RHS_STAT)
+        b.make_rhs_free(L.PROB_CUBE_WAVE, _lib.SLOT_RHS_DYN)
+        b.set_slot(_lib.SLOT_U_PREV, ck[0])
+        b.set_slot(_lib.SLOT_U_CUR, ck[1])
+        assert b.step_n(L.PROB_CUBE_WAVE, 60, ck[2], dt) == t_end
+        assert np.array_equal(b.get_slot(_lib.SLOT_U_CUR), full)
+
+
 @pytest.mark.parametrize("p", [2, 3])
 def test_receivers_and_traces(p):
     """Receiver values (EXPORT_SOLUTION's point evaluation) of the GPU state vs
@@ -382,7 +409,7 @@ def test_fp32_apply_on_ill_conditioned_agglomerates():
     assert np.linalg.norm(a32 - a64) < 1e-3 * np.linalg.norm(a64)
 
 
-@pytest.mark.parametrize("name,p", [("hetero", 1), ("hetero", 2), ("grown4", 4)])
+@pytest.mark.parametrize("name,p", [("hetero", 1), ("hetero", 2), ("hetero", 3), ("grown4", 4)])
 def test_step_variants_agree_bitwise(name, p):
     """Every step-kernel variant -- load group size, neighbour broadcast mode,
     waves per polyhedron, several polyhedra per wave at p <= 2 -- gives bit
