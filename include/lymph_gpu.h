@@ -182,7 +182,8 @@ int lymph_time_steps(LymphCtx* ctx, int problem, int nsteps, double t0,
  * variant i, unr[i] = column pairs (quads for the 32-bit apply) loaded per
  * unrolled group, xmode[i] = how the neighbour's coefficients reach the lanes
  * (0 scalar loads, 1 vector load + lane broadcast, 2 several polyhedra per
- * wave with the coefficients exchanged through LDS, p <= 2), wpp[i] = waves sharing one
+ * wave with the coefficients exchanged through LDS, p <= 2, 3 the same with the
+ * wave's polyhedra walking their blocks jointly), wpp[i] = waves sharing one
  * polyhedron's blocks (their block products are added in block order, so
  * results do not depend on it), ms[i] = milliseconds per step (-1: not run,
  * the longest block row does not fit the variant's LDS); arrays of at least

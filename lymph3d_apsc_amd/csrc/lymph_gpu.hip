@@ -124,14 +124,14 @@ int dispatch_p(int p, F&& f) {
 // polyhedron; index 0 is the default until lymph_autotune picks one.
 template <int P> struct StepVar;
 template <> struct StepVar<1> {
-  static constexpr int unr[] = {6, 6, 3, 2, 6, 1};
-  static constexpr int xs[] = {1, 0, 1, 1, 1, 2};
-  static constexpr int wpp[] = {1, 1, 1, 1, 2, 1};
+  static constexpr int unr[] = {6, 6, 3, 2, 6, 1, 1};
+  static constexpr int xs[] = {1, 0, 1, 1, 1, 2, 3};
+  static constexpr int wpp[] = {1, 1, 1, 1, 2, 1, 1};
 };
 template <> struct StepVar<2> {
-  static constexpr int unr[] = {5, 5, 15, 3, 15, 5, 5, 1};
-  static constexpr int xs[] = {1, 0, 1, 1, 0, 1, 1, 2};
-  static constexpr int wpp[] = {1, 1, 1, 1, 1, 2, 4, 1};
+  static constexpr int unr[] = {5, 5, 15, 3, 15, 5, 5, 1, 1};
+  static constexpr int xs[] = {1, 0, 1, 1, 0, 1, 1, 2, 3};
+  static constexpr int wpp[] = {1, 1, 1, 1, 1, 2, 4, 1, 1};
 };
 template <> struct StepVar<3> {
   static constexpr int unr[] = {10, 10, 5, 6, 15, 30, 30, 3, 30, 10, 30, 15};
@@ -162,6 +162,16 @@ void launch_leapfrog(int variant, int grid, hipStream_t st, const double* K, con
         k_leapfrog_packed<P, MODE, 64><<<step_grid((grid + G - 1) / G), 64, 0, st>>>(
             K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid);
         return;
+      } else if constexpr (StepVar<P>::xs[I] == 3) {
+        constexpr int G = 64 / (3 * np_of(P));
+        if ((size_t)G * dyn > kMaxDynLds) {
+          launch_leapfrog<P, MODE, 0>(0, grid, st, K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2,
+                                      gsel, gtab, ord, dyn);
+          return;
+        }
+        k_leapfrog_joint<P, MODE, 64><<<step_grid((grid + G - 1) / G), 64, G * dyn, st>>>(
+            K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid);
+        return;
       } else {
       if (W > 1 && dyn > kMaxDynLds) {
         launch_leapfrog<P, MODE, 0>(0, grid, st, K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2,
@@ -181,14 +191,14 @@ void launch_leapfrog(int variant, int grid, hipStream_t st, const double* K, con
 
 template <int P> struct StepVar32;
 template <> struct StepVar32<1> {
-  static constexpr int unr[] = {3, 1, 3, 1};
-  static constexpr int wpp[] = {1, 1, 2, 1};
-  static constexpr int pk[] = {0, 0, 0, 1};
+  static constexpr int unr[] = {3, 1, 3, 1, 1};
+  static constexpr int wpp[] = {1, 1, 2, 1, 1};
+  static constexpr int pk[] = {0, 0, 0, 1, 2};
 };
 template <> struct StepVar32<2> {
-  static constexpr int unr[] = {4, 2, 8, 4, 4, 1};
-  static constexpr int wpp[] = {1, 1, 1, 2, 4, 1};
-  static constexpr int pk[] = {0, 0, 0, 0, 0, 1};
+  static constexpr int unr[] = {4, 2, 8, 4, 4, 1, 1};
+  static constexpr int wpp[] = {1, 1, 1, 2, 4, 1, 1};
+  static constexpr int pk[] = {0, 0, 0, 0, 0, 1, 2};
 };
 template <> struct StepVar32<3> {
   static constexpr int unr[] = {5, 3, 15, 5, 5, 15, 15, 8};
@@ -218,6 +228,16 @@ void launch_leapfrog32(int variant, int grid, hipStream_t st, const float* K, co
       if constexpr (StepVar32<P>::pk[I] == 1) {
         constexpr int G = 64 / (3 * np_of(P));
         k_leapfrog_packed<P, MODE, 32><<<step_grid((grid + G - 1) / G), 64, 0, st>>>(
+            K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid);
+        return;
+      } else if constexpr (StepVar32<P>::pk[I] == 2) {
+        constexpr int G = 64 / (3 * np_of(P));
+        if ((size_t)G * dyn > kMaxDynLds) {
+          launch_leapfrog32<P, MODE, 0>(0, grid, st, K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2,
+                                        gsel, gtab, ord, dyn);
+          return;
+        }
+        k_leapfrog_joint<P, MODE, 32><<<step_grid((grid + G - 1) / G), 64, G * dyn, st>>>(
             K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid);
         return;
       } else {
@@ -991,7 +1011,7 @@ int lymph_autotune(LymphCtx* c, int reps, int* n_variants, int* unr, int* xmode,
     for (int v = 0; v < NV; ++v) {
       const int w = b32 ? StepVar32<P>::wpp[v] : StepVar<P>::wpp[v];
       if (unr) unr[v] = b32 ? StepVar32<P>::unr[v] : StepVar<P>::unr[v];
-      if (xmode) xmode[v] = b32 ? (StepVar32<P>::pk[v] ? 2 : 1) : StepVar<P>::xs[v];
+      if (xmode) xmode[v] = b32 ? 1 + StepVar32<P>::pk[v] : StepVar<P>::xs[v];
       if (wpp) wpp[v] = w;
       if (w > 1 && !fits) {  // would run variant 0: not a candidate
         if (ms) ms[v] = -1.0;

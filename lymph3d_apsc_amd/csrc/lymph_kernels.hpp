@@ -1064,6 +1064,144 @@ __global__ __launch_bounds__(64) void k_leapfrog_packed(const void* __restrict__
   }
 }
 
+// Packed step with a joint block walk (3Np <= 32): the G polyhedra of a wave
+// own the contiguous block range row_ptr[P0] .. row_ptr[P0 + G]; at loop
+// iteration j lane group s takes block B0 + j G + s, so each iteration
+// streams G consecutive blocks and no cache line is left half read for a
+// later iteration (32-bit blocks of 576 B at p = 1 end mid-line).  Each
+// block's product goes to LDS; each row then adds its own blocks in block
+// order, so results are bit-identical to k_leapfrog.  Dynamic LDS: the
+// products of the wave's longest block range (G x longest row x 3Np doubles).
+template <int P, int MODE, int TB>
+__global__ __launch_bounds__(64) void k_leapfrog_joint(const void* __restrict__ Kv,
+                                                       const int* __restrict__ row_ptr,
+                                                       const int* __restrict__ col,
+                                                       const double* __restrict__ Rp,
+                                                       const double* __restrict__ rs,
+                                                       const double* __restrict__ rd,
+                                                       const double* __restrict__ u,
+                                                       const double* __restrict__ wa,
+                                                       double* __restrict__ out, double g,
+                                                       double c1, double c2,
+                                                       const int* __restrict__ gsel,
+                                                       const double* __restrict__ gtab,
+                                                       int n_poly) {
+  using D = Dims<P>;
+  constexpr int NP = D::NP, N3 = D::N3, NPAIR = D::NPAIR;
+  static_assert(N3 <= 32, "joint packed step needs 3Np <= 32");
+  constexpr int G = 64 / N3;
+  constexpr int NQUAD = Dims32<P>::NQ, TAIL = Dims32<P>::T;
+  __shared__ double sx[2][G][N3 + 1];
+  __shared__ double sw[G][N3];
+  __shared__ double sR[G][D::NPK];
+  extern __shared__ double prod[];  // [block - B0][row]
+  const int lane = threadIdx.x;
+  const int sub = lane / N3, r = lane % N3;
+  const int P0 = step_poly() * G;
+  const int Pp = P0 + sub;
+  const bool act = sub < G && Pp < n_poly;
+  if (MODE != 2 && TB == 64)
+    for (int k = lane; k < G * D::NPK; k += 64) {
+      const int sp = k / D::NPK, e = k % D::NPK, Q = P0 + sp;
+      if (Q < n_poly) sR[sp][e] = Rp[(size_t)Q * D::NPK + e];
+    }
+  if (P0 >= n_poly) return;  // padding workgroup: uniform exit, no barrier passed
+  const int B0 = row_ptr[P0], B1 = row_ptr[min(P0 + G, n_poly)];
+  const int nblk = B1 - B0;
+  int bj = B0 + sub;  // this lane group's block at iteration 0
+  double xn = (sub < G && bj < B1) ? u[(size_t)col[bj] * N3 + r] : 0.0;
+  for (int j = 0; j * G < nblk; ++j, bj += G) {
+    const bool on = sub < G && bj < B1;
+    const int buf = j & 1;
+    double2 kd[TB == 64 ? NPAIR : 1];
+    float4 kf[TB == 32 ? NQUAD : 1];
+    float kt[TB == 32 && TAIL > 0 ? TAIL : 1];
+    if (on) {
+      if constexpr (TB == 64) {
+        const double* Kb = static_cast<const double*>(Kv) + (size_t)bj * D::BS;
+#pragma unroll
+        for (int c2i = 0; c2i < NPAIR; ++c2i) kd[c2i] = load_stream(Kb + ((size_t)c2i * N3 + r) * 2);
+      } else {
+        const float* Kb = static_cast<const float*>(Kv) + (size_t)bj * Dims32<P>::BS;
+#pragma unroll
+        for (int q = 0; q < NQUAD; ++q) kf[q] = load_stream(Kb + ((size_t)q * N3 + r) * 4);
+        if constexpr (TAIL > 0) load_tail<TAIL>(Kb + (size_t)4 * NQUAD * N3 + (size_t)r * TAIL, kt);
+      }
+      sx[buf][sub][r] = xn;
+      if (bj + G < B1) xn = u[(size_t)col[bj + G] * N3 + r];
+    }
+    __syncthreads();
+    if (on) {
+      double s = 0.0;
+      if constexpr (TB == 64) {
+#pragma unroll
+        for (int c2i = 0; c2i < NPAIR; ++c2i) {
+          const bool odd_tail = (N3 % 2 == 1) && (c2i == NPAIR - 1);
+          s = s + kd[c2i].x * sx[buf][sub][2 * c2i];
+          if (!odd_tail) s = s + kd[c2i].y * sx[buf][sub][2 * c2i + 1];
+        }
+      } else {
+        float sf = 0.0f;
+#pragma unroll
+        for (int q = 0; q < NQUAD; ++q) {
+          float xq[4];
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj) xq[jj] = (float)sx[buf][sub][4 * q + jj];
+          sf = sf + kf[q].x * xq[0];
+          sf = sf + kf[q].y * xq[1];
+          sf = sf + kf[q].z * xq[2];
+          sf = sf + kf[q].w * xq[3];
+        }
+        if constexpr (TAIL > 0) {
+#pragma unroll
+          for (int jj = 0; jj < TAIL; ++jj) sf = sf + kt[jj] * (float)sx[buf][sub][4 * NQUAD + jj];
+        }
+        s = (double)sf;
+      }
+      prod[(size_t)(bj - B0) * N3 + r] = s;
+    }
+  }
+  __syncthreads();
+  double acc = 0.0;
+  if (act) {  // this row's blocks, in block order
+    const int a0 = row_ptr[Pp] - B0, a1 = row_ptr[Pp + 1] - B0;
+    acc = prod[(size_t)a0 * N3 + r];
+    for (int k = a0 + 1; k < a1; ++k) acc = acc + prod[(size_t)k * N3 + r];
+  }
+  const size_t o = (size_t)Pp * N3 + r;
+  if (MODE == 2) {
+    if (act) sw[sub][r] = acc;
+    __syncthreads();
+    if (act && r == 0) {
+      double el = 0.0;
+      for (int k = 0; k < N3; ++k) el = el + u[(size_t)Pp * N3 + k] * sw[sub][k];
+      out[Pp] = el;
+    }
+    return;
+  }
+  double w = 0.0;
+  if (act) {
+    const double gp = gsel ? gtab[gsel[Pp]] : g;
+    w = -acc + rs[o] + rd[o] * gp;
+  }
+  if constexpr (TB == 64) {
+    if (act) sw[sub][r] = w;
+    __syncthreads();
+    if (lane < 3 * G) {
+      const int sp = lane / 3, comp = lane % 3;
+      if (P0 + sp < n_poly) solve_packed<NP>(sR[sp], sw[sp] + comp * NP);
+    }
+    __syncthreads();
+    if (act) w = sw[sub][r];
+  }
+  if (act) {
+    if (MODE == 0)
+      out[o] = c1 * w + 2.0 * u[o] - wa[o];
+    else
+      out[o] = c1 * w + u[o] + c2 * wa[o];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // FP32-apply variant of the step: the operator blocks, taken to the
 // mass-orthonormal basis in FP64 (k_ortho_operator below), are held in FP32 in
