@@ -46,17 +46,21 @@ def log(*a):
 
 def stream_probe_gbps():
     """Achievable HBM bandwidth of this box: (copy GB/s, read GB/s) of 2 GiB
-    STREAM-style kernels (liblymph_bwprobe.so), or (None, None)."""
+    STREAM-style kernels (liblymph_bwprobe.so), or (None, None).  The read
+    figure is the better of plain loads (k_read, 4 per thread) and
+    non-temporal loads (k_read_nt, 8 per thread: the way the step kernel
+    streams its blocks)."""
     path = os.path.join(ROOT, "lymph3d_apsc_amd", "liblymph_bwprobe.so")
     if not os.path.exists(path):
         return None, None
     L = C.CDLL(path)
     out = []
-    for fn in (L.lymph_bw_copy, L.lymph_bw_read):
+    for fn in (L.lymph_bw_copy, L.lymph_bw_read, L.lymph_bw_read_nt):
         fn.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         g, ms = C.c_double(), C.c_double()
         out.append(g.value if fn(2 << 30, 20, C.byref(g), C.byref(ms)) == 0 else None)
-    return tuple(out)
+    reads = [r for r in out[1:] if r]
+    return out[0], (max(reads) if reads else None)
 
 
 def pmc_traffic(n, p, kernel="k_leapfrog"):

@@ -49,9 +49,39 @@ __global__ __launch_bounds__(kThreads) void k_read(const double2* __restrict__ a
   }
 }
 
-int run(bool copy, size_t bytes, int reps, double* gbps, double* ms_per) {
+// Read ceiling as the step kernel streams its blocks: non-temporal 16-byte
+// loads, V per thread (8, 16 or 32), all issued before any use.
+typedef double dv2_t __attribute__((ext_vector_type(2)));
+template <int V>
+__global__ __launch_bounds__(kThreads) void k_read_nt(const double2* __restrict__ a,
+                                                      double* __restrict__ out, size_t n) {
+  const size_t base = (size_t)blockIdx.x * kThreads * V + threadIdx.x;
+  dv2_t v[V];
+#pragma unroll
+  for (int k = 0; k < V; ++k) {
+    const size_t i = base + (size_t)k * kThreads;
+    v[k] = i < n ? __builtin_nontemporal_load(reinterpret_cast<const dv2_t*>(a + i)) : dv2_t{0.0, 0.0};
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < V; ++k) s += v[k].x + v[k].y;
+  __shared__ double red[kThreads / 64];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kThreads / 64; ++w) t += red[w];
+    out[blockIdx.x] = t;
+  }
+}
+
+// kind: 0 copy, 1 read, 8/16/32 non-temporal read with that many loads per thread
+int run(int kind, size_t bytes, int reps, double* gbps, double* ms_per) {
+  const bool copy = kind == 0;
+  const int per = kind >= 8 ? kind : kVec;
   const size_t n = bytes / sizeof(double2);
-  const size_t grid = (n + (size_t)kThreads * kVec - 1) / ((size_t)kThreads * kVec);
+  const size_t grid = (n + (size_t)kThreads * per - 1) / ((size_t)kThreads * per);
   if (n == 0 || reps <= 0 || grid > 0x7fffffffu) return 1;
   double2* a = nullptr;
   double2* b = nullptr;
@@ -69,8 +99,14 @@ int run(bool copy, size_t bytes, int reps, double* gbps, double* ms_per) {
   auto launch = [&]() {
     if (copy)
       k_copy<<<(unsigned)grid, kThreads>>>(a, b, n);
-    else
+    else if (kind == 1)
       k_read<<<(unsigned)grid, kThreads>>>(a, out, n);
+    else if (kind == 8)
+      k_read_nt<8><<<(unsigned)grid, kThreads>>>(a, out, n);
+    else if (kind == 16)
+      k_read_nt<16><<<(unsigned)grid, kThreads>>>(a, out, n);
+    else
+      k_read_nt<32><<<(unsigned)grid, kThreads>>>(a, out, n);
   };
   launch();
   (void)hipEventRecord(e0, 0);
@@ -94,10 +130,27 @@ int run(bool copy, size_t bytes, int reps, double* gbps, double* ms_per) {
 
 // Copy `bytes` (read + write counted) `reps` times; GB/s and ms per launch.
 extern "C" int lymph_bw_copy(size_t bytes, int reps, double* gbps, double* ms_per) {
-  return run(true, bytes, reps, gbps, ms_per);
+  return run(0, bytes, reps, gbps, ms_per);
 }
 
 // Read `bytes` `reps` times (one double written per 256-thread block).
 extern "C" int lymph_bw_read(size_t bytes, int reps, double* gbps, double* ms_per) {
-  return run(false, bytes, reps, gbps, ms_per);
+  return run(1, bytes, reps, gbps, ms_per);
+}
+
+// The same read with non-temporal 16-byte loads, 8, 16 and 32 in flight per
+// thread; the fastest is reported.
+extern "C" int lymph_bw_read_nt(size_t bytes, int reps, double* gbps, double* ms_per) {
+  double best = 0.0, best_ms = 0.0;
+  for (int v : {8, 16, 32}) {
+    double g = 0.0, ms = 0.0;
+    if (int rc = run(v, bytes, reps, &g, &ms)) return rc;
+    if (g > best) {
+      best = g;
+      best_ms = ms;
+    }
+  }
+  *gbps = best;
+  *ms_per = best_ms;
+  return 0;
 }

@@ -219,13 +219,16 @@ def test_reference_known_answers(name, build):
     assert rel(r["u"], ro["u"]) < 1e-11, rel(r["u"], ro["u"])
 
 
-@pytest.mark.parametrize("n,p,steps", [(4, 2, 300), (3, 3, 200), (2, 4, 100)])
-def test_agglomerated_cube_wave_run(n, p, steps):
+@pytest.mark.parametrize("n,p,steps,dt", [(4, 2, 300, None), (3, 3, 200, None), (2, 4, 100, None),
+                                         (5, 2, 1000, 1e-4)])
+def test_agglomerated_cube_wave_run(n, p, steps, dt):
     """Agglomerated polyhedra (one per cube), the standing-wave data set, a
-    CFL-derived dt: final vector and errors vs the oracle."""
-    say(f"[cube wave] n={n} p={p}")
+    CFL-derived dt (or the given one: 5^3 cubes, p=2, dt=1e-4, 1000 steps is
+    the CPU-runnable configuration of the plain method): final vector and
+    errors vs the oracle."""
+    say(f"[cube wave] n={n} p={p} steps={steps}")
     m = M.cube_mesh(n, agglomerate=True)
-    dt = 0.2 * (1.0 / n) / (np.sqrt(3.0) * p * p)
+    dt = dt or 0.2 * (1.0 / n) / (np.sqrt(3.0) * p * p)
     with L.Context(m, p) as ctx:
         r = ctx.run(L.PROB_CUBE_WAVE, dt, 1e9, max_steps=steps, want_u=True)
     ro = oracle_for(m, p).run(oracle.PROB_CUBE_WAVE, dt, 1e9, max_steps=steps, want_u=True)
