@@ -124,14 +124,14 @@ int dispatch_p(int p, F&& f) {
 // polyhedron; index 0 is the default until lymph_autotune picks one.
 template <int P> struct StepVar;
 template <> struct StepVar<1> {
-  static constexpr int unr[] = {6, 6, 3, 2, 6, 1, 1};
-  static constexpr int xs[] = {1, 0, 1, 1, 1, 2, 3};
-  static constexpr int wpp[] = {1, 1, 1, 1, 2, 1, 1};
+  static constexpr int unr[] = {6, 6, 3, 2, 6, 1, 1, 1, 1};
+  static constexpr int xs[] = {1, 0, 1, 1, 1, 2, 3, 2, 3};
+  static constexpr int wpp[] = {1, 1, 1, 1, 2, 1, 1, 4, 4};
 };
 template <> struct StepVar<2> {
-  static constexpr int unr[] = {5, 5, 15, 3, 15, 5, 5, 1, 1};
-  static constexpr int xs[] = {1, 0, 1, 1, 0, 1, 1, 2, 3};
-  static constexpr int wpp[] = {1, 1, 1, 1, 1, 2, 4, 1, 1};
+  static constexpr int unr[] = {5, 5, 15, 3, 15, 5, 5, 1, 1, 1, 1};
+  static constexpr int xs[] = {1, 0, 1, 1, 0, 1, 1, 2, 3, 2, 3};
+  static constexpr int wpp[] = {1, 1, 1, 1, 1, 2, 4, 1, 1, 4, 4};
 };
 template <> struct StepVar<3> {
   static constexpr int unr[] = {10, 10, 5, 6, 15, 30, 30, 3, 30, 10, 30, 15};
@@ -159,18 +159,20 @@ void launch_leapfrog(int variant, int grid, hipStream_t st, const double* K, con
       constexpr int W = StepVar<P>::wpp[I];
       if constexpr (StepVar<P>::xs[I] == 2) {
         constexpr int G = 64 / (3 * np_of(P));
-        k_leapfrog_packed<P, MODE, 64><<<step_grid((grid + G - 1) / G), 64, 0, st>>>(
+        const int groups = (grid + G - 1) / G;
+        k_leapfrog_packed<P, MODE, 64, W><<<step_grid((groups + W - 1) / W), 64 * W, 0, st>>>(
             K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid);
         return;
       } else if constexpr (StepVar<P>::xs[I] == 3) {
         constexpr int G = 64 / (3 * np_of(P));
-        if ((size_t)G * dyn > kMaxDynLds) {
+        if ((size_t)W * G * dyn > kMaxDynLds) {
           launch_leapfrog<P, MODE, 0>(0, grid, st, K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2,
                                       gsel, gtab, ord, dyn);
           return;
         }
-        k_leapfrog_joint<P, MODE, 64><<<step_grid((grid + G - 1) / G), 64, G * dyn, st>>>(
-            K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid);
+        const int groups = (grid + G - 1) / G;
+        k_leapfrog_joint<P, MODE, 64, W><<<step_grid((groups + W - 1) / W), 64 * W, W * G * dyn, st>>>(
+            K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid, (int)(G * dyn / 8));
         return;
       } else {
       if (W > 1 && dyn > kMaxDynLds) {
@@ -191,14 +193,14 @@ void launch_leapfrog(int variant, int grid, hipStream_t st, const double* K, con
 
 template <int P> struct StepVar32;
 template <> struct StepVar32<1> {
-  static constexpr int unr[] = {3, 1, 3, 1, 1};
-  static constexpr int wpp[] = {1, 1, 2, 1, 1};
-  static constexpr int pk[] = {0, 0, 0, 1, 2};
+  static constexpr int unr[] = {3, 1, 3, 1, 1, 1, 1};
+  static constexpr int wpp[] = {1, 1, 2, 1, 1, 4, 4};
+  static constexpr int pk[] = {0, 0, 0, 1, 2, 1, 2};
 };
 template <> struct StepVar32<2> {
-  static constexpr int unr[] = {4, 2, 8, 4, 4, 1, 1};
-  static constexpr int wpp[] = {1, 1, 1, 2, 4, 1, 1};
-  static constexpr int pk[] = {0, 0, 0, 0, 0, 1, 2};
+  static constexpr int unr[] = {4, 2, 8, 4, 4, 1, 1, 1, 1};
+  static constexpr int wpp[] = {1, 1, 1, 2, 4, 1, 1, 4, 4};
+  static constexpr int pk[] = {0, 0, 0, 0, 0, 1, 2, 1, 2};
 };
 template <> struct StepVar32<3> {
   static constexpr int unr[] = {5, 3, 15, 5, 5, 15, 15, 8};
@@ -227,18 +229,20 @@ void launch_leapfrog32(int variant, int grid, hipStream_t st, const float* K, co
       constexpr int W = StepVar32<P>::wpp[I];
       if constexpr (StepVar32<P>::pk[I] == 1) {
         constexpr int G = 64 / (3 * np_of(P));
-        k_leapfrog_packed<P, MODE, 32><<<step_grid((grid + G - 1) / G), 64, 0, st>>>(
+        const int groups = (grid + G - 1) / G;
+        k_leapfrog_packed<P, MODE, 32, W><<<step_grid((groups + W - 1) / W), 64 * W, 0, st>>>(
             K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid);
         return;
       } else if constexpr (StepVar32<P>::pk[I] == 2) {
         constexpr int G = 64 / (3 * np_of(P));
-        if ((size_t)G * dyn > kMaxDynLds) {
+        if ((size_t)W * G * dyn > kMaxDynLds) {
           launch_leapfrog32<P, MODE, 0>(0, grid, st, K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2,
                                         gsel, gtab, ord, dyn);
           return;
         }
-        k_leapfrog_joint<P, MODE, 32><<<step_grid((grid + G - 1) / G), 64, G * dyn, st>>>(
-            K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid);
+        const int groups = (grid + G - 1) / G;
+        k_leapfrog_joint<P, MODE, 32, W><<<step_grid((groups + W - 1) / W), 64 * W, W * G * dyn, st>>>(
+            K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid, (int)(G * dyn / 8));
         return;
       } else {
       if (W > 1 && dyn > kMaxDynLds) {

@@ -932,6 +932,22 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog(const double* __restrict_
   }
 }
 
+// Barrier of the packed kernels: a whole workgroup for one-wave workgroups;
+// with several waves per workgroup each wave owns its own LDS region and
+// polyhedra (and its own loop trip count), so only the wave's own LDS
+// traffic is ordered (the rocPRIM wave barrier: fences at wavefront scope
+// around a code-motion barrier; a wave's LDS operations complete in order).
+template <int WPW>
+__device__ inline void lsync() {
+  if constexpr (WPW == 1) {
+    __syncthreads();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
 // Packed variant of the step for small blocks (3Np <= 32: p = 1, 2): one
 // wave carries G = 64 / 3Np polyhedra, lane = (polyhedron, row), so lanes are
 // not left idle by short rows.  Each polyhedron walks its own block row; the
@@ -940,8 +956,8 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog(const double* __restrict_
 // ones of k_leapfrog, so results are bit-identical to it.  TB = 64 (FP64
 // operator in column pairs) or 32 (mass-orthonormal FP32 operator in column
 // quads, no solves).
-template <int P, int MODE, int TB>
-__global__ __launch_bounds__(64) void k_leapfrog_packed(const void* __restrict__ Kv,
+template <int P, int MODE, int TB, int WPW = 1>
+__global__ __launch_bounds__(64 * WPW) void k_leapfrog_packed(const void* __restrict__ Kv,
                                                         const int* __restrict__ row_ptr,
                                                         const int* __restrict__ col,
                                                         const double* __restrict__ Rp,
@@ -960,16 +976,21 @@ __global__ __launch_bounds__(64) void k_leapfrog_packed(const void* __restrict__
   constexpr int G = 64 / N3;
   constexpr int NQUAD = Dims32<P>::NQ, TAIL = Dims32<P>::T;
   // neighbour coefficients, double-buffered by block parity: one barrier per block
-  __shared__ double sx[2][G][N3 + 1];
-  __shared__ double sw[G][N3];
-  __shared__ double sR[G][D::NPK];
-  const int lane = threadIdx.x;
+  __shared__ double sxa[WPW][2][G][N3 + 1];
+  __shared__ double swa[WPW][G][N3];
+  __shared__ double sRa[WPW][G][D::NPK];
+  const int wv = threadIdx.x >> 6;
+  double(&sx)[2][G][N3 + 1] = sxa[wv];
+  double(&sw)[G][N3] = swa[wv];
+  double(&sR)[G][D::NPK] = sRa[wv];
+  const int lane = threadIdx.x & 63;
   const int sub = lane / N3, r = lane % N3;
-  const int Pp = step_poly() * G + sub;
+  const int P0 = (step_poly() * WPW + wv) * G;
+  const int Pp = P0 + sub;
   const bool act = sub < G && Pp < n_poly;
   if (MODE != 2 && TB == 64)
     for (int k = lane; k < G * D::NPK; k += 64) {
-      const int sp = k / D::NPK, e = k % D::NPK, Q = step_poly() * G + sp;
+      const int sp = k / D::NPK, e = k % D::NPK, Q = P0 + sp;
       if (Q < n_poly) sR[sp][e] = Rp[(size_t)Q * D::NPK + e];
     }
   const int b0 = act ? row_ptr[Pp] : 0, cnt = act ? row_ptr[Pp + 1] - b0 : 0;
@@ -998,7 +1019,7 @@ __global__ __launch_bounds__(64) void k_leapfrog_packed(const void* __restrict__
       sx[buf][sub][r] = xn;
       if (j + 1 < cnt) xn = u[(size_t)col[b + 1] * N3 + r];
     }
-    __syncthreads();
+    lsync<WPW>();
     if (on) {
       double s = 0.0;
       if constexpr (TB == 64) {
@@ -1033,7 +1054,7 @@ __global__ __launch_bounds__(64) void k_leapfrog_packed(const void* __restrict__
   const size_t o = (size_t)Pp * N3 + r;
   if (MODE == 2) {
     if (act) sw[sub][r] = acc;
-    __syncthreads();
+    lsync<WPW>();
     if (act && r == 0) {
       double el = 0.0;
       for (int k = 0; k < N3; ++k) el = el + u[(size_t)Pp * N3 + k] * sw[sub][k];
@@ -1048,12 +1069,12 @@ __global__ __launch_bounds__(64) void k_leapfrog_packed(const void* __restrict__
   }
   if constexpr (TB == 64) {
     if (act) sw[sub][r] = w;
-    __syncthreads();
+    lsync<WPW>();
     if (lane < 3 * G) {
       const int sp = lane / 3, comp = lane % 3;
-      if (step_poly() * G + sp < n_poly) solve_packed<NP>(sR[sp], sw[sp] + comp * NP);
+      if (P0 + sp < n_poly) solve_packed<NP>(sR[sp], sw[sp] + comp * NP);
     }
-    __syncthreads();
+    lsync<WPW>();
     if (act) w = sw[sub][r];
   }
   if (act) {
@@ -1072,8 +1093,8 @@ __global__ __launch_bounds__(64) void k_leapfrog_packed(const void* __restrict__
 // block's product goes to LDS; each row then adds its own blocks in block
 // order, so results are bit-identical to k_leapfrog.  Dynamic LDS: the
 // products of the wave's longest block range (G x longest row x 3Np doubles).
-template <int P, int MODE, int TB>
-__global__ __launch_bounds__(64) void k_leapfrog_joint(const void* __restrict__ Kv,
+template <int P, int MODE, int TB, int WPW = 1>
+__global__ __launch_bounds__(64 * WPW) void k_leapfrog_joint(const void* __restrict__ Kv,
                                                        const int* __restrict__ row_ptr,
                                                        const int* __restrict__ col,
                                                        const double* __restrict__ Rp,
@@ -1085,19 +1106,23 @@ __global__ __launch_bounds__(64) void k_leapfrog_joint(const void* __restrict__ 
                                                        double c1, double c2,
                                                        const int* __restrict__ gsel,
                                                        const double* __restrict__ gtab,
-                                                       int n_poly) {
+                                                       int n_poly, int prod_stride) {
   using D = Dims<P>;
   constexpr int NP = D::NP, N3 = D::N3, NPAIR = D::NPAIR;
   static_assert(N3 <= 32, "joint packed step needs 3Np <= 32");
   constexpr int G = 64 / N3;
   constexpr int NQUAD = Dims32<P>::NQ, TAIL = Dims32<P>::T;
-  __shared__ double sx[2][G][N3 + 1];
-  __shared__ double sw[G][N3];
-  __shared__ double sR[G][D::NPK];
-  extern __shared__ double prod[];  // [block - B0][row]
-  const int lane = threadIdx.x;
+  __shared__ double sxa[WPW][2][G][N3 + 1];
+  __shared__ double swa[WPW][G][N3];
+  __shared__ double sRa[WPW][G][D::NPK];
+  const int wv = threadIdx.x >> 6;
+  double(&sx)[2][G][N3 + 1] = sxa[wv];
+  double(&sw)[G][N3] = swa[wv];
+  double(&sR)[G][D::NPK] = sRa[wv];
+  extern __shared__ double prod_all[];  // per wave: [block - B0][row]
+  const int lane = threadIdx.x & 63;
   const int sub = lane / N3, r = lane % N3;
-  const int P0 = step_poly() * G;
+  const int P0 = (step_poly() * WPW + wv) * G;
   const int Pp = P0 + sub;
   const bool act = sub < G && Pp < n_poly;
   if (MODE != 2 && TB == 64)
@@ -1105,7 +1130,8 @@ __global__ __launch_bounds__(64) void k_leapfrog_joint(const void* __restrict__ 
       const int sp = k / D::NPK, e = k % D::NPK, Q = P0 + sp;
       if (Q < n_poly) sR[sp][e] = Rp[(size_t)Q * D::NPK + e];
     }
-  if (P0 >= n_poly) return;  // padding workgroup: uniform exit, no barrier passed
+  if (P0 >= n_poly) return;  // padding group: the wave exits before any barrier
+  double* prod = prod_all + (size_t)wv * prod_stride;
   const int B0 = row_ptr[P0], B1 = row_ptr[min(P0 + G, n_poly)];
   const int nblk = B1 - B0;
   int bj = B0 + sub;  // this lane group's block at iteration 0
@@ -1130,7 +1156,7 @@ __global__ __launch_bounds__(64) void k_leapfrog_joint(const void* __restrict__ 
       sx[buf][sub][r] = xn;
       if (bj + G < B1) xn = u[(size_t)col[bj + G] * N3 + r];
     }
-    __syncthreads();
+    lsync<WPW>();
     if (on) {
       double s = 0.0;
       if constexpr (TB == 64) {
@@ -1161,7 +1187,7 @@ __global__ __launch_bounds__(64) void k_leapfrog_joint(const void* __restrict__ 
       prod[(size_t)(bj - B0) * N3 + r] = s;
     }
   }
-  __syncthreads();
+  lsync<WPW>();
   double acc = 0.0;
   if (act) {  // this row's blocks, in block order
     const int a0 = row_ptr[Pp] - B0, a1 = row_ptr[Pp + 1] - B0;
@@ -1171,7 +1197,7 @@ __global__ __launch_bounds__(64) void k_leapfrog_joint(const void* __restrict__ 
   const size_t o = (size_t)Pp * N3 + r;
   if (MODE == 2) {
     if (act) sw[sub][r] = acc;
-    __syncthreads();
+    lsync<WPW>();
     if (act && r == 0) {
       double el = 0.0;
       for (int k = 0; k < N3; ++k) el = el + u[(size_t)Pp * N3 + k] * sw[sub][k];
@@ -1186,12 +1212,12 @@ __global__ __launch_bounds__(64) void k_leapfrog_joint(const void* __restrict__ 
   }
   if constexpr (TB == 64) {
     if (act) sw[sub][r] = w;
-    __syncthreads();
+    lsync<WPW>();
     if (lane < 3 * G) {
       const int sp = lane / 3, comp = lane % 3;
       if (P0 + sp < n_poly) solve_packed<NP>(sR[sp], sw[sp] + comp * NP);
     }
-    __syncthreads();
+    lsync<WPW>();
     if (act) w = sw[sub][r];
   }
   if (act) {
