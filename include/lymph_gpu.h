@@ -141,6 +141,25 @@ int lymph_export_operator(const LymphCtx* ctx, double* K, double* A_dg,
 /* MAKE_RHS_FREE (src/matrix_free.f90:467-843) for a built-in data set into a
  * device slot (LYMPH_SLOT_RHS_STAT / _DYN); copied to rhs_out if non-NULL. */
 int lymph_build_rhs(LymphCtx* ctx, int problem, int slot, double* rhs_out);
+/* MAKE_RHS_FREE with caller-supplied data, the drop-in for the procedure
+ * arguments f_forcing, f_dirichlet, f_neumann (src/matrix_free.f90:474-493):
+ * the library computes the physical points where the reference evaluates
+ * them (volume quadrature points of every tet; data points of every
+ * Dirichlet and Neumann face) and calls each function once with all its
+ * points; the values then enter the same RHS kernel as the built-in data
+ * sets.  Arrays are n x 3 row-major (xyz[3*i+d], normal, outputs) or n
+ * entries (material of the point's tet, space-function tag of the face).
+ * A NULL function stands for zero data.  Functions run on the calling host
+ * thread before this call returns; `user` is passed through. */
+typedef void (*lymph_forcing_fn)(int n, const double* lambda, const double* mu, const double* rho,
+                                 const double* xyz, double* f, void* user);
+typedef void (*lymph_dirichlet_fn)(int n, const double* xyz, const int32_t* space_tag, double* gd,
+                                   void* user);
+typedef void (*lymph_neumann_fn)(int n, const double* lambda, const double* mu,
+                                 const double* normal, const double* xyz,
+                                 const int32_t* space_tag, double* gn, void* user);
+int lymph_build_rhs_fn(LymphCtx* ctx, lymph_forcing_fn f, lymph_dirichlet_fn gd,
+                       lymph_neumann_fn gn, void* user, int slot, double* rhs_out);
 /* COMPUTE_MODAL_COEFFICIENTS_FREE (src/matrix_free.f90:847-997): L2
  * projection of field `fn` into a device slot; copied to out if non-NULL. */
 int lymph_project(LymphCtx* ctx, int fn, int slot, double* out);

@@ -31,6 +31,14 @@ int fail(int code, const std::string& msg) {
       return fail(LYMPH_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));      \
   } while (0)
 
+// HIP_TRY that runs a clean-up first: `handler(code)` returns the code
+#define HIP_TRY_OR(expr, handler)                                                         \
+  do {                                                                                    \
+    hipError_t _e = (expr);                                                               \
+    if (_e != hipSuccess)                                                                 \
+      return handler(fail(LYMPH_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e))); \
+  } while (0)
+
 template <class T>
 int dalloc(T** p, size_t n) {
   *p = nullptr;
@@ -39,6 +47,14 @@ int dalloc(T** p, size_t n) {
   if (e != hipSuccess)
     return fail(LYMPH_ERR_HIP, "hipMalloc(" + std::to_string(n * sizeof(T)) + " B): " +
                                    hipGetErrorString(e));
+  return 0;
+}
+
+// host copy of a device array (set-up paths only)
+template <class T>
+int d2h(std::vector<T>& h, const T* d, size_t n) {
+  h.resize(n);
+  if (n) HIP_TRY(hipMemcpy(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -816,13 +832,101 @@ int lymph_build_rhs(LymphCtx* c, int problem, int slot, double* rhs_out) {
     return fail(LYMPH_ERR_ARG, "unknown problem");
   int rc = dispatch_p(c->p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
-    k_rhs<P><<<c->n_poly, 128, 0, c->stream>>>(c->md, problem, c->alpha, c->theta, c->slot[slot]);
+    k_rhs<P><<<c->n_poly, 128, 0, c->stream>>>(c->md, problem, c->alpha, c->theta, c->slot[slot],
+                                                nullptr, nullptr);
     return check_launch("k_rhs");
   });
   if (rc) return rc;
   if (rhs_out) return d2h_vec(c, c->slot[slot], rhs_out);
   HIP_TRY(hipStreamSynchronize(c->stream));
   return 0;
+}
+
+int lymph_build_rhs_fn(LymphCtx* c, lymph_forcing_fn f, lymph_dirichlet_fn gd, lymph_neumann_fn gn,
+                       void* user, int slot, double* rhs_out) {
+  if (int rc = need_ctx(c)) return rc;
+  if (slot < 0 || slot >= LYMPH_NUM_SLOTS) return fail(LYMPH_ERR_ARG, "bad slot");
+  const int nt = c->n_tet, nq3 = c->nq3, nq2 = c->nq2;
+  const size_t nv = (size_t)nt * nq3, nf = (size_t)nt * 4 * nq2;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  double *dv = nullptr, *df = nullptr;
+  if (int rc = dalloc(&dv, 3 * nv)) return rc;
+  if (int rc = dalloc(&df, 3 * nf)) {
+    (void)hipFree(dv);
+    return rc;
+  }
+  auto done = [&](int rc) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(dv);
+    (void)hipFree(df);
+    return rc;
+  };
+  int rc = dispatch_p(c->p, [&](auto pc) -> int {
+    constexpr int P = decltype(pc)::value;
+    const size_t n = (size_t)nt * (nq3 + 4 * nq2);
+    k_rhs_points<P><<<grid1(n, 256), 256, 0, c->stream>>>(c->md, nt, dv, df);
+    return check_launch("k_rhs_points");
+  });
+  if (rc) return done(rc);
+  HIP_TRY_OR(hipStreamSynchronize(c->stream), done);
+  std::vector<double> vp, fp, lam, mu, rho, nrm;
+  std::vector<int> neigh, tag;
+  if ((rc = d2h(vp, dv, 3 * nv)) || (rc = d2h(fp, df, 3 * nf)) || (rc = d2h(lam, c->md.lam, nt)) ||
+      (rc = d2h(mu, c->md.mu, nt)) || (rc = d2h(rho, c->md.rho, nt)) ||
+      (rc = d2h(nrm, c->md.normal, 12 * (size_t)nt)) || (rc = d2h(neigh, c->md.neigh, 4 * (size_t)nt)) ||
+      (rc = d2h(tag, c->md.bc_space, 4 * (size_t)nt)))
+    return done(rc);
+  // forcing at every volume point (material of the point's tet)
+  std::vector<double> tv(3 * nv, 0.0), tf(3 * nf, 0.0);
+  if (f) {
+    std::vector<double> l(nv), m(nv), r(nv);
+    for (size_t i = 0; i < nv; ++i) {
+      const size_t t = i / nq3;
+      l[i] = lam[t];
+      m[i] = mu[t];
+      r[i] = rho[t];
+    }
+    f((int)nv, l.data(), m.data(), r.data(), vp.data(), tv.data(), user);
+  }
+  // Dirichlet (neighbour -1) and Neumann (-2) data at the boundary faces' points
+  for (int kind : {-1, -2}) {
+    if ((kind == -1 && !gd) || (kind == -2 && !gn)) continue;
+    std::vector<size_t> at;  // index into the per-face point arrays
+    for (size_t tf4 = 0; tf4 < 4 * (size_t)nt; ++tf4)
+      if (neigh[tf4] == kind)
+        for (int q = 0; q < nq2; ++q) at.push_back(tf4 * nq2 + q);
+    if (at.empty()) continue;
+    const size_t n = at.size();
+    std::vector<double> x(3 * n), out(3 * n, 0.0), nn(3 * n), l(n), m(n);
+    std::vector<int> tg(n);
+    for (size_t k = 0; k < n; ++k) {
+      const size_t tf4 = at[k] / nq2, t = tf4 / 4;
+      for (int d = 0; d < 3; ++d) {
+        x[3 * k + d] = fp[3 * at[k] + d];
+        nn[3 * k + d] = nrm[3 * tf4 + d];
+      }
+      tg[k] = tag[tf4];
+      l[k] = lam[t];
+      m[k] = mu[t];
+    }
+    if (kind == -1)
+      gd((int)n, x.data(), tg.data(), out.data(), user);
+    else
+      gn((int)n, l.data(), m.data(), nn.data(), x.data(), tg.data(), out.data(), user);
+    for (size_t k = 0; k < n; ++k)
+      for (int d = 0; d < 3; ++d) tf[3 * at[k] + d] = out[3 * k + d];
+  }
+  HIP_TRY_OR(hipMemcpy(dv, tv.data(), 3 * nv * sizeof(double), hipMemcpyHostToDevice), done);
+  HIP_TRY_OR(hipMemcpy(df, tf.data(), 3 * nf * sizeof(double), hipMemcpyHostToDevice), done);
+  rc = dispatch_p(c->p, [&](auto pc) -> int {
+    constexpr int P = decltype(pc)::value;
+    k_rhs<P><<<c->n_poly, 128, 0, c->stream>>>(c->md, LYMPH_PROB_NULL, c->alpha, c->theta,
+                                                c->slot[slot], dv, df);
+    return check_launch("k_rhs");
+  });
+  if (rc) return done(rc);
+  if (rhs_out) rc = d2h_vec(c, c->slot[slot], rhs_out);
+  return done(rc);
 }
 
 int lymph_project(LymphCtx* c, int fn, int slot, double* out) {

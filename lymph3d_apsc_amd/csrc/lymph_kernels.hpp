@@ -564,7 +564,9 @@ __global__ void k_cholesky(const double* __restrict__ M, const double* __restric
 // ---------------------------------------------------------------------------
 template <int P>
 __global__ __launch_bounds__(128) void k_rhs(MeshDev m, int prob, double alpha, double theta,
-                                             double* __restrict__ rhs) {
+                                             double* __restrict__ rhs,
+                                             const double* __restrict__ tabv = nullptr,
+                                             const double* __restrict__ tabf = nullptr) {
   using D = Dims<P>;
   constexpr int NP = D::NP, N3 = D::N3, NQ3 = D::NQ3, NQ2 = D::NQ2;
   __shared__ double phi[NP * NQ3];
@@ -589,7 +591,11 @@ __global__ __launch_bounds__(128) void k_rhs(MeshDev m, int prob, double alpha, 
     __syncthreads();
     for (int q = tid; q < NQ3; q += blockDim.x) {
       map_point(Fk, &m.qtet[q * 4], &spt[q * 3]);
-      forcing(prob, lam, mu, &spt[q * 3], rho, &sval[q * 3]);
+      if (tabv) {  // values of the caller's forcing at k_rhs_points' volume points
+        for (int c = 0; c < 3; ++c) sval[q * 3 + c] = tabv[((size_t)t * NQ3 + q) * 3 + c];
+      } else {
+        forcing(prob, lam, mu, &spt[q * 3], rho, &sval[q * 3]);
+      }
     }
     __syncthreads();
     for (int e = tid; e < NP * NQ3; e += blockDim.x) {
@@ -614,8 +620,13 @@ __global__ __launch_bounds__(128) void k_rhs(MeshDev m, int prob, double alpha, 
         double xr[3];
         face_point(Fk, f, &m.qtri[q * 4], &spt[q * 3]);
         face_point_rhs(Fk, f, &m.qtri[q * 4], xr);
-        if (e2 == -1) dirichlet(prob, xr, tag, &sval[q * 3]);
-        else neumann(prob, lam, mu, nrm, xr, tag, &sval[q * 3]);
+        if (tabf) {  // the caller's Dirichlet / Neumann data at k_rhs_points' face points
+          for (int c = 0; c < 3; ++c) sval[q * 3 + c] = tabf[(((size_t)t * 4 + f) * NQ2 + q) * 3 + c];
+        } else if (e2 == -1) {
+          dirichlet(prob, xr, tag, &sval[q * 3]);
+        } else {
+          neumann(prob, lam, mu, nrm, xr, tag, &sval[q * 3]);
+        }
       }
       __syncthreads();
       for (int e = tid; e < NP * NQ2; e += blockDim.x) {
@@ -668,6 +679,29 @@ __global__ __launch_bounds__(128) void k_rhs(MeshDev m, int prob, double alpha, 
     }
   }
   if (tid < N3) r[tid] = racc;
+}
+
+// Physical points where MAKE_RHS_FREE evaluates its data (as k_rhs computes
+// them): volume quadrature points vpts[(t*NQ3 + q)*3 + d] (forcing) and the
+// data points of every tet face fpts[((t*4 + f)*NQ2 + q)*3 + d] (Dirichlet /
+// Neumann data; src/matrix_free.f90:711, :781-801).  One thread per point.
+template <int P>
+__global__ void k_rhs_points(MeshDev m, int n_tet, double* __restrict__ vpts,
+                             double* __restrict__ fpts) {
+  using D = Dims<P>;
+  constexpr int NQ3 = D::NQ3, NQ2 = D::NQ2, PER = NQ3 + 4 * NQ2;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)n_tet * PER) return;
+  const int t = (int)(i / PER), k = (int)(i % PER);
+  double v[12], Fk[12], Jdet;
+  for (int j = 0; j < 12; ++j) v[j] = m.tet_xyz[(size_t)t * 12 + j];
+  affine_map(v, Fk, Jdet);
+  if (k < NQ3) {
+    map_point(Fk, &m.qtet[k * 4], vpts + ((size_t)t * NQ3 + k) * 3);
+  } else {
+    const int f = (k - NQ3) / NQ2, q = (k - NQ3) % NQ2;
+    face_point_rhs(Fk, f, &m.qtri[q * 4], fpts + (((size_t)t * 4 + f) * NQ2 + q) * 3);
+  }
 }
 
 // Forward/backward substitution with the packed factor in the reference's

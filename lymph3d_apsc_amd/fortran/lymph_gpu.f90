@@ -320,6 +320,17 @@ module lymph_gpu
       integer(c_int), value :: slot_uh, slot_uex
       integer(c_int) :: rc
     end function
+    ! f_forcing / f_dirichlet / f_neumann as C functions of n points at once:
+    ! the host wraps its own procedures in bind(C) routines of these shapes
+    ! and passes c_funloc of them (c_null_funptr = zero data).
+    function lymph_build_rhs_fn(ctx, f, gd, gn, user, slot, rhs_out) &
+        bind(C, name='lymph_build_rhs_fn') result(rc)
+      import :: c_int, c_ptr, c_funptr
+      type(c_ptr), value :: ctx, user, rhs_out
+      type(c_funptr), value :: f, gd, gn
+      integer(c_int), value :: slot
+      integer(c_int) :: rc
+    end function
     function lymph_check_finite(ctx, slot, finite) bind(C, name='lymph_check_finite') result(rc)
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

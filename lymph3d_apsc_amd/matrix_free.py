@@ -102,6 +102,43 @@ class Context:
         check(lib().lymph_build_rhs(self.h, problem, slot, ptr(out)), "lymph_build_rhs")
         return out
 
+    def make_rhs_free_fn(self, forcing=None, dirichlet=None, neumann=None, slot=_lib.SLOT_RHS_DYN):
+        """MAKE_RHS_FREE with caller data, the procedure arguments of
+        src/matrix_free.f90:474-493 as vectorised numpy functions:
+        forcing(lam, mu, rho, xyz), dirichlet(xyz, tag), neumann(lam, mu,
+        normal, xyz, tag), each returning an (n, 3) array for n points (xyz
+        and normal (n, 3); lam, mu, rho, tag (n,)); None = zero data."""
+        import numpy.ctypeslib as npc
+
+        def vec(p, n, k=3):
+            return npc.as_array(p, shape=(n, k)) if k > 1 else npc.as_array(p, shape=(n,))
+
+        keep = []
+        if forcing is not None:
+            def f_c(n, lam, mu, rho, xyz, out, user):
+                vec(out, n)[:] = forcing(vec(lam, n, 1), vec(mu, n, 1), vec(rho, n, 1), vec(xyz, n))
+            keep.append(_lib.FORCING_FN(f_c))
+        else:
+            keep.append(None)
+        if dirichlet is not None:
+            def d_c(n, xyz, tag, out, user):
+                vec(out, n)[:] = dirichlet(vec(xyz, n), vec(tag, n, 1))
+            keep.append(_lib.DIRICHLET_FN(d_c))
+        else:
+            keep.append(None)
+        if neumann is not None:
+            def n_c(n, lam, mu, nrm, xyz, tag, out, user):
+                vec(out, n)[:] = neumann(vec(lam, n, 1), vec(mu, n, 1), vec(nrm, n), vec(xyz, n),
+                                         vec(tag, n, 1))
+            keep.append(_lib.NEUMANN_FN(n_c))
+        else:
+            keep.append(None)
+        fps = [C.cast(k, C.c_void_p) if k is not None else None for k in keep]
+        out = self.vec()
+        check(lib().lymph_build_rhs_fn(self.h, fps[0], fps[1], fps[2], None, slot, ptr(out)),
+              "lymph_build_rhs_fn")
+        return out
+
     def compute_modal_coefficients_free(self, fn, slot=_lib.SLOT_UEX):
         out = self.vec()
         check(lib().lymph_project(self.h, fn, slot, ptr(out)), "lymph_project")

@@ -239,6 +239,56 @@ def test_agglomerated_cube_wave_run(n, p, steps, dt):
     assert np.isfinite(r["err_l2"]) and r["err_l2"] < 0.5
 
 
+def _mms_data():
+    """The reference's manufactured data set (problem_data_and_properties.f90:
+    f_time, gd, gn) written as the vectorised functions a caller would pass."""
+    pi = np.pi
+
+    def forcing(lam, mu, rho, x):
+        c = (-2.0 * rho + 3.0 * (lam + 2.0 * mu)) * pi * pi
+        sx, sy, sz = np.sin(pi * x[:, 0]), np.sin(pi * x[:, 1]), np.sin(pi * x[:, 2])
+        cx, cy, cz = np.cos(pi * x[:, 0]), np.cos(pi * x[:, 1]), np.cos(pi * x[:, 2])
+        return np.stack([c * cx * sy * sz, c * sx * cy * sz, c * sx * sy * cz], axis=1)
+
+    def dirichlet(x, tag):
+        sx, sy, sz = np.sin(pi * x[:, 0]), np.sin(pi * x[:, 1]), np.sin(pi * x[:, 2])
+        cx, cy, cz = np.cos(pi * x[:, 0]), np.cos(pi * x[:, 1]), np.cos(pi * x[:, 2])
+        on = ((tag >= 1) & (tag <= 3))[:, None]
+        return np.where(on, np.stack([cx * sy * sz, sx * cy * sz, sx * sy * cz], axis=1), 0.0)
+
+    def neumann(lam, mu, n, x, tag):
+        sx, sy, sz = np.sin(pi * x[:, 0]), np.sin(pi * x[:, 1]), np.sin(pi * x[:, 2])
+        cx, cy, cz = np.cos(pi * x[:, 0]), np.cos(pi * x[:, 1]), np.cos(pi * x[:, 2])
+        d = -pi * sx * sy * sz * (3.0 * lam + 2.0 * mu)
+        s01 = 2.0 * mu * pi * cx * cy * sz
+        s02 = 2.0 * mu * pi * cx * cz * sy
+        s12 = 2.0 * mu * pi * cy * cz * sx
+        r = np.stack([d * n[:, 0] + s01 * n[:, 1] + s02 * n[:, 2],
+                      s01 * n[:, 0] + d * n[:, 1] + s12 * n[:, 2],
+                      s02 * n[:, 0] + s12 * n[:, 1] + d * n[:, 2]], axis=1)
+        return np.where(((tag >= 1) & (tag <= 3))[:, None], r, 0.0)
+
+    return forcing, dirichlet, neumann
+
+
+@pytest.mark.parametrize("name,p", [("hetero", 2), ("tet_25", 1), ("cube2agg", 3)])
+def test_rhs_from_caller_functions(name, p):
+    """MAKE_RHS_FREE with the data passed as functions (the drop-in for its
+    procedure arguments) equals the built-in data set it restates, on meshes
+    with Dirichlet and Neumann faces and per-tet materials; no functions at
+    all gives a zero RHS."""
+    say(f"[rhs functions] {name} p={p}")
+    m = make_mesh(name)
+    f, gd, gn = _mms_data()
+    with L.Context(m, p) as ctx:
+        ref = ctx.make_rhs_free(L.PROB_REF_MMS)
+        got = ctx.make_rhs_free_fn(f, gd, gn)
+        assert rel(got, ref) < 1e-13, rel(got, ref)
+        assert not np.any(ctx.make_rhs_free_fn())
+        # the slot written is the one the stepping uses
+        assert np.array_equal(ctx.get_slot(_lib.SLOT_RHS_DYN), np.zeros_like(ref))
+
+
 def test_singular_mass_reports_exit_code():
     m = M.cube_mesh(1)
     m.rho = -np.ones(m.n_tet)  # host-forced: M = rho * mass is negative definite
