@@ -164,6 +164,14 @@ int lymph_build_rhs_fn(LymphCtx* ctx, lymph_forcing_fn f, lymph_dirichlet_fn gd,
  * projection of field `fn` into a device slot; copied to out if non-NULL. */
 int lymph_project(LymphCtx* ctx, int fn, int slot, double* out);
 
+/* COMPUTE_MODAL_COEFFICIENTS_FREE with a caller function, the drop-in for
+ * its procedure argument f_analytic: fld(n, xyz, values, user) gets the n
+ * volume quadrature points (n x 3 row-major) where the reference evaluates
+ * f_analytic and writes n x 3 values; then the same projection kernel as the
+ * built-in fields.  Needs the mass factors (lymph_build_operator). */
+typedef void (*lymph_field_fn)(int n, const double* xyz, double* values, void* user);
+int lymph_project_fn(LymphCtx* ctx, lymph_field_fn fld, void* user, int slot, double* out);
+
 /* Host <-> device slot copies (layout as set). */
 int lymph_set_slot(LymphCtx* ctx, int slot, const double* host);
 int lymph_get_slot(LymphCtx* ctx, int slot, double* host);

@@ -37,7 +37,7 @@ EXPORTS = [
     "lymph_get_slot", "lymph_slot_ptr", "lymph_first_step", "lymph_step",
     "lymph_first_step_dev", "lymph_step_dev", "lymph_step_n", "lymph_time_steps",
     "lymph_error_l2", "lymph_error_dg", "lymph_error_slots", "lymph_run", "lymph_check_finite",
-    "lymph_build_rhs_fn",
+    "lymph_build_rhs_fn", "lymph_project_fn",
     "lymph_time_function", "lymph_autotune", "lymph_set_step_variant", "lymph_set_receivers",
     "lymph_sample_receivers", "lymph_step_n_record", "lymph_get_traces",
     "lymph_set_apply_precision", "lymph_set_source_groups", "lymph_ctx_set_keep_adg", "lymph_ctx_set_build", "lymph_release_fp64_operator",
@@ -105,6 +105,7 @@ def lib():
         "lymph_error_slots": (_i, [_vp, _i, _i, _vp, _vp]),
         "lymph_check_finite": (_i, [_vp, _i, _vp]),
         "lymph_build_rhs_fn": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp]),
+        "lymph_project_fn": (_i, [_vp, _vp, _vp, _i, _vp]),
         "lymph_run": (_i, [_vp, _i, _d, _d, _i, _vp, _vp]),
         "lymph_time_function": (_d, [_i, _d]),
         "lymph_autotune": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -160,6 +161,7 @@ FORCING_FN = C.CFUNCTYPE(None, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_dou
                          C.c_void_p)
 DIRICHLET_FN = C.CFUNCTYPE(None, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int32),
                            C.POINTER(C.c_double), C.c_void_p)
+FIELD_FN = C.CFUNCTYPE(None, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p)
 NEUMANN_FN = C.CFUNCTYPE(None, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                          C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int32),
                          C.POINTER(C.c_double), C.c_void_p)

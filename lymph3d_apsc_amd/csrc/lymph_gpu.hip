@@ -842,6 +842,48 @@ int lymph_build_rhs(LymphCtx* c, int problem, int slot, double* rhs_out) {
   return 0;
 }
 
+int lymph_project_fn(LymphCtx* c, lymph_field_fn fld, void* user, int slot, double* out) {
+  if (int rc = need_built(c)) return rc;
+  if (slot < 0 || slot >= LYMPH_NUM_SLOTS) return fail(LYMPH_ERR_ARG, "bad slot");
+  if (!fld) return fail(LYMPH_ERR_ARG, "field function is null");
+  const int nt = c->n_tet, nq3 = c->nq3, nq2 = c->nq2;
+  const size_t nv = (size_t)nt * nq3, nf = (size_t)nt * 4 * nq2;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  double *dv = nullptr, *df = nullptr;
+  if (int rc = dalloc(&dv, 3 * nv)) return rc;
+  if (int rc = dalloc(&df, 3 * nf)) {
+    (void)hipFree(dv);
+    return rc;
+  }
+  auto done = [&](int rc) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(dv);
+    (void)hipFree(df);
+    return rc;
+  };
+  int rc = dispatch_p(c->p, [&](auto pc) -> int {
+    constexpr int P = decltype(pc)::value;
+    const size_t n = (size_t)nt * (nq3 + 4 * nq2);
+    k_rhs_points<P><<<grid1(n, 256), 256, 0, c->stream>>>(c->md, nt, dv, df);
+    return check_launch("k_rhs_points");
+  });
+  if (rc) return done(rc);
+  HIP_TRY_OR(hipStreamSynchronize(c->stream), done);
+  std::vector<double> vp;
+  if ((rc = d2h(vp, dv, 3 * nv))) return done(rc);
+  std::vector<double> tv(3 * nv, 0.0);
+  fld((int)nv, vp.data(), tv.data(), user);
+  HIP_TRY_OR(hipMemcpy(dv, tv.data(), 3 * nv * sizeof(double), hipMemcpyHostToDevice), done);
+  rc = dispatch_p(c->p, [&](auto pc) -> int {
+    constexpr int P = decltype(pc)::value;
+    k_project<P><<<c->n_poly, 128, 0, c->stream>>>(c->md, LYMPH_FN_ZERO, c->RMm, c->slot[slot], dv);
+    return check_launch("k_project");
+  });
+  if (rc) return done(rc);
+  if (out) rc = d2h_vec(c, c->slot[slot], out);
+  return done(rc);
+}
+
 int lymph_build_rhs_fn(LymphCtx* c, lymph_forcing_fn f, lymph_dirichlet_fn gd, lymph_neumann_fn gn,
                        void* user, int slot, double* rhs_out) {
   if (int rc = need_ctx(c)) return rc;
@@ -935,7 +977,7 @@ int lymph_project(LymphCtx* c, int fn, int slot, double* out) {
   if (fn < LYMPH_FN_ZERO || fn > LYMPH_FN_CUBE_U) return fail(LYMPH_ERR_ARG, "unknown field");
   int rc = dispatch_p(c->p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
-    k_project<P><<<c->n_poly, 128, 0, c->stream>>>(c->md, fn, c->RMm, c->slot[slot]);
+    k_project<P><<<c->n_poly, 128, 0, c->stream>>>(c->md, fn, c->RMm, c->slot[slot], nullptr);
     return check_launch("k_project");
   });
   if (rc) return rc;
@@ -1357,9 +1399,9 @@ int lymph_capture_solve(LymphCtx* c, int problem, double dt, int nsteps, int eve
   int rc = dispatch_p(c->p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
     k_project<P><<<c->n_poly, 128, 0, c->stream>>>(c->md, LYMPH_FN_ZERO, c->RMm,
-                                                   c->slot[LYMPH_SLOT_U_PREV]);
+                                                   c->slot[LYMPH_SLOT_U_PREV], nullptr);
     k_project<P><<<c->n_poly, 128, 0, c->stream>>>(c->md, LYMPH_FN_ZERO, c->RMm,
-                                                   c->slot[LYMPH_SLOT_V0]);
+                                                   c->slot[LYMPH_SLOT_V0], nullptr);
     return check_launch("k_project");
   });
   double t = 0.0;

@@ -730,7 +730,8 @@ __device__ __forceinline__ void solve_packed(const double* __restrict__ R, doubl
 
 template <int P>
 __global__ __launch_bounds__(128) void k_project(MeshDev m, int fn, const double* __restrict__ RMm,
-                                                 double* __restrict__ out) {
+                                                 double* __restrict__ out,
+                                                 const double* __restrict__ tabv) {
   using D = Dims<P>;
   constexpr int NP = D::NP, N3 = D::N3, NQ3 = D::NQ3;
   __shared__ double phi[NP * NQ3];
@@ -753,7 +754,11 @@ __global__ __launch_bounds__(128) void k_project(MeshDev m, int fn, const double
     __syncthreads();
     for (int q = tid; q < NQ3; q += blockDim.x) {
       map_point(Fk, &m.qtet[q * 4], &spt[q * 3]);
-      field(fn, &spt[q * 3], &sval[q * 3]);
+      if (tabv) {  // the caller's field at k_rhs_points' volume points
+        for (int c = 0; c < 3; ++c) sval[q * 3 + c] = tabv[((size_t)t * NQ3 + q) * 3 + c];
+      } else {
+        field(fn, &spt[q * 3], &sval[q * 3]);
+      }
     }
     __syncthreads();
     for (int e = tid; e < NP * NQ3; e += blockDim.x) {

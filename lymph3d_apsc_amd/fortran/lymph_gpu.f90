@@ -331,6 +331,13 @@ module lymph_gpu
       integer(c_int), value :: slot
       integer(c_int) :: rc
     end function
+    function lymph_project_fn(ctx, fld, user, slot, out) bind(C, name='lymph_project_fn') result(rc)
+      import :: c_int, c_ptr, c_funptr
+      type(c_ptr), value :: ctx, user, out
+      type(c_funptr), value :: fld
+      integer(c_int), value :: slot
+      integer(c_int) :: rc
+    end function
     function lymph_check_finite(ctx, slot, finite) bind(C, name='lymph_check_finite') result(rc)
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

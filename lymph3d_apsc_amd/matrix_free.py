@@ -139,6 +139,20 @@ class Context:
               "lymph_build_rhs_fn")
         return out
 
+    def compute_modal_coefficients_free_fn(self, field, slot=_lib.SLOT_UEX):
+        """COMPUTE_MODAL_COEFFICIENTS_FREE of a caller function (the
+        reference's f_analytic argument) as a vectorised numpy function
+        field(xyz) -> (n, 3) at the (n, 3) volume quadrature points."""
+        import numpy.ctypeslib as npc
+
+        def f_c(n, xyz, out, user):
+            npc.as_array(out, shape=(n, 3))[:] = field(npc.as_array(xyz, shape=(n, 3)))
+        cb = _lib.FIELD_FN(f_c)
+        out = self.vec()
+        check(lib().lymph_project_fn(self.h, C.cast(cb, C.c_void_p), None, slot, ptr(out)),
+              "lymph_project_fn")
+        return out
+
     def compute_modal_coefficients_free(self, fn, slot=_lib.SLOT_UEX):
         out = self.vec()
         check(lib().lymph_project(self.h, fn, slot, ptr(out)), "lymph_project")

@@ -273,10 +273,10 @@ def _mms_data():
 
 @pytest.mark.parametrize("name,p", [("hetero", 2), ("tet_25", 1), ("cube2agg", 3)])
 def test_rhs_from_caller_functions(name, p):
-    """MAKE_RHS_FREE with the data passed as functions (the drop-in for its
-    procedure arguments) equals the built-in data set it restates, on meshes
-    with Dirichlet and Neumann faces and per-tet materials; no functions at
-    all gives a zero RHS."""
+    """MAKE_RHS_FREE and COMPUTE_MODAL_COEFFICIENTS_FREE with the data passed
+    as functions (the drop-ins for their procedure arguments) equal the
+    built-in data set they restate, on meshes with Dirichlet and Neumann
+    faces and per-tet materials; no functions at all gives a zero RHS."""
     say(f"[rhs functions] {name} p={p}")
     m = make_mesh(name)
     f, gd, gn = _mms_data()
@@ -287,6 +287,21 @@ def test_rhs_from_caller_functions(name, p):
         assert not np.any(ctx.make_rhs_free_fn())
         # the slot written is the one the stepping uses
         assert np.array_equal(ctx.get_slot(_lib.SLOT_RHS_DYN), np.zeros_like(ref))
+        # COMPUTE_MODAL_COEFFICIENTS_FREE of a caller function: uex of the same data set
+        ctx.make_matrices_free()
+        pi = np.pi
+
+        def uex(x):
+            sx, sy, sz = np.sin(pi * x[:, 0]), np.sin(pi * x[:, 1]), np.sin(pi * x[:, 2])
+            cx, cy, cz = np.cos(pi * x[:, 0]), np.cos(pi * x[:, 1]), np.cos(pi * x[:, 2])
+            return np.stack([cx * sy * sz, sx * cy * sz, sx * sy * cz], axis=1)
+        pref = ctx.compute_modal_coefficients_free(L.FN_REF_UEX)
+        pgot = ctx.compute_modal_coefficients_free_fn(uex)
+        # compared in the L2 norm of the functions (the bounding-box basis makes
+        # coefficients a poor measure of ulp-level data differences)
+        d2 = ctx.compute_error_l2_matrix_free(pgot, pref)
+        n2 = ctx.compute_error_l2_matrix_free(pref, np.zeros_like(pref))
+        assert np.sqrt(d2 / n2) < 1e-13, np.sqrt(d2 / n2)
 
 
 def test_singular_mass_reports_exit_code():
