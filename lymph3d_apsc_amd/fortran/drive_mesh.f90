@@ -5,25 +5,139 @@
 ! by lymph3d_apsc_amd.mesh.Mesh.write_flat (it plays the part of the
 ! reference's Mesh_Structure after MAKE_PARTITION_AND_MPI_FILES).
 !
-! usage: drive_mesh <mesh.bin>
+! usage: drive_mesh <mesh.bin> [procedures]
+!   With "procedures" the data of MAKE_RHS_FREE and the fields of
+!   COMPUTE_MODAL_COEFFICIENTS_FREE come from the Fortran functions of module
+!   mms_procedures (the manufactured solution of
+!   src/problem_data_and_properties.f90, evaluated on the host) through
+!   lymph_build_rhs_fn / lymph_project_fn, as a host passes its own
+!   procedure arguments; otherwise from the built-in data set.
 ! exit status: 0 ok, 3 no HIP device, 1 any other failure.
+
+! The host's pointwise data functions, wrapped as the batched C callbacks of
+! lymph_build_rhs_fn / lymph_project_fn (points and values 3 x n).
+module mms_procedures
+  use iso_c_binding
+  implicit none
+  real(c_double), parameter :: pi = 3.141592653589793238462643383279502884d0
+  real(c_double), parameter :: sqrt2 = 1.414213562373095048801688724209698079d0
+contains
+  subroutine disp(x, r)  ! uex / gd pattern
+    real(c_double), intent(in) :: x(3)
+    real(c_double), intent(out) :: r(3)
+    r(1) = cos(pi * x(1)) * sin(pi * x(2)) * sin(pi * x(3))
+    r(2) = sin(pi * x(1)) * cos(pi * x(2)) * sin(pi * x(3))
+    r(3) = sin(pi * x(1)) * sin(pi * x(2)) * cos(pi * x(3))
+  end subroutine
+
+  subroutine forcing_c(n, lam, mu, rho, xyz, f, user) bind(C)
+    integer(c_int), value :: n
+    real(c_double), intent(in) :: lam(n), mu(n), rho(n), xyz(3, n)
+    real(c_double), intent(out) :: f(3, n)
+    type(c_ptr), value :: user
+    integer :: i
+    real(c_double) :: c
+    do i = 1, n
+      c = (-2.0d0 * rho(i) + 3.0d0 * (lam(i) + 2.0d0 * mu(i)))
+      f(1, i) = c * pi * pi * cos(pi * xyz(1, i)) * sin(pi * xyz(2, i)) * sin(pi * xyz(3, i))
+      f(2, i) = c * pi * pi * sin(pi * xyz(1, i)) * cos(pi * xyz(2, i)) * sin(pi * xyz(3, i))
+      f(3, i) = c * pi * pi * sin(pi * xyz(1, i)) * sin(pi * xyz(2, i)) * cos(pi * xyz(3, i))
+    end do
+  end subroutine
+
+  subroutine dirichlet_c(n, xyz, tag, gd, user) bind(C)
+    integer(c_int), value :: n
+    real(c_double), intent(in) :: xyz(3, n)
+    integer(c_int32_t), intent(in) :: tag(n)
+    real(c_double), intent(out) :: gd(3, n)
+    type(c_ptr), value :: user
+    integer :: i
+    do i = 1, n
+      gd(:, i) = 0.0d0
+      if (tag(i) >= 1 .and. tag(i) <= 3) call disp(xyz(:, i), gd(:, i))
+    end do
+  end subroutine
+
+  subroutine neumann_c(n, lam, mu, nrm, xyz, tag, gn, user) bind(C)
+    integer(c_int), value :: n
+    real(c_double), intent(in) :: lam(n), mu(n), nrm(3, n), xyz(3, n)
+    integer(c_int32_t), intent(in) :: tag(n)
+    real(c_double), intent(out) :: gn(3, n)
+    type(c_ptr), value :: user
+    integer :: i, j
+    real(c_double) :: st(3, 3), x(3)
+    do i = 1, n
+      gn(:, i) = 0.0d0
+      if (tag(i) < 1 .or. tag(i) > 3) cycle
+      x = xyz(:, i)
+      st(1, 1) = -pi * sin(pi * x(1)) * sin(pi * x(2)) * sin(pi * x(3)) * (3.0d0 * lam(i) + 2.0d0 * mu(i))
+      st(2, 2) = st(1, 1)
+      st(3, 3) = st(1, 1)
+      st(1, 2) = 2.0d0 * mu(i) * pi * cos(pi * x(1)) * cos(pi * x(2)) * sin(pi * x(3))
+      st(2, 1) = st(1, 2)
+      st(1, 3) = 2.0d0 * mu(i) * pi * cos(pi * x(1)) * cos(pi * x(3)) * sin(pi * x(2))
+      st(3, 1) = st(1, 3)
+      st(2, 3) = 2.0d0 * mu(i) * pi * cos(pi * x(2)) * cos(pi * x(3)) * sin(pi * x(1))
+      st(3, 2) = st(2, 3)
+      do j = 1, 3
+        gn(j, i) = st(j, 1) * nrm(1, i) + st(j, 2) * nrm(2, i) + st(j, 3) * nrm(3, i)
+      end do
+    end do
+  end subroutine
+
+  subroutine zero_c(n, xyz, v, user) bind(C)
+    integer(c_int), value :: n
+    real(c_double), intent(in) :: xyz(3, n)
+    real(c_double), intent(out) :: v(3, n)
+    type(c_ptr), value :: user
+    v = 0.0d0
+  end subroutine
+
+  subroutine velocity_c(n, xyz, v, user) bind(C)  ! ic_velocity
+    integer(c_int), value :: n
+    real(c_double), intent(in) :: xyz(3, n)
+    real(c_double), intent(out) :: v(3, n)
+    type(c_ptr), value :: user
+    integer :: i
+    do i = 1, n
+      call disp(xyz(:, i), v(:, i))
+      v(:, i) = sqrt2 * pi * v(:, i)
+    end do
+  end subroutine
+
+  subroutine uex_c(n, xyz, v, user) bind(C)
+    integer(c_int), value :: n
+    real(c_double), intent(in) :: xyz(3, n)
+    real(c_double), intent(out) :: v(3, n)
+    type(c_ptr), value :: user
+    integer :: i
+    do i = 1, n
+      call disp(xyz(:, i), v(:, i))
+    end do
+  end subroutine
+end module mms_procedures
+
 program drive_mesh
   use iso_c_binding
   use lymph_gpu
+  use mms_procedures
   implicit none
   integer(c_int32_t) :: hdr(5)
   real(c_double) :: dt, stop_time, t, e2l, e2d, half
   real(c_double), allocatable, target :: xyz(:), normal(:), area(:), bbox(:), hk(:)
   real(c_double), allocatable, target :: rho(:), lam(:), mu(:)
   integer(c_int32_t), allocatable, target :: tet_vert(:), tet_poly(:), neigh(:), bc(:)
-  real(c_double), allocatable :: u0(:,:), un(:,:), v0(:,:), uex(:,:), rhs_s(:,:), rhs_d(:,:)
+  real(c_double), allocatable, target :: u0(:,:), un(:,:), v0(:,:), uex(:,:), rhs_s(:,:), rhs_d(:,:)
   type(lymph_mesh) :: m
   type(c_ptr) :: ctx
   integer(c_int) :: rc, p
   integer :: nn, nt, npoly, np3, nsteps
-  character(len=1024) :: path
+  character(len=1024) :: path, mode
+  logical :: procs
 
   call get_command_argument(1, path)
+  call get_command_argument(2, mode)
+  procs = trim(mode) == 'procedures'
   open(10, file=trim(path), access='stream', form='unformatted', status='old')
   read(10) hdr
   read(10) dt, stop_time
@@ -52,11 +166,22 @@ program drive_mesh
   ! MAKE_MATRICES_FREE + cholesky (src/Lymph3D.f90:262-271)
   call ok(lymph_build_operator(ctx), 'lymph_build_operator')
   ! MAKE_RHS_FREE static (null data) and dynamic (f_time, gd, gn) (:284-286)
-  call ok(lymph_build_rhs(ctx, LYMPH_PROB_NULL, LYMPH_SLOT_RHS_STAT, rhs_s), 'rhs static')
-  call ok(lymph_build_rhs(ctx, LYMPH_PROB_REF_MMS, LYMPH_SLOT_RHS_DYN, rhs_d), 'rhs dynamic')
   ! COMPUTE_MODAL_COEFFICIENTS_FREE of the initial conditions (:328-329)
-  call ok(lymph_project(ctx, LYMPH_FN_ZERO, LYMPH_SLOT_UEX, u0), 'ic displacement')
-  call ok(lymph_project(ctx, LYMPH_FN_REF_VEL, LYMPH_SLOT_UEX, v0), 'ic velocity')
+  if (procs) then
+    call ok(lymph_build_rhs_fn(ctx, c_null_funptr, c_null_funptr, c_null_funptr, c_null_ptr, &
+                               LYMPH_SLOT_RHS_STAT, c_loc(rhs_s)), 'rhs static')
+    call ok(lymph_build_rhs_fn(ctx, c_funloc(forcing_c), c_funloc(dirichlet_c), c_funloc(neumann_c), &
+                               c_null_ptr, LYMPH_SLOT_RHS_DYN, c_loc(rhs_d)), 'rhs dynamic')
+    call ok(lymph_project_fn(ctx, c_funloc(zero_c), c_null_ptr, LYMPH_SLOT_UEX, c_loc(u0)), &
+            'ic displacement')
+    call ok(lymph_project_fn(ctx, c_funloc(velocity_c), c_null_ptr, LYMPH_SLOT_UEX, c_loc(v0)), &
+            'ic velocity')
+  else
+    call ok(lymph_build_rhs(ctx, LYMPH_PROB_NULL, LYMPH_SLOT_RHS_STAT, rhs_s), 'rhs static')
+    call ok(lymph_build_rhs(ctx, LYMPH_PROB_REF_MMS, LYMPH_SLOT_RHS_DYN, rhs_d), 'rhs dynamic')
+    call ok(lymph_project(ctx, LYMPH_FN_ZERO, LYMPH_SLOT_UEX, u0), 'ic displacement')
+    call ok(lymph_project(ctx, LYMPH_FN_REF_VEL, LYMPH_SLOT_UEX, v0), 'ic velocity')
+  end if
   ! FIRST_TIME_STEP_MATRIX_FREE (:368)
   t = 0.0d0
   nsteps = 1
@@ -72,7 +197,11 @@ program drive_mesh
     t = t + dt
   end do
   ! exact solution projection scaled by time_function(t) (:459-467), errors (:490-517)
-  call ok(lymph_project(ctx, LYMPH_FN_REF_UEX, LYMPH_SLOT_UEX, uex), 'uex')
+  if (procs) then
+    call ok(lymph_project_fn(ctx, c_funloc(uex_c), c_null_ptr, LYMPH_SLOT_UEX, c_loc(uex)), 'uex')
+  else
+    call ok(lymph_project(ctx, LYMPH_FN_REF_UEX, LYMPH_SLOT_UEX, uex), 'uex')
+  end if
   half = lymph_time_function(LYMPH_PROB_REF_MMS, t)
   uex = uex * half
   call ok(lymph_error_l2(ctx, un, uex, e2l), 'error L2')

@@ -18,18 +18,23 @@ GOLD = os.path.join(ROOT, "tests", "golden")
 EXE = os.path.join(ROOT, "lymph3d_apsc_amd", "fortran", "drive_mesh")
 
 
-@pytest.mark.parametrize("name", ["tet_5", "tet_25"])
-def test_fortran_reference_loop(name, tmp_path):
+@pytest.mark.parametrize("name,mode", [("tet_5", ""), ("tet_25", ""), ("tet_5", "procedures"),
+                                       ("tet_25", "procedures")])
+def test_fortran_reference_loop(name, mode, tmp_path):
+    """mode "procedures": the data sets come from Fortran functions passed as
+    callbacks (lymph_build_rhs_fn / lymph_project_fn), as the reference passes
+    its procedure arguments."""
     if L.device_count() == 0:
         pytest.skip("no HIP device")
     if not os.path.exists(EXE):
         pytest.fail("Fortran driver not built")
-    print(f"[fortran drop-in] {name}", flush=True)
+    print(f"[fortran drop-in] {name} {mode}", flush=True)
     ka = json.load(open(os.path.join(GOLD, "known_answers.json")))[name]
     m = M.read_deck(os.path.join(GOLD, "decks", name))
     f = tmp_path / "mesh.bin"
     m.write_flat(str(f), m.degree, m.dt, m.stop_time)
-    r = subprocess.run([EXE, str(f)], capture_output=True, text=True, timeout=300)
+    r = subprocess.run([EXE, str(f)] + ([mode] if mode else []), capture_output=True, text=True,
+                       timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     vals = {}
     for line in r.stdout.splitlines():
