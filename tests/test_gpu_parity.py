@@ -43,7 +43,7 @@ def rel(a, b):
     return 0.0 if s == 0 and np.abs(a).max() == 0 else np.abs(a - b).max() / max(s, 1e-300)
 
 
-def oracle_for(m, p):
+def oracle_for(m, p, alpha=10.0, theta=-1.0):
     kinds = np.zeros(8, np.int32)
     kinds[[2, 3, 4]] = -1
     kinds[[5, 6, 7]] = -2
@@ -51,7 +51,8 @@ def oracle_for(m, p):
     for t, s in m.tag_space.items():
         space[t] = s
     return oracle.Oracle(m.xyz, m.tet_vert, m.tria_vert, m.tria_tag, kinds, space, p,
-                         tet_poly=m.tet_poly, rho=m.rho, lam=m.lam, mu=m.mu)
+                         tet_poly=m.tet_poly, rho=m.rho, lam=m.lam, mu=m.mu, alpha=alpha,
+                         theta=theta)
 
 
 def hetero_mesh(n=3, seed=5):
@@ -237,6 +238,36 @@ def test_agglomerated_cube_wave_run(n, p, steps, dt):
     assert abs(r["err_l2"] / ro["err_l2"] - 1) < 1e-9
     assert abs(r["err_dg"] / ro["err_dg"] - 1) < 1e-9
     assert np.isfinite(r["err_l2"]) and r["err_l2"] < 0.5
+
+
+@pytest.mark.parametrize("theta,alpha", [(0.0, 10.0), (1.0, 10.0), (-1.0, 3.0)])
+@pytest.mark.parametrize("build", ["reference", "factored"])
+def test_penalty_variants_match_oracle(theta, alpha, build):
+    """IIP (theta = 0) and NIPG (theta = 1) face terms and another penalty
+    (alpha = 3): operator blocks, RHS and a step against the oracle built with
+    the same theta and alpha, for both operator builds."""
+    say(f"[penalty] theta={theta} alpha={alpha} {build}")
+    m = make_mesh("hetero")
+    p = 2
+    o = oracle_for(m, p, alpha=alpha, theta=theta)
+    o.build_operator()
+    oK, oA = o.K(), o.Adg()
+    rng = np.random.default_rng(9)
+    n = m.n_poly * 3 * o.Np
+    u0, u1 = rng.standard_normal(n), rng.standard_normal(n)
+    with L.Context(m, p, alpha=alpha, theta=theta) as ctx:
+        ctx.set_build(build)
+        ctx.make_matrices_free()
+        K, A, _, _ = ctx.export_operator()
+        for b in range(K.shape[0]):
+            assert rel(K[b], oK[b]) < 1e-13, (b, rel(K[b], oK[b]))
+            assert rel(A[b], oA[b]) < 1e-13
+        rs = ctx.make_rhs_free(L.PROB_NULL, _lib.SLOT_RHS_STAT)
+        rd = ctx.make_rhs_free(L.PROB_REF_MMS, _lib.SLOT_RHS_DYN)
+        assert rel(rd, o.rhs(L.PROB_REF_MMS)) < 1e-13
+        a = ctx.time_step_matrix_free(0.37, 1e-3, u0, u1)
+        b = o.step(0.37, 1e-3, u0, u1, o.rhs(L.PROB_NULL), o.rhs(L.PROB_REF_MMS))
+        assert rel(a, b) < 1e-13
 
 
 def _mms_data():
