@@ -270,6 +270,45 @@ def test_penalty_variants_match_oracle(theta, alpha, build):
         assert rel(a, b) < 1e-13
 
 
+def _single_tet():
+    xyz = np.array([[0.1, 0.2, 0.0], [1.0, 0.0, 0.1], [0.2, 0.9, 0.0], [0.3, 0.3, 1.1]])
+    tv = np.array([[0, 1, 2, 3]])
+    tri = np.sort(tv[0][M.FACE_SETS], axis=1)
+    return M.Mesh(xyz, tv, None, tri, np.full(4, 2), {2: 1})
+
+
+@pytest.mark.parametrize("case,p", [("single_tet", 1), ("single_tet", 4), ("all_neumann", 2)])
+def test_edge_meshes_match_oracle(case, p):
+    """Edge cases of the mesh: one tetrahedron with every face on the
+    (Dirichlet) boundary, and a cube with no Dirichlet face at all (Neumann
+    everywhere): operator, RHS and a step against the oracle; time loops
+    whose stop time ends before the second step."""
+    say(f"[edge mesh] {case} p={p}")
+    m = _single_tet() if case == "single_tet" else M.cube_mesh(2, dirichlet_faces=set(), agglomerate=True)
+    o = oracle_for(m, p)
+    o.build_operator()
+    rng = np.random.default_rng(3)
+    n = m.n_poly * 3 * o.Np
+    u0, u1 = rng.standard_normal(n), rng.standard_normal(n)
+    with L.Context(m, p) as ctx:
+        ctx.make_matrices_free()
+        K, A, _, _ = ctx.export_operator()
+        oK, oA = o.K(), o.Adg()
+        for b in range(K.shape[0]):
+            assert rel(K[b], oK[b]) < 1e-13 and rel(A[b], oA[b]) < 1e-13
+        ctx.make_rhs_free(L.PROB_NULL, _lib.SLOT_RHS_STAT)
+        rd = ctx.make_rhs_free(L.PROB_REF_MMS, _lib.SLOT_RHS_DYN)
+        assert rel(rd, o.rhs(L.PROB_REF_MMS)) < 1e-13
+        a = ctx.time_step_matrix_free(0.37, 1e-3, u0, u1)
+        b = o.step(0.37, 1e-3, u0, u1, o.rhs(L.PROB_NULL), o.rhs(L.PROB_REF_MMS))
+        assert rel(a, b) < 1e-13
+        # stop time before the second step: only the first step runs
+        r = ctx.run(L.PROB_REF_MMS, 1e-3, 5e-4)
+        assert r["nsteps"] == 1 and r["t_final"] == 1e-3
+        r = ctx.run(L.PROB_REF_MMS, 1e-3, 1.0, max_steps=1)
+        assert r["nsteps"] == 1
+
+
 def _mms_data():
     """The reference's manufactured data set (problem_data_and_properties.f90:
     f_time, gd, gn) written as the vectorised functions a caller would pass."""
