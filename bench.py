@@ -22,6 +22,7 @@ sample of the same workload (a smaller cube of the same structure).
 
 Usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--n 40] [--p 3]
        python bench.py --degree-sweep     (p = 1..4 step table instead)
+       python bench.py --uneven           (uneven polyhedra, p = 4, heterogeneous, instead)
 """
 import argparse
 import ctypes as C
@@ -213,6 +214,62 @@ def surrogate_leg(n_train, n_eval, epochs, threshold=1e-2):
     }
 
 
+def uneven_leg(n=30, p=4, seed=3, steps=200, run_steps=2000):
+    """Heterogeneous uneven polyhedra: n^3 cubes x 6 tets grown into polyhedra
+    of 4-40 tets (seeded), smooth lognormal lambda and mu, rho ~ U[0.8, 1.2],
+    Neumann on z=1 and Dirichlet elsewhere, a z-directed Ricker point force
+    (f0 = 5) at a seeded interior point, 64 seeded receivers, p = 4, dt from
+    CFL 0.2 with the fastest P-wave speed.  Times `steps` steps of the FP64
+    and of the 32-bit apply (each autotuned) from the same start, the
+    difference of the two states, and a `run_steps`-step 32-bit run
+    recording the receivers every step."""
+    import lymph3d_apsc_amd as L
+    from lymph3d_apsc_amd import _lib, mesh as M, ensemble as E
+    m = M.cube_mesh(n, dirichlet_faces={"x0", "x1", "y0", "y1", "z0"}, poly_size=(4, 40), seed=seed)
+    rng = np.random.default_rng(seed)
+    v = m.xyz[m.tet_vert]
+    bc = (((v[:, 0] + v[:, 1]) + v[:, 2]) + v[:, 3]) / 4
+    m.lam = E.lognormal_field(rng, bc)
+    m.mu = E.lognormal_field(rng, bc)
+    m.rho = rng.uniform(0.8, 1.2, m.n_tet)
+    dt = 0.2 * (1 / n) / (np.sqrt(((m.lam + 2 * m.mu) / m.rho).max()) * p * p)
+    src = rng.uniform(0.3, 0.7, (1, 3))
+    rec = rng.uniform(0.05, 0.95, (64, 3))
+    res = {"mesh": f"{n}^3 cubes x 6 tets grown into polyhedra of 4-40 tets (seed {seed})",
+           "n_tet": m.n_tet, "n_poly": m.n_poly, "p": p, "dt": dt}
+    with L.Context(m, p) as ctx:
+        ctx.set_keep_adg(False)
+        ctx.set_build("factored")
+        t0 = time.perf_counter()
+        ctx.make_matrices_free()
+        res["build_s"] = time.perf_counter() - t0
+        res["n_blk"] = ctx.n_blk
+        out = {}
+        for bits in (64, 32):
+            ctx.set_apply_precision(bits)
+            t = ctx.start_from_rest(dt, src, [[0.0, 0.0, 1.0]], f0=5.0)
+            chosen, _ = ctx.autotune(reps=10)
+            t = ctx.start_from_rest(dt, src, [[0.0, 0.0, 1.0]], f0=5.0)
+            ms = ctx.time_steps(L.PROB_RICKER, steps, t, dt) / steps
+            B = ctx.step_bytes_for(bits)
+            out[bits] = ctx.get_slot(_lib.SLOT_U_CUR)
+            res[f"fp{bits}"] = {"ms_per_step": ms, "variant": list(chosen), "GBps": B / ms / 1e6,
+                                "frac": B / ms / 1e6 / HBM_PEAK_GBS, "bytes_per_step": B}
+        res["fp32"]["rel_l2_diff_vs_fp64"] = float(np.linalg.norm(out[32] - out[64]) /
+                                                   np.linalg.norm(out[64]))
+        t = ctx.start_from_rest(dt, src, [[0.0, 0.0, 1.0]], f0=5.0)
+        ctx.set_receivers(rec, max_records=run_steps)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.step_n_record(L.PROB_RICKER, run_steps, t, dt, every=1)
+        ctx.sync()
+        res["fp32_run_with_receivers_s"] = time.perf_counter() - t0
+        res["run_steps"] = run_steps
+        res["traces_finite"] = bool(np.isfinite(ctx.traces()).all())
+    log(json.dumps(res))
+    return res
+
+
 def degree_sweep(steps=200):
     """Step throughput at p = 1..4 on agglomerated cubes of about 4 M DOFs
     (70^3, 52^3, 40^3, 33^3 cubes), FP64 and 32-bit apply, each autotuned."""
@@ -262,6 +319,8 @@ def main():
     ap.add_argument("--sur-train", type=int, default=1024)
     ap.add_argument("--sur-eval", type=int, default=128)
     ap.add_argument("--sur-epochs", type=int, default=300)
+    ap.add_argument("--uneven", action="store_true",
+                    help="instead of the bench line: heterogeneous uneven polyhedra at p=4 (one JSON line)")
     ap.add_argument("--degree-sweep", action="store_true",
                     help="instead of the bench line: step throughput at p = 1..4 (one JSON line)")
     a = ap.parse_args()
@@ -277,6 +336,9 @@ def main():
     if L.device_count() == 0:
         print(json.dumps({"error": "no HIP device visible"}))
         return 1
+    if a.uneven:
+        print(json.dumps({"uneven": uneven_leg()}))
+        return 0
     if a.degree_sweep:
         copy, read = stream_probe_gbps()
         print(json.dumps({"degree_sweep": degree_sweep(), "stream_copy_GBps": copy,
