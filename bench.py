@@ -272,7 +272,8 @@ def uneven_leg(n=30, p=4, seed=3, steps=200, run_steps=2000):
 
 def degree_sweep(steps=200):
     """Step throughput at p = 1..4 on agglomerated cubes of about 4 M DOFs
-    (70^3, 52^3, 40^3, 33^3 cubes), FP64 and 32-bit apply, each autotuned."""
+    (70^3, 52^3, 40^3, 33^3 cubes), FP64 and 32-bit apply, each autotuned;
+    and the time of both operator builds (keep_adg off, K and M only)."""
     import lymph3d_apsc_amd as L
     from lymph3d_apsc_amd import _lib, mesh as M
     rows = []
@@ -281,11 +282,18 @@ def degree_sweep(steps=200):
         dt = 0.2 * (1 / n) / (np.sqrt(3.0) * p * p)
         with L.Context(m, p) as ctx:
             ctx.set_keep_adg(False)
-            ctx.set_build("factored")
-            ctx.make_matrices_free()
+            build_s = {}
+            for build in ("reference", "factored"):  # the factored build is kept
+                ctx.set_build(build)
+                ctx.sync()
+                t0 = time.perf_counter()
+                ctx.make_matrices_free()
+                ctx.sync()
+                build_s[build] = time.perf_counter() - t0
             ctx.make_rhs_free(L.PROB_NULL, _lib.SLOT_RHS_STAT)
             ctx.make_rhs_free(L.PROB_CUBE_WAVE, _lib.SLOT_RHS_DYN)
-            row = dict(p=p, n=n, n_poly=m.n_poly, dofs=m.n_poly * ctx.n3, n_blk=ctx.n_blk)
+            row = dict(p=p, n=n, n_poly=m.n_poly, dofs=m.n_poly * ctx.n3, n_blk=ctx.n_blk,
+                       operator_build_s=build_s)
             for bits in (64, 32):
                 ctx.set_apply_precision(bits)
                 chosen, _ = ctx.autotune(reps=10)
