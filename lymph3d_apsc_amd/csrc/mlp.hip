@@ -162,7 +162,7 @@ __device__ inline void epilogue(const GemmArgs& g, int m, int n, float v) {
 // TM x TM output tile per 256-thread workgroup; 2 x 2 waves, each owning a
 // (TM/2) x (TM/2) quarter as (TM/32) x (TM/32) MFMA tiles of 16 x 16.
 template <bool A_KC, bool B_KC, int EPI, int TM>
-__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+__global__ __launch_bounds__(256, TM == 128 ? 3 : 1) void k_gemm(GemmArgs g) {
   constexpr int T = TM / 32;  // MFMA tiles per wave per dimension
   __shared__ __attribute__((aligned(16))) bf16_t sA[TM * LDK];
   __shared__ __attribute__((aligned(16))) bf16_t sB[TM * LDK];
