@@ -357,7 +357,10 @@ int tilde(LymphCtx* c, int mode, const double* in, double* out, const int* list 
   if (n == 0) return 0;
   return dispatch_p(c->p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
-    k_tilde<P><<<grid1(3 * (size_t)n, 128), 128, 0, c->stream>>>(mode, c->RM, in, out, list, n);
+    if (mode == 1)
+      k_tilde_back<P><<<(unsigned)n, 64, 0, c->stream>>>(c->RM, in, out, list, n);
+    else
+      k_tilde<P><<<grid1(3 * (size_t)n, 128), 128, 0, c->stream>>>(mode, c->RM, in, out, list, n);
     return check_launch("k_tilde");
   });
 }

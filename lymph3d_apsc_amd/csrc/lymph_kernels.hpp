@@ -1395,6 +1395,53 @@ __global__ void k_tilde(int mode, const double* __restrict__ RM, const double* _
   }
 }
 
+// u = R^-1 ~u (k_tilde's mode 1) with one wave per polyhedron: lane i keeps
+// row i's partial sums of the three components, and the columns are swept
+// from the last: y_j = s_j / R_jj is broadcast from lane j, then every row
+// above subtracts R_ij y_j.  NP steps instead of NP(NP+1)/2 dependent ones
+// per thread; row i subtracts its terms in descending column order.
+template <int P>
+__global__ __launch_bounds__(64) void k_tilde_back(const double* __restrict__ RM,
+                                                   const double* __restrict__ in,
+                                                   double* __restrict__ out,
+                                                   const int* __restrict__ list, int n) {
+  using D = Dims<P>;
+  constexpr int NP = D::NP;
+  static_assert(NP <= 64, "one lane per row");
+  const int w = blockIdx.x;
+  if (w >= n) return;
+  const int Pp = list ? list[w] : w;
+  const int i = threadIdx.x;
+  const bool row = i < NP;
+  const double* R = RM + (size_t)Pp * D::NPK;
+  const double* x = in + (size_t)Pp * D::N3;
+  double* y = out + (size_t)Pp * D::N3;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  if (row) {
+    s0 = x[i];
+    s1 = x[NP + i];
+    s2 = x[2 * NP + i];
+  }
+  const int ri = row ? pk_off(NP, i) : 0;
+#pragma unroll
+  for (int j = NP - 1; j >= 0; --j) {
+    const double rjj = R[pk_off(NP, j)];
+    const double y0 = __shfl(s0, j, 64) / rjj;
+    const double y1 = __shfl(s1, j, 64) / rjj;
+    const double y2 = __shfl(s2, j, 64) / rjj;
+    if (i == j) {
+      y[i] = y0;
+      y[NP + i] = y1;
+      y[2 * NP + i] = y2;
+    } else if (i < j) {
+      const double r = R[ri + (j - i)];
+      s0 = s0 - r * y0;
+      s1 = s1 - r * y1;
+      s2 = s2 - r * y2;
+    }
+  }
+}
+
 template <int P, int MODE, int UNR, int WPP = 1>
 __global__ __launch_bounds__(64 * WPP) void k_leapfrog32(const float* __restrict__ K,
                                                    const int* __restrict__ row_ptr,
