@@ -350,17 +350,20 @@ int launch_step(LymphCtx* c, int mode, double g, double c1, double c2, const dou
 }
 
 // Mass-orthonormal transforms (32-bit apply): mode 0 ~u = R u, 1 u = R^-1 ~u,
-// 2 ~F = R^-T F; `list` restricts mode 1 to n listed polyhedra.
+// 2 ~F = R^-T F, one wave per polyhedron; `list` restricts them to n listed
+// polyhedra.
 int tilde(LymphCtx* c, int mode, const double* in, double* out, const int* list = nullptr,
           int n = -1) {
   if (n < 0) n = c->n_poly;
   if (n == 0) return 0;
   return dispatch_p(c->p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
-    if (mode == 1)
+    if (mode == 0)
+      k_tilde<P, 0><<<(unsigned)n, 64, 0, c->stream>>>(c->RM, in, out, list, n);
+    else if (mode == 1)
       k_tilde_back<P><<<(unsigned)n, 64, 0, c->stream>>>(c->RM, in, out, list, n);
     else
-      k_tilde<P><<<grid1(3 * (size_t)n, 128), 128, 0, c->stream>>>(mode, c->RM, in, out, list, n);
+      k_tilde<P, 2><<<(unsigned)n, 64, 0, c->stream>>>(c->RM, in, out, list, n);
     return check_launch("k_tilde");
   });
 }

@@ -1360,42 +1360,77 @@ __global__ __launch_bounds__(256) void k_ortho_operator(const double* __restrict
     if (tid < PAD) Ob[(size_t)N3 * N3 + tid] = 0.0f;
 }
 
-// Vector transforms per (polyhedron, component): mode 0 ~u = R u, mode 1
-// u = R^-1 ~u, mode 2 ~F = R^-T F.  `list` (n entries) restricts mode 1 to
-// the listed polyhedra (receivers); null = all n polyhedra.
-template <int P>
-__global__ void k_tilde(int mode, const double* __restrict__ RM, const double* __restrict__ in,
-                        double* __restrict__ out, const int* __restrict__ list, int n) {
+// Mass-orthonormal transforms with one wave per polyhedron, lane i on row i
+// of all three components: MODE 0 ~u = R u (row i sums R_ik x_k over k >= i
+// in ascending k, x_k broadcast from lane k); MODE 2 ~F = R^-T F (forward
+// substitution swept by columns: y_j = s_j / R_jj from lane j, then every row
+// below subtracts R_ji y_j, so row i subtracts in ascending k as a serial
+// loop would).  `list` (n entries) restricts the transform to the listed
+// polyhedra; null = the first n.
+template <int P, int MODE>
+__global__ __launch_bounds__(64) void k_tilde(const double* __restrict__ RM,
+                                              const double* __restrict__ in,
+                                              double* __restrict__ out,
+                                              const int* __restrict__ list, int n) {
   using D = Dims<P>;
   constexpr int NP = D::NP;
-  const int id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= n * 3) return;
-  const int Pp = list ? list[id / 3] : id / 3, comp = id % 3;
+  static_assert(NP <= 64, "one lane per row");
+  static_assert(MODE == 0 || MODE == 2, "mode 1 is k_tilde_back");
+  const int w = blockIdx.x;
+  if (w >= n) return;
+  const int Pp = list ? list[w] : w;
+  const int i = threadIdx.x;
+  const bool row = i < NP;
   const double* R = RM + (size_t)Pp * D::NPK;
-  const double* x = in + (size_t)Pp * D::N3 + comp * NP;
-  double* y = out + (size_t)Pp * D::N3 + comp * NP;
-  if (mode == 0) {
-    for (int i = 0; i < NP; ++i) {
-      double s = 0.0;
-      for (int k = i; k < NP; ++k) s = s + R[pk_off(NP, i) + (k - i)] * x[k];
-      y[i] = s;
+  const double* x = in + (size_t)Pp * D::N3;
+  double* y = out + (size_t)Pp * D::N3;
+  double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+  if (row) {
+    x0 = x[i];
+    x1 = x[NP + i];
+    x2 = x[2 * NP + i];
+  }
+  if constexpr (MODE == 0) {
+    const int ri = row ? pk_off(NP, i) : 0;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      const double a0 = __shfl(x0, k, 64), a1 = __shfl(x1, k, 64), a2 = __shfl(x2, k, 64);
+      if (row && i <= k) {
+        const double r = R[ri + (k - i)];
+        s0 = s0 + r * a0;
+        s1 = s1 + r * a1;
+        s2 = s2 + r * a2;
+      }
     }
-  } else if (mode == 1) {
-    for (int i = NP - 1; i >= 0; --i) {
-      double s = x[i];
-      for (int k = i + 1; k < NP; ++k) s = s - R[pk_off(NP, i) + (k - i)] * y[k];
-      y[i] = s / R[pk_off(NP, i)];
+    if (row) {
+      y[i] = s0;
+      y[NP + i] = s1;
+      y[2 * NP + i] = s2;
     }
   } else {
-    for (int i = 0; i < NP; ++i) {
-      double s = x[i];
-      for (int k = 0; k < i; ++k) s = s - R[pk_off(NP, k) + (i - k)] * y[k];
-      y[i] = s / R[pk_off(NP, i)];
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      const int rj = pk_off(NP, j);
+      const double rjj = R[rj];
+      const double y0 = __shfl(x0, j, 64) / rjj;
+      const double y1 = __shfl(x1, j, 64) / rjj;
+      const double y2 = __shfl(x2, j, 64) / rjj;
+      if (i == j) {
+        y[i] = y0;
+        y[NP + i] = y1;
+        y[2 * NP + i] = y2;
+      } else if (row && i > j) {
+        const double r = R[rj + (i - j)];
+        x0 = x0 - r * y0;
+        x1 = x1 - r * y1;
+        x2 = x2 - r * y2;
+      }
     }
   }
 }
 
-// u = R^-1 ~u (k_tilde's mode 1) with one wave per polyhedron: lane i keeps
+// u = R^-1 ~u (mode 1 of the transforms above) with one wave per polyhedron: lane i keeps
 // row i's partial sums of the three components, and the columns are swept
 // from the last: y_j = s_j / R_jj is broadcast from lane j, then every row
 // above subtracts R_ij y_j.  NP steps instead of NP(NP+1)/2 dependent ones
