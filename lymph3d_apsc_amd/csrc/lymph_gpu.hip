@@ -297,6 +297,7 @@ double g_of(const LymphCtx* c, int problem, double t) {
 
 // host vector (user layout) -> device staging (poly layout)
 int h2d_vec(LymphCtx* c, const double* h, double* d) {
+  if (!h) return fail(LYMPH_ERR_ARG, "null vector");
   const size_t n = vec_n(c);
   if (c->layout == LYMPH_LAYOUT_POLY) {
     HIP_TRY(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -309,6 +310,7 @@ int h2d_vec(LymphCtx* c, const double* h, double* d) {
 }
 
 int d2h_vec(LymphCtx* c, const double* d, double* h) {
+  if (!h) return fail(LYMPH_ERR_ARG, "null vector");
   const size_t n = vec_n(c);
   if (c->layout == LYMPH_LAYOUT_POLY) {
     HIP_TRY(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1090,6 +1092,7 @@ int run_loop(LymphCtx* c, int problem, int nsteps, double t0, double dt, int eve
 int lymph_first_step_dev(LymphCtx* c, double g0, double dt, const double* u0, const double* v0,
                          double* u1) {
   if (int rc = need_built(c)) return rc;
+  if (!u0 || !v0 || !u1) return fail(LYMPH_ERR_ARG, "null vector");
   const double dt2 = dt * dt, half_dt2 = 0.5 * dt2;
   return step_dev(c, 1, g0, half_dt2, dt, u0, v0, u1, false);
 }
@@ -1097,6 +1100,7 @@ int lymph_first_step_dev(LymphCtx* c, double g0, double dt, const double* u0, co
 int lymph_step_dev(LymphCtx* c, double g, double dt, const double* um1, const double* u,
                    double* up1) {
   if (int rc = need_built(c)) return rc;
+  if (!um1 || !u || !up1) return fail(LYMPH_ERR_ARG, "null vector");
   return step_dev(c, 0, g, dt * dt, 0.0, u, um1, up1, false);
 }
 
@@ -1127,17 +1131,20 @@ int lymph_step_n(LymphCtx* c, int problem, int nsteps, double t0, double dt, dou
 
 int lymph_time_steps(LymphCtx* c, int problem, int nsteps, double t0, double dt, double* ms) {
   if (int rc = need_built(c)) return rc;
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
-  HIP_TRY(hipEventRecord(e0, c->stream));
-  int rc = lymph_step_n(c, problem, nsteps, t0, dt, nullptr);
-  HIP_TRY(hipEventRecord(e1, c->stream));
-  HIP_TRY(hipEventSynchronize(e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   float f = 0.f;
-  HIP_TRY(hipEventElapsedTime(&f, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  int rc = 0;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+      hipEventRecord(e0, c->stream) != hipSuccess) {
+    rc = fail(LYMPH_ERR_HIP, "event set-up failed");
+  } else {
+    rc = lymph_step_n(c, problem, nsteps, t0, dt, nullptr);
+    if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&f, e0, e1) != hipSuccess)
+      if (!rc) rc = fail(LYMPH_ERR_HIP, "event timing failed");
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   if (rc) return rc;
   if (ms) *ms = f;
   return 0;
