@@ -258,6 +258,22 @@ int lymph_ctx_set_keep_adg(LymphCtx* ctx, int keep);
 int lymph_ctx_set_build(LymphCtx* ctx, int mode);
 int lymph_release_fp64_operator(LymphCtx* ctx);
 
+/* Face pairs stored once (no reference counterpart; an operator storage
+ * choice behind TIME_STEP_MATRIX_FREE, src/matrix_free.f90:1230-1385).  With
+ * SIPG (theta = -1) and one material on both traces of every face -- the
+ * reference forces rho = lambda = mu = 1, src/Lymph3D.f90:196-203 -- the
+ * blocks satisfy K_QP = K_PQ^T up to the rounding of the two sides' face
+ * quadratures.  on = 1 checks every pair on the device (largest
+ * max |K_PQ - K_QP^T| / max |K_PQ| to *asym when asym is not null) and, when
+ * it is at most 1e-12, keeps the self blocks and K_PQ for Q > P only; the
+ * FP64 step (lymph_step*, lymph_first_step*, lymph_step_n, lymph_run, captured
+ * solves) then reads about 42% fewer operator bytes at p = 3 and applies
+ * K_QP as the transpose of P's copy.  An operator that fails the check
+ * returns LYMPH_ERR_ARG and the full rows stay in use.  Needs p <= 3 and the
+ * FP64 blocks (not released); lymph_build_operator drops the stored pairs;
+ * on = 0 frees them.  The 32-bit apply and the DG norm are unaffected. */
+int lymph_set_symmetric_storage(LymphCtx* ctx, int on, double* asym);
+
 /* Point forces f_k delta(x - x_k) added to a RHS slot (normally
  * LYMPH_SLOT_RHS_DYN): rhs[P][i*Np+m] += f_k,i phi_m(x_k) in the polyhedron of
  * tet[k].  No reference counterpart: the reference's point source is the

@@ -85,6 +85,11 @@ struct LymphCtx {
   double *gf0 = nullptr, *gt0 = nullptr, *gtab = nullptr;
   float* K32 = nullptr;     // 32-bit apply: ~K = R^-T K R^-1 (mass-orthonormal basis)
   double* tv[5] = {};       // 32-bit stepping: ~u rotation (0..2), ~F_s (3), ~F_d (4)
+  double* Ks = nullptr;     // face pairs stored once (lymph_set_symmetric_storage)
+  int* sidx = nullptr;      // per block of the full rows: stored block, ~ = transposed
+  int* tdst = nullptr;      // stored pair block -> its (Q, P) slot of Q's row, else -1
+  double* prod = nullptr;   // per-block products of one step (n_blk x 3Np)
+  size_t ks_bytes = 0;
   int* blk_row = nullptr;   // row polyhedron of each block
   int* order = nullptr;     // step dispatch order (step_grid(n_poly) entries)
   int max_row = 0;          // most blocks in one block row
@@ -340,6 +345,22 @@ int launch_step(LymphCtx* c, int mode, double g, double c1, double c2, const dou
       else
         launch_leapfrog32<P, 1>(c->step_variant, c->n_poly, c->stream, c->K32, c->md.row_ptr,
                                 c->md.col, c->RM, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->order, dyn);
+    } else if (c->Ks) {
+      if constexpr (3 * np_of(P) <= 64 && (3 * np_of(P)) % 2 == 0) {
+        k_sym_products<P><<<step_grid(c->n_poly), 64, 0, c->stream>>>(
+            c->Ks, c->md.row_ptr, c->md.col, c->sidx, c->tdst, u, c->prod, c->n_poly, c->order);
+        if (int rc = check_launch("k_sym_products")) return rc;
+        if (mode == 0)
+          k_sym_update<P, 0><<<step_grid(c->n_poly), 64, 0, c->stream>>>(
+              c->prod, c->md.row_ptr, c->RM, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->n_poly,
+              c->order);
+        else
+          k_sym_update<P, 1><<<step_grid(c->n_poly), 64, 0, c->stream>>>(
+              c->prod, c->md.row_ptr, c->RM, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->n_poly,
+              c->order);
+      } else {
+        return fail(LYMPH_ERR_STATE, "symmetric storage needs p <= 3");
+      }
     } else if (mode == 0) {
       launch_leapfrog<P, 0>(c->step_variant, c->n_poly, c->stream, c->K, c->md.row_ptr, c->md.col,
                             c->RM, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->order, dyn);
@@ -368,6 +389,129 @@ int tilde(LymphCtx* c, int mode, const double* in, double* out, const int* list 
       k_tilde<P, 2><<<(unsigned)n, 64, 0, c->stream>>>(c->RM, in, out, list, n);
     return check_launch("k_tilde");
   });
+}
+
+// Largest max |K_PQ - K_QP^T| / max |K_PQ| of a pair taken as symmetric: the
+// two blocks come from the two sides' own face quadratures, which agree to a
+// few ulps where the bilinear form is symmetric.
+constexpr double kSymTol = 1e-12;
+
+// Drop the once-stored face pairs (operator rebuilt, storage switched off).
+int sym_off(LymphCtx* c) {
+  if (!c->Ks && !c->sidx && !c->tdst && !c->prod) return 0;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (c->Ks) HIP_TRY(hipFree(c->Ks));
+  if (c->sidx) HIP_TRY(hipFree(c->sidx));
+  if (c->tdst) HIP_TRY(hipFree(c->tdst));
+  if (c->prod) HIP_TRY(hipFree(c->prod));
+  c->Ks = nullptr;
+  c->sidx = nullptr;
+  c->tdst = nullptr;
+  c->prod = nullptr;
+  c->bytes -= c->ks_bytes;
+  c->ks_bytes = 0;
+  return 0;
+}
+
+// Store every face pair once: check K_QP = K_PQ^T pair by pair on the device
+// (largest ratio to *asym), then gather the self blocks and the blocks of
+// neighbours Q > P into Ks and index the full rows into it.
+int sym_on(LymphCtx* c, double tol, double* asym) {
+  const int np = c->n_poly;
+  const std::vector<int>& rp = c->h_row_ptr;
+  const std::vector<int>& cl = c->h_col;
+  std::vector<int> partner(c->n_blk, -1), sid(c->n_blk, -1), src, pb, pt;
+  for (int P = 0; P < np; ++P)
+    for (int b = rp[P]; b < rp[P + 1]; ++b) {
+      const int Q = cl[b];
+      if (Q == P) continue;
+      for (int b2 = rp[Q]; b2 < rp[Q + 1]; ++b2)
+        if (cl[b2] == P) {
+          partner[b] = b2;
+          break;
+        }
+      if (partner[b] < 0) return fail(LYMPH_ERR_STATE, "neighbour rows do not match");
+      if (Q > P) {
+        pb.push_back(b);
+        pt.push_back(partner[b]);
+      }
+    }
+  for (int P = 0; P < np; ++P)
+    for (int b = rp[P]; b < rp[P + 1]; ++b)
+      if (cl[b] >= P) {
+        sid[b] = (int)src.size();
+        src.push_back(b);
+      }
+  std::vector<int> sx(c->n_blk), td(c->n_blk, -1);
+  for (int P = 0; P < np; ++P)
+    for (int b = rp[P]; b < rp[P + 1]; ++b) {
+      sx[b] = cl[b] >= P ? sid[b] : ~sid[partner[b]];
+      if (cl[b] > P) td[b] = partner[b];
+    }
+  const int n_pairs = (int)pb.size();
+  double worst = 0.0;
+  if (n_pairs > 0) {
+    int *dpb = nullptr, *dpt = nullptr;
+    double* dr = nullptr;
+    int rc = dalloc(&dpb, n_pairs);
+    if (!rc) rc = dalloc(&dpt, n_pairs);
+    if (!rc) rc = dalloc(&dr, n_pairs);
+    std::vector<double> ratio(n_pairs);
+    if (!rc && (hipMemcpy(dpb, pb.data(), n_pairs * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(dpt, pt.data(), n_pairs * sizeof(int), hipMemcpyHostToDevice) != hipSuccess))
+      rc = fail(LYMPH_ERR_HIP, "copy of the pair lists failed");
+    if (!rc)
+      rc = dispatch_p(c->p, [&](auto pc) -> int {
+        constexpr int P = decltype(pc)::value;
+        if constexpr ((3 * np_of(P)) % 2 == 0) {
+          k_pair_asym<P><<<(unsigned)n_pairs, 64, 0, c->stream>>>(c->K, dpb, dpt, n_pairs, dr);
+          return check_launch("k_pair_asym");
+        } else {
+          return fail(LYMPH_ERR_STATE, "symmetric storage needs p <= 3");
+        }
+      });
+    if (!rc && (hipMemcpyAsync(ratio.data(), dr, n_pairs * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream) != hipSuccess ||
+                hipStreamSynchronize(c->stream) != hipSuccess))
+      rc = fail(LYMPH_ERR_HIP, "copy of the pair ratios failed");
+    (void)hipFree(dpb);
+    (void)hipFree(dpt);
+    (void)hipFree(dr);
+    if (rc) return rc;
+    for (double r : ratio)
+      if (!(r <= worst)) worst = r;  // a NaN ratio sticks
+  }
+  if (asym) *asym = worst;
+  if (!(worst <= tol))
+    return fail(LYMPH_ERR_ARG, "operator is not symmetric: max |K_PQ - K_QP^T| / max |K_PQ| = " +
+                                   std::to_string(worst));
+  const int n_s = (int)src.size();
+  const size_t bs = blk_sz(c);
+  int* dsrc = nullptr;
+  if (int rc = dalloc(&c->Ks, (size_t)n_s * bs)) return rc;
+  int rc = dalloc(&c->sidx, c->n_blk);
+  if (!rc) rc = dalloc(&c->tdst, c->n_blk);
+  if (!rc) rc = dalloc(&c->prod, (size_t)c->n_blk * c->n3);
+  if (!rc) rc = dalloc(&dsrc, n_s);
+  if (!rc && (hipMemcpy(c->sidx, sx.data(), sx.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy(c->tdst, td.data(), td.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy(dsrc, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess))
+    rc = fail(LYMPH_ERR_HIP, "copy of the block index failed");
+  if (!rc) {
+    k_gather_blocks<<<4096, 256, 0, c->stream>>>(c->K, c->Ks, dsrc, n_s, bs);
+    rc = check_launch("k_gather_blocks");
+  }
+  if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(LYMPH_ERR_HIP, "gather failed");
+  if (dsrc) (void)hipFree(dsrc);
+  if (rc) {
+    c->ks_bytes = 0;
+    (void)sym_off(c);
+    return rc;
+  }
+  c->ks_bytes = (size_t)n_s * bs * sizeof(double) + 2 * (size_t)c->n_blk * sizeof(int) +
+                (size_t)c->n_blk * c->n3 * sizeof(double);
+  c->bytes += c->ks_bytes;
+  return 0;
 }
 
 // ~F_s, ~F_d of the current RHS slots (every 32-bit stepping call starts here)
@@ -483,6 +627,10 @@ int lymph_ctx_destroy(LymphCtx* c) {
     if (q) (void)hipFree(q);
   if (c->K) (void)hipFree(c->K);
   if (c->A) (void)hipFree(c->A);
+  if (c->Ks) (void)hipFree(c->Ks);
+  if (c->sidx) (void)hipFree(c->sidx);
+  if (c->tdst) (void)hipFree(c->tdst);
+  if (c->prod) (void)hipFree(c->prod);
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   for (void* p : c->owned) (void)hipFree(p);
@@ -729,6 +877,7 @@ int lymph_get_pattern(const LymphCtx* c, int32_t* row_ptr, int32_t* col) {
 
 int lymph_build_operator(LymphCtx* c) {
   if (int rc = need_ctx(c)) return rc;
+  if (int rc = sym_off(c)) return rc;  // the blocks are about to change
   const size_t bs = blk_sz(c), nk = (size_t)c->n_blk * bs;
   // K (and A_dg unless dropped) live outside the mesh allocations so that
   // they can be released for batched runs
@@ -806,6 +955,16 @@ int lymph_release_fp64_operator(LymphCtx* c) {
     c->bytes -= (size_t)c->n_blk * blk_sz(c) * sizeof(double);
   }
   return 0;
+}
+
+int lymph_set_symmetric_storage(LymphCtx* c, int on, double* asym) {
+  if (int rc = need_built(c)) return rc;
+  if (asym) *asym = 0.0;
+  if (!on) return sym_off(c);
+  if (c->Ks) return 0;
+  if (!c->K) return fail(LYMPH_ERR_STATE, "the FP64 operator was released");
+  if (c->p > 3) return fail(LYMPH_ERR_ARG, "symmetric storage needs p <= 3");
+  return sym_on(c, kSymTol, asym);
 }
 
 int lymph_export_operator(const LymphCtx* cc, double* K, double* A, double* M, double* Mm) {

@@ -971,6 +971,184 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog(const double* __restrict_
   }
 }
 
+// Step with every face pair's block stored once, in two launches.  With SIPG
+// and one material on both traces of each face, K_QP = K_PQ^T to rounding, so
+// only the self blocks and K_PQ for Q > P are kept (Ks; sidx[b] >= 0 the
+// stored block of full-row block b, < 0 a block of the neighbour's row).
+// k_sym_products: one wave per polyhedron P streams its stored blocks once;
+// each gives P's product K_PQ u_Q (FMA-free chain over ascending columns, as
+// k_leapfrog) into prod[b], and a pair block also gives the neighbour's
+// product K_QP u_P = K_PQ^T u_P into prod[tdst[b]], the slot of block (Q, P)
+// of Q's row.  Lane i holds row i of K_PQ and u_P[i]; the products
+// K_PQ[i][c] u_P[i] go to LDS in three chunks of CW columns, lane (c, t) of a
+// chunk sums rows [t CW, (t+1) CW) of column c, and lane c gathers the three
+// partial sums.  k_sym_update: per polyhedron, the row's products summed in
+// block order, then the solves and update of k_leapfrog.  p <= 3 (3Np <= 64
+// rows, whole column pairs).
+template <int P>
+__global__ __launch_bounds__(64) void k_sym_products(const double* __restrict__ Ks,
+                                                     const int* __restrict__ row_ptr,
+                                                     const int* __restrict__ col,
+                                                     const int* __restrict__ sidx,
+                                                     const int* __restrict__ tdst,
+                                                     const double* __restrict__ u,
+                                                     double* __restrict__ prod, int n_poly,
+                                                     const int* __restrict__ order) {
+  using D = Dims<P>;
+  constexpr int N3 = D::N3, NPAIR = D::NPAIR;
+  static_assert(N3 <= 64 && N3 % 2 == 0, "one lane per row, whole column pairs");
+  constexpr int CW = N3 / 3, CP = CW / 2;
+  static_assert(CW % 2 == 0, "whole column pairs per chunk");
+  constexpr int LDT = N3 + 1;  // odd stride: a column read spreads over the banks
+  __shared__ double sT[CW * LDT];
+  const int lane = threadIdx.x;
+  const bool row = lane < N3;
+  const int Pp = step_poly(order);
+  if (Pp >= n_poly) return;  // padding workgroups of the last XCD share
+  const int b0 = row_ptr[Pp], b1 = row_ptr[Pp + 1];
+  // the row's neighbours and indices, one block per lane, so a block's loads
+  // do not wait on its index (rows of more than 64 blocks read the rest as
+  // they come)
+  const int lcb = lane < b1 - b0 ? col[b0 + lane] : 0;
+  const int lsi = lane < b1 - b0 ? sidx[b0 + lane] : 0;
+  const int ltd = lane < b1 - b0 ? tdst[b0 + lane] : 0;
+  const double up = row ? u[(size_t)Pp * N3 + lane] : 0.0;
+  const int cl = lane % CW, t = lane / CW;
+  for (int b = b0; b < b1; ++b) {
+    const int j = b - b0;
+    const int si = j < 64 ? __builtin_amdgcn_readlane(lsi, j) : __builtin_amdgcn_readfirstlane(sidx[b]);
+    if (si < 0) continue;  // written by the neighbour's wave
+    const int cb = j < 64 ? __builtin_amdgcn_readlane(lcb, j) : __builtin_amdgcn_readfirstlane(col[b]);
+    const int td = j < 64 ? __builtin_amdgcn_readlane(ltd, j) : __builtin_amdgcn_readfirstlane(tdst[b]);
+    const double* __restrict__ Kb = Ks + (size_t)si * D::BS;
+    const double xl = row ? u[(size_t)cb * N3 + lane] : 0.0;
+    double2 kv[NPAIR];
+#pragma unroll
+    for (int c2i = 0; c2i < NPAIR; ++c2i)
+      kv[c2i] = row ? load_stream(Kb + ((size_t)c2i * N3 + lane) * 2) : make_double2(0.0, 0.0);
+    double s = 0.0;
+#pragma unroll
+    for (int c2i = 0; c2i < NPAIR; ++c2i) {
+      const double x0 = lane_bcast(xl, 2 * c2i), x1 = lane_bcast(xl, 2 * c2i + 1);
+      s = s + kv[c2i].x * x0;
+      s = s + kv[c2i].y * x1;
+    }
+    if (row) prod[(size_t)b * N3 + lane] = s;
+    if (cb == Pp) continue;  // self block: no partner
+    double st = 0.0;
+#pragma unroll
+    for (int h = 0; h < 3; ++h) {
+      __syncthreads();  // the previous chunk's reads are done
+      if (row) {
+#pragma unroll
+        for (int k = 0; k < CP; ++k) {
+          sT[(2 * k) * LDT + lane] = kv[h * CP + k].x * up;
+          sT[(2 * k + 1) * LDT + lane] = kv[h * CP + k].y * up;
+        }
+      }
+      __syncthreads();
+      double v = 0.0;
+      if (row) {
+        v = sT[cl * LDT + t * CW];
+#pragma unroll
+        for (int k = 1; k < CW; ++k) v = v + sT[cl * LDT + t * CW + k];
+      }
+      const int src = (lane - h * CW) & 63;  // lanes outside the chunk discard
+      const double v0 = __shfl(v, src % CW, 64);
+      const double v1 = __shfl(v, src % CW + CW, 64);
+      const double v2 = __shfl(v, src % CW + 2 * CW, 64);
+      if (lane >= h * CW && lane < (h + 1) * CW) st = (v0 + v1) + v2;
+    }
+    if (row) prod[(size_t)td * N3 + lane] = st;
+  }
+}
+
+template <int P, int MODE>
+__global__ __launch_bounds__(64) void k_sym_update(const double* __restrict__ prod,
+                                                   const int* __restrict__ row_ptr,
+                                                   const double* __restrict__ Rp,
+                                                   const double* __restrict__ rs,
+                                                   const double* __restrict__ rd,
+                                                   const double* __restrict__ u,
+                                                   const double* __restrict__ wa,
+                                                   double* __restrict__ out, double g, double c1,
+                                                   double c2, const int* __restrict__ gsel,
+                                                   const double* __restrict__ gtab, int n_poly,
+                                                   const int* __restrict__ order) {
+  using D = Dims<P>;
+  constexpr int NP = D::NP, N3 = D::N3;
+  static_assert(N3 <= 64, "one lane per row");
+  static_assert(MODE == 0 || MODE == 1, "leapfrog and first step");
+  __shared__ double sw[N3];
+  __shared__ double sR[D::NPK];
+  const int lane = threadIdx.x;
+  const bool row = lane < N3;
+  const int Pp = step_poly(order);
+  if (Pp >= n_poly) return;
+  for (int k = lane; k < D::NPK; k += 64) sR[k] = Rp[(size_t)Pp * D::NPK + k];
+  const int b0 = row_ptr[Pp], b1 = row_ptr[Pp + 1];
+  double acc = 0.0;
+  if (row) {
+    acc = prod[(size_t)b0 * N3 + lane];
+    for (int b = b0 + 1; b < b1; ++b) acc = acc + prod[(size_t)b * N3 + lane];
+  }
+  const size_t o = (size_t)Pp * N3;
+  const double gp = gsel ? gtab[gsel[Pp]] : g;
+  if (row) sw[lane] = -acc + rs[o + lane] + rd[o + lane] * gp;
+  __syncthreads();
+  if (lane < 3) solve_packed<NP>(sR, sw + lane * NP);
+  __syncthreads();
+  if (row) {
+    if (MODE == 0)
+      out[o + lane] = c1 * sw[lane] + 2.0 * u[o + lane] - wa[o + lane];
+    else
+      out[o + lane] = c1 * sw[lane] + u[o + lane] + c2 * wa[o + lane];
+  }
+}
+
+// Symmetry of the operator, pair by pair: ratio[w] = max |K_PQ - K_QP^T| /
+// max |K_PQ| for the blocks pb[w] = K_PQ and pt[w] = K_QP (column-pair
+// layout, whole pairs: 3Np even).
+template <int P>
+__global__ __launch_bounds__(64) void k_pair_asym(const double* __restrict__ K,
+                                                  const int* __restrict__ pb,
+                                                  const int* __restrict__ pt, int n_pairs,
+                                                  double* __restrict__ ratio) {
+  using D = Dims<P>;
+  constexpr int N3 = D::N3;
+  static_assert(N3 % 2 == 0, "whole column pairs");
+  const int w = blockIdx.x;
+  if (w >= n_pairs) return;
+  const double* A = K + (size_t)pb[w] * D::BS;
+  const double* B = K + (size_t)pt[w] * D::BS;
+  double dmax = 0.0, amax = 0.0;
+  for (int e = threadIdx.x; e < N3 * N3; e += 64) {
+    const int r = e % N3, c = e / N3;
+    const double a = A[((size_t)(c >> 1) * N3 + r) * 2 + (c & 1)];   // K_PQ[r][c]
+    const double bt = B[((size_t)(r >> 1) * N3 + c) * 2 + (r & 1)];  // K_QP[c][r]
+    const double d = fabs(a - bt);
+    dmax = (d > dmax || d != d) ? d : dmax;  // a NaN makes the pair fail
+    amax = fabs(a) > amax ? fabs(a) : amax;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const double od = __shfl_xor(dmax, off, 64), oa = __shfl_xor(amax, off, 64);
+    dmax = (od > dmax || od != od) ? od : dmax;
+    amax = oa > amax ? oa : amax;
+  }
+  if (threadIdx.x == 0) ratio[w] = dmax == 0.0 ? 0.0 : dmax / amax;
+}
+
+// Ks[s] = K[src[s]], whole blocks of bs doubles.
+__global__ void k_gather_blocks(const double* __restrict__ K, double* __restrict__ Ks,
+                                const int* __restrict__ src, int n, size_t bs) {
+  const size_t total = (size_t)n * bs;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t s = i / bs, k = i - s * bs;
+    Ks[i] = K[(size_t)src[s] * bs + k];
+  }
+}
+
 // Barrier of the packed kernels: a whole workgroup for one-wave workgroups;
 // with several waves per workgroup each wave owns its own LDS region and
 // polyhedra (and its own loop trip count), so only the wave's own LDS

@@ -238,6 +238,14 @@ module lymph_gpu
       type(c_ptr), value :: ctx
       integer(c_int) :: rc
     end function
+    function lymph_set_symmetric_storage(ctx, on, asym) &
+        bind(C, name='lymph_set_symmetric_storage') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+      real(c_double), intent(out) :: asym
+      integer(c_int) :: rc
+    end function
     function lymph_add_point_sources(ctx, n, xyz, tet, force, slot) &
         bind(C, name='lymph_add_point_sources') result(rc)
       import :: c_int, c_ptr, c_double, c_int32_t

@@ -239,6 +239,16 @@ class Context:
         """Free the FP64 K blocks once the 32-bit apply is selected."""
         check(lib().lymph_release_fp64_operator(self.h), "lymph_release_fp64_operator")
 
+    def set_symmetric_storage(self, on=True):
+        """Keep every face pair's block once (K_QP applied as the transpose of
+        K_PQ) for the FP64 step; the operator must pass the device's pair
+        check (SIPG, one material per face).  Returns the largest
+        max|K_PQ - K_QP^T| / max|K_PQ| found (0 when switching off)."""
+        asym = C.c_double()
+        check(lib().lymph_set_symmetric_storage(self.h, int(bool(on)), C.byref(asym)),
+              "lymph_set_symmetric_storage")
+        return asym.value
+
     @property
     def device_bytes(self):
         info = np.zeros(8, np.int64)

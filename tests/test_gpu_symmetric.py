@@ -1,0 +1,134 @@
+"""Face pairs stored once (lymph_set_symmetric_storage): with SIPG and one
+material on both traces of every face, K_QP = K_PQ^T to rounding, and the FP64
+step that applies K_QP as the transpose of P's copy must match the full-row
+step and the oracle.
+
+Tolerances: single steps within 1e-13 of the full-row step and of the oracle
+(relative to the vector's max; the two sides' blocks differ by a few ulps);
+full runs of the reference's decks within four times the known-answer
+tolerance of test_gpu_parity (the factored build's margin) and within 1e-11
+of the oracle's final vector.  Operators that are not symmetric (NIPG,
+per-tet materials) and p = 4 are refused with LYMPH_ERR_ARG and the full rows
+stay in use."""
+import os
+
+import numpy as np
+import pytest
+
+import lymph3d_apsc_amd as L
+from lymph3d_apsc_amd import _lib, mesh as M
+from oracle import oracle
+
+from tests.test_gpu_parity import DECKS, KA, hetero_mesh, oracle_for, rel, say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if L.device_count() == 0:
+        pytest.skip("no HIP device")
+
+
+def _mesh(name):
+    if name.startswith("tet_"):
+        return M.read_deck(os.path.join(DECKS, name))
+    return M.cube_mesh(int(name[4]), agglomerate=True)
+
+
+def _prepared(ctx):
+    ctx.make_matrices_free()
+    ctx.make_rhs_free(L.PROB_NULL, _lib.SLOT_RHS_STAT)
+    ctx.make_rhs_free(L.PROB_REF_MMS, _lib.SLOT_RHS_DYN)
+
+
+@pytest.mark.parametrize("name,p", [("cube3agg", 3), ("tet_25", 2), ("cube2agg", 1), ("cube4agg", 3)])
+def test_steps_match_full_rows_and_oracle(name, p):
+    say(f"[symmetric] steps {name} p={p}")
+    m = _mesh(name)
+    o = oracle_for(m, p)
+    o.build_operator()
+    rng = np.random.default_rng(7)
+    n = m.n_poly * 3 * o.Np
+    u0, u1 = rng.standard_normal(n), rng.standard_normal(n)
+    dt, g = 1e-3, 0.37
+    with L.Context(m, p) as ctx:
+        _prepared(ctx)
+        full = ctx.time_step_matrix_free(g, dt, u0, u1)
+        first_full = ctx.first_time_step_matrix_free(g, dt, u0, u1)
+        bytes_full = ctx.device_bytes
+        asym = ctx.set_symmetric_storage(True)
+        assert 0.0 <= asym < 1e-12, asym
+        assert ctx.device_bytes > bytes_full
+        sym = ctx.time_step_matrix_free(g, dt, u0, u1)
+        first_sym = ctx.first_time_step_matrix_free(g, dt, u0, u1)
+        ref = o.step(g, dt, u0, u1, o.rhs(L.PROB_NULL), o.rhs(L.PROB_REF_MMS))
+        acc = lambda a: a - (2 * u1 - u0)  # noqa: E731
+        assert rel(acc(sym), acc(full)) < 1e-13, rel(acc(sym), acc(full))
+        assert rel(acc(sym), acc(ref)) < 1e-13, rel(acc(sym), acc(ref))
+        assert rel(first_sym - u0, first_full - u0) < 1e-13
+        # device loop (slot rotation) agrees with the full rows over 40 steps
+        ctx.set_slot(_lib.SLOT_U_PREV, u0)
+        ctx.set_slot(_lib.SLOT_U_CUR, u1)
+        ctx.step_n(L.PROB_REF_MMS, 40, 0.0, 1e-4)
+        a = ctx.get_slot(_lib.SLOT_U_CUR)
+        assert ctx.set_symmetric_storage(False) == 0.0
+        ctx.set_slot(_lib.SLOT_U_PREV, u0)
+        ctx.set_slot(_lib.SLOT_U_CUR, u1)
+        ctx.step_n(L.PROB_REF_MMS, 40, 0.0, 1e-4)
+        b = ctx.get_slot(_lib.SLOT_U_CUR)
+        assert ctx.device_bytes == bytes_full
+        assert rel(a, b) < 1e-12, rel(a, b)
+
+
+@pytest.mark.parametrize("name", ["tet_25", "tet_100"])
+def test_reference_decks_with_pairs_stored_once(name):
+    say(f"[symmetric] known answer {name}")
+    ka = KA[name]
+    m = M.read_deck(os.path.join(DECKS, name))
+    with L.Context(m, m.degree) as ctx:
+        ctx.make_matrices_free()
+        ctx.set_symmetric_storage(True)
+        r = ctx.run(L.PROB_REF_MMS, m.dt, m.stop_time, want_u=True)
+    assert r["nsteps"] == ka["steps"]
+    tol = 4 * (5e-12 if name == "tet_100" else 5e-13)
+    assert abs(r["err_l2"] / ka["err_l2"] - 1) < tol, r
+    assert abs(r["err_dg"] / ka["err_dg"] - 1) < tol, r
+    o = oracle_for(m, m.degree)
+    ro = o.run(oracle.PROB_REF_MMS, m.dt, m.stop_time, want_u=True)
+    assert rel(r["u"], ro["u"]) < 1e-11, rel(r["u"], ro["u"])
+
+
+@pytest.mark.parametrize("case", ["nipg", "materials", "p4"])
+def test_non_symmetric_operators_are_refused(case):
+    say(f"[symmetric] refusal {case}")
+    if case == "materials":
+        m, p, theta = hetero_mesh(), 2, -1.0
+    else:
+        m, p, theta = M.cube_mesh(2, agglomerate=True), (4 if case == "p4" else 2), \
+            (1.0 if case == "nipg" else -1.0)
+    rng = np.random.default_rng(3)
+    with L.Context(m, p, theta=theta) as ctx:
+        _prepared(ctx)
+        n = m.n_poly * 3 * ((p + 1) * (p + 2) * (p + 3) // 6)
+        u0, u1 = rng.standard_normal(n), rng.standard_normal(n)
+        before = ctx.time_step_matrix_free(0.3, 1e-3, u0, u1)
+        with pytest.raises(L.LymphError) as e:
+            ctx.set_symmetric_storage(True)
+        assert e.value.code == _lib.ERR_ARG
+        msg = str(e.value)
+        assert ("not symmetric" in msg) if case != "p4" else ("p <= 3" in msg), msg
+        after = ctx.time_step_matrix_free(0.3, 1e-3, u0, u1)
+        assert np.array_equal(before, after)
+
+
+def test_rebuild_drops_the_stored_pairs():
+    say("[symmetric] rebuild")
+    m = M.cube_mesh(2, agglomerate=True)
+    with L.Context(m, 2) as ctx:
+        _prepared(ctx)
+        base = ctx.device_bytes
+        ctx.set_symmetric_storage(True)
+        assert ctx.device_bytes > base
+        ctx.make_matrices_free()
+        assert ctx.device_bytes == base
