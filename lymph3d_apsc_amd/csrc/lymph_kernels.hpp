@@ -1079,7 +1079,6 @@ __global__ __launch_bounds__(64) void k_sym_update(const double* __restrict__ pr
   constexpr int NP = D::NP, N3 = D::N3;
   static_assert(N3 <= 64, "one lane per row");
   static_assert(MODE == 0 || MODE == 1, "leapfrog and first step");
-  __shared__ double sw[N3];
   __shared__ double sR[D::NPK];
   const int lane = threadIdx.x;
   const bool row = lane < N3;
@@ -1094,15 +1093,36 @@ __global__ __launch_bounds__(64) void k_sym_update(const double* __restrict__ pr
   }
   const size_t o = (size_t)Pp * N3;
   const double gp = gsel ? gtab[gsel[Pp]] : g;
-  if (row) sw[lane] = -acc + rs[o + lane] + rd[o + lane] * gp;
-  __syncthreads();
-  if (lane < 3) solve_packed<NP>(sR, sw + lane * NP);
-  __syncthreads();
+  double w = row ? -acc + rs[o + lane] + rd[o + lane] * gp : 0.0;
+  __syncthreads();  // sR
+  // M^-1 w = R^-1 R^-T w with lane (component cc, row ii): the forward sweep
+  // broadcasts y_j = s_j / R_jj from lane (cc, j) and rows below subtract
+  // R_ji y_j (ascending j, as solve_packed); the backward sweep runs the
+  // columns from the last, rows above subtracting R_ij x_j
+  const int cc = lane / NP, ii = lane % NP;
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const int rj = pk_off(NP, j);
+    const double yj = __shfl(w, row ? cc * NP + j : j, 64) / sR[rj];
+    if (row && ii == j)
+      w = yj;
+    else if (row && ii > j)
+      w = w - sR[rj + (ii - j)] * yj;
+  }
+  const int ri = pk_off(NP, row ? ii : 0);
+#pragma unroll
+  for (int j = NP - 1; j >= 0; --j) {
+    const double xj = __shfl(w, row ? cc * NP + j : j, 64) / sR[pk_off(NP, j)];
+    if (row && ii == j)
+      w = xj;
+    else if (row && ii < j)
+      w = w - sR[ri + (j - ii)] * xj;
+  }
   if (row) {
     if (MODE == 0)
-      out[o + lane] = c1 * sw[lane] + 2.0 * u[o + lane] - wa[o + lane];
+      out[o + lane] = c1 * w + 2.0 * u[o + lane] - wa[o + lane];
     else
-      out[o + lane] = c1 * sw[lane] + u[o + lane] + c2 * wa[o + lane];
+      out[o + lane] = c1 * w + u[o + lane] + c2 * wa[o + lane];
   }
 }
 
