@@ -706,6 +706,37 @@ __global__ void k_rhs_points(MeshDev m, int n_tet, double* __restrict__ vpts,
 
 // Forward/backward substitution with the packed factor in the reference's
 // order (solve_LU with L = R^T, src/vet_mat_operations.f90:123-153); x in place.
+// M^-1 w = R^-1 R^-T w across one wave, lane = (component cc, row ii) of a
+// 3Np <= 64 vector held one entry per lane (lanes >= 3Np pass through): the
+// forward sweep broadcasts y_j = s_j / R_jj from lane (cc, j) and the rows
+// below subtract R_ji y_j (ascending j, as solve_packed); the backward sweep
+// runs the columns from the last, the rows above subtracting R_ij x_j.  NP
+// dependent steps per sweep instead of Np(Np+1)/2 on one lane.
+template <int NP>
+__device__ __forceinline__ double solve_wave(const double* __restrict__ R, double w, int lane) {
+  const bool row = lane < 3 * NP;
+  const int cc = lane / NP, ii = lane % NP;
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const int rj = pk_off(NP, j);
+    const double yj = __shfl(w, row ? cc * NP + j : j, 64) / R[rj];
+    if (row && ii == j)
+      w = yj;
+    else if (row && ii > j)
+      w = w - R[rj + (ii - j)] * yj;
+  }
+  const int ri = pk_off(NP, row ? ii : 0);
+#pragma unroll
+  for (int j = NP - 1; j >= 0; --j) {
+    const double xj = __shfl(w, row ? cc * NP + j : j, 64) / R[pk_off(NP, j)];
+    if (row && ii == j)
+      w = xj;
+    else if (row && ii < j)
+      w = w - R[ri + (j - ii)] * xj;
+  }
+  return w;
+}
+
 template <int NP>
 __device__ __forceinline__ void solve_packed(const double* __restrict__ R, double* x) {
   // loops kept rolled: fully unrolled, the compiler hoists all Np(Np+1)/2
@@ -951,6 +982,22 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog(const double* __restrict_
   // time factor: one scalar, or per source group (ensembles whose instances
   // carry their own wavelet; the table is filled for this step)
   const double gp = gsel ? gtab[gsel[Pp]] : g;
+  if constexpr (P == 3) {
+    // one row per lane: wave 0 solves across its lanes (p = 3 only: at p <= 2
+    // the packed variants keep solve_packed, and every variant of a degree
+    // must give the same bits)
+    __syncthreads();  // sR
+    if (wv != 0) return;  // no barrier follows
+    double w = lane < N3 ? -acc[0] + rs[o + lane] + rd[o + lane] * gp : 0.0;
+    w = solve_wave<NP>(sR, w, lane);
+    if (lane < N3) {
+      if (MODE == 0)
+        out[o + lane] = c1 * w + 2.0 * u[o + lane] - wa[o + lane];
+      else
+        out[o + lane] = c1 * w + u[o + lane] + c2 * wa[o + lane];
+    }
+    return;
+  }
 #pragma unroll
   for (int k = 0; k < RPL; ++k) {
     const int r = lane + 64 * k;
@@ -1095,29 +1142,7 @@ __global__ __launch_bounds__(64) void k_sym_update(const double* __restrict__ pr
   const double gp = gsel ? gtab[gsel[Pp]] : g;
   double w = row ? -acc + rs[o + lane] + rd[o + lane] * gp : 0.0;
   __syncthreads();  // sR
-  // M^-1 w = R^-1 R^-T w with lane (component cc, row ii): the forward sweep
-  // broadcasts y_j = s_j / R_jj from lane (cc, j) and rows below subtract
-  // R_ji y_j (ascending j, as solve_packed); the backward sweep runs the
-  // columns from the last, rows above subtracting R_ij x_j
-  const int cc = lane / NP, ii = lane % NP;
-#pragma unroll
-  for (int j = 0; j < NP; ++j) {
-    const int rj = pk_off(NP, j);
-    const double yj = __shfl(w, row ? cc * NP + j : j, 64) / sR[rj];
-    if (row && ii == j)
-      w = yj;
-    else if (row && ii > j)
-      w = w - sR[rj + (ii - j)] * yj;
-  }
-  const int ri = pk_off(NP, row ? ii : 0);
-#pragma unroll
-  for (int j = NP - 1; j >= 0; --j) {
-    const double xj = __shfl(w, row ? cc * NP + j : j, 64) / sR[pk_off(NP, j)];
-    if (row && ii == j)
-      w = xj;
-    else if (row && ii < j)
-      w = w - sR[ri + (j - ii)] * xj;
-  }
+  w = solve_wave<NP>(sR, w, lane);
   if (row) {
     if (MODE == 0)
       out[o + lane] = c1 * w + 2.0 * u[o + lane] - wa[o + lane];
