@@ -1030,8 +1030,8 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog(const double* __restrict_
 // K_PQ[i][c] u_P[i] go to LDS in three chunks of CW columns, lane (c, t) of a
 // chunk sums rows [t CW, (t+1) CW) of column c, and lane c gathers the three
 // partial sums.  k_sym_update: per polyhedron, the row's products summed in
-// block order, then the solves and update of k_leapfrog.  p <= 3 (3Np <= 64
-// rows, whole column pairs).
+// block order, then the mass solves across the wave (solve_wave) and the
+// update of k_leapfrog.  p <= 3 (3Np <= 64 rows, whole column pairs).
 template <int P>
 __global__ __launch_bounds__(64) void k_sym_products(const double* __restrict__ Ks,
                                                      const int* __restrict__ row_ptr,
