@@ -432,12 +432,15 @@ def main():
         # vectors, three int32 indices per block
         Bs = 8 * (n_st * n3 * n3 + 2 * ctx.n_blk * n3 + ctx.n_poly * npk + 5 * ctx.n_poly * n3) \
             + 12 * ctx.n_blk
+        traffic_s, traffic_s_src = pmc_traffic(a.n, p, "k_sym")
         res["fp64_pairs_once"] = {
             "value": dofs / (mss * 1e-3), "unit": "DOF-updates/s", "ms_per_step": mss,
             "speedup_vs_full_rows": ms_step / mss, "stored_blocks": n_st, "max_pair_asymmetry": asym,
             "roofline": {"bound": "hbm", "achieved": Bs / (mss * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": Bs / (mss * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                         "bytes_per_step": Bs, "kernel": f"k_sym_products<{p}> + k_sym_update<{p},0>"},
+                         "bytes_per_step": Bs, "kernel": f"k_sym_products<{p}> + k_sym_update<{p},0>",
+                         "traffic": traffic_s, "traffic_source": traffic_s_src,
+                         "traffic_over_algorithmic": (traffic_s / Bs) if traffic_s else None},
             "rel_diff_vs_full_rows": float(np.abs(us - u).max() / np.abs(u).max()),
             "err_l2": float(np.sqrt(ctx.compute_error_l2_matrix_free(us, uex))),
         }
