@@ -1133,21 +1133,35 @@ __global__ __launch_bounds__(64) void k_sym_update(const double* __restrict__ pr
   if (Pp >= n_poly) return;
   for (int k = lane; k < D::NPK; k += 64) sR[k] = Rp[(size_t)Pp * D::NPK + k];
   const int b0 = row_ptr[Pp], b1 = row_ptr[Pp + 1];
+  const size_t o = (size_t)Pp * N3;
+  double vs = 0.0, vd = 0.0, vu = 0.0, va = 0.0;
+  if (row) {
+    vs = rs[o + lane];
+    vd = rd[o + lane];
+    vu = u[o + lane];
+    va = wa[o + lane];
+  }
+  // the row's products, eight loads in flight at a time, added in block order
   double acc = 0.0;
   if (row) {
-    acc = prod[(size_t)b0 * N3 + lane];
-    for (int b = b0 + 1; b < b1; ++b) acc = acc + prod[(size_t)b * N3 + lane];
+    for (int b = b0; b < b1; b += 8) {
+      double v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v[k] = b + k < b1 ? prod[(size_t)(b + k) * N3 + lane] : 0.0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (b + k < b1) acc = (b + k == b0) ? v[k] : acc + v[k];
+    }
   }
-  const size_t o = (size_t)Pp * N3;
   const double gp = gsel ? gtab[gsel[Pp]] : g;
-  double w = row ? -acc + rs[o + lane] + rd[o + lane] * gp : 0.0;
+  double w = row ? -acc + vs + vd * gp : 0.0;
   __syncthreads();  // sR
   w = solve_wave<NP>(sR, w, lane);
   if (row) {
     if (MODE == 0)
-      out[o + lane] = c1 * w + 2.0 * u[o + lane] - wa[o + lane];
+      out[o + lane] = c1 * w + 2.0 * vu - va;
     else
-      out[o + lane] = c1 * w + u[o + lane] + c2 * wa[o + lane];
+      out[o + lane] = c1 * w + vu + c2 * va;
   }
 }
 
