@@ -272,8 +272,9 @@ def uneven_leg(n=30, p=4, seed=3, steps=200, run_steps=2000):
 
 def degree_sweep(steps=200):
     """Step throughput at p = 1..4 on agglomerated cubes of about 4 M DOFs
-    (70^3, 52^3, 40^3, 33^3 cubes), FP64 and 32-bit apply, each autotuned;
-    and the time of both operator builds (keep_adg off, K and M only)."""
+    (70^3, 52^3, 40^3, 33^3 cubes), FP64 and 32-bit apply, each autotuned,
+    FP64 with face pairs stored once at p <= 3; and the time of both operator
+    builds (keep_adg off, K and M only)."""
     import lymph3d_apsc_amd as L
     from lymph3d_apsc_amd import _lib, mesh as M
     rows = []
@@ -302,6 +303,18 @@ def degree_sweep(steps=200):
                 row[f"fp{bits}"] = dict(ms_per_step=ms, dof_updates_per_s=row["dofs"] / (ms * 1e-3),
                                         variant=list(chosen), GBps=B / ms / 1e6,
                                         frac=B / ms / 1e6 / HBM_PEAK_GBS)
+            if p <= 3:  # FP64 with face pairs stored once (p <= 3)
+                ctx.set_apply_precision(64)
+                ctx.set_symmetric_storage(True)
+                ms = ctx.time_steps(L.PROB_CUBE_WAVE, steps, dt, dt) / steps
+                n3, npk = ctx.n3, ctx.Np * (ctx.Np + 1) // 2
+                n_st = ctx.n_poly + (ctx.n_blk - ctx.n_poly) // 2
+                B = 8 * (n_st * n3 * n3 + 2 * ctx.n_blk * n3 + ctx.n_poly * npk + 5 * ctx.n_poly * n3) \
+                    + 12 * ctx.n_blk
+                row["fp64_pairs_once"] = dict(ms_per_step=ms, dof_updates_per_s=row["dofs"] / (ms * 1e-3),
+                                              speedup_vs_full_rows=row["fp64"]["ms_per_step"] / ms,
+                                              GBps=B / ms / 1e6, frac=B / ms / 1e6 / HBM_PEAK_GBS)
+                ctx.set_symmetric_storage(False)
             log(json.dumps(row))
             rows.append(row)
     return rows
