@@ -317,7 +317,10 @@ int lymph_get_traces(LymphCtx* ctx, double* out, int* n_records);
  * at capture time.  No reference counterpart (ensembles are new); the
  * captured work is the same kernels the uncaptured calls launch. */
 int lymph_capture_solve(LymphCtx* ctx, int problem, double dt, int nsteps, int every);
-/* Launch the captured solve (asynchronous; traces overwritten in place). */
+/* Launch the captured solve (asynchronous; traces overwritten in place).
+ * Changing the apply precision, the face-pair storage or releasing the FP64
+ * blocks drops the captured solve; lymph_replay then returns
+ * LYMPH_ERR_STATE until lymph_capture_solve runs again. */
 int lymph_replay(LymphCtx* ctx);
 /* Wait for all work of the context's stream. */
 int lymph_sync(LymphCtx* ctx);

@@ -396,10 +396,21 @@ int tilde(LymphCtx* c, int mode, const double* in, double* out, const int* list 
 // few ulps where the bilinear form is symmetric.
 constexpr double kSymTol = 1e-12;
 
+// A captured solve holds the step's buffers by address: drop it whenever they
+// change, so that lymph_replay reports "no captured solve" instead of reading
+// freed memory.
+void drop_graph(LymphCtx* c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+}
+
 // Drop the once-stored face pairs (operator rebuilt, storage switched off).
 int sym_off(LymphCtx* c) {
   if (!c->Ks && !c->sidx && !c->tdst && !c->prod) return 0;
   HIP_TRY(hipStreamSynchronize(c->stream));
+  drop_graph(c);
   if (c->Ks) HIP_TRY(hipFree(c->Ks));
   if (c->sidx) HIP_TRY(hipFree(c->sidx));
   if (c->tdst) HIP_TRY(hipFree(c->tdst));
@@ -418,6 +429,7 @@ int sym_off(LymphCtx* c) {
 // neighbours Q > P into Ks and index the full rows into it.
 int sym_on(LymphCtx* c, double tol, double* asym) {
   const int np = c->n_poly;
+  drop_graph(c);  // a captured solve would keep stepping the full rows
   const std::vector<int>& rp = c->h_row_ptr;
   const std::vector<int>& cl = c->h_col;
   std::vector<int> partner(c->n_blk, -1), sid(c->n_blk, -1), src, pb, pt;
@@ -950,6 +962,7 @@ int lymph_release_fp64_operator(LymphCtx* c) {
     return fail(LYMPH_ERR_STATE, "select the 32-bit apply (lymph_set_apply_precision) first");
   if (c->K) {
     HIP_TRY(hipStreamSynchronize(c->stream));
+    drop_graph(c);
     HIP_TRY(hipFree(c->K));
     c->K = nullptr;
     c->bytes -= (size_t)c->n_blk * blk_sz(c) * sizeof(double);
@@ -1383,6 +1396,7 @@ int lymph_set_apply_precision(LymphCtx* c, int bits) {
   if (bits != 32 && bits != 64) return fail(LYMPH_ERR_ARG, "precision must be 32 or 64");
   if (bits == 64 && !c->K)
     return fail(LYMPH_ERR_STATE, "FP64 K blocks were released: rebuild the operator");
+  if (bits != c->apply_bits) drop_graph(c);  // a captured solve steps the other apply
   if (bits == 32 && !c->K32) {
     // vectors first: K32 exists only once everything the 32-bit path uses does
     for (double*& q : c->tv) {
@@ -1559,10 +1573,7 @@ int lymph_capture_solve(LymphCtx* c, int problem, double dt, int nsteps, int eve
   if (c->n_recv > 0 && nsteps / every > c->max_rec)
     return fail(LYMPH_ERR_ARG, "receiver trace buffer too small for the captured solve");
   if (!c->own_stream) return fail(LYMPH_ERR_STATE, "capture needs the context's own stream");
-  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-  if (c->graph) (void)hipGraphDestroy(c->graph);
-  c->graph_exec = nullptr;
-  c->graph = nullptr;
+  drop_graph(c);
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->n_rec = 0;
   HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));

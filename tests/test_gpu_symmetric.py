@@ -132,3 +132,27 @@ def test_rebuild_drops_the_stored_pairs():
         assert ctx.device_bytes > base
         ctx.make_matrices_free()
         assert ctx.device_bytes == base
+
+
+def test_storage_change_drops_the_captured_solve():
+    """A solve captured with the pairs stored once holds their buffers by
+    address: switching the storage off drops it (replay refused), and a new
+    capture replays the full rows."""
+    say("[symmetric] captured solve")
+    m = M.cube_mesh(2, agglomerate=True)
+    with L.Context(m, 2) as ctx:
+        _prepared(ctx)
+        ctx.set_symmetric_storage(True)
+        ctx.capture_solve(L.PROB_REF_MMS, 1e-3, 20)
+        ctx.replay()
+        ctx.sync()
+        a = ctx.get_slot(_lib.SLOT_U_CUR)
+        ctx.set_symmetric_storage(False)
+        with pytest.raises(L.LymphError) as e:
+            ctx.replay()
+        assert e.value.code == _lib.ERR_STATE
+        ctx.capture_solve(L.PROB_REF_MMS, 1e-3, 20)
+        ctx.replay()
+        ctx.sync()
+        b = ctx.get_slot(_lib.SLOT_U_CUR)
+        assert rel(a, b) < 1e-12, rel(a, b)
