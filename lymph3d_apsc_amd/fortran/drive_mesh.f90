@@ -5,7 +5,7 @@
 ! by lymph3d_apsc_amd.mesh.Mesh.write_flat (it plays the part of the
 ! reference's Mesh_Structure after MAKE_PARTITION_AND_MPI_FILES).
 !
-! usage: drive_mesh <mesh.bin> [procedures]
+! usage: drive_mesh <mesh.bin> [procedures | symmetric]
 !   With "procedures" the data of MAKE_RHS_FREE and the fields of
 !   COMPUTE_MODAL_COEFFICIENTS_FREE come from the Fortran functions of module
 !   mms_procedures (the manufactured solution of
@@ -133,11 +133,13 @@ program drive_mesh
   integer(c_int) :: rc, p
   integer :: nn, nt, npoly, np3, nsteps
   character(len=1024) :: path, mode
-  logical :: procs
+  logical :: procs, sym
+  real(c_double) :: asym
 
   call get_command_argument(1, path)
   call get_command_argument(2, mode)
   procs = trim(mode) == 'procedures'
+  sym = trim(mode) == 'symmetric'
   open(10, file=trim(path), access='stream', form='unformatted', status='old')
   read(10) hdr
   read(10) dt, stop_time
@@ -165,6 +167,11 @@ program drive_mesh
 
   ! MAKE_MATRICES_FREE + cholesky (src/Lymph3D.f90:262-271)
   call ok(lymph_build_operator(ctx), 'lymph_build_operator')
+  ! optionally keep each face pair's block once (SIPG, one material per face)
+  if (sym) then
+    call ok(lymph_set_symmetric_storage(ctx, 1, asym), 'lymph_set_symmetric_storage')
+    print '(A,ES10.3)', ' face pairs stored once, largest pair asymmetry ', asym
+  end if
   ! MAKE_RHS_FREE static (null data) and dynamic (f_time, gd, gn) (:284-286)
   ! COMPUTE_MODAL_COEFFICIENTS_FREE of the initial conditions (:328-329)
   if (procs) then
