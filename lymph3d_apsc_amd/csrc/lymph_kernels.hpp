@@ -1032,6 +1032,15 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog(const double* __restrict_
 // partial sums.  k_sym_update: per polyhedron, the row's products summed in
 // block order, then the mass solves across the wave (solve_wave) and the
 // update of k_leapfrog.  p <= 3 (3Np <= 64 rows, whole column pairs).
+// LDS ordering within one wave (the rocPRIM wave barrier: fences at
+// wavefront scope around a code-motion barrier); enough for a workgroup of
+// one wave, and it does not wait on the wave's global loads and stores.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int P>
 __global__ __launch_bounds__(64) void k_sym_products(const double* __restrict__ Ks,
                                                      const int* __restrict__ row_ptr,
@@ -1085,7 +1094,7 @@ __global__ __launch_bounds__(64) void k_sym_products(const double* __restrict__ 
     double st = 0.0;
 #pragma unroll
     for (int h = 0; h < 3; ++h) {
-      __syncthreads();  // the previous chunk's reads are done
+      wave_lds_sync();  // the previous chunk's reads are done
       if (row) {
 #pragma unroll
         for (int k = 0; k < CP; ++k) {
@@ -1093,7 +1102,7 @@ __global__ __launch_bounds__(64) void k_sym_products(const double* __restrict__ 
           sT[(2 * k + 1) * LDT + lane] = kv[h * CP + k].y * up;
         }
       }
-      __syncthreads();
+      wave_lds_sync();
       double v = 0.0;
       if (row) {
         v = sT[cl * LDT + t * CW];
