@@ -33,6 +33,8 @@ def _gpu():
 def _mesh(name):
     if name.startswith("tet_"):
         return M.read_deck(os.path.join(DECKS, name))
+    if name == "grown":  # polyhedra of 4-40 tets: uneven rows, so the step runs in a dispatch order
+        return M.cube_mesh(4, dirichlet_faces={"x0", "x1", "y0", "y1", "z0"}, poly_size=(4, 40), seed=3)
     return M.cube_mesh(int(name[4]), agglomerate=True)
 
 
@@ -42,7 +44,8 @@ def _prepared(ctx):
     ctx.make_rhs_free(L.PROB_REF_MMS, _lib.SLOT_RHS_DYN)
 
 
-@pytest.mark.parametrize("name,p", [("cube3agg", 3), ("tet_25", 2), ("cube2agg", 1), ("cube4agg", 3)])
+@pytest.mark.parametrize("name,p", [("cube3agg", 3), ("tet_25", 2), ("cube2agg", 1), ("cube4agg", 3),
+                                    ("grown", 3), ("grown", 2)])
 def test_steps_match_full_rows_and_oracle(name, p):
     say(f"[symmetric] steps {name} p={p}")
     m = _mesh(name)
