@@ -88,6 +88,9 @@ struct LymphCtx {
   double* Ks = nullptr;     // face pairs stored once (lymph_set_symmetric_storage)
   int* sidx = nullptr;      // per block of the full rows: stored block, ~ = transposed
   int* tdst = nullptr;      // stored pair block -> its (Q, P) slot of Q's row, else -1
+  int* ssrc = nullptr;      // full-row slot of each stored block (gather list)
+  float* Ks32 = nullptr;    // the stored blocks of the 32-bit operator (p = 3)
+  size_t ks32_bytes = 0;
   double* prod = nullptr;   // per-block products of one step (n_blk x 3Np)
   size_t ks_bytes = 0;
   int* blk_row = nullptr;   // row polyhedron of each block
@@ -338,7 +341,21 @@ int launch_step(LymphCtx* c, int mode, double g, double c1, double c2, const dou
     // the 32-bit apply works on mass-orthonormal vectors: ~F from tilde_rhs
     const double* rs = c->apply_bits == 32 ? c->tv[3] : c->slot[LYMPH_SLOT_RHS_STAT];
     const double* rd = c->apply_bits == 32 ? c->tv[4] : c->slot[LYMPH_SLOT_RHS_DYN];
-    if (c->apply_bits == 32) {
+    if (c->apply_bits == 32 && c->Ks32) {
+      if constexpr (3 * np_of(P) % 12 == 0 && 3 * np_of(P) <= 64) {
+        k_sym_products32<P><<<step_grid(c->n_poly), 64, 0, c->stream>>>(
+            c->Ks32, c->md.row_ptr, c->md.col, c->sidx, c->tdst, u, c->prod, c->n_poly, c->order);
+        if (int rc = check_launch("k_sym_products32")) return rc;
+        if (mode == 0)
+          k_sym_update32<P, 0><<<step_grid(c->n_poly), 64, 0, c->stream>>>(
+              c->prod, c->md.row_ptr, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->n_poly, c->order);
+        else
+          k_sym_update32<P, 1><<<step_grid(c->n_poly), 64, 0, c->stream>>>(
+              c->prod, c->md.row_ptr, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->n_poly, c->order);
+      } else {
+        return fail(LYMPH_ERR_STATE, "32-bit stored pairs need 3Np a multiple of 12");
+      }
+    } else if (c->apply_bits == 32) {
       if (mode == 0)
         launch_leapfrog32<P, 0>(c->step_variant, c->n_poly, c->stream, c->K32, c->md.row_ptr,
                                 c->md.col, c->RM, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->order, dyn);
@@ -408,9 +425,15 @@ void drop_graph(LymphCtx* c) {
 
 // Drop the once-stored face pairs (operator rebuilt, storage switched off).
 int sym_off(LymphCtx* c) {
-  if (!c->Ks && !c->sidx && !c->tdst && !c->prod) return 0;
+  if (!c->Ks && !c->sidx && !c->tdst && !c->prod && !c->ssrc && !c->Ks32) return 0;
   HIP_TRY(hipStreamSynchronize(c->stream));
   drop_graph(c);
+  if (c->ssrc) HIP_TRY(hipFree(c->ssrc));
+  if (c->Ks32) HIP_TRY(hipFree(c->Ks32));
+  c->ssrc = nullptr;
+  c->Ks32 = nullptr;
+  c->bytes -= c->ks32_bytes;
+  c->ks32_bytes = 0;
   if (c->Ks) HIP_TRY(hipFree(c->Ks));
   if (c->sidx) HIP_TRY(hipFree(c->sidx));
   if (c->tdst) HIP_TRY(hipFree(c->tdst));
@@ -423,6 +446,8 @@ int sym_off(LymphCtx* c) {
   c->ks_bytes = 0;
   return 0;
 }
+
+int sym32_build(LymphCtx* c);
 
 // Store every face pair once: check K_QP = K_PQ^T pair by pair on the device
 // (largest ratio to *asym), then gather the self blocks and the blocks of
@@ -510,19 +535,35 @@ int sym_on(LymphCtx* c, double tol, double* asym) {
               hipMemcpy(dsrc, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess))
     rc = fail(LYMPH_ERR_HIP, "copy of the block index failed");
   if (!rc) {
-    k_gather_blocks<<<4096, 256, 0, c->stream>>>(c->K, c->Ks, dsrc, n_s, bs);
+    k_gather_blocks<double><<<4096, 256, 0, c->stream>>>(c->K, c->Ks, dsrc, n_s, bs);
     rc = check_launch("k_gather_blocks");
   }
   if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(LYMPH_ERR_HIP, "gather failed");
-  if (dsrc) (void)hipFree(dsrc);
+  c->ssrc = dsrc;  // kept for the 32-bit blocks (freed with the stored pairs)
   if (rc) {
     c->ks_bytes = 0;
     (void)sym_off(c);
     return rc;
   }
-  c->ks_bytes = (size_t)n_s * bs * sizeof(double) + 2 * (size_t)c->n_blk * sizeof(int) +
+  c->ks_bytes = (size_t)n_s * bs * sizeof(double) + (2 * (size_t)c->n_blk + n_s) * sizeof(int) +
                 (size_t)c->n_blk * c->n3 * sizeof(double);
   c->bytes += c->ks_bytes;
+  return sym32_build(c);  // the 32-bit operator's copy, if it exists already
+}
+
+// The stored pairs of the 32-bit operator, once both it and the stored list
+// exist (p = 3: whole quads per chunk); otherwise the 32-bit apply keeps its
+// full rows.
+int sym32_build(LymphCtx* c) {
+  if (!c->ssrc || !c->K32 || c->Ks32 || c->p != 3) return 0;
+  const size_t bs = (((size_t)c->n3 * c->n3 + 3) / 4) * 4;
+  const int n_s = c->n_poly + (c->n_blk - c->n_poly) / 2;
+  if (int rc = dalloc(&c->Ks32, (size_t)n_s * bs)) return rc;
+  k_gather_blocks<float><<<4096, 256, 0, c->stream>>>(c->K32, c->Ks32, c->ssrc, n_s, bs);
+  if (int rc = check_launch("k_gather_blocks")) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->ks32_bytes = (size_t)n_s * bs * sizeof(float);
+  c->bytes += c->ks32_bytes;
   return 0;
 }
 
@@ -643,6 +684,8 @@ int lymph_ctx_destroy(LymphCtx* c) {
   if (c->sidx) (void)hipFree(c->sidx);
   if (c->tdst) (void)hipFree(c->tdst);
   if (c->prod) (void)hipFree(c->prod);
+  if (c->ssrc) (void)hipFree(c->ssrc);
+  if (c->Ks32) (void)hipFree(c->Ks32);
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   for (void* p : c->owned) (void)hipFree(p);
@@ -1417,6 +1460,8 @@ int lymph_set_apply_precision(LymphCtx* c, int bits) {
       return r;
     }
   }
+  if (bits == 32)
+    if (int rc = sym32_build(c)) return rc;  // pairs stored once: the 32-bit copy too
   c->apply_bits = bits;
   c->step_variant = 0;
   return 0;

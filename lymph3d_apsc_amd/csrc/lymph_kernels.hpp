@@ -1174,6 +1174,131 @@ __global__ __launch_bounds__(64) void k_sym_update(const double* __restrict__ pr
   }
 }
 
+// The pairs-once step of the 32-bit apply (mass-orthonormal basis, no
+// solves): ~K_QP = R_Q^-T K_PQ^T R_P^-1 = ~K_PQ^T, so the same stored list
+// applies to the FP32 blocks (column quads, 3Np a multiple of 12 so that each
+// of the three chunks is whole quads: p = 3).  Products and per-block sums in
+// FP32 as k_leapfrog32, written as doubles; k_sym_update32 adds them in block
+// order in FP64 and applies k_leapfrog32's update.
+template <int P>
+__global__ __launch_bounds__(64) void k_sym_products32(const float* __restrict__ Ks,
+                                                       const int* __restrict__ row_ptr,
+                                                       const int* __restrict__ col,
+                                                       const int* __restrict__ sidx,
+                                                       const int* __restrict__ tdst,
+                                                       const double* __restrict__ u,
+                                                       double* __restrict__ prod, int n_poly,
+                                                       const int* __restrict__ order) {
+  constexpr int N3 = Dims32<P>::N3, NQ = Dims32<P>::NQ;
+  static_assert(N3 <= 64 && N3 % 12 == 0, "one lane per row, whole quads per chunk");
+  constexpr int CW = N3 / 3, CQ = CW / 4;
+  constexpr int LDT = N3 + 1;
+  __shared__ float sT[CW * LDT];
+  const int lane = threadIdx.x;
+  const bool row = lane < N3;
+  const int Pp = step_poly(order);
+  if (Pp >= n_poly) return;  // padding workgroups of the last XCD share
+  const int b0 = row_ptr[Pp], b1 = row_ptr[Pp + 1];
+  const int lcb = lane < b1 - b0 ? col[b0 + lane] : 0;
+  const int lsi = lane < b1 - b0 ? sidx[b0 + lane] : 0;
+  const int ltd = lane < b1 - b0 ? tdst[b0 + lane] : 0;
+  const float up = row ? (float)u[(size_t)Pp * N3 + lane] : 0.0f;
+  const int cl = lane % CW, t = lane / CW;
+  for (int b = b0; b < b1; ++b) {
+    const int j = b - b0;
+    const int si = j < 64 ? __builtin_amdgcn_readlane(lsi, j) : __builtin_amdgcn_readfirstlane(sidx[b]);
+    if (si < 0) continue;  // written by the neighbour's wave
+    const int cb = j < 64 ? __builtin_amdgcn_readlane(lcb, j) : __builtin_amdgcn_readfirstlane(col[b]);
+    const int td = j < 64 ? __builtin_amdgcn_readlane(ltd, j) : __builtin_amdgcn_readfirstlane(tdst[b]);
+    const float* __restrict__ Kb = Ks + (size_t)si * Dims32<P>::BS;
+    const float xl = row ? (float)u[(size_t)cb * N3 + lane] : 0.0f;
+    float4 kq[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+      kq[q] = row ? load_stream(Kb + ((size_t)q * N3 + lane) * 4) : make_float4(0.f, 0.f, 0.f, 0.f);
+    float s = 0.0f;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      float xq[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        xq[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xl), 4 * q + k));
+      s = s + kq[q].x * xq[0];
+      s = s + kq[q].y * xq[1];
+      s = s + kq[q].z * xq[2];
+      s = s + kq[q].w * xq[3];
+    }
+    if (row) prod[(size_t)b * N3 + lane] = (double)s;
+    if (cb == Pp) continue;  // self block: no partner
+    float st = 0.0f;
+#pragma unroll
+    for (int h = 0; h < 3; ++h) {
+      wave_lds_sync();  // the previous chunk's reads are done
+      if (row) {
+#pragma unroll
+        for (int k = 0; k < CQ; ++k) {
+          const float4 v = kq[h * CQ + k];
+          sT[(4 * k) * LDT + lane] = v.x * up;
+          sT[(4 * k + 1) * LDT + lane] = v.y * up;
+          sT[(4 * k + 2) * LDT + lane] = v.z * up;
+          sT[(4 * k + 3) * LDT + lane] = v.w * up;
+        }
+      }
+      wave_lds_sync();
+      float v = 0.0f;
+      if (row) {
+        v = sT[cl * LDT + t * CW];
+#pragma unroll
+        for (int k = 1; k < CW; ++k) v = v + sT[cl * LDT + t * CW + k];
+      }
+      const int src = (lane - h * CW) & 63;  // lanes outside the chunk discard
+      const float v0 = __shfl(v, src % CW, 64);
+      const float v1 = __shfl(v, src % CW + CW, 64);
+      const float v2 = __shfl(v, src % CW + 2 * CW, 64);
+      if (lane >= h * CW && lane < (h + 1) * CW) st = (v0 + v1) + v2;
+    }
+    if (row) prod[(size_t)td * N3 + lane] = (double)st;
+  }
+}
+
+template <int P, int MODE>
+__global__ __launch_bounds__(64) void k_sym_update32(const double* __restrict__ prod,
+                                                     const int* __restrict__ row_ptr,
+                                                     const double* __restrict__ rs,
+                                                     const double* __restrict__ rd,
+                                                     const double* __restrict__ u,
+                                                     const double* __restrict__ wa,
+                                                     double* __restrict__ out, double g, double c1,
+                                                     double c2, const int* __restrict__ gsel,
+                                                     const double* __restrict__ gtab, int n_poly,
+                                                     const int* __restrict__ order) {
+  constexpr int N3 = Dims32<P>::N3;
+  static_assert(N3 <= 64, "one lane per row");
+  static_assert(MODE == 0 || MODE == 1, "leapfrog and first step");
+  const int lane = threadIdx.x;
+  if (lane >= N3) return;  // no barrier in this kernel
+  const int Pp = step_poly(order);
+  if (Pp >= n_poly) return;
+  const int b0 = row_ptr[Pp], b1 = row_ptr[Pp + 1];
+  const size_t o = (size_t)Pp * N3;
+  const double vs = rs[o + lane], vd = rd[o + lane], vu = u[o + lane], va = wa[o + lane];
+  double acc = 0.0;
+  for (int b = b0; b < b1; b += 8) {
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = b + k < b1 ? prod[(size_t)(b + k) * N3 + lane] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (b + k < b1) acc = (b + k == b0) ? v[k] : acc + v[k];
+  }
+  const double gp = gsel ? gtab[gsel[Pp]] : g;
+  const double w = -acc + vs + vd * gp;
+  if (MODE == 0)
+    out[o + lane] = c1 * w + 2.0 * vu - va;
+  else
+    out[o + lane] = c1 * w + vu + c2 * va;
+}
+
 // Symmetry of the operator, pair by pair: ratio[w] = max |K_PQ - K_QP^T| /
 // max |K_PQ| for the blocks pb[w] = K_PQ and pt[w] = K_QP (column-pair
 // layout, whole pairs: 3Np even).
@@ -1206,8 +1331,9 @@ __global__ __launch_bounds__(64) void k_pair_asym(const double* __restrict__ K,
   if (threadIdx.x == 0) ratio[w] = dmax == 0.0 ? 0.0 : dmax / amax;
 }
 
-// Ks[s] = K[src[s]], whole blocks of bs doubles.
-__global__ void k_gather_blocks(const double* __restrict__ K, double* __restrict__ Ks,
+// Ks[s] = K[src[s]], whole blocks of bs entries.
+template <class T>
+__global__ void k_gather_blocks(const T* __restrict__ K, T* __restrict__ Ks,
                                 const int* __restrict__ src, int n, size_t bs) {
   const size_t total = (size_t)n * bs;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;

@@ -159,3 +159,54 @@ def test_storage_change_drops_the_captured_solve():
         ctx.sync()
         b = ctx.get_slot(_lib.SLOT_U_CUR)
         assert rel(a, b) < 1e-12, rel(a, b)
+
+
+@pytest.mark.parametrize("first", ["pairs", "fp32"])
+def test_fp32_apply_with_pairs_stored_once(first):
+    """The 32-bit apply with face pairs stored once (p = 3): whichever option
+    comes first, the step's acceleration term is within 1e-5 of the FP64
+    oracle and of the full-row 32-bit step, and the 32-bit copy of the stored
+    blocks is allocated (and freed with the stored pairs)."""
+    say(f"[symmetric] fp32 apply, {first} first")
+    m, p = M.cube_mesh(3, agglomerate=True), 3
+    o = oracle_for(m, p)
+    o.build_operator()
+    rng = np.random.default_rng(11)
+    n = m.n_poly * 3 * o.Np
+    u0, u1 = rng.standard_normal(n), rng.standard_normal(n)
+    dt, g = 1e-3, 0.37
+    ref = o.step(g, dt, u0, u1, o.rhs(L.PROB_NULL), o.rhs(L.PROB_REF_MMS))
+    acc = lambda a: a - (2 * u1 - u0)  # noqa: E731
+    with L.Context(m, p) as ctx:
+        _prepared(ctx)
+        ctx.set_apply_precision(32)
+        full32 = ctx.time_step_matrix_free(g, dt, u0, u1)
+        bytes32 = ctx.device_bytes
+        ctx.set_apply_precision(64)
+        if first == "pairs":
+            ctx.set_symmetric_storage(True)
+            ctx.set_apply_precision(32)
+        else:
+            ctx.set_apply_precision(32)
+            ctx.set_symmetric_storage(True)
+        rp, _ = ctx.pattern()
+        n_st = m.n_poly + (ctx.n_blk - m.n_poly) // 2
+        assert ctx.device_bytes - bytes32 >= n_st * (3 * o.Np) ** 2 * (8 + 4)
+        sym32 = ctx.time_step_matrix_free(g, dt, u0, u1)
+        assert rel(acc(sym32), acc(ref)) < 1e-5, rel(acc(sym32), acc(ref))
+        assert rel(acc(sym32), acc(full32)) < 1e-5, rel(acc(sym32), acc(full32))
+        # device loop and first step through the same kernels
+        ctx.set_slot(_lib.SLOT_U_PREV, u0)
+        ctx.set_slot(_lib.SLOT_U_CUR, u1)
+        ctx.step_n(L.PROB_REF_MMS, 20, 0.0, 1e-4)
+        a = ctx.get_slot(_lib.SLOT_U_CUR)
+        first_sym = ctx.first_time_step_matrix_free(g, dt, u0, u1)
+        ctx.set_symmetric_storage(False)
+        assert ctx.device_bytes == bytes32
+        ctx.set_slot(_lib.SLOT_U_PREV, u0)
+        ctx.set_slot(_lib.SLOT_U_CUR, u1)
+        ctx.step_n(L.PROB_REF_MMS, 20, 0.0, 1e-4)
+        b = ctx.get_slot(_lib.SLOT_U_CUR)
+        first_full = ctx.first_time_step_matrix_free(g, dt, u0, u1)
+        assert rel(a, b) < 1e-5, rel(a, b)
+        assert rel(first_sym - u0, first_full - u0) < 1e-5
