@@ -271,7 +271,8 @@ int lymph_release_fp64_operator(LymphCtx* ctx);
  * K_QP as the transpose of P's copy.  An operator that fails the check
  * returns LYMPH_ERR_ARG and the full rows stay in use.  Needs p <= 3 and the
  * FP64 blocks (not released); lymph_build_operator drops the stored pairs;
- * on = 0 frees them.  The 32-bit apply and the DG norm are unaffected. */
+ * on = 0 frees them.  At p = 3 the 32-bit apply (lymph_set_apply_precision)
+ * keeps its stored blocks once too; the DG norm is unaffected. */
 int lymph_set_symmetric_storage(LymphCtx* ctx, int on, double* asym);
 
 /* Point forces f_k delta(x - x_k) added to a RHS slot (normally
