@@ -498,13 +498,16 @@ def main():
             # stored FP32 blocks once, per-block products written and read (FP64), five vectors,
             # three int32 indices per block
             B32s = 4 * n_st * n3 * n3 + 8 * (2 * ctx.n_blk * n3 + 5 * ctx.n_poly * n3) + 12 * ctx.n_blk
+            traffic32s, traffic32s_src = pmc_traffic(a.n, p, "k_sym32")
             res["fp32_apply"]["pairs_once"] = {
                 "value": dofs / (ms32s * 1e-3), "unit": "DOF-updates/s", "ms_per_step": ms32s,
                 "speedup_vs_full_rows": ms32_step / ms32s,
                 "roofline": {"bound": "hbm", "achieved": B32s / (ms32s * 1e-3) / 1e9,
                              "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": B32s / (ms32s * 1e-3) / 1e9 / HBM_PEAK_GBS, "bytes_per_step": B32s,
-                             "kernel": f"k_sym_products32<{p}> + k_sym_update32<{p},0>"},
+                             "kernel": f"k_sym_products32<{p}> + k_sym_update32<{p},0>",
+                             "traffic": traffic32s, "traffic_source": traffic32s_src,
+                             "traffic_over_algorithmic": (traffic32s / B32s) if traffic32s else None},
                 "rel_diff_vs_full_rows": float(np.abs(u32s - u32).max() / np.abs(u32).max()),
                 "rel_l2_diff_vs_fp64": float(np.linalg.norm(u32s - u) / np.linalg.norm(u)),
             }
