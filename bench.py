@@ -490,7 +490,7 @@ def main():
             ctx.set_apply_precision(64)
             ctx.run(prob, dt, 1e30, max_steps=1 + a.warmup)
             ctx.set_apply_precision(32)
-            ctx.set_symmetric_storage(True)
+            ctx.set_symmetric_storage(True, fp32=True)
             ms32s = ctx.time_steps(prob, a.steps, t_start, dt) / a.steps
             u32s = ctx.get_slot(_lib.SLOT_U_CUR)
             n3 = 3 * ctx.Np

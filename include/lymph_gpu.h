@@ -258,6 +258,11 @@ int lymph_ctx_set_keep_adg(LymphCtx* ctx, int keep);
 int lymph_ctx_set_build(LymphCtx* ctx, int mode);
 int lymph_release_fp64_operator(LymphCtx* ctx);
 
+/* Modes of lymph_set_symmetric_storage. */
+#define LYMPH_SYM_OFF 0       /* full block rows (default) */
+#define LYMPH_SYM_FP64 1      /* FP64 step reads each face pair once */
+#define LYMPH_SYM_FP64_FP32 2 /* and, at p = 3, so does the 32-bit apply */
+
 /* Face pairs stored once (no reference counterpart; an operator storage
  * choice behind TIME_STEP_MATRIX_FREE, src/matrix_free.f90:1230-1385).  With
  * SIPG (theta = -1) and one material on both traces of every face -- the
@@ -271,8 +276,12 @@ int lymph_release_fp64_operator(LymphCtx* ctx);
  * K_QP as the transpose of P's copy.  An operator that fails the check
  * returns LYMPH_ERR_ARG and the full rows stay in use.  Needs p <= 3 and the
  * FP64 blocks (not released); lymph_build_operator drops the stored pairs;
- * on = 0 frees them.  At p = 3 the 32-bit apply (lymph_set_apply_precision)
- * keeps its stored blocks once too; the DG norm is unaffected. */
+ * on = LYMPH_SYM_OFF frees them.  Only with on = LYMPH_SYM_FP64_FP32 (p = 3)
+ * does the 32-bit apply (lymph_set_apply_precision) keep its blocks once too;
+ * with LYMPH_SYM_FP64 it keeps its full rows.  A failing call leaves no stored
+ * pairs switched on.  While the step in use reads stored pairs,
+ * lymph_autotune returns LYMPH_ERR_STATE (the full-row variants are not run).
+ * The DG norm is unaffected. */
 int lymph_set_symmetric_storage(LymphCtx* ctx, int on, double* asym);
 
 /* Point forces f_k delta(x - x_k) added to a RHS slot (normally
@@ -314,14 +323,18 @@ int lymph_get_traces(LymphCtx* ctx, double* out, int* n_records);
 /* Capture one whole solve from rest into a HIP graph on the context stream:
  * zero initial displacement and velocity (projection kernels), first step at
  * t=0, then `nsteps` steps (recording receivers every `every` steps when
- * receivers are set).  Uses the RHS slots and Ricker/time factors as they are
- * at capture time.  No reference counterpart (ensembles are new); the
+ * receivers are set).  The time factors (single Ricker wavelet or the data
+ * set's g(t)) are those at capture time; the RHS slots are read at replay.  No reference counterpart (ensembles are new); the
  * captured work is the same kernels the uncaptured calls launch. */
 int lymph_capture_solve(LymphCtx* ctx, int problem, double dt, int nsteps, int every);
 /* Launch the captured solve (asynchronous; traces overwritten in place).
- * Changing the apply precision, the face-pair storage or releasing the FP64
- * blocks drops the captured solve; lymph_replay then returns
- * LYMPH_ERR_STATE until lymph_capture_solve runs again. */
+ * Changing the apply precision, the face-pair storage, the receivers, the
+ * source groups or the single Ricker wavelet, or releasing the FP64 blocks,
+ * drops the captured solve; lymph_replay then returns LYMPH_ERR_STATE until
+ * lymph_capture_solve runs again.  The RHS slots are read when the replay
+ * runs, so point sources added or RHS slots rebuilt after the capture take
+ * effect in the next replay.  After a replay the state slots hold the
+ * captured solve's last two states, whatever uncaptured calls ran between. */
 int lymph_replay(LymphCtx* ctx);
 /* Wait for all work of the context's stream. */
 int lymph_sync(LymphCtx* ctx);

@@ -91,6 +91,7 @@ struct LymphCtx {
   int* ssrc = nullptr;      // full-row slot of each stored block (gather list)
   float* Ks32 = nullptr;    // the stored blocks of the 32-bit operator (p = 3)
   size_t ks32_bytes = 0;
+  bool sym32_wanted = false;  // LYMPH_SYM_FP64_FP32: the 32-bit apply stores pairs once too
   double* prod = nullptr;   // per-block products of one step (n_blk x 3Np)
   size_t ks_bytes = 0;
   int* blk_row = nullptr;   // row polyhedron of each block
@@ -99,6 +100,7 @@ struct LymphCtx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int graph_records = 0;
+  double* graph_slot[LYMPH_NUM_SLOTS] = {};  // slot pointers as the captured solve leaves them
   hipStream_t stream = nullptr;
   bool own_stream = false;
   bool built = false;
@@ -548,20 +550,34 @@ int sym_on(LymphCtx* c, double tol, double* asym) {
   c->ks_bytes = (size_t)n_s * bs * sizeof(double) + (2 * (size_t)c->n_blk + n_s) * sizeof(int) +
                 (size_t)c->n_blk * c->n3 * sizeof(double);
   c->bytes += c->ks_bytes;
-  return sym32_build(c);  // the 32-bit operator's copy, if it exists already
+  // the 32-bit operator's copy, if asked for and it exists already; a failure
+  // leaves no stored pairs switched on
+  if (int rc2 = sym32_build(c)) {
+    (void)sym_off(c);
+    return rc2;
+  }
+  return 0;
 }
 
 // The stored pairs of the 32-bit operator, once both it and the stored list
-// exist (p = 3: whole quads per chunk); otherwise the 32-bit apply keeps its
-// full rows.
+// exist and LYMPH_SYM_FP64_FP32 asked for them (p = 3: whole quads per
+// chunk); otherwise the 32-bit apply keeps its full rows.  On failure no
+// partial copy is kept.
 int sym32_build(LymphCtx* c) {
-  if (!c->ssrc || !c->K32 || c->Ks32 || c->p != 3) return 0;
+  if (!c->sym32_wanted || !c->ssrc || !c->K32 || c->Ks32 || c->p != 3) return 0;
   const size_t bs = (((size_t)c->n3 * c->n3 + 3) / 4) * 4;
   const int n_s = c->n_poly + (c->n_blk - c->n_poly) / 2;
-  if (int rc = dalloc(&c->Ks32, (size_t)n_s * bs)) return rc;
-  k_gather_blocks<float><<<4096, 256, 0, c->stream>>>(c->K32, c->Ks32, c->ssrc, n_s, bs);
-  if (int rc = check_launch("k_gather_blocks")) return rc;
-  HIP_TRY(hipStreamSynchronize(c->stream));
+  float* ks = nullptr;
+  if (int rc = dalloc(&ks, (size_t)n_s * bs)) return rc;
+  k_gather_blocks<float><<<4096, 256, 0, c->stream>>>(c->K32, ks, c->ssrc, n_s, bs);
+  int rc = check_launch("k_gather_blocks");
+  if (!rc && hipStreamSynchronize(c->stream) != hipSuccess)
+    rc = fail(LYMPH_ERR_HIP, "gather of the 32-bit stored pairs failed");
+  if (rc) {
+    (void)hipFree(ks);
+    return rc;
+  }
+  c->Ks32 = ks;
   c->ks32_bytes = (size_t)n_s * bs * sizeof(float);
   c->bytes += c->ks32_bytes;
   return 0;
@@ -1016,11 +1032,20 @@ int lymph_release_fp64_operator(LymphCtx* c) {
 int lymph_set_symmetric_storage(LymphCtx* c, int on, double* asym) {
   if (int rc = need_built(c)) return rc;
   if (asym) *asym = 0.0;
-  if (!on) return sym_off(c);
-  if (c->Ks) return 0;
+  if (on != LYMPH_SYM_OFF && on != LYMPH_SYM_FP64 && on != LYMPH_SYM_FP64_FP32)
+    return fail(LYMPH_ERR_ARG, "unknown storage mode");
+  if (on == LYMPH_SYM_OFF) {
+    c->sym32_wanted = false;
+    return sym_off(c);
+  }
+  if (c->Ks && (on == LYMPH_SYM_FP64_FP32) == c->sym32_wanted) return 0;
   if (!c->K) return fail(LYMPH_ERR_STATE, "the FP64 operator was released");
   if (c->p > 3) return fail(LYMPH_ERR_ARG, "symmetric storage needs p <= 3");
-  return sym_on(c, kSymTol, asym);
+  if (int rc = sym_off(c)) return rc;  // switching modes: start from the full rows
+  c->sym32_wanted = on == LYMPH_SYM_FP64_FP32;
+  const int rc = sym_on(c, kSymTol, asym);
+  if (rc) c->sym32_wanted = false;
+  return rc;
 }
 
 int lymph_export_operator(const LymphCtx* cc, double* K, double* A, double* M, double* Mm) {
@@ -1369,6 +1394,10 @@ int lymph_autotune(LymphCtx* c, int reps, int* n_variants, int* unr, int* xmode,
                    double* ms, int* chosen) {
   if (int rc = need_built(c)) return rc;
   if (reps < 1) return fail(LYMPH_ERR_ARG, "reps < 1");
+  // with face pairs stored once the step runs k_sym_* whatever the variant:
+  // timing the variants would only time those kernels again
+  if ((c->apply_bits == 64 && c->Ks) || (c->apply_bits == 32 && c->Ks32))
+    return fail(LYMPH_ERR_STATE, "face pairs are stored once: the full-row variants are not in use");
   return dispatch_p(c->p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
     const bool b32 = c->apply_bits == 32;
@@ -1480,6 +1509,7 @@ int lymph_set_source_groups(LymphCtx* c, int n_groups, const int32_t* poly_group
       if (!(f0[i] > 0)) return fail(LYMPH_ERR_ARG, "f0 must be positive");
   }
   HIP_TRY(hipStreamSynchronize(c->stream));
+  drop_graph(c);  // a captured solve reads these buffers by address
   for (void** q : {(void**)&c->gsel, (void**)&c->gf0, (void**)&c->gt0, (void**)&c->gtab})
     if (*q) {
       (void)hipFree(*q);
@@ -1498,6 +1528,11 @@ int lymph_set_source_groups(LymphCtx* c, int n_groups, const int32_t* poly_group
 int lymph_set_ricker(LymphCtx* c, double f0, double t0) {
   if (int rc = need_ctx(c)) return rc;
   if (!(f0 > 0)) return fail(LYMPH_ERR_ARG, "f0 must be positive");
+  if (f0 != c->ricker_f0 || t0 != c->ricker_t0) {
+    // a captured solve holds the old wavelet's factors as kernel arguments
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    drop_graph(c);
+  }
   c->ricker_f0 = f0;
   c->ricker_t0 = t0;
   return 0;
@@ -1546,6 +1581,7 @@ int lymph_set_receivers(LymphCtx* c, int n, const double* xyz, const int32_t* te
     rp[i] = c->h_tet_poly[tet[i]];
   }
   HIP_TRY(hipStreamSynchronize(c->stream));
+  drop_graph(c);  // a captured solve samples into these buffers by address
   if (c->rpoly) (void)hipFree(c->rpoly);
   if (c->rxyz) (void)hipFree(c->rxyz);
   if (c->rnow) (void)hipFree(c->rnow);
@@ -1654,6 +1690,7 @@ int lymph_capture_solve(LymphCtx* c, int problem, double dt, int nsteps, int eve
   c->graph = g;
   HIP_TRY(hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
   c->graph_records = c->n_rec;
+  for (int k = 0; k < LYMPH_NUM_SLOTS; ++k) c->graph_slot[k] = c->slot[k];
   return 0;
 }
 
@@ -1662,6 +1699,9 @@ int lymph_replay(LymphCtx* c) {
   if (!c->graph_exec) return fail(LYMPH_ERR_STATE, "no captured solve: call lymph_capture_solve");
   HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
   c->n_rec = c->graph_records;
+  // the 64-bit loop rotates the state slots; the replay ends where the
+  // captured solve ended, whatever uncaptured steps rotated since
+  for (int k = 0; k < LYMPH_NUM_SLOTS; ++k) c->slot[k] = c->graph_slot[k];
   return 0;
 }
 

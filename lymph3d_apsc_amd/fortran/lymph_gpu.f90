@@ -17,6 +17,7 @@ module lymph_gpu
   integer(c_int), parameter :: LYMPH_EXIT_NEIGHBOUR = 18
   integer(c_int), parameter :: LYMPH_ERR_ARG = 101, LYMPH_ERR_HIP = 102, LYMPH_ERR_STATE = 103
   integer(c_int), parameter :: LYMPH_LAYOUT_POLY = 0, LYMPH_LAYOUT_FORTRAN = 1
+  integer(c_int), parameter :: LYMPH_SYM_OFF = 0, LYMPH_SYM_FP64 = 1, LYMPH_SYM_FP64_FP32 = 2
   integer(c_int), parameter :: LYMPH_PROB_NULL = 0, LYMPH_PROB_REF_MMS = 1, LYMPH_PROB_CUBE_WAVE = 2
   integer(c_int), parameter :: LYMPH_FN_ZERO = 0, LYMPH_FN_REF_VEL = 1, LYMPH_FN_REF_UEX = 2
   integer(c_int), parameter :: LYMPH_FN_CUBE_U = 3

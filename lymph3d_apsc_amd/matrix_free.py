@@ -239,13 +239,16 @@ class Context:
         """Free the FP64 K blocks once the 32-bit apply is selected."""
         check(lib().lymph_release_fp64_operator(self.h), "lymph_release_fp64_operator")
 
-    def set_symmetric_storage(self, on=True):
+    def set_symmetric_storage(self, on=True, fp32=False):
         """Keep every face pair's block once (K_QP applied as the transpose of
         K_PQ) for the FP64 step; the operator must pass the device's pair
-        check (SIPG, one material per face).  Returns the largest
-        max|K_PQ - K_QP^T| / max|K_PQ| found (0 when switching off)."""
+        check (SIPG, one material per face).  With fp32=True (p = 3) the
+        32-bit apply keeps its blocks once too; otherwise it keeps its full
+        rows.  Returns the largest max|K_PQ - K_QP^T| / max|K_PQ| found (0
+        when switching off)."""
+        mode = 0 if not on else (2 if fp32 else 1)
         asym = C.c_double()
-        check(lib().lymph_set_symmetric_storage(self.h, int(bool(on)), C.byref(asym)),
+        check(lib().lymph_set_symmetric_storage(self.h, mode, C.byref(asym)),
               "lymph_set_symmetric_storage")
         return asym.value
 
@@ -277,7 +280,7 @@ class Context:
         self.ricker = (f0, 1.2 / f0 if t0 is None else t0)
         check(lib().lymph_set_ricker(self.h, *self.ricker), "lymph_set_ricker")
 
-    def set_source_groups(self, poly_group, f0, t0=None):
+    def set_source_groups(self, poly_group, f0=None, t0=None):
         """Per-group Ricker wavelets: polyhedron i uses (f0[g], t0[g]) with
         g = poly_group[i]; t0 defaults to 1.2/f0.  None clears."""
         if poly_group is None:

@@ -718,3 +718,50 @@ def test_ensemble_union_equals_standalone_runs():
     for a, b in zip(tr32, tr1):
         assert rel(a, b) < 1e-4
     ens32.close()
+
+
+def test_captured_solve_dropped_when_its_buffers_change():
+    """A captured solve holds the receivers' and source groups' buffers and
+    the wavelet's factors: replacing any of them drops it, and lymph_replay
+    then returns LYMPH_ERR_STATE instead of touching freed memory.  After an
+    uncaptured step has rotated the state slots, a replay still ends with
+    the captured solve's state in U_CUR."""
+    say("[capture] dropped on buffer changes")
+    m, p = M.cube_mesh(2, agglomerate=True), 2
+    dt, nsteps = 2e-3, 12
+    rec = np.array([[0.3, 0.4, 0.5], [0.6, 0.2, 0.7]])
+    src, force = np.array([[0.45, 0.55, 0.5]]), np.array([[0.0, 0.0, 1.0]])
+
+    def capture(ctx):
+        ctx.set_receivers(rec, max_records=nsteps)
+        ctx.capture_solve(L.PROB_RICKER, dt, nsteps, every=1)
+
+    with L.Context(m, p) as ctx:
+        ctx.start_from_rest(dt, src, force, f0=5.0)
+        capture(ctx)
+        ctx.replay()
+        ctx.sync()
+        u_ref = ctx.get_slot(_lib.SLOT_U_CUR)
+        tr_ref = ctx.traces()
+        assert np.abs(tr_ref).max() > 0
+        ctx.step_n(L.PROB_RICKER, 5, 1.0, dt)  # uncaptured steps rotate the slots
+        ctx.replay()
+        ctx.sync()
+        assert np.array_equal(ctx.get_slot(_lib.SLOT_U_CUR), u_ref)
+        assert np.array_equal(ctx.traces(), tr_ref)
+        changes = [lambda: ctx.set_receivers(rec[:1], max_records=nsteps),
+                   lambda: ctx.set_source_groups(np.zeros(m.n_poly, np.int32), [4.0]),
+                   lambda: ctx.set_source_groups(None),
+                   lambda: ctx.set_ricker(6.0)]
+        for change in changes:
+            capture(ctx)
+            change()
+            with pytest.raises(L.LymphError) as e:
+                ctx.replay()
+            assert e.value.code == _lib.ERR_STATE
+        # setting the same wavelet again keeps the captured solve
+        ctx.set_ricker(5.0)
+        capture(ctx)
+        ctx.set_ricker(5.0)
+        ctx.replay()
+        ctx.sync()

@@ -161,14 +161,15 @@ def test_storage_change_drops_the_captured_solve():
         assert rel(a, b) < 1e-12, rel(a, b)
 
 
-@pytest.mark.parametrize("first", ["pairs", "fp32"])
-def test_fp32_apply_with_pairs_stored_once(first):
-    """The 32-bit apply with face pairs stored once (p = 3): whichever option
-    comes first, the step's acceleration term is within 1e-5 of the FP64
-    oracle and of the full-row 32-bit step, and the 32-bit copy of the stored
-    blocks is allocated (and freed with the stored pairs)."""
-    say(f"[symmetric] fp32 apply, {first} first")
-    m, p = M.cube_mesh(3, agglomerate=True), 3
+@pytest.mark.parametrize("name,first", [("cube3agg", "pairs"), ("cube3agg", "fp32"), ("grown", "pairs")])
+def test_fp32_apply_with_pairs_stored_once(name, first):
+    """The 32-bit apply with face pairs stored once (p = 3, asked for with
+    fp32=True): whichever option comes first, the step's acceleration term is
+    within 1e-5 of the FP64 oracle and of the full-row 32-bit step, and the
+    32-bit copy of the stored blocks is allocated (and freed with the stored
+    pairs).  The grown mesh has uneven rows and a dispatch order."""
+    say(f"[symmetric] fp32 apply {name}, {first} first")
+    m, p = _mesh(name), 3
     o = oracle_for(m, p)
     o.build_operator()
     rng = np.random.default_rng(11)
@@ -184,11 +185,11 @@ def test_fp32_apply_with_pairs_stored_once(first):
         bytes32 = ctx.device_bytes
         ctx.set_apply_precision(64)
         if first == "pairs":
-            ctx.set_symmetric_storage(True)
+            ctx.set_symmetric_storage(True, fp32=True)
             ctx.set_apply_precision(32)
         else:
             ctx.set_apply_precision(32)
-            ctx.set_symmetric_storage(True)
+            ctx.set_symmetric_storage(True, fp32=True)
         rp, _ = ctx.pattern()
         n_st = m.n_poly + (ctx.n_blk - m.n_poly) // 2
         assert ctx.device_bytes - bytes32 >= n_st * (3 * o.Np) ** 2 * (8 + 4)
@@ -210,3 +211,67 @@ def test_fp32_apply_with_pairs_stored_once(first):
         first_full = ctx.first_time_step_matrix_free(g, dt, u0, u1)
         assert rel(a, b) < 1e-5, rel(a, b)
         assert rel(first_sym - u0, first_full - u0) < 1e-5
+
+
+def test_fp32_pairs_once_is_opt_in():
+    """Face pairs stored once for the FP64 step only (the default mode) leave
+    the 32-bit apply on its full rows: no 32-bit copy of the stored blocks is
+    made and its step equals the full-row 32-bit step bit for bit;
+    lymph_autotune is refused while the step in use reads stored pairs."""
+    say("[symmetric] fp32 pairs once is opt-in")
+    m, p = M.cube_mesh(3, agglomerate=True), 3
+    rng = np.random.default_rng(5)
+    with L.Context(m, p) as ctx:
+        _prepared(ctx)
+        n = m.n_poly * 3 * ctx.Np
+        u0, u1 = rng.standard_normal(n), rng.standard_normal(n)
+        ctx.set_apply_precision(32)
+        full32 = ctx.time_step_matrix_free(0.37, 1e-3, u0, u1)
+        ctx.set_apply_precision(64)
+        bytes0 = ctx.device_bytes
+        ctx.set_symmetric_storage(True)
+        with pytest.raises(L.LymphError) as e:
+            ctx.autotune(reps=2)
+        assert e.value.code == _lib.ERR_STATE
+        bytes_fp64 = ctx.device_bytes
+        assert bytes_fp64 > bytes0
+        ctx.set_apply_precision(32)  # the 32-bit blocks exist already: nothing is added
+        assert ctx.device_bytes == bytes_fp64
+        a = ctx.time_step_matrix_free(0.37, 1e-3, u0, u1)
+        assert np.array_equal(a, full32)
+        ctx.autotune(reps=2)  # the 32-bit full rows are in use: allowed
+        # switching to both precisions stores the 32-bit pairs once
+        before = ctx.device_bytes
+        ctx.set_symmetric_storage(True, fp32=True)
+        n_st = m.n_poly + (ctx.n_blk - m.n_poly) // 2
+        assert ctx.device_bytes - before >= n_st * (3 * ctx.Np) ** 2 * 4
+        ctx.set_symmetric_storage(False)
+
+
+def test_fp32_pairs_once_long_run_drift():
+    """1000 steps of the 32-bit apply from the same start on the 4^3
+    agglomerated cube at p = 3: the pairs-once state stays within a small
+    factor of the full rows' distance from the FP64 state."""
+    say("[symmetric] fp32 pairs once, 1000-step drift")
+    m, p = M.cube_mesh(4, agglomerate=True), 3
+    dt = 0.2 * (1 / 4) / (np.sqrt(3.0) * p * p)
+    with L.Context(m, p) as ctx:
+        ctx.set_build("factored")
+        ctx.make_matrices_free()
+        res = {}
+        for key in ("fp64", "full32", "pairs32"):
+            ctx.set_symmetric_storage(False)
+            ctx.set_apply_precision(64)
+            ctx.run(L.PROB_CUBE_WAVE, dt, 1e30, max_steps=1)
+            if key != "fp64":
+                ctx.set_apply_precision(32)
+            if key == "pairs32":
+                ctx.set_symmetric_storage(True, fp32=True)
+            ctx.step_n(L.PROB_CUBE_WAVE, 1000, dt, dt)
+            res[key] = ctx.get_slot(_lib.SLOT_U_CUR)
+        ctx.set_symmetric_storage(False)
+    nrm = np.linalg.norm(res["fp64"])
+    d_full = np.linalg.norm(res["full32"] - res["fp64"]) / nrm
+    d_pairs = np.linalg.norm(res["pairs32"] - res["fp64"]) / nrm
+    say(f"  drift vs fp64: full rows {d_full:.3e}, pairs once {d_pairs:.3e}")
+    assert d_full < 1e-4 and d_pairs < 3 * d_full + 1e-7, (d_full, d_pairs)
