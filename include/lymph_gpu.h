@@ -291,6 +291,19 @@ int lymph_set_symmetric_storage(LymphCtx* ctx, int on, double* asym);
  * its Ricker keyword is parsed but unused (src/Poly_readfile.f90:589-680). */
 int lymph_add_point_sources(LymphCtx* ctx, int n, const double* xyz, const int32_t* tet,
                             const double* force, int slot);
+/* lymph_add_point_sources without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The sources are staged in one file-scope device array of the
+ * library (n <= 16384 sources per call, 832 KiB, held by every GPU the
+ * library runs on until the process ends) instead of per-call device
+ * buffers; it calls no hipMalloc/hipFree.  The context's device must be
+ * current.  n > 16384 returns LYMPH_ERR_ARG.  Every call on one GPU shares
+ * that GPU's array, so calls that overlap on one GPU (other host threads,
+ * other contexts) race; calls return only once the update is done.  Same
+ * result as lymph_add_point_sources, bit for bit. */
+int lymph_add_point_sources_noalloc(LymphCtx* ctx, int n, const double* xyz, const int32_t* tet,
+                                    const double* force, int slot);
 /* Time factor of LYMPH_PROB_RICKER for this context: Ricker wavelet of peak
  * frequency f0 centred at t0 (default f0 = 5, t0 = 1.2/f0). */
 int lymph_set_ricker(LymphCtx* ctx, double f0, double t0);

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -66,6 +67,37 @@ int upload(T** p, const T* h, size_t n) {
   return 0;
 }
 
+// Staging of lymph_add_point_sources_noalloc: one file-scope device array,
+// so that the entry point allocates nothing.  Every GPU the library runs on
+// holds its own copy until the process ends; host code never passes the
+// array's name (a host-side stand-in) to a kernel or a copy, only the device
+// address hipGetSymbolAddress returned for the current device.  Layout:
+// xyz[3n] doubles, force[3n] doubles, then tet[n] int32.
+constexpr int kNoAllocMaxSources = 16384;
+constexpr size_t kScratchBytes = (size_t)kNoAllocMaxSources * (6 * sizeof(double) + sizeof(int32_t));
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// Device address of `scratch` on device `dev` (which must be current),
+// resolved once per device.
+int scratch_ptr(int dev, unsigned char** out) {
+  *out = nullptr;
+  if (dev < 0 || dev >= kMaxDevices) return fail(LYMPH_ERR_ARG, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(LYMPH_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = (unsigned char*)g_scratch_ptr[dev];
+  return 0;
+}
+
 }  // namespace
 
 struct LymphCtx {
@@ -103,6 +135,7 @@ struct LymphCtx {
   double* graph_slot[LYMPH_NUM_SLOTS] = {};  // slot pointers as the captured solve leaves them
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  int device = 0;  // the HIP device current at lymph_ctx_create
   bool built = false;
   std::vector<int> h_row_ptr, h_col, h_tet_poly;
   // device
@@ -898,6 +931,17 @@ int lymph_ctx_create(const LymphMesh* mesh, int p, double alpha, double theta, L
     lymph_ctx_destroy(c);
     return fail(LYMPH_ERR_HIP, "stream sync failed");
   }
+  // set-up of the allocation-free entry points: the scratch array's address
+  // on this device, resolved once per device with the device current
+  unsigned char* sp = nullptr;
+  if (hipGetDevice(&c->device) != hipSuccess) {
+    lymph_ctx_destroy(c);
+    return fail(LYMPH_ERR_HIP, "hipGetDevice failed");
+  }
+  if (int r = scratch_ptr(c->device, &sp)) {
+    lymph_ctx_destroy(c);
+    return r;
+  }
   *out = c;
   return 0;
 }
@@ -1560,7 +1604,7 @@ int lymph_add_point_sources(LymphCtx* c, int n, const double* xyz, const int32_t
     rc = dispatch_p(c->p, [&](auto pc) -> int {
       constexpr int P = decltype(pc)::value;
       k_point_sources<P><<<1, 128, 0, c->stream>>>(dx, dp, df, n, c->md.bbox, c->md.blist,
-                                                  c->slot[slot]);
+                                                  c->slot[slot], nullptr);
       return check_launch("k_point_sources");
     });
   if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(LYMPH_ERR_HIP, "sync failed");
@@ -1568,6 +1612,40 @@ int lymph_add_point_sources(LymphCtx* c, int n, const double* xyz, const int32_t
   if (df) (void)hipFree(df);
   if (dp) (void)hipFree(dp);
   return rc;
+}
+
+int lymph_add_point_sources_noalloc(LymphCtx* c, int n, const double* xyz, const int32_t* tet,
+                                    const double* force, int slot) {
+  if (int rc = need_ctx(c)) return rc;
+  if (n < 0 || slot < 0 || slot >= LYMPH_NUM_SLOTS || (n > 0 && (!xyz || !tet || !force)))
+    return fail(LYMPH_ERR_ARG, "bad point-source arguments");
+  if (n > kNoAllocMaxSources)
+    return fail(LYMPH_ERR_ARG, "lymph_add_point_sources_noalloc takes at most " +
+                                   std::to_string(kNoAllocMaxSources) + " sources per call, got " +
+                                   std::to_string(n));
+  if (n == 0) return 0;
+  for (int i = 0; i < n; ++i)
+    if (tet[i] < 0 || tet[i] >= c->n_tet) return fail(LYMPH_ERR_ARG, "source tet out of range");
+  int cur = -1;
+  HIP_TRY(hipGetDevice(&cur));
+  if (cur != c->device) return fail(LYMPH_ERR_STATE, "the context's device is not the current device");
+  unsigned char* base = nullptr;
+  if (int rc = scratch_ptr(c->device, &base)) return rc;
+  double* sx = (double*)base;
+  double* sf = sx + 3 * (size_t)kNoAllocMaxSources;
+  int* st = (int*)(sf + 3 * (size_t)kNoAllocMaxSources);
+  HIP_TRY(hipMemcpyAsync(sx, xyz, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(sf, force, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(st, tet, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  int rc = dispatch_p(c->p, [&](auto pc) -> int {
+    constexpr int P = decltype(pc)::value;
+    k_point_sources<P><<<1, 128, 0, c->stream>>>(sx, st, sf, n, c->md.bbox, c->md.blist,
+                                                c->slot[slot], c->md.tet_poly);
+    return check_launch("k_point_sources");
+  });
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));  // the host arrays are only borrowed
+  return 0;
 }
 
 int lymph_set_receivers(LymphCtx* c, int n, const double* xyz, const int32_t* tet,

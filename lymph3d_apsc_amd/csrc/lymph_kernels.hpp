@@ -2098,11 +2098,13 @@ __global__ __launch_bounds__(128) void k_point_sources(const double* __restrict_
                                                        const double* __restrict__ sf, int n,
                                                        const double* __restrict__ bbox,
                                                        const int* __restrict__ blist,
-                                                       double* __restrict__ rhs) {
+                                                       double* __restrict__ rhs,
+                                                       const int* __restrict__ tet_poly) {
   using D = Dims<P>;
   constexpr int NP = D::NP, N3 = D::N3;
   for (int k = 0; k < n; ++k) {
-    const int Pp = spoly[k];
+    // spoly holds polyhedra, or tets when their polyhedron map is given
+    const int Pp = tet_poly ? tet_poly[spoly[k]] : spoly[k];
     const double* bb = bbox + (size_t)Pp * 6;
     for (int e = threadIdx.x; e < N3; e += blockDim.x) {
       const int i = e / NP, m = e % NP;
