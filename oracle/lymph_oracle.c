@@ -43,6 +43,10 @@ struct OCtx {
   int* blist;
   double *K, *A, *M, *Mm, *RM, *RMm;
   int built;
+  /* O_FN_POLY: u_i = sum_k poly_coef[k*3+i] x^a y^b z^c, (a,b,c) = poly_exp[k] */
+  int poly_n;
+  int poly_exp[O_MAX_POLY_TERMS * 3];
+  double poly_coef[O_MAX_POLY_TERMS * 3];
 };
 
 int o_num_threads(int n) {
@@ -681,9 +685,18 @@ static void neumann(int problem, double lam, double mu, const double* n,
 }
 
 /* fields for projection */
-static void field(int fn, const double* p, double* r) {
+static void field(const OCtx* c, int fn, const double* p, double* r) {
   double PI = o_pi();
   r[0] = r[1] = r[2] = 0.0;
+  if (fn == O_FN_POLY) {
+    for (int k = 0; k < c->poly_n; ++k) {
+      double m = 1.0;
+      for (int d = 0; d < 3; ++d)
+        for (int e = 0; e < c->poly_exp[k * 3 + d]; ++e) m = m * p[d];
+      for (int i = 0; i < 3; ++i) r[i] = r[i] + c->poly_coef[k * 3 + i] * m;
+    }
+    return;
+  }
   if (fn == O_FN_REF_VEL) {
     double c = o_sqrt2() * PI;
     r[0] = c * cos(PI * p[0]) * sin(PI * p[1]) * sin(PI * p[2]);
@@ -1247,7 +1260,7 @@ int o_project(const OCtx* c, int fn, double* coeff) {
         for (int q = 0; q < nq3; ++q)
           for (int m = 0; m < Np; ++m) {
             double e[3];
-            field(fn, &pts[q * 3], e);
+            field(c, fn, &pts[q * 3], e);
             for (int i = 0; i < 3; ++i)
               acc[i * Np + m] = acc[i * Np + m] + fabs(Jdet) * c->weitet3[q] * e[i] * phi[m * nq3 + q];
           }
@@ -1494,4 +1507,24 @@ double o_ricker(double f0, double t0, double t) {
   double a = o_pi() * f0 * (t - t0);
   a = a * a;
   return (1.0 - 2.0 * a) * exp(-a);
+}
+
+/* ---------------- test access to the data sets and the polynomial field ---- */
+void o_problem_data(int problem, int n, const double* xyz, const double* normal, double lam,
+                    double mu, double rho, int tag, double* f, double* gd, double* gn) {
+  for (int k = 0; k < n; ++k) {
+    if (f) forcing(problem, lam, mu, xyz + 3 * k, rho, f + 3 * k);
+    if (gd) dirichlet(problem, xyz + 3 * k, tag, gd + 3 * k);
+    if (gn) neumann(problem, lam, mu, normal + 3 * k, xyz + 3 * k, tag, gn + 3 * k);
+  }
+}
+
+int o_set_poly_field(OCtx* c, int nterms, const int* exps, const double* coef) {
+  if (nterms < 0 || nterms > O_MAX_POLY_TERMS) return 1;
+  c->poly_n = nterms;
+  for (int k = 0; k < 3 * nterms; ++k) {
+    c->poly_exp[k] = exps[k];
+    c->poly_coef[k] = coef[k];
+  }
+  return 0;
 }

@@ -54,8 +54,10 @@ enum {
   O_FN_REF_VEL = 1,      /* ic_velocity */
   O_FN_REF_UEX = 2,      /* uex */
   O_FN_CUBE_U = 3,       /* sin sin sin (1,1,1) */
-  O_FN_CUBE_V = 4        /* zero */
+  O_FN_CUBE_V = 4,       /* zero */
+  O_FN_POLY = 5          /* the polynomial field set by o_set_poly_field */
 };
+#define O_MAX_POLY_TERMS 64
 
 /* Build the mesh context.
  *  xyz[n_node*3], tet_vert[n_tet*4] (0-based node ids, file order v1..v4),
@@ -117,6 +119,15 @@ void o_eval_points(const OCtx* c, int n, const double* pts, const int* tet, cons
 void o_point_sources(const OCtx* c, int n, const double* pts, const int* tet, const double* f,
                      double* rhs);
 double o_ricker(double f0, double t0, double t);
+
+/* Point values of a data set (the reference's f_forcing, f_dirichlet,
+ * f_neumann of src/problem_data_and_properties.f90, or the cube wave):
+ * f[3n], gd[3n], gn[3n] (any may be NULL) at xyz[3n], normals normal[3n]. */
+void o_problem_data(int problem, int n, const double* xyz, const double* normal, double lam,
+                    double mu, double rho, int tag, double* f, double* gd, double* gn);
+/* Polynomial field for o_project(c, O_FN_POLY, ...): component i is
+ * sum_k coef[3k+i] x^exps[3k] y^exps[3k+1] z^exps[3k+2]; nterms <= 64. */
+int o_set_poly_field(OCtx* c, int nterms, const int* exps, const double* coef);
 
 /* ---- element-level restatements (checked against the reference's own
  *      Fortran routines by the tests) ---- */

@@ -12,7 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 PROB_NULL, PROB_REF_MMS, PROB_CUBE_WAVE = 0, 1, 2
-FN_ZERO, FN_REF_VEL, FN_REF_UEX, FN_CUBE_U, FN_CUBE_V = 0, 1, 2, 3, 4
+FN_ZERO, FN_REF_VEL, FN_REF_UEX, FN_CUBE_U, FN_CUBE_V, FN_POLY = 0, 1, 2, 3, 4, 5
 
 _dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -77,6 +77,10 @@ def lib():
         L.o_cholesky.argtypes = [C.c_int, _dp, _dp]
         L.o_cholesky.restype = C.c_int
         L.o_solve_lu.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.o_problem_data.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_double,
+                                     C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.o_set_poly_field.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
+        L.o_set_poly_field.restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -211,6 +215,14 @@ class Oracle:
                               np.ascontiguousarray(forces, np.float64).reshape(-1, 3), out)
         return out
 
+    def set_poly_field(self, exps, coef):
+        """Field FN_POLY: component i = sum_k coef[k, i] x^a y^b z^c with
+        (a, b, c) = exps[k]."""
+        e = np.ascontiguousarray(exps, np.int32).reshape(-1, 3)
+        cf = np.ascontiguousarray(coef, np.float64).reshape(-1, 3)
+        if lib().o_set_poly_field(self.h, e.shape[0], e, cf):
+            raise ValueError("too many polynomial terms")
+
     def run(self, problem, dt, stop_time, max_steps=-1, want_u=False):
         out = np.zeros(4)
         u = np.zeros(self.n_poly * self.n3) if want_u else None
@@ -234,3 +246,13 @@ def from_deck(deck, p=None, **kw):
 
 def ricker(f0, t0, t):
     return lib().o_ricker(f0, t0, t)
+
+
+def problem_data(problem, xyz, normal=None, lam=1.0, mu=1.0, rho=1.0, tag=1):
+    """Point values (f, gd, gn), each (n, 3), of a data set at xyz (n, 3)."""
+    xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+    n = xyz.shape[0]
+    nrm = np.zeros_like(xyz) if normal is None else np.ascontiguousarray(normal, np.float64).reshape(-1, 3)
+    f, gd, gn = (np.zeros((n, 3)) for _ in range(3))
+    lib().o_problem_data(problem, n, xyz, nrm, lam, mu, rho, tag, _ptr(f), _ptr(gd), _ptr(gn))
+    return f, gd, gn
