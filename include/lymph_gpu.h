@@ -291,6 +291,19 @@ int lymph_set_symmetric_storage(LymphCtx* ctx, int on, double* asym);
  * its Ricker keyword is parsed but unused (src/Poly_readfile.f90:589-680). */
 int lymph_add_point_sources(LymphCtx* ctx, int n, const double* xyz, const int32_t* tet,
                             const double* force, int slot);
+/* Double-couple point sources, the SISM branch of MAKE_RHS_FREE
+ * (src/matrix_free.f90:600-664; MAKE_DOUBLE_COUPLE, src/assemble_element.f90:
+ * 116-167): for source k, in order, located in tet[k] (as FIND_ELEM_FROM_POINT
+ * locates it, nearest barycentre), the moment density
+ * M = (s n^T + n s^T) M0 / V with strike s = strike[3k..], fault normal
+ * n = normal[3k..], moment M0 = m0[k] and V the tet's volume, integrated
+ * against the gradients of the polyhedron's basis over the tet:
+ * rhs[P][i*Np+m] += sum_q w_q |J| sum_j M_ij d_j phi_m(x_q).  The reference
+ * takes the strike, normal and moment of its first source for every source
+ * and adds the couples in every MAKE_RHS_FREE call (static and dynamic RHS);
+ * its drop-in passes them that way. */
+int lymph_add_double_couples(LymphCtx* ctx, int n, const int32_t* tet, const double* strike,
+                             const double* normal, const double* m0, int slot);
 /* lymph_add_point_sources without any allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid

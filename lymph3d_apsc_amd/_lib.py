@@ -41,7 +41,7 @@ EXPORTS = [
     "lymph_time_function", "lymph_autotune", "lymph_set_step_variant", "lymph_set_receivers",
     "lymph_sample_receivers", "lymph_step_n_record", "lymph_get_traces",
     "lymph_set_apply_precision", "lymph_set_source_groups", "lymph_ctx_set_keep_adg", "lymph_ctx_set_build", "lymph_release_fp64_operator", "lymph_set_symmetric_storage",
-    "lymph_add_point_sources", "lymph_add_point_sources_noalloc", "lymph_set_ricker", "lymph_ricker",
+    "lymph_add_point_sources", "lymph_add_point_sources_noalloc", "lymph_add_double_couples", "lymph_set_ricker", "lymph_ricker",
     "lymph_capture_solve", "lymph_replay", "lymph_sync",
 ]
 
@@ -122,6 +122,7 @@ def lib():
         "lymph_set_symmetric_storage": (_i, [_vp, _i, _vp]),
         "lymph_add_point_sources": (_i, [_vp, _i, _vp, _vp, _vp, _i]),
         "lymph_add_point_sources_noalloc": (_i, [_vp, _i, _vp, _vp, _vp, _i]),
+        "lymph_add_double_couples": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i]),
         "lymph_set_ricker": (_i, [_vp, _d, _d]),
         "lymph_ricker": (_d, [_d, _d, _d]),
         "lymph_capture_solve": (_i, [_vp, _i, _d, _i, _i]),

@@ -1614,6 +1614,32 @@ int lymph_add_point_sources(LymphCtx* c, int n, const double* xyz, const int32_t
   return rc;
 }
 
+int lymph_add_double_couples(LymphCtx* c, int n, const int32_t* tet, const double* strike,
+                             const double* normal, const double* m0, int slot) {
+  if (int rc = need_ctx(c)) return rc;
+  if (n < 0 || slot < 0 || slot >= LYMPH_NUM_SLOTS || (n > 0 && (!tet || !strike || !normal || !m0)))
+    return fail(LYMPH_ERR_ARG, "bad double-couple arguments");
+  if (n == 0) return 0;
+  for (int i = 0; i < n; ++i)
+    if (tet[i] < 0 || tet[i] >= c->n_tet) return fail(LYMPH_ERR_ARG, "source tet out of range");
+  int* dt = nullptr;
+  double *ds = nullptr, *dn = nullptr, *dm = nullptr;
+  int rc = upload(&dt, tet, (size_t)n);
+  if (!rc) rc = upload(&ds, strike, 3 * (size_t)n);
+  if (!rc) rc = upload(&dn, normal, 3 * (size_t)n);
+  if (!rc) rc = upload(&dm, m0, (size_t)n);
+  if (!rc)
+    rc = dispatch_p(c->p, [&](auto pc) -> int {
+      constexpr int P = decltype(pc)::value;
+      k_double_couples<P><<<1, 128, 0, c->stream>>>(c->md, dt, ds, dn, dm, n, c->slot[slot]);
+      return check_launch("k_double_couples");
+    });
+  if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(LYMPH_ERR_HIP, "sync failed");
+  for (void* q : {(void*)dt, (void*)ds, (void*)dn, (void*)dm})
+    if (q) (void)hipFree(q);
+  return rc;
+}
+
 int lymph_add_point_sources_noalloc(LymphCtx* c, int n, const double* xyz, const int32_t* tet,
                                     const double* force, int slot) {
   if (int rc = need_ctx(c)) return rc;

@@ -1041,6 +1041,16 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The row's stored blocks are streamed in chunks of CW = 3Np/3 columns (CP
+// column pairs): a chunk gives the row product's terms for its columns (the
+// same ascending-column chain as k_leapfrog) and, for a pair block, the
+// transposed products of exactly those columns.  Chunks are software
+// pipelined: the loads of the next chunk -- of this block or the next stored
+// one, with that block's neighbour coefficients -- are issued before the
+// current chunk's row product and LDS transpose run, so the wave keeps a
+// chunk of loads in flight through its compute (two chunks of registers
+// instead of a whole block).  Same operations in the same order as one pass
+// per block, so the products are the same bits.
 template <int P>
 __global__ __launch_bounds__(64) void k_sym_products(const double* __restrict__ Ks,
                                                      const int* __restrict__ row_ptr,
@@ -1051,7 +1061,7 @@ __global__ __launch_bounds__(64) void k_sym_products(const double* __restrict__ 
                                                      double* __restrict__ prod, int n_poly,
                                                      const int* __restrict__ order) {
   using D = Dims<P>;
-  constexpr int N3 = D::N3, NPAIR = D::NPAIR;
+  constexpr int N3 = D::N3;
   static_assert(N3 <= 64 && N3 % 2 == 0, "one lane per row, whole column pairs");
   constexpr int CW = N3 / 3, CP = CW / 2;
   static_assert(CW % 2 == 0, "whole column pairs per chunk");
@@ -1062,61 +1072,134 @@ __global__ __launch_bounds__(64) void k_sym_products(const double* __restrict__ 
   const int Pp = step_poly(order);
   if (Pp >= n_poly) return;  // padding workgroups of the last XCD share
   const int b0 = row_ptr[Pp], b1 = row_ptr[Pp + 1];
-  // the row's neighbours and indices, one block per lane, so a block's loads
-  // do not wait on its index (rows of more than 64 blocks read the rest as
-  // they come)
-  const int lcb = lane < b1 - b0 ? col[b0 + lane] : 0;
-  const int lsi = lane < b1 - b0 ? sidx[b0 + lane] : 0;
-  const int ltd = lane < b1 - b0 ? tdst[b0 + lane] : 0;
   const double up = row ? u[(size_t)Pp * N3 + lane] : 0.0;
   const int cl = lane % CW, t = lane / CW;
-  for (int b = b0; b < b1; ++b) {
-    const int j = b - b0;
-    const int si = j < 64 ? __builtin_amdgcn_readlane(lsi, j) : __builtin_amdgcn_readfirstlane(sidx[b]);
-    if (si < 0) continue;  // written by the neighbour's wave
-    const int cb = j < 64 ? __builtin_amdgcn_readlane(lcb, j) : __builtin_amdgcn_readfirstlane(col[b]);
-    const int td = j < 64 ? __builtin_amdgcn_readlane(ltd, j) : __builtin_amdgcn_readfirstlane(tdst[b]);
-    const double* __restrict__ Kb = Ks + (size_t)si * D::BS;
-    const double xl = row ? u[(size_t)cb * N3 + lane] : 0.0;
-    double2 kv[NPAIR];
+  // blocks of the row in batches of 64, one block's indices per lane
+  for (int bb = b0; bb < b1; bb += 64) {
+    const int nb = b1 - bb < 64 ? b1 - bb : 64;
+    const bool mine = lane < nb;
+    const int lcb = mine ? col[bb + lane] : 0;
+    const int lsi = mine ? sidx[bb + lane] : -1;
+    const int ltd = mine ? tdst[bb + lane] : 0;
+    const unsigned long long stored = __ballot(lsi >= 0);  // blocks this wave streams
+    if (!stored) continue;
+    // load cursor (next chunk to load) and process cursor (next to compute):
+    // the lowest block left in the mask, chunk h of it
+    unsigned long long ml = stored, mp = stored;
+    int hl = 0, hp = 0;
+    double xnext = 0.0, xcur = 0.0, s = 0.0, st = 0.0;
+    double2 A[CP], B[CP];
+    auto load_chunk = [&](double2(&kv)[CP]) {
+      const int j = __builtin_ctzll(ml);
+      const int si = __builtin_amdgcn_readlane(lsi, j);
+      if (hl == 0) {  // a new block: its neighbour's coefficients come with its first chunk
+        const int cb = __builtin_amdgcn_readlane(lcb, j);
+        xnext = row ? u[(size_t)cb * N3 + lane] : 0.0;
+      }
+      const double* __restrict__ Kb = Ks + (size_t)si * D::BS + (size_t)hl * CP * N3 * 2;
 #pragma unroll
-    for (int c2i = 0; c2i < NPAIR; ++c2i)
-      kv[c2i] = row ? load_stream(Kb + ((size_t)c2i * N3 + lane) * 2) : make_double2(0.0, 0.0);
-    double s = 0.0;
+      for (int k = 0; k < CP; ++k)
+        kv[k] = row ? load_stream(Kb + ((size_t)k * N3 + lane) * 2) : make_double2(0.0, 0.0);
+      if (++hl == 3) {
+        hl = 0;
+        ml &= ml - 1;
+      }
+    };
+    auto process_chunk = [&](const double2(&kv)[CP]) {
+      const int j = __builtin_ctzll(mp);
+      const int h = hp;
+      const int cb = __builtin_amdgcn_readlane(lcb, j);
+      if (h == 0) {
+        xcur = xnext;
+        s = 0.0;
+      }
 #pragma unroll
-    for (int c2i = 0; c2i < NPAIR; ++c2i) {
-      const double x0 = lane_bcast(xl, 2 * c2i), x1 = lane_bcast(xl, 2 * c2i + 1);
-      s = s + kv[c2i].x * x0;
-      s = s + kv[c2i].y * x1;
-    }
-    if (row) prod[(size_t)b * N3 + lane] = s;
-    if (cb == Pp) continue;  // self block: no partner
-    double st = 0.0;
+      for (int k = 0; k < CP; ++k) {
+        const int c2 = h * CP + k;
+        const double x0 = lane_bcast(xcur, 2 * c2), x1 = lane_bcast(xcur, 2 * c2 + 1);
+        s = s + kv[k].x * x0;
+        s = s + kv[k].y * x1;
+      }
+      if (h == 2 && row) prod[(size_t)(bb + j) * N3 + lane] = s;
+      if (cb != Pp) {  // a pair block: the neighbour's product K_PQ^T u_P, chunk h
+        wave_lds_sync();  // the previous chunk's reads are done
+        if (row) {
 #pragma unroll
-    for (int h = 0; h < 3; ++h) {
-      wave_lds_sync();  // the previous chunk's reads are done
-      if (row) {
+          for (int k = 0; k < CP; ++k) {
+            sT[(2 * k) * LDT + lane] = kv[k].x * up;
+            sT[(2 * k + 1) * LDT + lane] = kv[k].y * up;
+          }
+        }
+        wave_lds_sync();
+        double v = 0.0;
+        if (row) {
+          v = sT[cl * LDT + t * CW];
 #pragma unroll
-        for (int k = 0; k < CP; ++k) {
-          sT[(2 * k) * LDT + lane] = kv[h * CP + k].x * up;
-          sT[(2 * k + 1) * LDT + lane] = kv[h * CP + k].y * up;
+          for (int k = 1; k < CW; ++k) v = v + sT[cl * LDT + t * CW + k];
+        }
+        const int src = (lane - h * CW) & 63;  // lanes outside the chunk discard
+        const double v0 = __shfl(v, src % CW, 64);
+        const double v1 = __shfl(v, src % CW + CW, 64);
+        const double v2 = __shfl(v, src % CW + 2 * CW, 64);
+        if (lane >= h * CW && lane < (h + 1) * CW) st = (v0 + v1) + v2;
+        if (h == 2 && row) {
+          const int td = __builtin_amdgcn_readlane(ltd, j);
+          prod[(size_t)td * N3 + lane] = st;
         }
       }
-      wave_lds_sync();
-      double v = 0.0;
-      if (row) {
-        v = sT[cl * LDT + t * CW];
-#pragma unroll
-        for (int k = 1; k < CW; ++k) v = v + sT[cl * LDT + t * CW + k];
+      if (++hp == 3) {
+        hp = 0;
+        mp &= mp - 1;
       }
-      const int src = (lane - h * CW) & 63;  // lanes outside the chunk discard
-      const double v0 = __shfl(v, src % CW, 64);
-      const double v1 = __shfl(v, src % CW + CW, 64);
-      const double v2 = __shfl(v, src % CW + 2 * CW, 64);
-      if (lane >= h * CW && lane < (h + 1) * CW) st = (v0 + v1) + v2;
+    };
+    load_chunk(A);
+    while (true) {
+      if (ml) load_chunk(B);
+      process_chunk(A);
+      if (!mp) break;
+      if (ml) load_chunk(A);
+      process_chunk(B);
+      if (!mp) break;
     }
-    if (row) prod[(size_t)td * N3 + lane] = st;
   }
+}
+
+// M^-1 w = R^-1 R^-T w across one wave as solve_wave, with each lane's
+// entries of R in registers (E[j] = R_{j,ii} below the diagonal for the
+// forward sweep, R_{ii,j} above it for the backward one) and the pivots as
+// products with the reciprocal of the lane's own diagonal entry, broadcast
+// with v_readlane: each of the 2 NP dependent steps is a multiply, three lane
+// reads and an FMA instead of a division, a cross-lane permute and two LDS
+// reads.  Results differ from solve_wave by rounding only.
+template <int NP>
+__device__ __forceinline__ double solve_wave_rcp(const double* __restrict__ R, double w, int lane) {
+  const bool row = lane < 3 * NP;
+  const int cc = row ? lane / NP : 0, ii = lane % NP;
+  double E[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) E[j] = R[ii > j ? pk_off(NP, j) + (ii - j) : pk_off(NP, ii) + (j - ii)];
+  const double rinv = 1.0 / R[pk_off(NP, ii)];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const double yv = w * rinv;
+    const double y0 = lane_bcast(yv, j), y1 = lane_bcast(yv, NP + j), y2 = lane_bcast(yv, 2 * NP + j);
+    const double y = cc == 0 ? y0 : (cc == 1 ? y1 : y2);
+    if (ii == j)
+      w = y;
+    else if (ii > j)
+      w = w - E[j] * y;
+  }
+#pragma unroll
+  for (int j = NP - 1; j >= 0; --j) {
+    const double xv = w * rinv;
+    const double x0 = lane_bcast(xv, j), x1 = lane_bcast(xv, NP + j), x2 = lane_bcast(xv, 2 * NP + j);
+    const double x = cc == 0 ? x0 : (cc == 1 ? x1 : x2);
+    if (ii == j)
+      w = x;
+    else if (ii < j)
+      w = w - E[j] * x;
+  }
+  return w;
 }
 
 template <int P, int MODE>
@@ -1165,7 +1248,7 @@ __global__ __launch_bounds__(64) void k_sym_update(const double* __restrict__ pr
   const double gp = gsel ? gtab[gsel[Pp]] : g;
   double w = row ? -acc + vs + vd * gp : 0.0;
   __syncthreads();  // sR
-  w = solve_wave<NP>(sR, w, lane);
+  w = solve_wave_rcp<NP>(sR, w, lane);
   if (row) {
     if (MODE == 0)
       out[o + lane] = c1 * w + 2.0 * vu - va;
@@ -2120,6 +2203,77 @@ __global__ __launch_bounds__(128) void k_point_sources(const double* __restrict_
   }
 }
 
+
+// Double-couple sources, the SISM branch of MAKE_RHS_FREE
+// (src/matrix_free.f90:600-664): for source k in order, in tet stet[k] of
+// polyhedron P, the moment density M = ((s n^T + n s^T) M0) / V with V =
+// |det4| / 6 of the tet (moment_density, src/problem_data_and_properties.f90:
+// 317-335; volume, src/find_tet.f90:184-210), and MAKE_DOUBLE_COUPLE
+// (src/assemble_element.f90:116-167): couple[i*Np+m] = sum_q (w_q |J|)
+// sum_j M_ij d_j phi_m(x_q) over the tet's quadrature points with P's basis,
+// then rhs[P] += couple.  One workgroup, thread = entry (i, m), sources in
+// order so that sources sharing a polyhedron add as the reference adds them.
+__device__ inline double det4_rows(const double* v) {  // rows (x_r, y_r, z_r, 1)
+#define A_(i, j) ((j) == 4 ? 1.0 : v[((i) - 1) * 3 + ((j) - 1)])
+  return A_(1, 1) * (A_(2, 2) * (A_(3, 3) * A_(4, 4) - A_(3, 4) * A_(4, 3)) -
+                     A_(2, 3) * (A_(3, 2) * A_(4, 4) - A_(3, 4) * A_(4, 2)) +
+                     A_(2, 4) * (A_(3, 2) * A_(4, 3) - A_(3, 3) * A_(4, 2))) -
+         A_(1, 2) * (A_(2, 1) * (A_(3, 3) * A_(4, 4) - A_(3, 4) * A_(4, 3)) -
+                     A_(2, 3) * (A_(3, 1) * A_(4, 4) - A_(3, 4) * A_(4, 1)) +
+                     A_(2, 4) * (A_(3, 1) * A_(4, 3) - A_(3, 3) * A_(4, 1))) +
+         A_(1, 3) * (A_(2, 1) * (A_(3, 2) * A_(4, 4) - A_(3, 4) * A_(4, 2)) -
+                     A_(2, 2) * (A_(3, 1) * A_(4, 4) - A_(3, 4) * A_(4, 1)) +
+                     A_(2, 4) * (A_(3, 1) * A_(4, 2) - A_(3, 2) * A_(4, 1))) -
+         A_(1, 4) * (A_(2, 1) * (A_(3, 2) * A_(4, 3) - A_(3, 3) * A_(4, 2)) -
+                     A_(2, 2) * (A_(3, 1) * A_(4, 3) - A_(3, 3) * A_(4, 1)) +
+                     A_(2, 3) * (A_(3, 1) * A_(4, 2) - A_(3, 2) * A_(4, 1)));
+#undef A_
+}
+
+template <int P>
+__global__ __launch_bounds__(128) void k_double_couples(MeshDev m, const int* __restrict__ stet,
+                                                        const double* __restrict__ strike,
+                                                        const double* __restrict__ normal,
+                                                        const double* __restrict__ m0, int n,
+                                                        double* __restrict__ rhs) {
+  using D = Dims<P>;
+  constexpr int NP = D::NP, N3 = D::N3, NQ3 = D::NQ3;
+  __shared__ double spt[NQ3 * 3];
+  __shared__ int sbl[NP * 3];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < NP * 3; i += blockDim.x) sbl[i] = m.blist[i];
+  for (int k = 0; k < n; ++k) {
+    const int t = stet[k];
+    const int Pp = m.tet_poly[t];
+    double v[12], Fk[12], Jdet;
+    for (int i = 0; i < 12; ++i) v[i] = m.tet_xyz[(size_t)t * 12 + i];
+    affine_map(v, Fk, Jdet);
+    const double vol = fabs(det4_rows(v)) / 6.0;
+    double mom[9];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        mom[i * 3 + j] = ((strike[3 * k + i] * normal[3 * k + j] + strike[3 * k + j] * normal[3 * k + i]) *
+                          m0[k]) / vol;
+    __syncthreads();  // the previous source's points are no longer read
+    for (int q = tid; q < NQ3; q += blockDim.x) map_point(Fk, &m.qtet[q * 4], &spt[q * 3]);
+    __syncthreads();
+    double bb[6];
+    for (int d = 0; d < 6; ++d) bb[d] = m.bbox[(size_t)Pp * 6 + d];
+    for (int e = tid; e < N3; e += blockDim.x) {
+      const int i = e / NP, mm = e % NP;
+      double acc = 0.0;
+      for (int q = 0; q < NQ3; ++q) {
+        double ph, g[3];
+        basis_at(&sbl[mm * 3], bb, &spt[q * 3], ph, g[0], g[1], g[2]);
+        double reduct = 0.0;
+        for (int j = 0; j < 3; ++j) reduct = reduct + mom[i * 3 + j] * g[j];
+        acc = acc + m.wtet[q] * fabs(Jdet) * reduct;
+      }
+      const size_t o = (size_t)Pp * N3 + e;
+      rhs[o] = rhs[o] + acc;
+    }
+  }
+}
 
 // Per-group Ricker time factors of one step: gtab[i] = ricker(f0[i], t0[i], t).
 __global__ void k_ricker_table(const double* __restrict__ f0, const double* __restrict__ t0,

@@ -256,6 +256,15 @@ module lymph_gpu
       integer(c_int32_t), intent(in) :: tet(*)
       integer(c_int) :: rc
     end function
+    function lymph_add_double_couples(ctx, n, tet, strike, normal, m0, slot) &
+        bind(C, name='lymph_add_double_couples') result(rc)
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n, slot
+      integer(c_int32_t), intent(in) :: tet(*)
+      real(c_double), intent(in) :: strike(*), normal(*), m0(*)
+      integer(c_int) :: rc
+    end function
     function lymph_add_point_sources_noalloc(ctx, n, xyz, tet, force, slot) &
         bind(C, name='lymph_add_point_sources_noalloc') result(rc)
       import :: c_int, c_ptr, c_double, c_int32_t

@@ -276,6 +276,22 @@ class Context:
               "lymph_add_point_sources")
         return tets
 
+    def add_double_couples(self, points, strike, normal, m0, slot=_lib.SLOT_RHS_DYN, tets=None):
+        """Double-couple sources (the SISM branch of MAKE_RHS_FREE) at
+        `points` (n, 3), located like FIND_ELEM_FROM_POINT unless `tets` is
+        given; strike and fault normal (n, 3), moment m0 (n,)."""
+        if tets is None:
+            tets = self.mesh.locate(np.ascontiguousarray(points, np.float64).reshape(-1, 3))
+        tets = np.ascontiguousarray(tets, np.int32)
+        s = np.ascontiguousarray(strike, np.float64).reshape(-1, 3)
+        nrm = np.ascontiguousarray(normal, np.float64).reshape(-1, 3)
+        m0 = np.ascontiguousarray(m0, np.float64).reshape(-1)
+        if not (s.shape[0] == nrm.shape[0] == m0.size == tets.size):
+            raise ValueError("one strike, normal and moment per source")
+        check(lib().lymph_add_double_couples(self.h, tets.size, ptr(tets), ptr(s), ptr(nrm), ptr(m0), slot),
+              "lymph_add_double_couples")
+        return tets
+
     def set_ricker(self, f0, t0=None):
         self.ricker = (f0, 1.2 / f0 if t0 is None else t0)
         check(lib().lymph_set_ricker(self.h, *self.ricker), "lymph_set_ricker")

@@ -1528,3 +1528,78 @@ int o_set_poly_field(OCtx* c, int nterms, const int* exps, const double* coef) {
   }
   return 0;
 }
+
+/* ------------------------- double-couple sources ------------------------ */
+/* 4x4 determinant by cofactors along the first row, in the term order of
+ * det4 (src/vet_mat_operations.f90:52-69); A row-major. */
+double o_det4(const double* A) {
+#define A_(i, j) A[((i) - 1) * 4 + ((j) - 1)]
+  return A_(1, 1) * (A_(2, 2) * (A_(3, 3) * A_(4, 4) - A_(3, 4) * A_(4, 3)) -
+                     A_(2, 3) * (A_(3, 2) * A_(4, 4) - A_(3, 4) * A_(4, 2)) +
+                     A_(2, 4) * (A_(3, 2) * A_(4, 3) - A_(3, 3) * A_(4, 2))) -
+         A_(1, 2) * (A_(2, 1) * (A_(3, 3) * A_(4, 4) - A_(3, 4) * A_(4, 3)) -
+                     A_(2, 3) * (A_(3, 1) * A_(4, 4) - A_(3, 4) * A_(4, 1)) +
+                     A_(2, 4) * (A_(3, 1) * A_(4, 3) - A_(3, 3) * A_(4, 1))) +
+         A_(1, 3) * (A_(2, 1) * (A_(3, 2) * A_(4, 4) - A_(3, 4) * A_(4, 2)) -
+                     A_(2, 2) * (A_(3, 1) * A_(4, 4) - A_(3, 4) * A_(4, 1)) +
+                     A_(2, 4) * (A_(3, 1) * A_(4, 2) - A_(3, 2) * A_(4, 1))) -
+         A_(1, 4) * (A_(2, 1) * (A_(3, 2) * A_(4, 3) - A_(3, 3) * A_(4, 2)) -
+                     A_(2, 2) * (A_(3, 1) * A_(4, 3) - A_(3, 3) * A_(4, 1)) +
+                     A_(2, 3) * (A_(3, 1) * A_(4, 2) - A_(3, 2) * A_(4, 1)));
+#undef A_
+}
+
+/* moment_density (src/problem_data_and_properties.f90:317-335):
+ * M_ij = ((s_i n_j + s_j n_i) M0) / V; mom row-major 3x3. */
+void o_moment_density(double M0, double V, const double* s, const double* n, double* mom) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) mom[i * 3 + j] = ((s[i] * n[j] + s[j] * n[i]) * M0) / V;
+}
+
+/* MAKE_DOUBLE_COUPLE (src/assemble_element.f90:116-167): out[i*Np+m] =
+ * sum_q (w_q |J|) sum_j M_ij d_j phi_m(x_q); dphi[(d*Np + m)*nq3 + q]. */
+void o_double_couple(int p, double Jdet, const double* dphi, const double* mom, double* out) {
+  int Np = o_np(p), nq3 = (p + 1) * (p + 1) * (p + 1);
+  double t3[4 * 125], w3[125], t2[4 * 25], w2[25];
+  o_quadrature(p, t3, w3, t2, w2);
+  for (int r = 0; r < 3 * Np; ++r) out[r] = 0.0;
+  for (int q = 0; q < nq3; ++q)
+    for (int m = 0; m < Np; ++m)
+      for (int i = 0; i < 3; ++i) {
+        double reduct = 0.0;
+        for (int j = 0; j < 3; ++j) reduct = reduct + mom[i * 3 + j] * dphi[(j * Np + m) * nq3 + q];
+        out[i * Np + m] = out[i * Np + m] + w3[q] * fabs(Jdet) * reduct;
+      }
+}
+
+/* The SISM loop of MAKE_RHS_FREE (src/matrix_free.f90:600-664): for every
+ * source, in order, the couple of its tet (volume |det4|/6 of the tet,
+ * find_tet.f90:184-210; basis on the polyhedron's bounding box) is added to
+ * the polyhedron's entries of rhs.  strike/normal[3n], m0[n] per source. */
+void o_double_couples(const OCtx* c, int n, const int* tet, const double* strike, const double* normal,
+                      const double* m0, double* rhs) {
+  int Np = c->Np, nq3 = c->nq3, n3 = c->n3;
+  double* phi = malloc(sizeof(double) * Np * nq3);
+  double* dphi = malloc(sizeof(double) * 3 * Np * nq3);
+  double* pts = malloc(sizeof(double) * 3 * nq3);
+  double* cpl = malloc(sizeof(double) * n3);
+  for (int k = 0; k < n; ++k) {
+    int t = tet[k], P = c->tet_poly[t];
+    double x[4], y[4], z[4], Fk[12], Jdet, A[16], mom[9];
+    tet_xyz(c, t, x, y, z);
+    o_jacobians(x, y, z, Fk, &Jdet);
+    for (int v = 0; v < 4; ++v) {
+      A[v * 4 + 0] = x[v];
+      A[v * 4 + 1] = y[v];
+      A[v * 4 + 2] = z[v];
+      A[v * 4 + 3] = 1.0;
+    }
+    double vol = fabs(o_det4(A)) / 6.0;
+    o_moment_density(m0[k], vol, strike + 3 * k, normal + 3 * k, mom);
+    tet_points(nq3, Fk, c->nodtet3, pts);
+    eval_basis_pts(Np, c->blist, &c->bbox[P * 6], nq3, pts, phi, dphi);
+    o_double_couple(c->p, Jdet, dphi, mom, cpl);
+    for (int r = 0; r < n3; ++r) rhs[(size_t)P * n3 + r] = rhs[(size_t)P * n3 + r] + cpl[r];
+  }
+  free(phi); free(dphi); free(pts); free(cpl);
+}

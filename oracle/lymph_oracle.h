@@ -25,6 +25,9 @@
  *   normals/areas/bbox/hk      src/MAKE_PARTITION_AND_MPI_FILES.f90:1086-1095,1429-1580
  *   face matching, BC tags     src/MAKE_PARTITION_AND_MPI_FILES.f90:1914-2200
  *   problem data               src/problem_data_and_properties.f90
+ *   double couples (SISM)      src/matrix_free.f90:600-664, src/assemble_element.f90:116-167,
+ *                              moment_density src/problem_data_and_properties.f90:317-335,
+ *                              det4 src/vet_mat_operations.f90:52-69, volume src/find_tet.f90:184-210
  *
  * Polyhedra: the reference stores one block row per tetrahedron.  Here each
  * polyhedron owns one block row (self block first, then one block per
@@ -119,6 +122,16 @@ void o_eval_points(const OCtx* c, int n, const double* pts, const int* tet, cons
 void o_point_sources(const OCtx* c, int n, const double* pts, const int* tet, const double* f,
                      double* rhs);
 double o_ricker(double f0, double t0, double t);
+
+/* Double-couple sources (the SISM branch of MAKE_RHS_FREE): det4, the
+ * moment density, MAKE_DOUBLE_COUPLE on one tet, and the loop over sources
+ * adding each couple to its polyhedron's entries of rhs[n_poly*3Np]. */
+double o_det4(const double* A /* 4x4 row-major */);
+void o_moment_density(double M0, double V, const double* s, const double* n, double* mom /* 3x3 */);
+void o_double_couple(int p, double Jdet, const double* dphi /* [3][Np][nq3] */, const double* mom,
+                     double* out /* 3Np */);
+void o_double_couples(const OCtx* c, int n, const int* tet, const double* strike, const double* normal,
+                      const double* m0, double* rhs);
 
 /* Point values of a data set (the reference's f_forcing, f_dirichlet,
  * f_neumann of src/problem_data_and_properties.f90, or the cube wave):

@@ -79,6 +79,11 @@ def lib():
         L.o_solve_lu.argtypes = [C.c_int, _dp, _dp, _dp]
         L.o_problem_data.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_double,
                                      C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.o_det4.argtypes = [_dp]
+        L.o_det4.restype = C.c_double
+        L.o_moment_density.argtypes = [C.c_double, C.c_double, _dp, _dp, _dp]
+        L.o_double_couple.argtypes = [C.c_int, C.c_double, _dp, _dp, _dp]
+        L.o_double_couples.argtypes = [C.c_void_p, C.c_int, _ip, _dp, _dp, _dp, _dp]
         L.o_set_poly_field.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
         L.o_set_poly_field.restype = C.c_int
         _LIB = L
@@ -213,6 +218,17 @@ class Oracle:
         out = np.zeros(self.n_poly * self.n3)
         lib().o_point_sources(self.h, pts.shape[0], pts, np.ascontiguousarray(tets, np.int32),
                               np.ascontiguousarray(forces, np.float64).reshape(-1, 3), out)
+        return out
+
+    def double_couples(self, tets, strike, normal, m0):
+        """RHS entries of double-couple sources in the given tets (the SISM
+        branch of MAKE_RHS_FREE)."""
+        tets = np.ascontiguousarray(tets, np.int32)
+        out = np.zeros(self.n_poly * self.n3)
+        lib().o_double_couples(self.h, tets.size, tets,
+                               np.ascontiguousarray(strike, np.float64).reshape(-1),
+                               np.ascontiguousarray(normal, np.float64).reshape(-1),
+                               np.ascontiguousarray(m0, np.float64).reshape(-1), out)
         return out
 
     def set_poly_field(self, exps, coef):

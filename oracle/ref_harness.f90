@@ -179,4 +179,24 @@ contains
     g = time_function(tt)
   end function
 
+  ! MAKE_DOUBLE_COUPLE with the moment of moment_density(m0, vol, s, n)
+  subroutine rh_double_couple(p, np, jdet, dphi, m0, vol, s, n, rhs) bind(C, name='rh_double_couple')
+    integer(c_int), value :: p, np
+    real(c_double), value :: jdet, m0, vol
+    real(c_double) :: dphi(3,np,(p+1)**3), s(3), n(3), rhs(3*np)
+    integer :: nq3, nq2
+    real(8), allocatable :: a(:,:), b(:), c(:,:), d(:)
+    real(8) :: mom(3,3), mm, vv, ss(3), nn(3)
+    call quad_tables(p, nq3, nq2, a, b, c, d)
+    mm = m0; vv = vol; ss = s; nn = n
+    mom = moment_density(mm, vv, ss, nn)
+    call MAKE_DOUBLE_COUPLE(np, jdet, b, nq3, dphi, mom, rhs)
+  end subroutine
+
+  function rh_det4(a) bind(C, name='rh_det4') result(d)
+    real(c_double) :: a(4,4)
+    real(c_double) :: d
+    d = det4(a)
+  end function
+
 end module lymph_ref_harness

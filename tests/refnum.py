@@ -144,3 +144,21 @@ def fields(x, lam, mu, rho, nrm):
 
 def time_function(t):
     return lib().rh_time_function(t)
+
+
+def double_couple(p, jdet, dphi, m0, vol, s, n):
+    """MAKE_DOUBLE_COUPLE with moment_density(m0, vol, s, n); dphi (3, Np, nq3)."""
+    np1 = np_(p)
+    out = F(3 * np1)
+    d = np.asfortranarray(dphi, np.float64)
+    s_, n_ = (np.asfortranarray(v, np.float64) for v in (s, n))
+    lib().rh_double_couple(C.c_int(p), C.c_int(np1), C.c_double(jdet), _p(d), C.c_double(m0),
+                           C.c_double(vol), _p(s_), _p(n_), _p(out))
+    return out
+
+
+def det4(a):
+    """det4 of a 4x4 array (Fortran order)."""
+    lib().rh_det4.restype = C.c_double
+    a = np.asfortranarray(a, np.float64)
+    return lib().rh_det4(_p(a))

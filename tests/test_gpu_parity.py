@@ -220,6 +220,29 @@ def test_reference_known_answers(name, build):
     assert rel(r["u"], ro["u"]) < 1e-11, rel(r["u"], ro["u"])
 
 
+def test_largest_reference_deck():
+    """tet_625, the largest deck the reference ships (28,606 tets, p = 1,
+    1000 steps; no printed answer): the GPU run against the oracle's final
+    vector (1e-11) and errors (1e-11), in both operator builds, and the face
+    pairs stored once against the full rows."""
+    say("[tet_625]")
+    m = M.read_deck(os.path.join(DECKS, "tet_625"))
+    ro = oracle_for(m, m.degree).run(oracle.PROB_REF_MMS, m.dt, m.stop_time, want_u=True)
+    assert ro["nsteps"] == 1000
+    for build in ("reference", "factored"):
+        with L.Context(m, m.degree) as ctx:
+            ctx.set_build(build)
+            r = ctx.run(L.PROB_REF_MMS, m.dt, m.stop_time, want_u=True)
+            if build == "factored":
+                ctx.set_symmetric_storage(True)
+                r2 = ctx.run(L.PROB_REF_MMS, m.dt, m.stop_time, want_u=True)
+                assert rel(r2["u"], r["u"]) < 1e-11
+        assert r["t_final"] == ro["t_final"] and r["nsteps"] == 1000
+        assert rel(r["u"], ro["u"]) < 1e-11, (build, rel(r["u"], ro["u"]))
+        assert abs(r["err_l2"] / ro["err_l2"] - 1) < 1e-11
+        assert abs(r["err_dg"] / ro["err_dg"] - 1) < 1e-11
+
+
 @pytest.mark.parametrize("n,p,steps,dt", [(4, 2, 300, None), (3, 3, 200, None), (2, 4, 100, None),
                                          (5, 2, 1000, 1e-4)])
 def test_agglomerated_cube_wave_run(n, p, steps, dt):
@@ -765,3 +788,54 @@ def test_captured_solve_dropped_when_its_buffers_change():
         ctx.set_ricker(5.0)
         ctx.replay()
         ctx.sync()
+
+
+@pytest.mark.parametrize("name,p", [("grown", 2), ("grown", 3), ("tet_25", 1), ("cube3agg", 4)])
+def test_double_couple_sources_match_oracle(name, p):
+    """The SISM branch of MAKE_RHS_FREE: double couples (several sources, two
+    of them in one polyhedron) added to a RHS slot on the device against the
+    oracle's o_double_couples (1e-13), then 30 leapfrog steps from rest with a
+    Ricker time factor against the oracle's step loop (1e-11)."""
+    say(f"[double couple] {name} p={p}")
+    if name == "grown":
+        m = M.cube_mesh(3, dirichlet_faces={"x0", "x1", "y0", "y1", "z0"}, poly_size=(4, 12), seed=9)
+    elif name.startswith("tet_"):
+        m = M.read_deck(os.path.join(DECKS, name))
+    else:
+        m = M.cube_mesh(3, agglomerate=True)
+    rng = np.random.default_rng(21 + p)
+    pts = rng.uniform(0.2, 0.8, (4, 3))
+    o = oracle_for(m, p)
+    o.build_operator()
+    tets = o.locate(pts)
+    same = np.flatnonzero(m.tet_poly == m.tet_poly[tets[0]]) if m.tet_poly is not None else [tets[0]]
+    tets = np.append(tets, np.int32(same[-1]))  # a second source in the first one's polyhedron
+    strike, normal = rng.standard_normal((5, 3)), rng.standard_normal((5, 3))
+    m0 = 10 ** rng.uniform(-1, 1, 5)
+    rs = o.rhs(oracle.PROB_NULL)
+    rd = o.rhs(oracle.PROB_NULL) + o.double_couples(tets, strike, normal, m0)
+    f0, steps = 5.0, 30
+    dt = 0.2 * o.hmax() / (np.sqrt(3.0) * p * p) / 4
+    z = np.zeros(o.n_poly * o.n3)
+    t = 0.0
+    un = o.first_step(oracle.ricker(f0, 1.2 / f0, t), dt, z, z, rs, rd)
+    up = z
+    t = t + dt
+    for _ in range(steps):
+        up, un = un, o.step(oracle.ricker(f0, 1.2 / f0, t), dt, up, un, rs, rd)
+        t = t + dt
+    with L.Context(m, p) as ctx:
+        ctx.make_matrices_free()
+        ctx.set_ricker(f0)
+        ctx.make_rhs_free(L.PROB_NULL, _lib.SLOT_RHS_STAT)
+        ctx.make_rhs_free(L.PROB_RICKER, _lib.SLOT_RHS_DYN)
+        ctx.add_double_couples(None, strike, normal, m0, tets=tets.astype(np.int32))
+        assert rel(ctx.get_slot(_lib.SLOT_RHS_DYN), rd) < 1e-13
+        u1 = ctx.first_time_step_matrix_free(L.ricker(f0, 1.2 / f0, 0.0), dt, z, z)
+        ctx.set_slot(_lib.SLOT_U_PREV, z)
+        ctx.set_slot(_lib.SLOT_U_CUR, u1)
+        t_gpu = ctx.step_n(L.PROB_RICKER, steps, 0.0 + dt, dt)
+        u = ctx.get_slot(_lib.SLOT_U_CUR)
+    assert t_gpu == t
+    assert np.abs(un).max() > 0
+    assert rel(u, un) < 1e-11, rel(u, un)

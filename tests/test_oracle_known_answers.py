@@ -28,3 +28,18 @@ def test_known_answer(name):
     tol = 5e-12 if name == "tet_100" else 5e-13
     assert abs(r["err_l2"] / ka["err_l2"] - 1) < tol
     assert abs(r["err_dg"] / ka["err_dg"] - 1) < tol
+
+
+def test_largest_deck_converges():
+    """tet_625 (28,606 tets, p = 1), the largest deck the reference ships, has
+    no printed answer: its errors against tet_125's known answers fall at the
+    rates p = 1 admits (L2 ~ h^2, DG ~ h)."""
+    import math
+    d = deck.read_deck(os.path.join(GOLD, "decks", "tet_625"))
+    o = oracle.from_deck(d)
+    r = o.run(oracle.PROB_REF_MMS, d["dt"], d["stop_time"])
+    assert r["nsteps"] == 1000
+    ka = KA["tet_125"]
+    h = math.log(ka["hmax"] / o.hmax())
+    assert 1.8 < math.log(ka["err_l2"] / r["err_l2"]) / h < 2.6
+    assert 0.9 < math.log(ka["err_dg"] / r["err_dg"]) / h < 1.5

@@ -169,3 +169,34 @@ def test_cholesky_solve_and_fields():
     assert L.o_cholesky(2, np.array([1.0, 2.0, 2.0, 1.0]), np.zeros(4)) == 5
     for t in (0.0, 0.123, 1.0000000000000007):
         assert oracle.time_function(oracle.PROB_REF_MMS, t) == refnum.time_function(t)
+
+
+@pytest.mark.parametrize("name,tets", [("tet_5", [0, 13, 29]), ("tet_100", [3, 700])])
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_double_couple(name, tets, p):
+    """The SISM branch: det4, moment_density and MAKE_DOUBLE_COUPLE of the
+    oracle against the reference's own routines on real deck tets, then the
+    mesh-level loop (o_double_couples) against the element routine."""
+    d = deck.read_deck(os.path.join(DECKS, name))
+    o = oracle.from_deck(d, p=p)
+    cn = o.conn()
+    L = oracle.lib()
+    Np, n3q = o.Np, (p + 1) ** 3
+    rng = np.random.default_rng(p)
+    for t in tets:
+        x, y, z = _tet(d, t)
+        A = np.stack([x, y, z, np.ones(4)], axis=1)
+        assert L.o_det4(np.ascontiguousarray(A).ravel()) == refnum.det4(A)
+        vol = abs(refnum.det4(A)) / 6.0
+        rfk, rjd = refnum.jacobians(x, y, z)
+        _, rdphi = refnum.basis(p, _bbox_f(cn["bbox"][t]), rfk)
+        s, n, m0 = rng.standard_normal(3), rng.standard_normal(3), 10 ** rng.uniform(-2, 3)
+        mom = np.zeros(9)
+        L.o_moment_density(m0, vol, s, n, mom)
+        out = np.zeros(3 * Np)
+        L.o_double_couple(p, rjd, np.ascontiguousarray(rdphi).ravel(), mom, out)
+        ref = refnum.double_couple(p, rjd, rdphi, m0, vol, s, n)
+        assert rel(out, ref) < 1e-14, rel(out, ref)
+        # the mesh-level loop puts the same vector into the tet's row
+        rhs = o.double_couples([t], s, n, [m0])
+        assert rel(rhs.reshape(o.n_poly, -1)[t], ref) < 1e-14
