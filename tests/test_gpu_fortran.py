@@ -51,3 +51,84 @@ def test_fortran_reference_loop(name, mode, tmp_path):
     assert abs(vals["dg"] / ka["err_dg"] - 1) < tol
     if mode == "symmetric":
         assert "face pairs stored once" in r.stdout
+
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "dropin_harness")
+
+
+def _harness_or_skip():
+    if L.device_count() == 0:
+        pytest.skip("no HIP device")
+    if not os.path.exists(HARNESS):
+        pytest.skip("oracle/_ref/dropin_harness not built (needs the reference tree at build time)")
+
+
+@pytest.mark.parametrize("name", ["tet_5", "tet_25", "tet_100"])
+def test_reference_named_dropins(name, tmp_path):
+    """src/Lymph3D.f90:262-518 run by oracle/_ref/dropin_harness through the
+    module matrix_free_gpu (MAKE_MATRICES_FREE, host Cholesky of its M_loc,
+    MAKE_RHS_FREE with the reference's f_null/gd_null/gn_null and
+    f_time/gd/gn, COMPUTE_MODAL_COEFFICIENTS_FREE with ic_displacement,
+    ic_velocity, uex and uex_stat, FIRST_TIME_STEP_MATRIX_FREE and
+    TIME_STEP_MATRIX_FREE with the reference's arrays and time_function,
+    COMPUTE_ERROR_*), on a Mesh_Structure filled in the reference's
+    conventions: printed errors vs the known answers."""
+    from tests import dropin_files
+    _harness_or_skip()
+    ka = json.load(open(os.path.join(GOLD, "known_answers.json")))[name]
+    m = M.read_deck(os.path.join(GOLD, "decks", name))
+    f = tmp_path / "in.bin"
+    dropin_files.write(str(f), m, m.degree, m.dt, m.stop_time)
+    print(f"[dropin] {name}", flush=True)
+    r = subprocess.run([HARNESS, "run", str(f)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    vals = {}
+    for line in r.stdout.splitlines():
+        if line.startswith("steps"):
+            vals["steps"] = int(line.split()[1])
+        elif "NORM L2" in line:
+            vals["l2"] = float(line.split(":")[1])
+        elif "NORM DG" in line:
+            vals["dg"] = float(line.split(":")[1])
+    assert vals["steps"] == ka["steps"]
+    tol = 5e-12 if name == "tet_100" else 5e-13
+    assert abs(vals["l2"] / ka["err_l2"] - 1) < tol, vals
+    assert abs(vals["dg"] / ka["err_dg"] - 1) < tol, vals
+
+
+def test_reference_named_rhs_with_sism_sources(tmp_path):
+    """MAKE_RHS_FREE's drop-in with three SISM lines: both RHS arrays (the
+    reference adds the couples in every call, with the first line's strike,
+    normal and moment for all sources) and M_loc against the oracle."""
+    import numpy as np
+    from oracle import oracle
+    from tests import dropin_files
+    from tests.test_gpu_parity import oracle_for, rel
+    _harness_or_skip()
+    m = M.read_deck(os.path.join(GOLD, "decks", "tet_25"))
+    p = 2
+    rng = np.random.default_rng(4)
+    sism = [dict(point=rng.uniform(0.2, 0.8, 3), strike=rng.standard_normal(3),
+                 normal=rng.standard_normal(3), m0=10 ** rng.uniform(-1, 1)) for _ in range(3)]
+    f, out = tmp_path / "in.bin", tmp_path / "rhs.bin"
+    dropin_files.write(str(f), m, p, m.dt, m.stop_time, sism=sism)
+    r = subprocess.run([HARNESS, "rhs", str(f), str(out)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    o = oracle_for(m, p)
+    o.build_operator()
+    ne, n3, Np = o.n_poly, o.n3, o.Np
+    raw = np.fromfile(str(out), np.float64)
+    rs = raw[:ne * n3].reshape(n3, ne).T.ravel()
+    rd = raw[ne * n3:2 * ne * n3].reshape(n3, ne).T.ravel()
+    Mf = raw[2 * ne * n3:].reshape(Np, Np, 3, ne)
+    pts = np.array([s["point"] for s in sism])
+    tets = o.locate(pts)
+    k = len(sism)
+    cpl = o.double_couples(tets, np.tile(sism[0]["strike"], (k, 1)), np.tile(sism[0]["normal"], (k, 1)),
+                           np.full(k, sism[0]["m0"]))
+    assert np.abs(cpl).max() > 0
+    assert rel(rs, o.rhs(oracle.PROB_NULL) + cpl) < 1e-13
+    assert rel(rd, o.rhs(oracle.PROB_REF_MMS) + cpl) < 1e-13
+    Mo = o.mass()[0]
+    for i in range(3):
+        assert rel(Mf[:, :, i, :].transpose(2, 1, 0), Mo) < 1e-14
