@@ -17,8 +17,10 @@ roofline: algorithmic bytes per step B = 8 [n_blk (3Np)^2 + N_poly Np(Np+1)/2
 + 5 N_poly 3Np] (operator blocks, packed mass factor, five vectors) divided by
 the average step-kernel time, against the MI355X HBM peak of 8 TB/s.
 cpu_baseline: the C oracle (a scalar restatement of the reference's
-algorithm, OpenMP over polyhedra) timed on this host's cores on a bounded
-sample of the same workload (a smaller cube of the same structure).
+algorithm, OpenMP over elements) on the reference's own per-tet layout of a
+mesh with the same DOF count (22^3 cubes x 6 tets at p = 3), timed for a
+bounded number of steps on every core this job may use and on one thread,
+with the GPU's FP64 step on that same mesh beside it.
 
 Usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--n 40] [--p 3]
        python bench.py --degree-sweep     (p = 1..4 step table instead)
@@ -77,43 +79,88 @@ def pmc_traffic(n, p, kernel="k_leapfrog"):
     return None, None
 
 
+def host_threads():
+    """The cores this job may use: the CPU affinity set, capped by
+    OMP_NUM_THREADS when the environment sets it."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n, (os.cpu_count() or 1)
+
+
 def cpu_baseline(p, n_sample, target_s):
-    """Time the oracle's leapfrog step on an n_sample^3-cube sample of the
-    same structure, using the host's cores (OpenMP)."""
+    """The C oracle's FP64 leapfrog step (the reference's per-element loops,
+    OpenMP over elements) on the reference's own layout: n_sample^3 cubes x 6
+    tets, one block row per tet (the reference's stepper does not agglomerate;
+    22^3 x 6 = 63,888 tets at p = 3 has the bench workload's 3.83 M DOFs),
+    timed for about target_s on every core this job may use and again on one
+    thread.  The GPU's FP64 step on the same mesh is timed beside it."""
     from lymph3d_apsc_amd import mesh as M
     from oracle import oracle
-    m = M.cube_mesh(n_sample, agglomerate=True)
+    import lymph3d_apsc_amd as L
+    from lymph3d_apsc_amd import _lib
+    m = M.cube_mesh(n_sample, agglomerate=False)
     kinds = np.zeros(8, np.int32)
     kinds[[2, 3, 4]] = -1
     kinds[[5, 6, 7]] = -2
     space = np.zeros(8, np.int32)
     space[2] = space[5] = 1
-    threads = min(16, os.cpu_count() or 1)
+    threads, visible = host_threads()
     oracle.lib().o_num_threads(threads)
-    o = oracle.Oracle(m.xyz, m.tet_vert, m.tria_vert, m.tria_tag, kinds, space, p,
-                      tet_poly=m.tet_poly)
+    o = oracle.Oracle(m.xyz, m.tet_vert, m.tria_vert, m.tria_tag, kinds, space, p)
+    t0 = time.perf_counter()
     o.build_operator()
+    t_build = time.perf_counter() - t0
     rs = o.rhs(oracle.PROB_NULL)
     rd = o.rhs(oracle.PROB_CUBE_WAVE)
     u0 = o.project(oracle.FN_CUBE_U)
-    u1 = u0.copy()
     dt = 0.2 * (1.0 / n_sample) / (np.sqrt(3.0) * p * p)
-    # one untimed step, then as many as fit in target_s
-    u2 = o.step(0.5, dt, u0, u1, rs, rd)
-    t0 = time.perf_counter()
-    k = 0
-    while True:
-        u0, u1 = u1, o.step(0.5, dt, u0, u1, rs, rd)
-        k += 1
-        el = time.perf_counter() - t0
-        if el >= target_s:
-            break
     dofs = 3 * o.Np * o.n_poly
-    return dict(value=dofs * k / el, unit="DOF-updates/s", cores=threads, kind="port",
-                sample=f"C oracle, {n_sample}^3 cubes x 6 tets agglomerated per cube "
-                       f"({o.n_poly} polyhedra, p={p}, {dofs} DOFs), {k} FP64 leapfrog steps "
-                       f"in {el:.1f} s on {threads} OpenMP threads", steps=k, seconds=el,
-                _unused=float(u2[0]))
+    B = 8 * (o.n_blk * o.n3 ** 2 + o.n_poly * o.Np * (o.Np + 1) // 2 + 5 * o.n_poly * o.n3)
+
+    def timed(nthreads, seconds):
+        oracle.lib().o_num_threads(nthreads)
+        a, b = u0.copy(), u0.copy()
+        o.step(0.5, dt, a, b, rs, rd)  # one untimed step
+        k, t1 = 0, time.perf_counter()
+        while True:
+            a, b = b, o.step(0.5, dt, a, b, rs, rd)
+            k += 1
+            el = time.perf_counter() - t1
+            if el >= seconds:
+                return k, el, b
+
+    k_all, el_all, u_cpu = timed(threads, target_s)
+    k_one, el_one, _ = timed(1, target_s)
+    oracle.lib().o_num_threads(threads)
+    # the GPU's FP64 step on the same per-tet mesh and state
+    gpu = None
+    if L.device_count() > 0:
+        with L.Context(m, p) as ctx:
+            ctx.set_keep_adg(False)
+            ctx.set_build("factored")
+            ctx.make_matrices_free()
+            ctx.make_rhs_free(L.PROB_NULL, _lib.SLOT_RHS_STAT)
+            ctx.make_rhs_free(L.PROB_CUBE_WAVE, _lib.SLOT_RHS_DYN)
+            ctx.set_slot(_lib.SLOT_U_PREV, u0)
+            ctx.set_slot(_lib.SLOT_U_CUR, u0)
+            ctx.autotune(reps=5)
+            ctx.time_steps(L.PROB_CUBE_WAVE, 10, 0.0, dt)
+            ms = ctx.time_steps(L.PROB_CUBE_WAVE, 200, 0.0, dt) / 200
+        gpu = {"value": dofs / (ms * 1e-3), "ms_per_step": ms, "roofline_frac": B / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    sample = (f"C oracle on {n_sample}^3 cubes x 6 tets, one block row per tet as the reference stores "
+              f"it ({o.n_poly} tets, p={p}, {dofs} DOFs, {o.n_blk} blocks, {B / 1e9:.2f} GB per step): "
+              f"{k_all} FP64 leapfrog steps in {el_all:.1f} s on {threads} OpenMP threads "
+              f"(the cores this job may use: affinity set capped by OMP_NUM_THREADS; the host shows "
+              f"{visible}); {k_one} steps in {el_one:.1f} s on 1 thread; operator build {t_build:.1f} s")
+    return dict(value=dofs * k_all / el_all, unit="DOF-updates/s", cores=threads, kind="port",
+                sample=sample, steps=k_all, seconds=el_all, ms_per_step=el_all / k_all * 1e3,
+                value_1thread=dofs * k_one / el_one, ms_per_step_1thread=el_one / k_one * 1e3,
+                host_cpus_visible=visible, operator_build_s=t_build, bytes_per_step=B,
+                gpu_fp64_same_mesh=gpu,
+                gpu_over_cpu_same_mesh=(gpu["value"] / (dofs * k_all / el_all)) if gpu else None,
+                _unused=float(u_cpu[0]))
 
 
 def ensemble_leg(n_inst, steps, plain_sample, f0=5.0, seed0=1000):
@@ -144,6 +191,16 @@ def ensemble_leg(n_inst, steps, plain_sample, f0=5.0, seed0=1000):
                "step_bytes": g["ctx"].step_bytes_for(32)} for g in ens.groups]
     ens.close()
     del ens
+    # the same ensemble with the FP64 apply (graph, no 32-bit operator): what
+    # batching alone gains over the plain path, apart from the halved bytes
+    t0 = time.perf_counter()
+    ens64 = E.Ensemble(inst, dt, steps, every=1, f0=f0, apply_bits=64, autotune=True)
+    t_setup64 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tr64 = ens64.run()  # one replay
+    t_ens64 = time.perf_counter() - t0
+    ens64.close()
+    del ens64
     # plain path on a sample: FP64, one instance at a time, uncaptured
     sample = list(range(0, n_inst, max(1, n_inst // plain_sample)))[:plain_sample]
     t_plain, t_plain_setup, errs = 0.0, 0.0, []
@@ -166,6 +223,8 @@ def ensemble_leg(n_inst, steps, plain_sample, f0=5.0, seed0=1000):
         errs.append(float(np.linalg.norm(tr[k] - ref) / max(np.linalg.norm(ref), 1e-300)))
     plain_rate = len(sample) / t_plain
     ens_rate = n_inst / t_ens
+    errs64 = [float(np.linalg.norm(tr[k] - tr64[k]) / max(np.linalg.norm(tr64[k]), 1e-300))
+              for k in range(n_inst)]
     return {
         "value": ens_rate, "unit": "ensemble solves/s",
         "instances": n_inst, "steps": steps, "dt": dt, "receivers_per_instance": 64,
@@ -177,9 +236,20 @@ def ensemble_leg(n_inst, steps, plain_sample, f0=5.0, seed0=1000):
         "plain_fp64": {"value": plain_rate, "unit": "solves/s", "sample_instances": len(sample),
                        "seconds": t_plain, "setup_seconds": t_plain_setup},
         "speedup_vs_plain": ens_rate / plain_rate,
+        "solves_per_s_including_setup": n_inst / (t_setup + t_ens),
         "trace_rel_l2_vs_fp64_max": max(errs), "trace_rel_l2_vs_fp64_mean": float(np.mean(errs)),
+        "fp64_ensemble": {
+            "value": n_inst / t_ens64, "unit": "ensemble solves/s", "seconds_per_ensemble_solve": t_ens64,
+            "setup_s": t_setup64, "solves_per_s_including_setup": n_inst / (t_setup64 + t_ens64),
+            "speedup_vs_plain": (n_inst / t_ens64) / plain_rate,
+            "fp32_trace_rel_l2_vs_this_max": max(errs64),
+            "fp32_trace_rel_l2_vs_this_mean": float(np.mean(errs64)),
+            "note": "the same instances, graph and launches with the FP64 apply: the gain of "
+                    "batching alone; the FP32 ensemble's further gain is its halved bytes"},
         "note": "solve = zero state, first step, then `steps` leapfrog steps recording 64 "
-                "receivers per step; setup (operator build, RHS) timed separately",
+                "receivers per step; value excludes the one-time setup (operator build, "
+                "transforms, RHS, capture), which solves_per_s_including_setup adds for one "
+                "ensemble solve",
     }
 
 
@@ -327,8 +397,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=40, help="cubes per side")
     ap.add_argument("--p", type=int, default=3)
-    ap.add_argument("--cpu-sample", type=int, default=12, help="cubes per side of the CPU sample")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-sample", type=int, default=22,
+                    help="cubes per side of the CPU sample (per-tet layout)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="seconds per thread count")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-apply measurement")
     ap.add_argument("--no-pairs-once", action="store_true",
@@ -469,6 +540,17 @@ def main():
         ms32 = ctx.time_steps(prob, a.steps, t_start, dt)
         u32 = ctx.get_slot(_lib.SLOT_U_CUR)
         ms32_step = ms32 / a.steps
+        # every 32-bit stepping call transforms the states and RHS into the
+        # mass-orthonormal basis and back once: twice the steps from the same
+        # start give the per-step time without that per-call cost
+        ctx.set_apply_precision(64)
+        ctx.run(prob, dt, 1e30, max_steps=1 + a.warmup)
+        ctx.set_apply_precision(32)
+        if not a.no_autotune:
+            ctx.set_step_variant(ctx.chosen_variant)
+        ms32_2 = ctx.time_steps(prob, 2 * a.steps, t_start, dt)
+        ms32_kernel = (ms32_2 - ms32) / a.steps
+        l2_32_vs_64 = float(np.sqrt(ctx.compute_error_l2_matrix_free(u32, u)))
         B32 = ctx.step_bytes_for(32)
         traffic32, traffic32_src = pmc_traffic(a.n, p, "k_leapfrog32")
         res["fp32_apply"] = {
@@ -480,8 +562,13 @@ def main():
                          "traffic": traffic32, "traffic_source": traffic32_src,
                          "traffic_over_algorithmic": (traffic32 / B32) if traffic32 else None,
                          "bytes_per_step": B32, "kernel": f"k_leapfrog32<{p},0>"},
+            "ms_per_step_includes": f"the basis transforms of one stepping call of {a.steps} steps",
+            "ms_per_step_without_transforms": ms32_kernel,
             "rel_l2_diff_vs_fp64": float(np.linalg.norm(u32 - u) / np.linalg.norm(u)),
             "err_l2": float(np.sqrt(ctx.compute_error_l2_matrix_free(u32, uex))),
+            "l2_diff_vs_fp64": l2_32_vs_64,
+            "l2_diff_vs_fp64_over_fp64_err_l2": l2_32_vs_64 / e_l2,
+            "steps_from_rest": 1 + a.warmup + a.steps,
             "autotune": {"chosen": list(ch32),
                          "ms_per_step": {f"unr{k[0]}_w{k[2]}": v for k, v in tune32.items()}},
         }

@@ -204,6 +204,7 @@ class Context:
                                    C.byref(ch)), "lymph_autotune")
         table = {(int(unr[i]), int(xm[i]), int(wp[i])): float(ms[i]) for i in range(nv.value)}
         c = ch.value
+        self.chosen_variant = c  # the index set_step_variant takes
         return (int(unr[c]), int(xm[c]), int(wp[c])), table
 
     def set_step_variant(self, index):
