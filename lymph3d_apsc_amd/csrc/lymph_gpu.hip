@@ -399,8 +399,9 @@ int launch_step(LymphCtx* c, int mode, double g, double c1, double c2, const dou
                                 c->md.col, c->RM, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->order, dyn);
     } else if (c->Ks) {
       if constexpr (3 * np_of(P) <= 64 && (3 * np_of(P)) % 2 == 0) {
-        k_sym_products<P><<<step_grid(c->n_poly), 64, 0, c->stream>>>(
-            c->Ks, c->md.row_ptr, c->md.col, c->sidx, c->tdst, u, c->prod, c->n_poly, c->order);
+        const int n_s = c->n_poly + (c->n_blk - c->n_poly) / 2;  // stored blocks
+        k_sym_products<P><<<step_grid(n_s), 64, 0, c->stream>>>(
+            c->Ks, c->ssrc, c->blk_row, c->md.col, c->tdst, u, c->prod, n_s);
         if (int rc = check_launch("k_sym_products")) return rc;
         if (mode == 0)
           k_sym_update<P, 0><<<step_grid(c->n_poly), 64, 0, c->stream>>>(

@@ -1022,16 +1022,18 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog(const double* __restrict_
 // and one material on both traces of each face, K_QP = K_PQ^T to rounding, so
 // only the self blocks and K_PQ for Q > P are kept (Ks; sidx[b] >= 0 the
 // stored block of full-row block b, < 0 a block of the neighbour's row).
-// k_sym_products: one wave per polyhedron P streams its stored blocks once;
-// each gives P's product K_PQ u_Q (FMA-free chain over ascending columns, as
-// k_leapfrog) into prod[b], and a pair block also gives the neighbour's
-// product K_QP u_P = K_PQ^T u_P into prod[tdst[b]], the slot of block (Q, P)
-// of Q's row.  Lane i holds row i of K_PQ and u_P[i]; the products
-// K_PQ[i][c] u_P[i] go to LDS in three chunks of CW columns, lane (c, t) of a
-// chunk sums rows [t CW, (t+1) CW) of column c, and lane c gathers the three
-// partial sums.  k_sym_update: per polyhedron, the row's products summed in
-// block order, then the mass solves across the wave (solve_wave) and the
-// update of k_leapfrog.  p <= 3 (3Np <= 64 rows, whole column pairs).
+// k_sym_products: one wave per stored block K_PQ (ssrc[s] its full-row slot
+// b in P's row): P's product K_PQ u_Q (FMA-free chain over ascending columns,
+// as k_leapfrog) into prod[b], and for a pair block the neighbour's product
+// K_QP u_P = K_PQ^T u_P into prod[tdst[b]], the slot of block (Q, P) of Q's
+// row.  Lane i holds row i of K_PQ and u_P[i]; the products K_PQ[i][c] u_P[i]
+// go to LDS in three chunks of CW columns and lane c adds column c's
+// products over ascending rows: the same chain Q's own row would run over
+// K_QP's columns, so with K_QP = K_PQ^T exactly (the 32-bit copies) the
+// transposed product is the full-row product bit for bit.  Each wave's two
+// stores come after all of its loads.  k_sym_update: per polyhedron, the
+// row's products summed in block order, then the mass solves across the wave
+// and the update of k_leapfrog.  p <= 3 (3Np <= 64 rows, whole column pairs).
 // LDS ordering within one wave (the rocPRIM wave barrier: fences at
 // wavefront scope around a code-motion barrier); enough for a workgroup of
 // one wave, and it does not wait on the wave's global loads and stores.
@@ -1041,27 +1043,16 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The row's stored blocks are streamed in chunks of CW = 3Np/3 columns (CP
-// column pairs): a chunk gives the row product's terms for its columns (the
-// same ascending-column chain as k_leapfrog) and, for a pair block, the
-// transposed products of exactly those columns.  Chunks are software
-// pipelined: the loads of the next chunk -- of this block or the next stored
-// one, with that block's neighbour coefficients -- are issued before the
-// current chunk's row product and LDS transpose run, so the wave keeps a
-// chunk of loads in flight through its compute (two chunks of registers
-// instead of a whole block).  Same operations in the same order as one pass
-// per block, so the products are the same bits.
 template <int P>
 __global__ __launch_bounds__(64) void k_sym_products(const double* __restrict__ Ks,
-                                                     const int* __restrict__ row_ptr,
+                                                     const int* __restrict__ ssrc,
+                                                     const int* __restrict__ blk_row,
                                                      const int* __restrict__ col,
-                                                     const int* __restrict__ sidx,
                                                      const int* __restrict__ tdst,
                                                      const double* __restrict__ u,
-                                                     double* __restrict__ prod, int n_poly,
-                                                     const int* __restrict__ order) {
+                                                     double* __restrict__ prod, int n_stored) {
   using D = Dims<P>;
-  constexpr int N3 = D::N3;
+  constexpr int N3 = D::N3, NPAIR = D::NPAIR;
   static_assert(N3 <= 64 && N3 % 2 == 0, "one lane per row, whole column pairs");
   constexpr int CW = N3 / 3, CP = CW / 2;
   static_assert(CW % 2 == 0, "whole column pairs per chunk");
@@ -1069,98 +1060,53 @@ __global__ __launch_bounds__(64) void k_sym_products(const double* __restrict__ 
   __shared__ double sT[CW * LDT];
   const int lane = threadIdx.x;
   const bool row = lane < N3;
-  const int Pp = step_poly(order);
-  if (Pp >= n_poly) return;  // padding workgroups of the last XCD share
-  const int b0 = row_ptr[Pp], b1 = row_ptr[Pp + 1];
-  const double up = row ? u[(size_t)Pp * N3 + lane] : 0.0;
-  const int cl = lane % CW, t = lane / CW;
-  // blocks of the row in batches of 64, one block's indices per lane
-  for (int bb = b0; bb < b1; bb += 64) {
-    const int nb = b1 - bb < 64 ? b1 - bb : 64;
-    const bool mine = lane < nb;
-    const int lcb = mine ? col[bb + lane] : 0;
-    const int lsi = mine ? sidx[bb + lane] : -1;
-    const int ltd = mine ? tdst[bb + lane] : 0;
-    const unsigned long long stored = __ballot(lsi >= 0);  // blocks this wave streams
-    if (!stored) continue;
-    // load cursor (next chunk to load) and process cursor (next to compute):
-    // the lowest block left in the mask, chunk h of it
-    unsigned long long ml = stored, mp = stored;
-    int hl = 0, hp = 0;
-    double xnext = 0.0, xcur = 0.0, s = 0.0, st = 0.0;
-    double2 A[CP], B[CP];
-    auto load_chunk = [&](double2(&kv)[CP]) {
-      const int j = __builtin_ctzll(ml);
-      const int si = __builtin_amdgcn_readlane(lsi, j);
-      if (hl == 0) {  // a new block: its neighbour's coefficients come with its first chunk
-        const int cb = __builtin_amdgcn_readlane(lcb, j);
-        xnext = row ? u[(size_t)cb * N3 + lane] : 0.0;
-      }
-      const double* __restrict__ Kb = Ks + (size_t)si * D::BS + (size_t)hl * CP * N3 * 2;
+  const int sb = step_poly();  // XCD-aware: each XCD streams a contiguous share of the rows
+  if (sb >= n_stored) return;  // padding workgroups of the last XCD share
+  const int b = ssrc[sb];
+  const int Pp = blk_row[b], Qq = col[b];
+  const int td = tdst[b];  // -1: the self block
+  const double* __restrict__ Kb = Ks + (size_t)sb * D::BS;
+  double2 kv[NPAIR];
 #pragma unroll
-      for (int k = 0; k < CP; ++k)
-        kv[k] = row ? load_stream(Kb + ((size_t)k * N3 + lane) * 2) : make_double2(0.0, 0.0);
-      if (++hl == 3) {
-        hl = 0;
-        ml &= ml - 1;
-      }
-    };
-    auto process_chunk = [&](const double2(&kv)[CP]) {
-      const int j = __builtin_ctzll(mp);
-      const int h = hp;
-      const int cb = __builtin_amdgcn_readlane(lcb, j);
-      if (h == 0) {
-        xcur = xnext;
-        s = 0.0;
-      }
+  for (int c2i = 0; c2i < NPAIR; ++c2i)
+    kv[c2i] = row ? load_stream(Kb + ((size_t)c2i * N3 + lane) * 2) : make_double2(0.0, 0.0);
+  const double xl = row ? u[(size_t)Qq * N3 + lane] : 0.0;
+  double s = 0.0;
+#pragma unroll
+  for (int c2i = 0; c2i < NPAIR; ++c2i) {
+    const double x0 = lane_bcast(xl, 2 * c2i), x1 = lane_bcast(xl, 2 * c2i + 1);
+    s = s + kv[c2i].x * x0;
+    s = s + kv[c2i].y * x1;
+  }
+  if (td < 0) {
+    if (row) prod[(size_t)b * N3 + lane] = s;
+    return;
+  }
+  // P's own coefficients, loaded once the row product has freed xl's registers
+  const double up = row ? u[(size_t)Pp * N3 + lane] : 0.0;
+  double st = 0.0;
+#pragma unroll
+  for (int h = 0; h < 3; ++h) {
+    wave_lds_sync();  // the previous chunk's reads are done
+    if (row) {
 #pragma unroll
       for (int k = 0; k < CP; ++k) {
-        const int c2 = h * CP + k;
-        const double x0 = lane_bcast(xcur, 2 * c2), x1 = lane_bcast(xcur, 2 * c2 + 1);
-        s = s + kv[k].x * x0;
-        s = s + kv[k].y * x1;
+        sT[(2 * k) * LDT + lane] = kv[h * CP + k].x * up;
+        sT[(2 * k + 1) * LDT + lane] = kv[h * CP + k].y * up;
       }
-      if (h == 2 && row) prod[(size_t)(bb + j) * N3 + lane] = s;
-      if (cb != Pp) {  // a pair block: the neighbour's product K_PQ^T u_P, chunk h
-        wave_lds_sync();  // the previous chunk's reads are done
-        if (row) {
-#pragma unroll
-          for (int k = 0; k < CP; ++k) {
-            sT[(2 * k) * LDT + lane] = kv[k].x * up;
-            sT[(2 * k + 1) * LDT + lane] = kv[k].y * up;
-          }
-        }
-        wave_lds_sync();
-        double v = 0.0;
-        if (row) {
-          v = sT[cl * LDT + t * CW];
-#pragma unroll
-          for (int k = 1; k < CW; ++k) v = v + sT[cl * LDT + t * CW + k];
-        }
-        const int src = (lane - h * CW) & 63;  // lanes outside the chunk discard
-        const double v0 = __shfl(v, src % CW, 64);
-        const double v1 = __shfl(v, src % CW + CW, 64);
-        const double v2 = __shfl(v, src % CW + 2 * CW, 64);
-        if (lane >= h * CW && lane < (h + 1) * CW) st = (v0 + v1) + v2;
-        if (h == 2 && row) {
-          const int td = __builtin_amdgcn_readlane(ltd, j);
-          prod[(size_t)td * N3 + lane] = st;
-        }
-      }
-      if (++hp == 3) {
-        hp = 0;
-        mp &= mp - 1;
-      }
-    };
-    load_chunk(A);
-    while (true) {
-      if (ml) load_chunk(B);
-      process_chunk(A);
-      if (!mp) break;
-      if (ml) load_chunk(A);
-      process_chunk(B);
-      if (!mp) break;
     }
+    wave_lds_sync();
+    if (lane >= h * CW && lane < (h + 1) * CW) {
+      const double* __restrict__ colp = sT + (lane - h * CW) * LDT;
+      double v = 0.0;
+#pragma unroll
+      for (int r = 0; r < N3; ++r) v = v + colp[r];
+      st = v;
+    }
+  }
+  if (row) {
+    prod[(size_t)b * N3 + lane] = s;
+    prod[(size_t)td * N3 + lane] = st;
   }
 }
 
@@ -1261,8 +1207,14 @@ __global__ __launch_bounds__(64) void k_sym_update(const double* __restrict__ pr
 // solves): ~K_QP = R_Q^-T K_PQ^T R_P^-1 = ~K_PQ^T, so the same stored list
 // applies to the FP32 blocks (column quads, 3Np a multiple of 12 so that each
 // of the three chunks is whole quads: p = 3).  Products and per-block sums in
-// FP32 as k_leapfrog32, written as doubles; k_sym_update32 adds them in block
-// order in FP64 and applies k_leapfrog32's update.
+// FP32 as k_leapfrog32, written as doubles; the transposed product of a
+// column runs over ascending rows, the chain Q's full row would run, so where
+// the two rounded blocks are exact transposes the products equal the full
+// rows' bit for bit.  k_sym_update32 adds them in block order in FP64 and
+// applies k_leapfrog32's update.  (Summing the transposed products in three
+// row groups instead made the state drift from FP64 faster over long runs,
+// increasingly with the mesh size: 7x the full rows' drift after 1000 steps
+// at 40^3.)
 template <int P>
 __global__ __launch_bounds__(64) void k_sym_products32(const float* __restrict__ Ks,
                                                        const int* __restrict__ row_ptr,
@@ -1286,7 +1238,6 @@ __global__ __launch_bounds__(64) void k_sym_products32(const float* __restrict__
   const int lsi = lane < b1 - b0 ? sidx[b0 + lane] : 0;
   const int ltd = lane < b1 - b0 ? tdst[b0 + lane] : 0;
   const float up = row ? (float)u[(size_t)Pp * N3 + lane] : 0.0f;
-  const int cl = lane % CW, t = lane / CW;
   for (int b = b0; b < b1; ++b) {
     const int j = b - b0;
     const int si = j < 64 ? __builtin_amdgcn_readlane(lsi, j) : __builtin_amdgcn_readfirstlane(sidx[b]);
@@ -1328,17 +1279,13 @@ __global__ __launch_bounds__(64) void k_sym_products32(const float* __restrict__
         }
       }
       wave_lds_sync();
-      float v = 0.0f;
-      if (row) {
-        v = sT[cl * LDT + t * CW];
+      if (lane >= h * CW && lane < (h + 1) * CW) {  // column's products over ascending rows
+        const float* __restrict__ colp = sT + (lane - h * CW) * LDT;
+        float v = 0.0f;
 #pragma unroll
-        for (int k = 1; k < CW; ++k) v = v + sT[cl * LDT + t * CW + k];
+        for (int r = 0; r < N3; ++r) v = v + colp[r];
+        st = v;
       }
-      const int src = (lane - h * CW) & 63;  // lanes outside the chunk discard
-      const float v0 = __shfl(v, src % CW, 64);
-      const float v1 = __shfl(v, src % CW + CW, 64);
-      const float v2 = __shfl(v, src % CW + 2 * CW, 64);
-      if (lane >= h * CW && lane < (h + 1) * CW) st = (v0 + v1) + v2;
     }
     if (row) prod[(size_t)td * N3 + lane] = (double)st;
   }

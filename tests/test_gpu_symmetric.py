@@ -195,7 +195,10 @@ def test_fp32_apply_with_pairs_stored_once(name, first):
         assert ctx.device_bytes - bytes32 >= n_st * (3 * o.Np) ** 2 * (8 + 4)
         sym32 = ctx.time_step_matrix_free(g, dt, u0, u1)
         assert rel(acc(sym32), acc(ref)) < 1e-5, rel(acc(sym32), acc(ref))
-        assert rel(acc(sym32), acc(full32)) < 1e-5, rel(acc(sym32), acc(full32))
+        # the transposed products run the full rows' chain over the same rounded
+        # blocks: the step equals the full-row 32-bit step (to the rare entry
+        # whose two FP64 sides round to different floats)
+        assert rel(acc(sym32), acc(full32)) < 1e-12, rel(acc(sym32), acc(full32))
         # device loop and first step through the same kernels
         ctx.set_slot(_lib.SLOT_U_PREV, u0)
         ctx.set_slot(_lib.SLOT_U_CUR, u1)
@@ -250,8 +253,9 @@ def test_fp32_pairs_once_is_opt_in():
 
 def test_fp32_pairs_once_long_run_drift():
     """1000 steps of the 32-bit apply from the same start on the 4^3
-    agglomerated cube at p = 3: the pairs-once state stays within a small
-    factor of the full rows' distance from the FP64 state."""
+    agglomerated cube at p = 3: the pairs-once state is as far from the FP64
+    state as the full rows' (the products are the full rows' bit for bit but
+    for rare entries whose two FP64 sides round apart)."""
     say("[symmetric] fp32 pairs once, 1000-step drift")
     m, p = M.cube_mesh(4, agglomerate=True), 3
     dt = 0.2 * (1 / 4) / (np.sqrt(3.0) * p * p)
@@ -274,4 +278,4 @@ def test_fp32_pairs_once_long_run_drift():
     d_full = np.linalg.norm(res["full32"] - res["fp64"]) / nrm
     d_pairs = np.linalg.norm(res["pairs32"] - res["fp64"]) / nrm
     say(f"  drift vs fp64: full rows {d_full:.3e}, pairs once {d_pairs:.3e}")
-    assert d_full < 1e-4 and d_pairs < 3 * d_full + 1e-7, (d_full, d_pairs)
+    assert d_full < 1e-4 and abs(d_pairs - d_full) < 0.02 * d_full, (d_full, d_pairs)
