@@ -512,10 +512,12 @@ def main():
         us = ctx.get_slot(_lib.SLOT_U_CUR)
         n3, npk = 3 * ctx.Np, ctx.Np * (ctx.Np + 1) // 2
         n_st = ctx.n_poly + (ctx.n_blk - ctx.n_poly) // 2
-        # stored blocks read once, per-block products written and read back, mass factor, five
-        # vectors, three int32 indices per block
-        Bs = 8 * (n_st * n3 * n3 + 2 * ctx.n_blk * n3 + ctx.n_poly * npk + 5 * ctx.n_poly * n3) \
-            + 12 * ctx.n_blk
+        # stored pair blocks read once and the self blocks' upper triangles (column pair k's
+        # rows 0..2k+1), per-block products written and read back, the mass factor's inverse,
+        # five vectors, three int32 indices per block
+        n_self_e = (n3 // 2) * (n3 // 2 + 1) * 2
+        Bs = 8 * ((n_st - ctx.n_poly) * n3 * n3 + ctx.n_poly * n_self_e + 2 * ctx.n_blk * n3
+                  + ctx.n_poly * npk + 5 * ctx.n_poly * n3) + 12 * ctx.n_blk
         traffic_s, traffic_s_src = pmc_traffic(a.n, p, "k_sym")
         res["fp64_pairs_once"] = {
             "value": dofs / (mss * 1e-3), "unit": "DOF-updates/s", "ms_per_step": mss,

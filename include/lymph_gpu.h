@@ -268,12 +268,12 @@ int lymph_release_fp64_operator(LymphCtx* ctx);
  * SIPG (theta = -1) and one material on both traces of every face -- the
  * reference forces rho = lambda = mu = 1, src/Lymph3D.f90:196-203 -- the
  * blocks satisfy K_QP = K_PQ^T up to the rounding of the two sides' face
- * quadratures.  on = 1 checks every pair on the device (largest
- * max |K_PQ - K_QP^T| / max |K_PQ| to *asym when asym is not null) and, when
- * it is at most 1e-12, keeps the self blocks and K_PQ for Q > P only; the
- * FP64 step (lymph_step*, lymph_first_step*, lymph_step_n, lymph_run, captured
- * solves) then reads about 42% fewer operator bytes at p = 3 and applies
- * K_QP as the transpose of P's copy.  An operator that fails the check
+ * quadratures.  on = 1 checks every pair and self block on the device
+ * (largest max |K_PQ - K_QP^T| / max |K_PQ| to *asym when asym is not null)
+ * and, when it is at most 1e-12, keeps K_PQ for Q > P and the upper triangle
+ * of each self block only; the FP64 step (lymph_step*, lymph_first_step*,
+ * lymph_step_n, lymph_run, captured solves) then reads about half the
+ * operator bytes at p = 3 and applies K_QP as the transpose of P's copy.  An operator that fails the check
  * returns LYMPH_ERR_ARG and the full rows stay in use.  Needs p <= 3 and the
  * FP64 blocks (not released); lymph_build_operator drops the stored pairs;
  * on = LYMPH_SYM_OFF frees them.  Only with on = LYMPH_SYM_FP64_FP32 (p = 3)

@@ -121,6 +121,7 @@ struct LymphCtx {
   int* sidx = nullptr;      // per block of the full rows: stored block, ~ = transposed
   int* tdst = nullptr;      // stored pair block -> its (Q, P) slot of Q's row, else -1
   int* ssrc = nullptr;      // full-row slot of each stored block (gather list)
+  double* Sinv = nullptr;   // R^-1 of each mass block, packed like R (pairs-once FP64 update)
   float* Ks32 = nullptr;    // the stored blocks of the 32-bit operator (p = 3)
   size_t ks32_bytes = 0;
   bool sym32_wanted = false;  // LYMPH_SYM_FP64_FP32: the 32-bit apply stores pairs once too
@@ -405,11 +406,11 @@ int launch_step(LymphCtx* c, int mode, double g, double c1, double c2, const dou
         if (int rc = check_launch("k_sym_products")) return rc;
         if (mode == 0)
           k_sym_update<P, 0><<<step_grid(c->n_poly), 64, 0, c->stream>>>(
-              c->prod, c->md.row_ptr, c->RM, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->n_poly,
+              c->prod, c->md.row_ptr, c->Sinv, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->n_poly,
               c->order);
         else
           k_sym_update<P, 1><<<step_grid(c->n_poly), 64, 0, c->stream>>>(
-              c->prod, c->md.row_ptr, c->RM, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->n_poly,
+              c->prod, c->md.row_ptr, c->Sinv, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->n_poly,
               c->order);
       } else {
         return fail(LYMPH_ERR_STATE, "symmetric storage needs p <= 3");
@@ -461,7 +462,7 @@ void drop_graph(LymphCtx* c) {
 
 // Drop the once-stored face pairs (operator rebuilt, storage switched off).
 int sym_off(LymphCtx* c) {
-  if (!c->Ks && !c->sidx && !c->tdst && !c->prod && !c->ssrc && !c->Ks32) return 0;
+  if (!c->Ks && !c->sidx && !c->tdst && !c->prod && !c->ssrc && !c->Ks32 && !c->Sinv) return 0;
   HIP_TRY(hipStreamSynchronize(c->stream));
   drop_graph(c);
   if (c->ssrc) HIP_TRY(hipFree(c->ssrc));
@@ -474,6 +475,8 @@ int sym_off(LymphCtx* c) {
   if (c->sidx) HIP_TRY(hipFree(c->sidx));
   if (c->tdst) HIP_TRY(hipFree(c->tdst));
   if (c->prod) HIP_TRY(hipFree(c->prod));
+  if (c->Sinv) HIP_TRY(hipFree(c->Sinv));
+  c->Sinv = nullptr;
   c->Ks = nullptr;
   c->sidx = nullptr;
   c->tdst = nullptr;
@@ -485,19 +488,24 @@ int sym_off(LymphCtx* c) {
 
 int sym32_build(LymphCtx* c);
 
-// Store every face pair once: check K_QP = K_PQ^T pair by pair on the device
-// (largest ratio to *asym), then gather the self blocks and the blocks of
-// neighbours Q > P into Ks and index the full rows into it.
+// Store every face pair once: check K_QP = K_PQ^T pair by pair and K_PP =
+// K_PP^T on the device (largest ratio to *asym), then gather the self blocks
+// (upper triangles) and the blocks of neighbours Q > P into Ks, index the full
+// rows into it, and form S = R^-1 of each mass block for the update.
 int sym_on(LymphCtx* c, double tol, double* asym) {
   const int np = c->n_poly;
   drop_graph(c);  // a captured solve would keep stepping the full rows
   const std::vector<int>& rp = c->h_row_ptr;
   const std::vector<int>& cl = c->h_col;
-  std::vector<int> partner(c->n_blk, -1), sid(c->n_blk, -1), src, pb, pt;
+  std::vector<int> partner(c->n_blk, -1), sid(c->n_blk, -1), src, self_s, pb, pt;
   for (int P = 0; P < np; ++P)
     for (int b = rp[P]; b < rp[P + 1]; ++b) {
       const int Q = cl[b];
-      if (Q == P) continue;
+      if (Q == P) {
+        pb.push_back(b);
+        pt.push_back(b);
+        continue;
+      }
       for (int b2 = rp[Q]; b2 < rp[Q + 1]; ++b2)
         if (cl[b2] == P) {
           partner[b] = b2;
@@ -513,6 +521,7 @@ int sym_on(LymphCtx* c, double tol, double* asym) {
     for (int b = rp[P]; b < rp[P + 1]; ++b)
       if (cl[b] >= P) {
         sid[b] = (int)src.size();
+        if (cl[b] == P) self_s.push_back(sid[b]);
         src.push_back(b);
       }
   std::vector<int> sx(c->n_blk), td(c->n_blk, -1);
@@ -574,7 +583,29 @@ int sym_on(LymphCtx* c, double tol, double* asym) {
     k_gather_blocks<double><<<4096, 256, 0, c->stream>>>(c->K, c->Ks, dsrc, n_s, bs);
     rc = check_launch("k_gather_blocks");
   }
+  // the self blocks' upper triangles over their slots' heads, and S = R^-1
+  const int n_self = (int)self_s.size();
+  const size_t npk = (size_t)c->Np * (c->Np + 1) / 2;
+  int* dself = nullptr;
+  if (!rc) rc = dalloc(&dself, n_self);
+  if (!rc) rc = dalloc(&c->Sinv, (size_t)np * npk);
+  if (!rc && hipMemcpy(dself, self_s.data(), n_self * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(LYMPH_ERR_HIP, "copy of the self-block list failed");
+  if (!rc)
+    rc = dispatch_p(c->p, [&](auto pc) -> int {
+      constexpr int P = decltype(pc)::value;
+      if constexpr ((3 * np_of(P)) % 2 == 0) {
+        const size_t ne = (size_t)n_self * self_upper_off(3 * np_of(P) / 2);
+        k_pack_self_upper<P><<<grid1(ne, 256), 256, 0, c->stream>>>(c->K, c->Ks, dsrc, dself, n_self);
+        if (int r = check_launch("k_pack_self_upper")) return r;
+        k_inv_upper<P><<<grid1((size_t)np * c->Np, 64), 64, 0, c->stream>>>(c->RM, c->Sinv, np);
+        return check_launch("k_inv_upper");
+      } else {
+        return fail(LYMPH_ERR_STATE, "symmetric storage needs p <= 3");
+      }
+    });
   if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(LYMPH_ERR_HIP, "gather failed");
+  if (dself) (void)hipFree(dself);
   c->ssrc = dsrc;  // kept for the 32-bit blocks (freed with the stored pairs)
   if (rc) {
     c->ks_bytes = 0;
@@ -582,7 +613,7 @@ int sym_on(LymphCtx* c, double tol, double* asym) {
     return rc;
   }
   c->ks_bytes = (size_t)n_s * bs * sizeof(double) + (2 * (size_t)c->n_blk + n_s) * sizeof(int) +
-                (size_t)c->n_blk * c->n3 * sizeof(double);
+                (size_t)c->n_blk * c->n3 * sizeof(double) + (size_t)np * npk * sizeof(double);
   c->bytes += c->ks_bytes;
   // the 32-bit operator's copy, if asked for and it exists already; a failure
   // leaves no stored pairs switched on
@@ -736,6 +767,7 @@ int lymph_ctx_destroy(LymphCtx* c) {
   if (c->prod) (void)hipFree(c->prod);
   if (c->ssrc) (void)hipFree(c->ssrc);
   if (c->Ks32) (void)hipFree(c->Ks32);
+  if (c->Sinv) (void)hipFree(c->Sinv);
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   for (void* p : c->owned) (void)hipFree(p);

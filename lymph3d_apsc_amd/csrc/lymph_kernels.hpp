@@ -1021,7 +1021,10 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog(const double* __restrict_
 // Step with every face pair's block stored once, in two launches.  With SIPG
 // and one material on both traces of each face, K_QP = K_PQ^T to rounding, so
 // only the self blocks and K_PQ for Q > P are kept (Ks; sidx[b] >= 0 the
-// stored block of full-row block b, < 0 a block of the neighbour's row).
+// stored block of full-row block b, < 0 a block of the neighbour's row), and
+// of each self block (K_PP = K_PP^T to rounding) only its upper triangle:
+// column pair c2i's rows 0..2c2i+1 packed from entry 2 c2i (c2i + 1)
+// (self_upper_off), half the bytes of the full block.
 // k_sym_products: one wave per stored block K_PQ (ssrc[s] its full-row slot
 // b in P's row): P's product K_PQ u_Q (FMA-free chain over ascending columns,
 // as k_leapfrog) into prod[b], and for a pair block the neighbour's product
@@ -1030,10 +1033,15 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog(const double* __restrict_
 // go to LDS in three chunks of CW columns and lane c adds column c's
 // products over ascending rows: the same chain Q's own row would run over
 // K_QP's columns, so with K_QP = K_PQ^T exactly (the 32-bit copies) the
-// transposed product is the full-row product bit for bit.  Each wave's two
-// stores come after all of its loads.  k_sym_update: per polyhedron, the
-// row's products summed in block order, then the mass solves across the wave
-// and the update of k_leapfrog.  p <= 3 (3Np <= 64 rows, whole column pairs).
+// transposed product is the full-row product bit for bit.  A self block's
+// product is its upper triangle's row product plus the transposed product
+// of its strict upper triangle, both with u_P.  Each wave's stores come after
+// all of its loads.  k_sym_update: per polyhedron, the row's products summed
+// in block order, then M^-1 = S S^T with S = R^-1 (packed like R) as two
+// triangular products through LDS, and the update of k_leapfrog.  p <= 3
+// (3Np <= 64 rows, whole column pairs).
+__host__ __device__ constexpr int self_upper_off(int c2i) { return 2 * c2i * (c2i + 1); }
+
 // LDS ordering within one wave (the rocPRIM wave barrier: fences at
 // wavefront scope around a code-motion barrier); enough for a workgroup of
 // one wave, and it does not wait on the wave's global loads and stores.
@@ -1056,6 +1064,7 @@ __global__ __launch_bounds__(64) void k_sym_products(const double* __restrict__ 
   static_assert(N3 <= 64 && N3 % 2 == 0, "one lane per row, whole column pairs");
   constexpr int CW = N3 / 3, CP = CW / 2;
   static_assert(CW % 2 == 0, "whole column pairs per chunk");
+  static_assert(self_upper_off(NPAIR) <= D::BS, "packed self block fits its slot");
   constexpr int LDT = N3 + 1;  // odd stride: a column read spreads over the banks
   __shared__ double sT[CW * LDT];
   const int lane = threadIdx.x;
@@ -1064,12 +1073,17 @@ __global__ __launch_bounds__(64) void k_sym_products(const double* __restrict__ 
   if (sb >= n_stored) return;  // padding workgroups of the last XCD share
   const int b = ssrc[sb];
   const int Pp = blk_row[b], Qq = col[b];
-  const int td = tdst[b];  // -1: the self block
+  const int td = tdst[b];  // -1: the self block (upper triangle stored)
+  const bool self = td < 0;
   const double* __restrict__ Kb = Ks + (size_t)sb * D::BS;
   double2 kv[NPAIR];
 #pragma unroll
-  for (int c2i = 0; c2i < NPAIR; ++c2i)
-    kv[c2i] = row ? load_stream(Kb + ((size_t)c2i * N3 + lane) * 2) : make_double2(0.0, 0.0);
+  for (int c2i = 0; c2i < NPAIR; ++c2i) {
+    const bool ld = row && (!self || lane <= 2 * c2i + 1);
+    const int off = self ? self_upper_off(c2i) + 2 * lane : (c2i * N3 + lane) * 2;
+    kv[c2i] = ld ? load_stream(Kb + off) : make_double2(0.0, 0.0);
+    if (self && lane == 2 * c2i + 1) kv[c2i].x = 0.0;  // K_PP[lane][lane - 1]: below the diagonal
+  }
   const double xl = row ? u[(size_t)Qq * N3 + lane] : 0.0;
   double s = 0.0;
 #pragma unroll
@@ -1078,12 +1092,9 @@ __global__ __launch_bounds__(64) void k_sym_products(const double* __restrict__ 
     s = s + kv[c2i].x * x0;
     s = s + kv[c2i].y * x1;
   }
-  if (td < 0) {
-    if (row) prod[(size_t)b * N3 + lane] = s;
-    return;
-  }
-  // P's own coefficients, loaded once the row product has freed xl's registers
-  const double up = row ? u[(size_t)Pp * N3 + lane] : 0.0;
+  // P's own coefficients (the self block's are xl), loaded once the row
+  // product has freed registers
+  const double up = self ? xl : (row ? u[(size_t)Pp * N3 + lane] : 0.0);
   double st = 0.0;
 #pragma unroll
   for (int h = 0; h < 3; ++h) {
@@ -1094,6 +1105,9 @@ __global__ __launch_bounds__(64) void k_sym_products(const double* __restrict__ 
         sT[(2 * k) * LDT + lane] = kv[h * CP + k].x * up;
         sT[(2 * k + 1) * LDT + lane] = kv[h * CP + k].y * up;
       }
+      // the self block's diagonal is in its row product only (a wave's LDS
+      // writes land in program order)
+      if (self && lane >= h * CW && lane < (h + 1) * CW) sT[(lane - h * CW) * LDT + lane] = 0.0;
     }
     wave_lds_sync();
     if (lane >= h * CW && lane < (h + 1) * CW) {
@@ -1105,53 +1119,38 @@ __global__ __launch_bounds__(64) void k_sym_products(const double* __restrict__ 
     }
   }
   if (row) {
-    prod[(size_t)b * N3 + lane] = s;
-    prod[(size_t)td * N3 + lane] = st;
+    if (self) {
+      prod[(size_t)b * N3 + lane] = s + st;
+    } else {
+      prod[(size_t)b * N3 + lane] = s;
+      prod[(size_t)td * N3 + lane] = st;
+    }
   }
 }
 
-// M^-1 w = R^-1 R^-T w across one wave as solve_wave, with each lane's
-// entries of R in registers (E[j] = R_{j,ii} below the diagonal for the
-// forward sweep, R_{ii,j} above it for the backward one) and the pivots as
-// products with the reciprocal of the lane's own diagonal entry, broadcast
-// with v_readlane: each of the 2 NP dependent steps is a multiply, three lane
-// reads and an FMA instead of a division, a cross-lane permute and two LDS
-// reads.  Results differ from solve_wave by rounding only.
-template <int NP>
-__device__ __forceinline__ double solve_wave_rcp(const double* __restrict__ R, double w, int lane) {
-  const bool row = lane < 3 * NP;
-  const int cc = row ? lane / NP : 0, ii = lane % NP;
-  double E[NP];
-#pragma unroll
-  for (int j = 0; j < NP; ++j) E[j] = R[ii > j ? pk_off(NP, j) + (ii - j) : pk_off(NP, ii) + (j - ii)];
-  const double rinv = 1.0 / R[pk_off(NP, ii)];
-#pragma unroll
-  for (int j = 0; j < NP; ++j) {
-    const double yv = w * rinv;
-    const double y0 = lane_bcast(yv, j), y1 = lane_bcast(yv, NP + j), y2 = lane_bcast(yv, 2 * NP + j);
-    const double y = cc == 0 ? y0 : (cc == 1 ? y1 : y2);
-    if (ii == j)
-      w = y;
-    else if (ii > j)
-      w = w - E[j] * y;
-  }
-#pragma unroll
-  for (int j = NP - 1; j >= 0; --j) {
-    const double xv = w * rinv;
-    const double x0 = lane_bcast(xv, j), x1 = lane_bcast(xv, NP + j), x2 = lane_bcast(xv, 2 * NP + j);
-    const double x = cc == 0 ? x0 : (cc == 1 ? x1 : x2);
-    if (ii == j)
-      w = x;
-    else if (ii < j)
-      w = w - E[j] * x;
-  }
-  return w;
+// K_PP's upper triangle in the packed layout of k_sym_products, one thread
+// per stored entry: the self blocks listed in self_s (stored index s, taken
+// from full-row slot src[s]).
+template <int P>
+__global__ void k_pack_self_upper(const double* __restrict__ K, double* __restrict__ Ks,
+                                  const int* __restrict__ src, const int* __restrict__ self_s,
+                                  int n_self) {
+  using D = Dims<P>;
+  constexpr int N3 = D::N3, NPAIR = D::NPAIR, NE = self_upper_off(NPAIR);
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (size_t)n_self * NE) return;
+  const int k = (int)(id / NE), e = (int)(id % NE);
+  int c2i = 0;  // column pair of packed entry e: self_upper_off(c2i) <= e < self_upper_off(c2i + 1)
+  while (self_upper_off(c2i + 1) <= e) ++c2i;
+  const int r = (e - self_upper_off(c2i)) >> 1, h = e & 1;
+  const int s = self_s[k];
+  Ks[(size_t)s * D::BS + e] = K[(size_t)src[s] * D::BS + ((size_t)c2i * N3 + r) * 2 + h];
 }
 
 template <int P, int MODE>
 __global__ __launch_bounds__(64) void k_sym_update(const double* __restrict__ prod,
                                                    const int* __restrict__ row_ptr,
-                                                   const double* __restrict__ Rp,
+                                                   const double* __restrict__ Sp,
                                                    const double* __restrict__ rs,
                                                    const double* __restrict__ rd,
                                                    const double* __restrict__ u,
@@ -1164,12 +1163,13 @@ __global__ __launch_bounds__(64) void k_sym_update(const double* __restrict__ pr
   constexpr int NP = D::NP, N3 = D::N3;
   static_assert(N3 <= 64, "one lane per row");
   static_assert(MODE == 0 || MODE == 1, "leapfrog and first step");
-  __shared__ double sR[D::NPK];
+  __shared__ double sS[D::NPK];
+  __shared__ double sv[N3];
   const int lane = threadIdx.x;
   const bool row = lane < N3;
   const int Pp = step_poly(order);
   if (Pp >= n_poly) return;
-  for (int k = lane; k < D::NPK; k += 64) sR[k] = Rp[(size_t)Pp * D::NPK + k];
+  for (int k = lane; k < D::NPK; k += 64) sS[k] = Sp[(size_t)Pp * D::NPK + k];
   const int b0 = row_ptr[Pp], b1 = row_ptr[Pp + 1];
   const size_t o = (size_t)Pp * N3;
   double vs = 0.0, vd = 0.0, vu = 0.0, va = 0.0;
@@ -1192,10 +1192,26 @@ __global__ __launch_bounds__(64) void k_sym_update(const double* __restrict__ pr
     }
   }
   const double gp = gsel ? gtab[gsel[Pp]] : g;
-  double w = row ? -acc + vs + vd * gp : 0.0;
-  __syncthreads();  // sR
-  w = solve_wave_rcp<NP>(sR, w, lane);
+  const int cc = row ? lane / NP : 0, ii = lane % NP;
+  const double* __restrict__ vc = sv + cc * NP;  // this lane's component
+  if (row) sv[lane] = -acc + vs + vd * gp;
+  __syncthreads();  // sS, sv
+  // z = S^T w: z_i = sum_{j <= i} S_ji w_j
+  double z = 0.0;
   if (row) {
+#pragma unroll
+    for (int j = 0; j < NP; ++j)
+      if (j <= ii) z = z + sS[pk_off(NP, j) + (ii - j)] * vc[j];
+  }
+  __syncthreads();
+  if (row) sv[lane] = z;
+  __syncthreads();
+  // M^-1 w = S z: x_i = sum_{j >= i} S_ij z_j
+  double w = 0.0;
+  if (row) {
+#pragma unroll
+    for (int j = 0; j < NP; ++j)
+      if (j >= ii) w = w + sS[pk_off(NP, ii) + (j - ii)] * vc[j];
     if (MODE == 0)
       out[o + lane] = c1 * w + 2.0 * vu - va;
     else
