@@ -121,6 +121,7 @@ struct LymphCtx {
   int* sidx = nullptr;      // per block of the full rows: stored block, ~ = transposed
   int* tdst = nullptr;      // stored pair block -> its (Q, P) slot of Q's row, else -1
   int* ssrc = nullptr;      // full-row slot of each stored block (gather list)
+  int4* pinfo = nullptr;    // per stored pair block: full-row slot, P, Q, slot of (Q, P)
   double* Sinv = nullptr;   // R^-1 of each mass block, packed like R (pairs-once FP64 update)
   float* Ks32 = nullptr;    // the stored blocks of the 32-bit operator (p = 3)
   size_t ks32_bytes = 0;
@@ -400,9 +401,10 @@ int launch_step(LymphCtx* c, int mode, double g, double c1, double c2, const dou
                                 c->md.col, c->RM, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->order, dyn);
     } else if (c->Ks) {
       if constexpr (3 * np_of(P) <= 64 && (3 * np_of(P)) % 2 == 0) {
-        const int n_s = c->n_poly + (c->n_blk - c->n_poly) / 2;  // stored blocks
-        k_sym_products<P><<<step_grid(n_s), 64, 0, c->stream>>>(
-            c->Ks, c->ssrc, c->blk_row, c->md.col, c->tdst, u, c->prod, n_s);
+        const int n_pair = (c->n_blk - c->n_poly) / 2;
+        const int self_per = (c->n_poly + 7) / 8, pair_per = (n_pair + 7) / 8;
+        k_sym_products<P><<<8u * (unsigned)(self_per + pair_per), 64, 0, c->stream>>>(
+            c->Ks, c->ssrc, c->pinfo, u, c->prod, c->n_poly, n_pair, self_per, pair_per);
         if (int rc = check_launch("k_sym_products")) return rc;
         if (mode == 0)
           k_sym_update<P, 0><<<step_grid(c->n_poly), 64, 0, c->stream>>>(
@@ -462,7 +464,8 @@ void drop_graph(LymphCtx* c) {
 
 // Drop the once-stored face pairs (operator rebuilt, storage switched off).
 int sym_off(LymphCtx* c) {
-  if (!c->Ks && !c->sidx && !c->tdst && !c->prod && !c->ssrc && !c->Ks32 && !c->Sinv) return 0;
+  if (!c->Ks && !c->sidx && !c->tdst && !c->prod && !c->ssrc && !c->Ks32 && !c->Sinv && !c->pinfo)
+    return 0;
   HIP_TRY(hipStreamSynchronize(c->stream));
   drop_graph(c);
   if (c->ssrc) HIP_TRY(hipFree(c->ssrc));
@@ -476,7 +479,9 @@ int sym_off(LymphCtx* c) {
   if (c->tdst) HIP_TRY(hipFree(c->tdst));
   if (c->prod) HIP_TRY(hipFree(c->prod));
   if (c->Sinv) HIP_TRY(hipFree(c->Sinv));
+  if (c->pinfo) HIP_TRY(hipFree(c->pinfo));
   c->Sinv = nullptr;
+  c->pinfo = nullptr;
   c->Ks = nullptr;
   c->sidx = nullptr;
   c->tdst = nullptr;
@@ -517,12 +522,25 @@ int sym_on(LymphCtx* c, double tol, double* asym) {
         pt.push_back(partner[b]);
       }
     }
+  // stored order: the self blocks first (stored index = polyhedron), then the
+  // pair blocks in row order, so that a stored block's kind follows from its
+  // index alone
+  for (int P = 0; P < np; ++P) {
+    int self = -1;
+    for (int b = rp[P]; b < rp[P + 1]; ++b)
+      if (cl[b] == P) self = b;
+    if (self < 0) return fail(LYMPH_ERR_STATE, "a block row without its self block");
+    sid[self] = P;
+    self_s.push_back(P);
+    src.push_back(self);
+  }
+  std::vector<int4> pi;  // the pair blocks' (slot, P, Q, slot of (Q, P))
   for (int P = 0; P < np; ++P)
     for (int b = rp[P]; b < rp[P + 1]; ++b)
-      if (cl[b] >= P) {
+      if (cl[b] > P) {
         sid[b] = (int)src.size();
-        if (cl[b] == P) self_s.push_back(sid[b]);
         src.push_back(b);
+        pi.push_back(make_int4(b, P, cl[b], partner[b]));
       }
   std::vector<int> sx(c->n_blk), td(c->n_blk, -1);
   for (int P = 0; P < np; ++P)
@@ -573,8 +591,12 @@ int sym_on(LymphCtx* c, double tol, double* asym) {
   if (int rc = dalloc(&c->Ks, (size_t)n_s * bs)) return rc;
   int rc = dalloc(&c->sidx, c->n_blk);
   if (!rc) rc = dalloc(&c->tdst, c->n_blk);
-  if (!rc) rc = dalloc(&c->prod, (size_t)c->n_blk * c->n3);
+  if (!rc) rc = dalloc(&c->prod, (size_t)c->n_blk * prod_stride(c->n3));
   if (!rc) rc = dalloc(&dsrc, n_s);
+  if (!rc && !pi.empty()) rc = dalloc(&c->pinfo, pi.size());
+  if (!rc && !pi.empty() &&
+      hipMemcpy(c->pinfo, pi.data(), pi.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(LYMPH_ERR_HIP, "copy of the pair list failed");
   if (!rc && (hipMemcpy(c->sidx, sx.data(), sx.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
               hipMemcpy(c->tdst, td.data(), td.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
               hipMemcpy(dsrc, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess))
@@ -613,7 +635,8 @@ int sym_on(LymphCtx* c, double tol, double* asym) {
     return rc;
   }
   c->ks_bytes = (size_t)n_s * bs * sizeof(double) + (2 * (size_t)c->n_blk + n_s) * sizeof(int) +
-                (size_t)c->n_blk * c->n3 * sizeof(double) + (size_t)np * npk * sizeof(double);
+                (size_t)c->n_blk * prod_stride(c->n3) * sizeof(double) + (size_t)np * npk * sizeof(double) +
+                pi.size() * sizeof(int4);
   c->bytes += c->ks_bytes;
   // the 32-bit operator's copy, if asked for and it exists already; a failure
   // leaves no stored pairs switched on
@@ -768,6 +791,7 @@ int lymph_ctx_destroy(LymphCtx* c) {
   if (c->ssrc) (void)hipFree(c->ssrc);
   if (c->Ks32) (void)hipFree(c->Ks32);
   if (c->Sinv) (void)hipFree(c->Sinv);
+  if (c->pinfo) (void)hipFree(c->pinfo);
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   for (void* p : c->owned) (void)hipFree(p);

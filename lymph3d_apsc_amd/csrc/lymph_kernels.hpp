@@ -76,6 +76,10 @@ LYMPH_HD size_t kpos32(int N3, int r, int c) {
   return (size_t)c0 * N3 + (size_t)r * (N3 % 4) + (c - c0);
 }
 LYMPH_HD int pk_off(int NP, int i) { return i * NP - i * (i - 1) / 2; }
+// Doubles per block product of the pairs-once steps: whole 128-byte lines, so
+// that each product's store covers its lines entirely (the padding lanes
+// write zeros) instead of leaving partial lines that other waves complete.
+LYMPH_HD constexpr int prod_stride(int n3) { return (n3 + 15) & ~15; }
 
 struct MeshDev {
   const double* tet_xyz;  // n_tet*12
@@ -1025,8 +1029,11 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog(const double* __restrict_
 // of each self block (K_PP = K_PP^T to rounding) only its upper triangle:
 // column pair c2i's rows 0..2c2i+1 packed from entry 2 c2i (c2i + 1)
 // (self_upper_off), half the bytes of the full block.
-// k_sym_products: one wave per stored block K_PQ (ssrc[s] its full-row slot
-// b in P's row): P's product K_PQ u_Q (FMA-free chain over ascending columns,
+// k_sym_products: one wave per stored block K_PQ, the self blocks first
+// (stored index = polyhedron, ssrc[P] its full-row slot) and then the pair
+// blocks (pinfo: full-row slot b in P's row, P, Q, slot of (Q, P)), each
+// XCD taking a contiguous share of both, so that the block's loads depend on
+// nothing loaded: P's product K_PQ u_Q (FMA-free chain over ascending columns,
 // as k_leapfrog) into prod[b], and for a pair block the neighbour's product
 // K_QP u_P = K_PQ^T u_P into prod[tdst[b]], the slot of block (Q, P) of Q's
 // row.  Lane i holds row i of K_PQ and u_P[i]; the products K_PQ[i][c] u_P[i]
@@ -1051,40 +1058,30 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// One stored block of k_sym_products (self: a self block's packed upper
+// triangle; otherwise a pair block K_PQ with its transposed product).  Kb is
+// the block's base plus the lane's offset.  A wave holds its registers until
+// its stores are acknowledged, so a pair block's row product is stored before
+// the LDS transpose (u_P is loaded with the block: no load follows a store).
 template <int P>
-__global__ __launch_bounds__(64) void k_sym_products(const double* __restrict__ Ks,
-                                                     const int* __restrict__ ssrc,
-                                                     const int* __restrict__ blk_row,
-                                                     const int* __restrict__ col,
-                                                     const int* __restrict__ tdst,
-                                                     const double* __restrict__ u,
-                                                     double* __restrict__ prod, int n_stored) {
+__device__ __forceinline__ void sym_block(bool self, const double* __restrict__ Kb, int b, int Pp, int Qq,
+                                          int td, const double* __restrict__ u,
+                                          double* __restrict__ prod, double* __restrict__ sT,
+                                          int lane) {
   using D = Dims<P>;
-  constexpr int N3 = D::N3, NPAIR = D::NPAIR;
-  static_assert(N3 <= 64 && N3 % 2 == 0, "one lane per row, whole column pairs");
-  constexpr int CW = N3 / 3, CP = CW / 2;
-  static_assert(CW % 2 == 0, "whole column pairs per chunk");
-  static_assert(self_upper_off(NPAIR) <= D::BS, "packed self block fits its slot");
-  constexpr int LDT = N3 + 1;  // odd stride: a column read spreads over the banks
-  __shared__ double sT[CW * LDT];
-  const int lane = threadIdx.x;
+  constexpr int N3 = D::N3, NPAIR = D::NPAIR, PS = prod_stride(N3);
+  constexpr int CW = N3 / 3, CP = CW / 2, LDT = N3 + 1;
   const bool row = lane < N3;
-  const int sb = step_poly();  // XCD-aware: each XCD streams a contiguous share of the rows
-  if (sb >= n_stored) return;  // padding workgroups of the last XCD share
-  const int b = ssrc[sb];
-  const int Pp = blk_row[b], Qq = col[b];
-  const int td = tdst[b];  // -1: the self block (upper triangle stored)
-  const bool self = td < 0;
-  const double* __restrict__ Kb = Ks + (size_t)sb * D::BS;
   double2 kv[NPAIR];
 #pragma unroll
   for (int c2i = 0; c2i < NPAIR; ++c2i) {
     const bool ld = row && (!self || lane <= 2 * c2i + 1);
-    const int off = self ? self_upper_off(c2i) + 2 * lane : (c2i * N3 + lane) * 2;
-    kv[c2i] = ld ? load_stream(Kb + off) : make_double2(0.0, 0.0);
+    kv[c2i] = ld ? load_stream(Kb + (self ? self_upper_off(c2i) : c2i * N3 * 2))
+                 : make_double2(0.0, 0.0);
     if (self && lane == 2 * c2i + 1) kv[c2i].x = 0.0;  // K_PP[lane][lane - 1]: below the diagonal
   }
   const double xl = row ? u[(size_t)Qq * N3 + lane] : 0.0;
+  const double up = self ? xl : (row ? u[(size_t)Pp * N3 + lane] : 0.0);  // P's own coefficients
   double s = 0.0;
 #pragma unroll
   for (int c2i = 0; c2i < NPAIR; ++c2i) {
@@ -1092,9 +1089,7 @@ __global__ __launch_bounds__(64) void k_sym_products(const double* __restrict__ 
     s = s + kv[c2i].x * x0;
     s = s + kv[c2i].y * x1;
   }
-  // P's own coefficients (the self block's are xl), loaded once the row
-  // product has freed registers
-  const double up = self ? xl : (row ? u[(size_t)Pp * N3 + lane] : 0.0);
+  if (!self && lane < PS) prod[(size_t)b * PS + lane] = s;  // lanes past the rows hold zeros
   double st = 0.0;
 #pragma unroll
   for (int h = 0; h < 3; ++h) {
@@ -1118,14 +1113,47 @@ __global__ __launch_bounds__(64) void k_sym_products(const double* __restrict__ 
       st = v;
     }
   }
-  if (row) {
-    if (self) {
-      prod[(size_t)b * N3 + lane] = s + st;
-    } else {
-      prod[(size_t)b * N3 + lane] = s;
-      prod[(size_t)td * N3 + lane] = st;
-    }
+  if (lane < PS) {
+    if (self)
+      prod[(size_t)b * PS + lane] = s + st;
+    else
+      prod[(size_t)td * PS + lane] = st;
   }
+}
+
+template <int P>
+__global__ __launch_bounds__(64) void k_sym_products(const double* __restrict__ Ks,
+                                                     const int* __restrict__ ssrc,
+                                                     const int4* __restrict__ pinfo,
+                                                     const double* __restrict__ u,
+                                                     double* __restrict__ prod, int n_poly,
+                                                     int n_pair, int self_per, int pair_per) {
+  using D = Dims<P>;
+  constexpr int N3 = D::N3, NPAIR = D::NPAIR;
+  static_assert(N3 <= 64 && N3 % 2 == 0, "one lane per row, whole column pairs");
+  static_assert((N3 / 3) % 2 == 0, "whole column pairs per chunk");
+  static_assert(self_upper_off(NPAIR) <= D::BS, "packed self block fits its slot");
+  __shared__ double sT[(N3 / 3) * (N3 + 1)];  // odd stride: a column read spreads over the banks
+  const int lane = threadIdx.x;
+  // XCD-aware: workgroup i of XCD x takes self block x self_per + i, or pair
+  // x pair_per + (i - self_per); padding workgroups of the last shares leave
+  const int xcd = (int)(blockIdx.x & 7), i = (int)(blockIdx.x >> 3);
+  const bool self = i < self_per;
+  int sb;
+  if (self) {
+    sb = xcd * self_per + i;
+    if (sb >= n_poly) return;
+  } else {
+    const int j = xcd * pair_per + (i - self_per);
+    if (j >= n_pair) return;
+    sb = n_poly + j;
+  }
+  int4 q;
+  if (self)
+    q = make_int4(ssrc[sb], sb, sb, -1);
+  else
+    q = pinfo[sb - n_poly];
+  sym_block<P>(self, Ks + (size_t)sb * D::BS + 2 * lane, q.x, q.y, q.z, q.w, u, prod, sT, lane);
 }
 
 // K_PP's upper triangle in the packed layout of k_sym_products, one thread
@@ -1185,7 +1213,8 @@ __global__ __launch_bounds__(64) void k_sym_update(const double* __restrict__ pr
     for (int b = b0; b < b1; b += 8) {
       double v[8];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) v[k] = b + k < b1 ? prod[(size_t)(b + k) * N3 + lane] : 0.0;
+      for (int k = 0; k < 8; ++k)
+        v[k] = b + k < b1 ? prod[(size_t)(b + k) * prod_stride(N3) + lane] : 0.0;
 #pragma unroll
       for (int k = 0; k < 8; ++k)
         if (b + k < b1) acc = (b + k == b0) ? v[k] : acc + v[k];
@@ -1278,7 +1307,7 @@ __global__ __launch_bounds__(64) void k_sym_products32(const float* __restrict__
       s = s + kq[q].z * xq[2];
       s = s + kq[q].w * xq[3];
     }
-    if (row) prod[(size_t)b * N3 + lane] = (double)s;
+    if (lane < prod_stride(N3)) prod[(size_t)b * prod_stride(N3) + lane] = (double)s;  // zeros past the rows
     if (cb == Pp) continue;  // self block: no partner
     float st = 0.0f;
 #pragma unroll
@@ -1303,7 +1332,7 @@ __global__ __launch_bounds__(64) void k_sym_products32(const float* __restrict__
         st = v;
       }
     }
-    if (row) prod[(size_t)td * N3 + lane] = (double)st;
+    if (lane < prod_stride(N3)) prod[(size_t)td * prod_stride(N3) + lane] = (double)st;
   }
 }
 
@@ -1332,7 +1361,8 @@ __global__ __launch_bounds__(64) void k_sym_update32(const double* __restrict__ 
   for (int b = b0; b < b1; b += 8) {
     double v[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = b + k < b1 ? prod[(size_t)(b + k) * N3 + lane] : 0.0;
+    for (int k = 0; k < 8; ++k)
+      v[k] = b + k < b1 ? prod[(size_t)(b + k) * prod_stride(N3) + lane] : 0.0;
 #pragma unroll
     for (int k = 0; k < 8; ++k)
       if (b + k < b1) acc = (b + k == b0) ? v[k] : acc + v[k];
