@@ -253,12 +253,13 @@ def ensemble_leg(n_inst, steps, plain_sample, f0=5.0, seed0=1000):
     }
 
 
-def surrogate_leg(n_train, n_eval, epochs, threshold=1e-2):
+def surrogate_leg(n_train, n_eval, epochs, threshold=5e-2):
     """Learned surrogate on a bounded sample of its workload: n_train + n_eval
     parameter sets on 16^3 cubes at p=2 (1000 steps, 64 receivers every 5
     steps), solved by the 32-bit ensemble path (truth: the FP64 ensemble), four
-    8-512-512-512-38400 networks trained on the device; throughput and error
-    against the plain FP64 path one instance at a time."""
+    per-receiver networks (32 physics features, 5 hidden layers of 512, 64 SVD modes
+    of the traces) trained on the device; throughput and error against the
+    plain FP64 path one instance at a time."""
     from lymph3d_apsc_amd import surrogate as S
     r = S.run(n_train=n_train, n_eval=n_eval, epochs=epochs, threshold=threshold, plain_sample=4,
               log=log)
@@ -267,7 +268,7 @@ def surrogate_leg(n_train, n_eval, epochs, threshold=1e-2):
     return {
         "value": thr["surrogate"], "unit": "surrogate predictions/s (4 networks, inputs resident)",
         "sample": f"{n_train} training / {n_eval} held-out instances, {epochs} epochs "
-                  "(the full-size run uses 4096 / 512 and 1000 epochs)",
+                  "(the full-size run uses 4096 / 512 and 300 epochs)",
         "plain_fp64_solves_per_s": thr["plain_fp64"],
         "ensemble_fp32_solves_per_s": thr["ensemble_fp32"],
         "hybrid_solves_per_s": thr["hybrid_with_fallback"], "threshold": threshold,
@@ -412,7 +413,7 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg")
     ap.add_argument("--sur-train", type=int, default=1024)
     ap.add_argument("--sur-eval", type=int, default=128)
-    ap.add_argument("--sur-epochs", type=int, default=300)
+    ap.add_argument("--sur-epochs", type=int, default=150)
     ap.add_argument("--uneven", action="store_true",
                     help="instead of the bench line: heterogeneous uneven polyhedra at p=4 (one JSON line)")
     ap.add_argument("--degree-sweep", action="store_true",

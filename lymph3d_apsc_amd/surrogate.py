@@ -12,12 +12,20 @@ steps.
 
 Pipeline: the training and held-out sets are solved by the batched ensemble
 path (ensemble.Ensemble: 32-bit apply, HIP-graph captured); the held-out
-truth by the same path with the FP64 apply.  An ensemble of networks (widths
-8-512-512-512-out, GELU) is trained on the GPU by liblymph_mlp.so (BF16 MFMA,
-FP32 master weights, Adam) on standardised inputs and centred traces
-scaled by one global factor.  At evaluation the spread of the networks' predictions
-is the error estimate: a prediction whose relative spread exceeds the
-threshold is replaced by a solver run.
+truth by the same path with the FP64 apply.  An ensemble of networks (GELU)
+is trained on the GPU by liblymph_mlp.so (BF16 MFMA, FP32 master weights,
+Adam).  Two models:
+- "receiver" (default): one example per (instance, receiver).  Inputs are
+  the parameters together with what the physics fixes for that pair
+  (receiver position, source-receiver direction and distance, P and S travel
+  times along the straight path through the instance's field, the local wave
+  speeds); the output is the receiver's trace, times its distance (geometric
+  spreading), in the leading SVD modes of the training traces.
+- "global": the parameters to all 64 receivers' traces at once (widths
+  8-512-512-512-38400), centred and scaled by one global factor.
+At evaluation the spread of the networks' predictions is the error
+estimate: a prediction whose relative spread exceeds the threshold is
+replaced by a solver run.
 
 No reference counterpart: LYMPH3D has no learned component.
 """
@@ -152,6 +160,159 @@ class Surrogate:
             net.close()
 
 
+N_FEAT = 32
+R_MIN = 0.05  # distance floor of the geometric-spreading factor
+
+
+def _gauss01(n):
+    x, w = np.polynomial.legendre.leggauss(n)
+    return 0.5 * (x + 1), 0.5 * w
+
+
+def receiver_features(thetas, rec, n_cells, n_q=8):
+    """(N, R, N_FEAT) inputs of the receiver model: source xyz, receiver xyz,
+    unit source-receiver direction, distance, log distance, f0, the four
+    field coefficients, P and S travel times along the straight segment
+    (distance times the segment's mean slowness, Gauss-Legendre in n_q
+    points; rho = 1 so c_P = sqrt(lambda + 2 mu), c_S = sqrt(mu)), the same
+    times f0, c_P and c_S at the source and at the receiver, and the source's
+    and the receiver's positions inside their cells of the n_cells^3 cube
+    mesh (a point force's discrete response near the source depends on
+    them), and two zeros."""
+    th = np.asarray(thetas, np.float64)
+    src, f0, xi = th[:, :3], th[:, 3], th[:, 4:8]
+    d = rec[None, :, :] - src[:, None, :]
+    dist = np.linalg.norm(d, axis=2)
+    gam = d / np.maximum(dist, 1e-12)[..., None]
+    sq, wq = _gauss01(n_q)
+    pts = src[:, None, None, :] + sq[None, None, :, None] * d[:, :, None, :]
+
+    def speeds(x, z, k):
+        lam = np.exp(SIGMA * (xi[:, 0].reshape(k) + xi[:, 1].reshape(k) * (2 * x - 1)))
+        mu = np.exp(SIGMA * (xi[:, 2].reshape(k) + xi[:, 3].reshape(k) * (2 * z - 1)))
+        return np.sqrt(lam + 2 * mu), np.sqrt(mu)
+
+    cp, cs = speeds(pts[..., 0], pts[..., 2], (-1, 1, 1))
+    tp = dist * (wq / cp).sum(axis=2)
+    ts = dist * (wq / cs).sum(axis=2)
+    cps, css = speeds(src[:, 0], src[:, 2], (-1,))
+    cpr, csr = speeds(rec[None, :, 0], rec[None, :, 2], (-1, 1))
+    N, R = dist.shape
+    f = np.zeros((N, R, N_FEAT))
+    f[..., 0:3] = src[:, None, :]
+    f[..., 3:6] = rec[None, :, :]
+    f[..., 6:9] = gam
+    f[..., 9] = dist
+    f[..., 10] = np.log(np.maximum(dist, R_MIN))
+    f[..., 11] = f0[:, None]
+    f[..., 12:16] = xi[:, None, :]
+    f[..., 16] = tp
+    f[..., 17] = ts
+    f[..., 18] = tp * f0[:, None]
+    f[..., 19] = ts * f0[:, None]
+    f[..., 20] = cps[:, None]
+    f[..., 21] = css[:, None]
+    f[..., 22] = cpr
+    f[..., 23] = csr
+    f[..., 24:27] = (n_cells * src - np.floor(n_cells * src))[:, None, :]
+    f[..., 27:30] = (n_cells * rec - np.floor(n_cells * rec))[None, :, :]
+    return f
+
+
+class ReceiverSurrogate:
+    """Networks from (parameters, receiver) features to one receiver's trace
+    in n_modes SVD modes; rec is the fixed receiver array."""
+
+    def __init__(self, rec, n_t, n_cells, n_nets=4, width=512, depth=6, n_modes=64, max_batch=256,
+                 seed=0):
+        self.rec = np.asarray(rec, np.float64)
+        self.n_cells = n_cells
+        self.n_t, self.n_modes = n_t, n_modes
+        self.widths = [N_FEAT] + [width] * (depth - 1) + [n_modes]
+        self.nets = [Mlp(self.widths, max_batch=max_batch, seed=seed + 17 * i) for i in range(n_nets)]
+        self.seed = seed
+
+    def _x(self, thetas):
+        f = receiver_features(thetas, self.rec, self.n_cells)
+        return ((f - self.f_mean) / self.f_std).reshape(-1, N_FEAT).astype(np.float32), f[..., 9]
+
+    def fit(self, thetas, traces, epochs=100, batch=256, lr=1e-3, log=None):
+        N, T, R, _ = traces.shape
+        f = receiver_features(thetas, self.rec, self.n_cells)
+        self.f_mean = f.reshape(-1, N_FEAT).mean(axis=0)
+        self.f_std = np.maximum(f.reshape(-1, N_FEAT).std(axis=0), 1e-12)
+        x, dist = self._x(thetas)
+        # one example per (instance, receiver): the trace times max(distance, R_MIN)
+        y = np.ascontiguousarray(traces.transpose(0, 2, 1, 3)).reshape(N * R, T * 3).astype(np.float64)
+        y *= np.maximum(dist, R_MIN).reshape(-1, 1)
+        self.y_mean = y.mean(axis=0)
+        rng = np.random.default_rng(self.seed + 7)
+        sub = rng.choice(len(y), min(len(y), 20000), replace=False)
+        _, sv, vt = np.linalg.svd(y[sub] - self.y_mean, full_matrices=False)
+        self.basis = vt[:self.n_modes].T  # (T*3, n_modes), orthonormal columns
+        self.basis_energy = float((sv[:self.n_modes] ** 2).sum() / (sv ** 2).sum())
+        c = (y - self.y_mean) @ self.basis
+        self.c_std = max(float(np.sqrt(np.mean(c ** 2))), 1e-300)
+        cn = (c / self.c_std).astype(np.float32)
+        n = len(x)
+        steps_per_epoch = n // batch
+        hist = []
+        for i, net in enumerate(self.nets):
+            net.set_data(x, cn)
+            rng = np.random.default_rng(self.seed + 1000 + i)
+            losses = []
+            for ep in range(epochs):
+                frac = ep / max(1, epochs - 1)
+                rate = lr if frac < 0.6 else (lr * 0.3 if frac < 0.85 else lr * 0.1)
+                order = rng.permutation(n)[:steps_per_epoch * batch].astype(np.int32)
+                losses.append(float(net.train(order, batch, rate).mean()))
+            hist.append(losses)
+            if log:
+                log(f"net {i}: loss {losses[0]:.3e} -> {losses[-1]:.3e}")
+        return hist
+
+    def _decode(self, c, dist, N):
+        y = (c.astype(np.float64) * self.c_std) @ self.basis.T + self.y_mean
+        y /= np.maximum(dist, R_MIN).reshape(-1, 1)
+        R = len(self.rec)
+        return y.reshape(N, R, self.n_t, 3).transpose(0, 2, 1, 3).reshape(N, -1)
+
+    def project(self, thetas, traces):
+        """The traces as the SVD basis represents them (the floor of the
+        model's error)."""
+        N, T, R, _ = traces.shape
+        _, dist = self._x(thetas)
+        y = np.ascontiguousarray(traces.transpose(0, 2, 1, 3)).reshape(N * R, T * 3).astype(np.float64)
+        y *= np.maximum(dist, R_MIN).reshape(-1, 1)
+        return self._decode(((y - self.y_mean) @ self.basis) / self.c_std, dist, N)
+
+    def predict_all(self, thetas):
+        x, dist = self._x(thetas)
+        return np.stack([self._decode(net.predict(x), dist, len(thetas)) for net in self.nets])
+
+    def predict(self, thetas):
+        """(mean prediction, relative spread per sample)."""
+        P = self.predict_all(thetas)
+        mean = P.mean(axis=0)
+        spread = np.linalg.norm(P.std(axis=0), axis=1) / np.maximum(np.linalg.norm(mean, axis=1), 1e-300)
+        return mean, spread
+
+    def time_predict(self, thetas, reps=10):
+        """Seconds for one prediction of every sample by every network
+        (inputs resident on the device; the modes' decoding is not timed)."""
+        x, _ = self._x(thetas)
+        dummy = np.zeros((len(x), self.widths[-1]), np.float32)
+        total = 0.0
+        for net in self.nets:
+            net.set_data(x, dummy)
+            total += net.time_predict(len(x), reps) * 1e-3
+        return total
+
+    def close(self):
+        for net in self.nets:
+            net.close()
+
+
 def rel_err(a, b):
     a = a.reshape(len(a), -1)
     b = b.reshape(len(b), -1)
@@ -159,8 +320,8 @@ def rel_err(a, b):
 
 
 def run(n_train=4096, n_eval=512, n=16, p=2, steps=1000, every=5, n_receivers=64, n_nets=4,
-        width=512, epochs=200, batch=256, lr=1e-3, threshold=1e-2, chunk=512, plain_sample=8,
-        seed0=10_000, log=None):
+        width=512, epochs=300, batch=256, lr=1e-3, threshold=5e-2, chunk=512, plain_sample=8,
+        seed0=10_000, model="receiver", n_modes=64, log=None):
     """Whole pipeline; returns a dict of results (errors, throughputs)."""
     from . import _lib
     from .matrix_free import Context
@@ -195,14 +356,23 @@ def run(n_train=4096, n_eval=512, n=16, p=2, steps=1000, every=5, n_receivers=64
     plain_rate = plain_sample / t_plain
     # networks
     t0 = time.perf_counter()
-    sur = Surrogate(y_train[0].size, n_nets=n_nets, width=width, max_batch=max(batch, 64))
-    hist = sur.fit(th_train, y_train, epochs=epochs, batch=batch, lr=lr, log=log)
+    floor = None
+    if model == "receiver":
+        sur = ReceiverSurrogate(rec, steps // every, n, n_nets=n_nets, width=width, n_modes=n_modes,
+                                max_batch=max(batch, 64))
+        hist = sur.fit(th_train, y_train, epochs=epochs, batch=batch, lr=lr, log=log)
+        n_examples = n_train * n_receivers
+        floor = rel_err(sur.project(th_eval, y_eval64), y_eval64)
+    else:
+        sur = Surrogate(y_train[0].size, n_nets=n_nets, width=width, max_batch=max(batch, 64))
+        hist = sur.fit(th_train, y_train, epochs=epochs, batch=batch, lr=lr, log=log)
+        n_examples = n_train
     t_train = time.perf_counter() - t0
     w = sur.widths
     # GEMM flops of one training step: forward, weight gradient, input gradient
     # (none for the first layer's input)
     step_flops = 2.0 * batch * sum(k * n for k, n in zip(w[:-1], w[1:])) * 3 - 2.0 * batch * w[0] * w[1]
-    n_steps = n_nets * epochs * (n_train // batch)
+    n_steps = n_nets * epochs * (n_examples // batch)
     mean, spread = sur.predict(th_eval)
     t_pred = sur.time_predict(th_eval)
     err_sur = rel_err(mean, y_eval64)
@@ -216,7 +386,7 @@ def run(n_train=4096, n_eval=512, n=16, p=2, steps=1000, every=5, n_receivers=64
     # what each spread threshold buys: share answered by the networks, their
     # error, and the hybrid's throughput and error (fallbacks exact)
     tradeoff = []
-    for tau in (threshold, 0.02, 0.05, 0.1, 0.2, 0.3, 0.5, 1.0):
+    for tau in sorted({threshold, 0.01, 0.02, 0.05, 0.1, 0.2, 0.3, 0.5, 1.0}):
         acc = spread <= tau
         t_h = t_pred + int((~acc).sum()) / plain_rate
         tradeoff.append({"threshold": tau, "accepted": float(acc.mean()),
@@ -229,8 +399,12 @@ def run(n_train=4096, n_eval=512, n=16, p=2, steps=1000, every=5, n_receivers=64
     return {
         "config": {"n_train": n_train, "n_eval": n_eval, "mesh": f"{n}^3 cubes x 6 tets, agglomerated",
                    "p": p, "steps": steps, "every": every, "receivers": n_receivers,
-                   "outputs": int(y_train[0].size), "widths": sur.widths, "nets": n_nets,
-                   "epochs": epochs, "batch": batch, "lr": lr, "threshold": threshold, "dt": dt},
+                   "outputs": int(y_train[0].size), "model": model, "widths": sur.widths,
+                   "nets": n_nets, "epochs": epochs, "batch": batch, "lr": lr,
+                   "threshold": threshold, "dt": dt,
+                   "n_modes": n_modes if model == "receiver" else None},
+        "basis": None if floor is None else {
+            "energy_kept": sur.basis_energy, "projection_rel_l2_vs_fp64": q(floor)},
         "throughput_solves_per_s": {
             "plain_fp64": plain_rate,
             "ensemble_fp32": (n_train + n_eval) / (run32 + run32e),
@@ -262,10 +436,14 @@ if __name__ == "__main__":
     ap.add_argument("--n-train", type=int, default=4096)
     ap.add_argument("--n-eval", type=int, default=512)
     ap.add_argument("--n", type=int, default=16)
-    ap.add_argument("--epochs", type=int, default=200)
+    ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--nets", type=int, default=4)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--model", choices=["receiver", "global"], default="receiver")
+    ap.add_argument("--modes", type=int, default=64)
+    ap.add_argument("--batch", type=int, default=256)
     a = ap.parse_args()
     res = run(n_train=a.n_train, n_eval=a.n_eval, n=a.n, epochs=a.epochs, n_nets=a.nets, lr=a.lr,
+              model=a.model, n_modes=a.modes, batch=a.batch,
               log=lambda *m: print(*m, file=sys.stderr, flush=True))
     print(json.dumps(res))
