@@ -1080,7 +1080,7 @@ int lymph_build_operator(LymphCtx* c) {
                                                      c->Mm, p0);
       if (int r = check_launch("k_build_operator")) return r;
     }
-    k_cholesky<P><<<grid1(c->n_poly, 64), 64, 0, c->stream>>>(c->M, c->Mm, c->RM, c->RMm,
+    k_cholesky<P><<<(unsigned)c->n_poly, 64, 0, c->stream>>>(c->M, c->Mm, c->RM, c->RMm,
                                                               c->n_poly, c->status);
     if (int r = check_launch("k_cholesky")) return r;
     return 0;

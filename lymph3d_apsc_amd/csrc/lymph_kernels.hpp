@@ -526,38 +526,54 @@ __global__ __launch_bounds__(256) void k_build_factored(MeshDev m, double alpha,
 }
 
 // Cholesky of both mass blocks of a polyhedron (cholesky,
-// src/vet_mat_operations.f90:84-120), one thread per polyhedron; writes the
-// packed upper factors.  status[0] set to 1 if a block is not SPD.
+// src/vet_mat_operations.f90:84-120), one wave per polyhedron: the block is
+// staged in LDS by coalesced loads, each row j of the packed upper factor is
+// computed across the wave (every lane forms the pivot, lane i > j the entry
+// (j, i)) with each entry's sum over k in the serial order, and the factor
+// leaves as one coalesced store.  status[0] set to 1 if a block is not SPD.
 template <int P>
-__global__ void k_cholesky(const double* __restrict__ M, const double* __restrict__ Mm,
-                           double* __restrict__ RM, double* __restrict__ RMm, int n_poly,
-                           int* __restrict__ status) {
+__global__ __launch_bounds__(64) void k_cholesky(const double* __restrict__ M,
+                                                 const double* __restrict__ Mm,
+                                                 double* __restrict__ RM, double* __restrict__ RMm,
+                                                 int n_poly, int* __restrict__ status) {
   using D = Dims<P>;
-  constexpr int NP = D::NP;
-  const int Pp = blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr int NP = D::NP, NPK = D::NPK;
+  static_assert(NP <= 64, "one lane per row of the factor");
+  __shared__ double sA[NP * NP];
+  __shared__ double sR[NPK];
+  const int lane = threadIdx.x;
+  const int Pp = blockIdx.x;
   if (Pp >= n_poly) return;
   for (int which = 0; which < 2; ++which) {
     const double* A = (which ? Mm : M) + (size_t)Pp * NP * NP;
-    double* R = (which ? RMm : RM) + (size_t)Pp * D::NPK;
+    double* R = (which ? RMm : RM) + (size_t)Pp * NPK;
+    __syncthreads();  // the previous block's LDS reads are done
+    for (int e = lane; e < NP * NP; e += 64) sA[e] = A[e];
+    __syncthreads();
     for (int j = 0; j < NP; ++j) {
-      double s = A[j * NP + j];
+      double s = sA[j * NP + j];
       for (int k = 0; k < j; ++k) {
-        const double rkj = R[pk_off(NP, k) + (j - k)];
+        const double rkj = sR[pk_off(NP, k) + (j - k)];
         s = s - rkj * rkj;
       }
-      if (!(s > 0.0)) {
-        atomicOr(status, 1);
+      if (!(s > 0.0)) {  // the same value on every lane: the wave leaves together
+        if (lane == 0) atomicOr(status, 1);
         return;
       }
       const double rjj = sqrt(s);
-      R[pk_off(NP, j)] = rjj;
-      for (int i = j + 1; i < NP; ++i) {
-        double s2 = A[j * NP + i];
+      double rji = 0.0;
+      if (lane > j && lane < NP) {
+        double s2 = sA[j * NP + lane];
         for (int k = 0; k < j; ++k)
-          s2 = s2 - R[pk_off(NP, k) + (j - k)] * R[pk_off(NP, k) + (i - k)];
-        R[pk_off(NP, j) + (i - j)] = s2 / rjj;
+          s2 = s2 - sR[pk_off(NP, k) + (j - k)] * sR[pk_off(NP, k) + (lane - k)];
+        rji = s2 / rjj;
       }
+      // row j is read from the next step on (this step reads rows k < j only)
+      if (lane == j) sR[pk_off(NP, j)] = rjj;
+      if (lane > j && lane < NP) sR[pk_off(NP, j) + (lane - j)] = rji;
+      __syncthreads();
     }
+    for (int e = lane; e < NPK; e += 64) R[e] = sR[e];
   }
 }
 
