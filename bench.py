@@ -341,6 +341,18 @@ def uneven_leg(n=30, p=4, seed=3, steps=200, run_steps=2000):
     return res
 
 
+def pairs_once_bytes(ctx):
+    """Algorithmic bytes of one FP64 step with face pairs stored once: the
+    pair blocks and the self blocks' upper triangles (column pair k's rows
+    0..2k+1) read once, the per-block products written and read back, the
+    mass factor's inverse, five vectors and three int32 indices per block."""
+    n3, npk = ctx.n3, ctx.Np * (ctx.Np + 1) // 2
+    n_pair = (ctx.n_blk - ctx.n_poly) // 2
+    h = n3 // 2
+    return 8 * (n_pair * n3 * n3 + ctx.n_poly * 2 * h * (h + 1) + 2 * ctx.n_blk * n3
+                + ctx.n_poly * npk + 5 * ctx.n_poly * n3) + 12 * ctx.n_blk
+
+
 def degree_sweep(steps=200):
     """Step throughput at p = 1..4 on agglomerated cubes of about 4 M DOFs
     (70^3, 52^3, 40^3, 33^3 cubes), FP64 and 32-bit apply, each autotuned,
@@ -378,10 +390,7 @@ def degree_sweep(steps=200):
                 ctx.set_apply_precision(64)
                 ctx.set_symmetric_storage(True)
                 ms = ctx.time_steps(L.PROB_CUBE_WAVE, steps, dt, dt) / steps
-                n3, npk = ctx.n3, ctx.Np * (ctx.Np + 1) // 2
-                n_st = ctx.n_poly + (ctx.n_blk - ctx.n_poly) // 2
-                B = 8 * (n_st * n3 * n3 + 2 * ctx.n_blk * n3 + ctx.n_poly * npk + 5 * ctx.n_poly * n3) \
-                    + 12 * ctx.n_blk
+                B = pairs_once_bytes(ctx)
                 row["fp64_pairs_once"] = dict(ms_per_step=ms, dof_updates_per_s=row["dofs"] / (ms * 1e-3),
                                               speedup_vs_full_rows=row["fp64"]["ms_per_step"] / ms,
                                               GBps=B / ms / 1e6, frac=B / ms / 1e6 / HBM_PEAK_GBS)
@@ -511,14 +520,8 @@ def main():
         asym = ctx.set_symmetric_storage(True)
         mss = ctx.time_steps(prob, a.steps, t_start, dt) / a.steps
         us = ctx.get_slot(_lib.SLOT_U_CUR)
-        n3, npk = 3 * ctx.Np, ctx.Np * (ctx.Np + 1) // 2
         n_st = ctx.n_poly + (ctx.n_blk - ctx.n_poly) // 2
-        # stored pair blocks read once and the self blocks' upper triangles (column pair k's
-        # rows 0..2k+1), per-block products written and read back, the mass factor's inverse,
-        # five vectors, three int32 indices per block
-        n_self_e = (n3 // 2) * (n3 // 2 + 1) * 2
-        Bs = 8 * ((n_st - ctx.n_poly) * n3 * n3 + ctx.n_poly * n_self_e + 2 * ctx.n_blk * n3
-                  + ctx.n_poly * npk + 5 * ctx.n_poly * n3) + 12 * ctx.n_blk
+        Bs = pairs_once_bytes(ctx)
         traffic_s, traffic_s_src = pmc_traffic(a.n, p, "k_sym")
         res["fp64_pairs_once"] = {
             "value": dofs / (mss * 1e-3), "unit": "DOF-updates/s", "ms_per_step": mss,
