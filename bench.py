@@ -588,9 +588,9 @@ def main():
             u32s = ctx.get_slot(_lib.SLOT_U_CUR)
             n3 = 3 * ctx.Np
             n_st = ctx.n_poly + (ctx.n_blk - ctx.n_poly) // 2
-            # stored FP32 blocks once, per-block products written and read (FP64), five vectors,
-            # three int32 indices per block
-            B32s = 4 * n_st * n3 * n3 + 8 * (2 * ctx.n_blk * n3 + 5 * ctx.n_poly * n3) + 12 * ctx.n_blk
+            # stored FP32 blocks once, per-block FP32 products written and read, five FP64
+            # vectors, three int32 indices per block
+            B32s = 4 * n_st * n3 * n3 + 4 * 2 * ctx.n_blk * n3 + 8 * 5 * ctx.n_poly * n3 + 12 * ctx.n_blk
             traffic32s, traffic32s_src = pmc_traffic(a.n, p, "k_sym32")
             res["fp32_apply"]["pairs_once"] = {
                 "value": dofs / (ms32s * 1e-3), "unit": "DOF-updates/s", "ms_per_step": ms32s,

@@ -380,15 +380,16 @@ int launch_step(LymphCtx* c, int mode, double g, double c1, double c2, const dou
     const double* rd = c->apply_bits == 32 ? c->tv[4] : c->slot[LYMPH_SLOT_RHS_DYN];
     if (c->apply_bits == 32 && c->Ks32) {
       if constexpr (3 * np_of(P) % 12 == 0 && 3 * np_of(P) <= 64) {
+        float* prod32 = reinterpret_cast<float*>(c->prod);  // the 32-bit products in the same buffer
         k_sym_products32<P><<<step_grid(c->n_poly), 64, 0, c->stream>>>(
-            c->Ks32, c->md.row_ptr, c->md.col, c->sidx, c->tdst, u, c->prod, c->n_poly, c->order);
+            c->Ks32, c->md.row_ptr, c->md.col, c->sidx, c->tdst, u, prod32, c->n_poly, c->order);
         if (int rc = check_launch("k_sym_products32")) return rc;
         if (mode == 0)
           k_sym_update32<P, 0><<<step_grid(c->n_poly), 64, 0, c->stream>>>(
-              c->prod, c->md.row_ptr, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->n_poly, c->order);
+              prod32, c->md.row_ptr, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->n_poly, c->order);
         else
           k_sym_update32<P, 1><<<step_grid(c->n_poly), 64, 0, c->stream>>>(
-              c->prod, c->md.row_ptr, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->n_poly, c->order);
+              prod32, c->md.row_ptr, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->n_poly, c->order);
       } else {
         return fail(LYMPH_ERR_STATE, "32-bit stored pairs need 3Np a multiple of 12");
       }

@@ -1268,7 +1268,8 @@ __global__ __launch_bounds__(64) void k_sym_update(const double* __restrict__ pr
 // solves): ~K_QP = R_Q^-T K_PQ^T R_P^-1 = ~K_PQ^T, so the same stored list
 // applies to the FP32 blocks (column quads, 3Np a multiple of 12 so that each
 // of the three chunks is whole quads: p = 3).  Products and per-block sums in
-// FP32 as k_leapfrog32, written as doubles; the transposed product of a
+// FP32 as k_leapfrog32, stored as floats (prod_stride(3Np) per block, so a
+// 32-bit product moves half the bytes of a 64-bit one); the transposed product of a
 // column runs over ascending rows, the chain Q's full row would run, so where
 // the two rounded blocks are exact transposes the products equal the full
 // rows' bit for bit.  k_sym_update32 adds them in block order in FP64 and
@@ -1283,7 +1284,7 @@ __global__ __launch_bounds__(64) void k_sym_products32(const float* __restrict__
                                                        const int* __restrict__ sidx,
                                                        const int* __restrict__ tdst,
                                                        const double* __restrict__ u,
-                                                       double* __restrict__ prod, int n_poly,
+                                                       float* __restrict__ prod, int n_poly,
                                                        const int* __restrict__ order) {
   constexpr int N3 = Dims32<P>::N3, NQ = Dims32<P>::NQ;
   static_assert(N3 <= 64 && N3 % 12 == 0, "one lane per row, whole quads per chunk");
@@ -1323,7 +1324,7 @@ __global__ __launch_bounds__(64) void k_sym_products32(const float* __restrict__
       s = s + kq[q].z * xq[2];
       s = s + kq[q].w * xq[3];
     }
-    if (lane < prod_stride(N3)) prod[(size_t)b * prod_stride(N3) + lane] = (double)s;  // zeros past the rows
+    if (lane < prod_stride(N3)) prod[(size_t)b * prod_stride(N3) + lane] = s;  // zeros past the rows
     if (cb == Pp) continue;  // self block: no partner
     float st = 0.0f;
 #pragma unroll
@@ -1348,12 +1349,12 @@ __global__ __launch_bounds__(64) void k_sym_products32(const float* __restrict__
         st = v;
       }
     }
-    if (lane < prod_stride(N3)) prod[(size_t)td * prod_stride(N3) + lane] = (double)st;
+    if (lane < prod_stride(N3)) prod[(size_t)td * prod_stride(N3) + lane] = st;
   }
 }
 
 template <int P, int MODE>
-__global__ __launch_bounds__(64) void k_sym_update32(const double* __restrict__ prod,
+__global__ __launch_bounds__(64) void k_sym_update32(const float* __restrict__ prod,
                                                      const int* __restrict__ row_ptr,
                                                      const double* __restrict__ rs,
                                                      const double* __restrict__ rd,
@@ -1378,7 +1379,7 @@ __global__ __launch_bounds__(64) void k_sym_update32(const double* __restrict__ 
     double v[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k)
-      v[k] = b + k < b1 ? prod[(size_t)(b + k) * prod_stride(N3) + lane] : 0.0;
+      v[k] = b + k < b1 ? (double)prod[(size_t)(b + k) * prod_stride(N3) + lane] : 0.0;
 #pragma unroll
     for (int k = 0; k < 8; ++k)
       if (b + k < b1) acc = (b + k == b0) ? v[k] : acc + v[k];
