@@ -48,20 +48,22 @@ def log(*a):
 
 
 def stream_probe_gbps():
-    """Achievable HBM bandwidth of this box: (copy GB/s, read GB/s) of 2 GiB
-    STREAM-style kernels (liblymph_bwprobe.so), or (None, None).  The read
-    figure is the better of plain loads (k_read, 4 per thread) and
-    non-temporal loads (k_read_nt, 8 per thread: the way the step kernel
-    streams its blocks)."""
+    """Achievable HBM bandwidth of this box: (copy GB/s, read GB/s) of
+    STREAM-style kernels (liblymph_bwprobe.so), or (None, None).  Copy: 2 GiB.
+    Read: the best of plain loads (k_read, 4 per thread) and non-temporal
+    loads (k_read_nt, 8-32 per thread; k_read_persist, long-lived readers
+    with 16 or 32 per thread: the way the step kernel streams its blocks)
+    over 2 GiB and over 12 GiB (the step's size)."""
     path = os.path.join(ROOT, "lymph3d_apsc_amd", "liblymph_bwprobe.so")
     if not os.path.exists(path):
         return None, None
     L = C.CDLL(path)
     out = []
-    for fn in (L.lymph_bw_copy, L.lymph_bw_read, L.lymph_bw_read_nt):
+    for fn, size, reps in ((L.lymph_bw_copy, 2 << 30, 20), (L.lymph_bw_read, 2 << 30, 20),
+                           (L.lymph_bw_read_nt, 2 << 30, 20), (L.lymph_bw_read_nt, 12 << 30, 10)):
         fn.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         g, ms = C.c_double(), C.c_double()
-        out.append(g.value if fn(2 << 30, 20, C.byref(g), C.byref(ms)) == 0 else None)
+        out.append(g.value if fn(size, reps, C.byref(g), C.byref(ms)) == 0 else None)
     reads = [r for r in out[1:] if r]
     return out[0], (max(reads) if reads else None)
 

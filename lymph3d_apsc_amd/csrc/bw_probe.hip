@@ -76,12 +76,53 @@ __global__ __launch_bounds__(kThreads) void k_read_nt(const double2* __restrict_
   }
 }
 
-// kind: 0 copy, 1 read, 8/16/32 non-temporal read with that many loads per thread
+// Long-lived readers, as the step's waves are: `wgs` workgroups (a multiple
+// of 8), each XCD's share walking a contiguous eighth of the buffer in
+// chunks of kThreads x V double2, V non-temporal loads per thread in flight
+// per chunk.
+template <int V>
+__global__ __launch_bounds__(kThreads) void k_read_persist(const double2* __restrict__ a,
+                                                           double* __restrict__ out, size_t n) {
+  const unsigned per_xcd = gridDim.x >> 3;
+  const unsigned xcd = blockIdx.x & 7, w = blockIdx.x >> 3;
+  const size_t chunk = (size_t)kThreads * V;
+  const size_t n_chunks = (n + chunk - 1) / chunk;
+  const size_t c_per_xcd = (n_chunks + 7) / 8;
+  const size_t c0 = xcd * c_per_xcd, c1 = c0 + c_per_xcd < n_chunks ? c0 + c_per_xcd : n_chunks;
+  double s = 0.0;
+  for (size_t c = c0 + w; c < c1; c += per_xcd) {
+    const size_t base = c * chunk + threadIdx.x;
+    dv2_t v[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      const size_t i = base + (size_t)k * kThreads;
+      v[k] = i < n ? __builtin_nontemporal_load(reinterpret_cast<const dv2_t*>(a + i)) : dv2_t{0.0, 0.0};
+    }
+#pragma unroll
+    for (int k = 0; k < V; ++k) s += v[k].x + v[k].y;
+  }
+  __shared__ double red[kThreads / 64];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w2 = 0; w2 < kThreads / 64; ++w2) t += red[w2];
+    out[blockIdx.x] = t;
+  }
+}
+
+constexpr unsigned kPersistWgs = 256 * 8;  // 8 workgroups of 4 waves per CU
+
+// kind: 0 copy, 1 read, 8/16/32 non-temporal read with that many loads per
+// thread, 116/132 the long-lived reader with 16/32 loads per thread
 int run(int kind, size_t bytes, int reps, double* gbps, double* ms_per) {
   const bool copy = kind == 0;
-  const int per = kind >= 8 ? kind : kVec;
+  const bool persist = kind > 100;
+  const int per = persist ? kind - 100 : (kind >= 8 ? kind : kVec);
   const size_t n = bytes / sizeof(double2);
-  const size_t grid = (n + (size_t)kThreads * per - 1) / ((size_t)kThreads * per);
+  const size_t grid = persist ? kPersistWgs
+                              : (n + (size_t)kThreads * per - 1) / ((size_t)kThreads * per);
   if (n == 0 || reps <= 0 || grid > 0x7fffffffu) return 1;
   double2* a = nullptr;
   double2* b = nullptr;
@@ -105,8 +146,12 @@ int run(int kind, size_t bytes, int reps, double* gbps, double* ms_per) {
       k_read_nt<8><<<(unsigned)grid, kThreads>>>(a, out, n);
     else if (kind == 16)
       k_read_nt<16><<<(unsigned)grid, kThreads>>>(a, out, n);
-    else
+    else if (kind == 32)
       k_read_nt<32><<<(unsigned)grid, kThreads>>>(a, out, n);
+    else if (kind == 116)
+      k_read_persist<16><<<(unsigned)grid, kThreads>>>(a, out, n);
+    else
+      k_read_persist<32><<<(unsigned)grid, kThreads>>>(a, out, n);
   };
   launch();
   (void)hipEventRecord(e0, 0);
@@ -139,10 +184,10 @@ extern "C" int lymph_bw_read(size_t bytes, int reps, double* gbps, double* ms_pe
 }
 
 // The same read with non-temporal 16-byte loads, 8, 16 and 32 in flight per
-// thread; the fastest is reported.
+// thread, and by long-lived readers with 16 and 32; the fastest is reported.
 extern "C" int lymph_bw_read_nt(size_t bytes, int reps, double* gbps, double* ms_per) {
   double best = 0.0, best_ms = 0.0;
-  for (int v : {8, 16, 32}) {
+  for (int v : {8, 16, 32, 116, 132}) {
     double g = 0.0, ms = 0.0;
     if (int rc = run(v, bytes, reps, &g, &ms)) return rc;
     if (g > best) {
