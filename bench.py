@@ -414,6 +414,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="seconds per thread count")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-apply measurement")
+    ap.add_argument("--no-fp48", action="store_true",
+                    help="skip the 48-bit apply (FP32 blocks plus 16-bit corrections) measurement")
     ap.add_argument("--no-pairs-once", action="store_true",
                     help="skip the FP64 step with face pairs stored once")
     ap.add_argument("--no-autotune", action="store_true", help="use the default step variant")
@@ -608,6 +610,34 @@ def main():
             }
             ctx.set_symmetric_storage(False)
         log(f"fp32 apply: {ms32_step:.3f} ms/step, rel diff vs fp64 {res['fp32_apply']['rel_l2_diff_vs_fp64']:.2e}")
+    if not a.no_fp48:
+        # 48-bit apply (FP32 blocks plus a 16-bit correction per entry, FP64 sums): same start
+        # state and steps as the FP64 timed run
+        ctx.set_apply_precision(64)
+        ctx.run(prob, dt, 1e30, max_steps=1 + a.warmup)
+        ctx.set_apply_precision(48)
+        ch48, tune48 = ((None, None, None), {}) if a.no_autotune else ctx.autotune(reps=20)
+        ms48 = ctx.time_steps(prob, a.steps, t_start, dt) / a.steps
+        u48 = ctx.get_slot(_lib.SLOT_U_CUR)
+        B48 = ctx.step_bytes_for(48)
+        l2_48 = float(np.sqrt(ctx.compute_error_l2_matrix_free(u48, u)))
+        res["fp48_apply"] = {
+            "value": dofs / (ms48 * 1e-3), "unit": "DOF-updates/s", "ms_per_step": ms48,
+            "speedup_vs_fp64": ms_step / ms48,
+            "roofline": {"bound": "hbm", "achieved": B48 / (ms48 * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": B48 / (ms48 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "bytes_per_step": B48, "kernel": f"k_leapfrog48<{p},0>"},
+            "ms_per_step_includes": f"the basis transforms of one stepping call of {a.steps} steps",
+            "rel_l2_diff_vs_fp64": float(np.linalg.norm(u48 - u) / np.linalg.norm(u)),
+            "err_l2": float(np.sqrt(ctx.compute_error_l2_matrix_free(u48, uex))),
+            "l2_diff_vs_fp64": l2_48,
+            "l2_diff_vs_fp64_over_fp64_err_l2": l2_48 / e_l2,
+            "steps_from_rest": 1 + a.warmup + a.steps,
+            "autotune": {"chosen": list(ch48),
+                         "ms_per_step": {f"unr{k[0]}_w{k[2]}": v for k, v in tune48.items()}},
+        }
+        ctx.set_apply_precision(64)
+        log(f"fp48 apply: {ms48:.3f} ms/step, rel diff vs fp64 {res['fp48_apply']['rel_l2_diff_vs_fp64']:.2e}")
     del ctx
     if not a.no_ensemble:
         try:

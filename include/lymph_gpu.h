@@ -224,16 +224,22 @@ int lymph_autotune(LymphCtx* ctx, int reps, int* n_variants, int* unr, int* xmod
  * No reference counterpart. */
 int lymph_set_step_variant(LymphCtx* ctx, int index, int* n_variants);
 
-/* Operator precision of the step's block apply: 64 (default) or 32.  With 32
- * the steps run in the mass-orthonormal basis u~ = R u (M = R^T R per
+/* Operator precision of the step's block apply: 64 (default), 32 or 48.
+ * With 32 the steps run in the mass-orthonormal basis u~ = R u (M = R^T R per
  * polyhedron): the blocks R_P^-T K_PQ R_Q^-1 are formed in FP64 and rounded
  * once to FP32 (a second copy, half the bytes per step), products and
  * per-block sums are FP32, the sum over blocks, the RHS, the basis transforms
  * at the start and end of every stepping call and the update stay FP64, and
  * the step needs no triangular solves.  Rounding K itself would perturb
  * M^-1 K by the rounding times cond(M_P), which diverges on ill-conditioned
- * agglomerates.  No reference counterpart (the reference is FP64 only).
- * Call after lymph_build_operator; resets the autotuned variant. */
+ * agglomerates.  48 keeps, beside each FP32 entry hi of that operator, the
+ * 16-bit correction t = (k - hi) / hi * 2^24 (FP16), so that hi (1 + t 2^-24)
+ * carries about 35 bits (6 bytes per entry), and sums the products in FP64:
+ * its state stays within about 1e-9 of the 64-bit step's over 1000 steps
+ * where the 32-bit one drifts to 1e-5.  Selecting 48 needs the FP64 blocks
+ * once (the corrections are formed from them).  No reference counterpart
+ * (the reference is FP64 only).  Call after lymph_build_operator; resets the
+ * autotuned variant. */
 int lymph_set_apply_precision(LymphCtx* ctx, int bits);
 
 /* Device-memory options for large batched runs (no reference counterpart:
@@ -241,9 +247,10 @@ int lymph_set_apply_precision(LymphCtx* ctx, int bits);
  * lymph_ctx_set_keep_adg(ctx, 0) before lymph_build_operator skips the
  * A_dg blocks (needed only by lymph_error_dg, which then returns
  * LYMPH_ERR_STATE); after a build it frees them.
- * lymph_release_fp64_operator frees the FP64 K blocks once the 32-bit apply
- * is selected (lymph_set_apply_precision(ctx, 32)); 64-bit apply and K export
- * then return LYMPH_ERR_STATE until lymph_build_operator runs again. */
+ * lymph_release_fp64_operator frees the FP64 K blocks once the 32- or 48-bit
+ * apply is selected (lymph_set_apply_precision(ctx, 32 or 48)); 64-bit apply,
+ * K export and a first selection of the 48-bit apply then return
+ * LYMPH_ERR_STATE until lymph_build_operator runs again. */
 int lymph_ctx_set_keep_adg(LymphCtx* ctx, int keep);
 
 /* How lymph_build_operator evaluates MAKE_MATRICES_FREE's sums:

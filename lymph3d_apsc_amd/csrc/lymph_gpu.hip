@@ -116,6 +116,7 @@ struct LymphCtx {
   int* gsel = nullptr;
   double *gf0 = nullptr, *gt0 = nullptr, *gtab = nullptr;
   float* K32 = nullptr;     // 32-bit apply: ~K = R^-T K R^-1 (mass-orthonormal basis)
+  _Float16* K16 = nullptr;  // 48-bit apply: per entry of K32 its 16-bit correction
   double* tv[5] = {};       // 32-bit stepping: ~u rotation (0..2), ~F_s (3), ~F_d (4)
   double* Ks = nullptr;     // face pairs stored once (lymph_set_symmetric_storage)
   int* sidx = nullptr;      // per block of the full rows: stored block, ~ = transposed
@@ -274,11 +275,34 @@ template <> struct StepVar32<4> {
   static constexpr int pk[] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 template <int P> constexpr int n_var32() { return (int)(sizeof(StepVar32<P>::unr) / sizeof(int)); }
+// 48-bit apply (k_leapfrog48): column quads per load group, waves per polyhedron
+template <int P> struct StepVar48;
+template <> struct StepVar48<1> {
+  static constexpr int unr[] = {3, 3, 3};
+  static constexpr int wpp[] = {1, 2, 4};
+};
+template <> struct StepVar48<2> {
+  static constexpr int unr[] = {7, 7, 7};
+  static constexpr int wpp[] = {1, 2, 4};
+};
+template <> struct StepVar48<3> {
+  static constexpr int unr[] = {5, 15, 5, 15, 5, 15};
+  static constexpr int wpp[] = {1, 1, 2, 2, 4, 4};
+};
+template <> struct StepVar48<4> {
+  static constexpr int unr[] = {13, 13, 13};
+  static constexpr int wpp[] = {1, 2, 4};
+};
+template <int P> constexpr int n_var48() { return (int)(sizeof(StepVar48<P>::unr) / sizeof(int)); }
 // lymph_autotune reports into caller arrays of 16 entries
 static_assert(n_var<1>() <= 16 && n_var<2>() <= 16 && n_var<3>() <= 16 && n_var<4>() <= 16 &&
                   n_var32<1>() <= 16 && n_var32<2>() <= 16 && n_var32<3>() <= 16 &&
-                  n_var32<4>() <= 16,
+                  n_var32<4>() <= 16 && n_var48<1>() <= 16 && n_var48<2>() <= 16 &&
+                  n_var48<3>() <= 16 && n_var48<4>() <= 16,
               "at most 16 step variants per degree");
+template <int P> int n_var_of(int bits) {
+  return bits == 64 ? n_var<P>() : (bits == 32 ? n_var32<P>() : n_var48<P>());
+}
 
 template <int P, int MODE, int I = 0>
 void launch_leapfrog32(int variant, int grid, hipStream_t st, const float* K, const int* rp,
@@ -318,6 +342,28 @@ void launch_leapfrog32(int variant, int grid, hipStream_t st, const float* K, co
       }
     }
     launch_leapfrog32<P, MODE, I + 1>(variant, grid, st, K, rp, col, Rp, rs, rd, u, wa, out, g, c1,
+                                      c2, gsel, gtab, ord, dyn);
+  }
+}
+
+template <int P, int MODE, int I = 0>
+void launch_leapfrog48(int variant, int grid, hipStream_t st, const float* K, const _Float16* Kr,
+                       const int* rp, const int* col, const double* rs, const double* rd,
+                       const double* u, const double* wa, double* out, double g, double c1,
+                       double c2, const int* gsel, const double* gtab, const int* ord, size_t dyn) {
+  if constexpr (I < n_var48<P>()) {
+    if (variant == I) {
+      constexpr int W = StepVar48<P>::wpp[I];
+      if (W > 1 && dyn > kMaxDynLds) {
+        launch_leapfrog48<P, MODE, 0>(0, grid, st, K, Kr, rp, col, rs, rd, u, wa, out, g, c1, c2,
+                                      gsel, gtab, ord, dyn);
+        return;
+      }
+      k_leapfrog48<P, MODE, StepVar48<P>::unr[I], W><<<step_grid(grid), 64 * W, W > 1 ? dyn : 0, st>>>(
+          K, Kr, rp, col, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid, ord);
+      return;
+    }
+    launch_leapfrog48<P, MODE, I + 1>(variant, grid, st, K, Kr, rp, col, rs, rd, u, wa, out, g, c1,
                                       c2, gsel, gtab, ord, dyn);
   }
 }
@@ -376,8 +422,8 @@ int launch_step(LymphCtx* c, int mode, double g, double c1, double c2, const dou
   return dispatch_p(c->p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
     // the 32-bit apply works on mass-orthonormal vectors: ~F from tilde_rhs
-    const double* rs = c->apply_bits == 32 ? c->tv[3] : c->slot[LYMPH_SLOT_RHS_STAT];
-    const double* rd = c->apply_bits == 32 ? c->tv[4] : c->slot[LYMPH_SLOT_RHS_DYN];
+    const double* rs = c->apply_bits != 64 ? c->tv[3] : c->slot[LYMPH_SLOT_RHS_STAT];
+    const double* rd = c->apply_bits != 64 ? c->tv[4] : c->slot[LYMPH_SLOT_RHS_DYN];
     if (c->apply_bits == 32 && c->Ks32) {
       if constexpr (3 * np_of(P) % 12 == 0 && 3 * np_of(P) <= 64) {
         float* prod32 = reinterpret_cast<float*>(c->prod);  // the 32-bit products in the same buffer
@@ -393,6 +439,13 @@ int launch_step(LymphCtx* c, int mode, double g, double c1, double c2, const dou
       } else {
         return fail(LYMPH_ERR_STATE, "32-bit stored pairs need 3Np a multiple of 12");
       }
+    } else if (c->apply_bits == 48) {
+      if (mode == 0)
+        launch_leapfrog48<P, 0>(c->step_variant, c->n_poly, c->stream, c->K32, c->K16, c->md.row_ptr,
+                                c->md.col, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->order, dyn);
+      else
+        launch_leapfrog48<P, 1>(c->step_variant, c->n_poly, c->stream, c->K32, c->K16, c->md.row_ptr,
+                                c->md.col, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->order, dyn);
     } else if (c->apply_bits == 32) {
       if (mode == 0)
         launch_leapfrog32<P, 0>(c->step_variant, c->n_poly, c->stream, c->K32, c->md.row_ptr,
@@ -678,8 +731,8 @@ int tilde_rhs(LymphCtx* c) {
   return tilde(c, 2, c->slot[LYMPH_SLOT_RHS_DYN], c->tv[4]);
 }
 
-// ~K from the FP64 blocks and the mass factors (K32 and the tilde vectors
-// allocated by the caller)
+// ~K from the FP64 blocks and the mass factors (K32, the tilde vectors and,
+// for the 48-bit apply, K16 allocated by the caller)
 int build_ortho(LymphCtx* c) {
   double* S = nullptr;
   const size_t npk = (size_t)c->Np * (c->Np + 1) / 2;
@@ -691,7 +744,8 @@ int build_ortho(LymphCtx* c) {
     const size_t chunk = 1u << 20;  // workgroups per launch (one per block)
     for (size_t b0 = 0; b0 < (size_t)c->n_blk; b0 += chunk) {
       const unsigned nb = (unsigned)std::min(chunk, (size_t)c->n_blk - b0);
-      k_ortho_operator<P><<<nb, 256, 0, c->stream>>>(c->K, S, c->blk_row, c->md.col, c->K32, b0);
+      k_ortho_operator<P><<<nb, 256, 0, c->stream>>>(c->K, S, c->blk_row, c->md.col, c->K32, c->K16,
+                                                     b0);
       if (int r = check_launch("k_ortho_operator")) return r;
     }
     return 0;
@@ -778,6 +832,7 @@ int lymph_ctx_destroy(LymphCtx* c) {
   if (c->rnow) (void)hipFree(c->rnow);
   if (c->trace) (void)hipFree(c->trace);
   if (c->K32) (void)hipFree(c->K32);
+  if (c->K16) (void)hipFree(c->K16);
   for (double* q : c->tv)
     if (q) (void)hipFree(q);
   if (c->rpoly_u) (void)hipFree(c->rpoly_u);
@@ -1119,8 +1174,8 @@ int lymph_ctx_set_keep_adg(LymphCtx* c, int keep) {
 
 int lymph_release_fp64_operator(LymphCtx* c) {
   if (int rc = need_built(c)) return rc;
-  if (c->apply_bits != 32 || !c->K32)
-    return fail(LYMPH_ERR_STATE, "select the 32-bit apply (lymph_set_apply_precision) first");
+  if (c->apply_bits == 64 || !c->K32)
+    return fail(LYMPH_ERR_STATE, "select the 32- or 48-bit apply (lymph_set_apply_precision) first");
   if (c->K) {
     HIP_TRY(hipStreamSynchronize(c->stream));
     drop_graph(c);
@@ -1361,7 +1416,7 @@ namespace {
 // 32-bit apply steps the mass-orthonormal vectors and transforms back.
 int step_dev(LymphCtx* c, int mode, double g, double c1, double c2, const double* a,
              const double* b, double* out, bool grouped) {
-  if (c->apply_bits != 32) return launch_step(c, mode, g, c1, c2, a, b, out, grouped);
+  if (c->apply_bits == 64) return launch_step(c, mode, g, c1, c2, a, b, out, grouped);
   if (int rc = tilde_rhs(c)) return rc;
   if (int rc = tilde(c, 0, a, c->tv[0])) return rc;
   if (int rc = tilde(c, 0, b, c->tv[1])) return rc;
@@ -1377,7 +1432,7 @@ int sample_into(LymphCtx* c, const double* u, double* d_out);
 // copies and writes the last two states back into U_PREV, U_CUR.
 int run_loop(LymphCtx* c, int problem, int nsteps, double t0, double dt, int every, bool record,
              double* t_out) {
-  const bool b32 = c->apply_bits == 32;
+  const bool b32 = c->apply_bits != 64;  // mass-orthonormal copies (32- and 48-bit apply)
   double *prev, *cur, *next;
   if (b32) {
     if (int rc = tilde_rhs(c)) return rc;
@@ -1502,8 +1557,8 @@ int lymph_autotune(LymphCtx* c, int reps, int* n_variants, int* unr, int* xmode,
     return fail(LYMPH_ERR_STATE, "face pairs are stored once: the full-row variants are not in use");
   return dispatch_p(c->p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
-    const bool b32 = c->apply_bits == 32;
-    const int NV = b32 ? n_var32<P>() : n_var<P>();
+    const int bits = c->apply_bits;
+    const int NV = n_var_of<P>(bits);
     if (n_variants) *n_variants = NV;
     const bool fits = (size_t)c->max_row * c->n3 * sizeof(double) <= kMaxDynLds;
     hipEvent_t e0, e1;
@@ -1518,9 +1573,12 @@ int lymph_autotune(LymphCtx* c, int reps, int* n_variants, int* unr, int* xmode,
     double best = 1e300;
     int bi = 0;
     for (int v = 0; v < NV; ++v) {
-      const int w = b32 ? StepVar32<P>::wpp[v] : StepVar<P>::wpp[v];
-      if (unr) unr[v] = b32 ? StepVar32<P>::unr[v] : StepVar<P>::unr[v];
-      if (xmode) xmode[v] = b32 ? 1 + StepVar32<P>::pk[v] : StepVar<P>::xs[v];
+      const int w = bits == 64 ? StepVar<P>::wpp[v]
+                               : (bits == 32 ? StepVar32<P>::wpp[v] : StepVar48<P>::wpp[v]);
+      if (unr)
+        unr[v] = bits == 64 ? StepVar<P>::unr[v]
+                            : (bits == 32 ? StepVar32<P>::unr[v] : StepVar48<P>::unr[v]);
+      if (xmode) xmode[v] = bits == 64 ? StepVar<P>::xs[v] : (bits == 32 ? 1 + StepVar32<P>::pk[v] : 1);
       if (wpp) wpp[v] = w;
       if (w > 1 && !fits) {  // would run variant 0: not a candidate
         if (ms) ms[v] = -1.0;
@@ -1557,7 +1615,7 @@ int lymph_set_step_variant(LymphCtx* c, int index, int* n_variants) {
   if (int rc = need_ctx(c)) return rc;
   return dispatch_p(c->p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
-    const int NV = c->apply_bits == 32 ? n_var32<P>() : n_var<P>();
+    const int NV = n_var_of<P>(c->apply_bits);
     if (n_variants) *n_variants = NV;
     if (index < 0 || index >= NV) return fail(LYMPH_ERR_ARG, "no such step variant");
     c->step_variant = index;
@@ -1567,27 +1625,57 @@ int lymph_set_step_variant(LymphCtx* c, int index, int* n_variants) {
 
 int lymph_set_apply_precision(LymphCtx* c, int bits) {
   if (int rc = need_built(c)) return rc;
-  if (bits != 32 && bits != 64) return fail(LYMPH_ERR_ARG, "precision must be 32 or 64");
+  if (bits != 32 && bits != 48 && bits != 64)
+    return fail(LYMPH_ERR_ARG, "precision must be 32, 48 or 64");
   if (bits == 64 && !c->K)
     return fail(LYMPH_ERR_STATE, "FP64 K blocks were released: rebuild the operator");
+  if (bits == 48 && !c->K16 && !c->K)
+    return fail(LYMPH_ERR_STATE, "the 48-bit corrections need the FP64 K blocks: rebuild the operator");
   if (bits != c->apply_bits) drop_graph(c);  // a captured solve steps the other apply
-  if (bits == 32 && !c->K32) {
+  const size_t n16 = (size_t)c->n_blk * Dims32Any(c->p);
+  auto drop16 = [&]() {  // corrections allocated by this call but never written
+    (void)hipFree(c->K16);
+    c->K16 = nullptr;
+    c->bytes -= n16 * sizeof(_Float16);
+  };
+  bool new16 = false;
+  if (bits == 48 && !c->K16) {
+    // the corrections come with a fresh ~K: both are written by one pass
+    _Float16* k16 = nullptr;
+    if (int r = dalloc(&k16, n16)) return r;
+    c->K16 = k16;
+    c->bytes += n16 * sizeof(_Float16);
+    new16 = true;
+    if (c->K32)
+      if (int r = build_ortho(c)) {
+        drop16();
+        return r;
+      }
+  }
+  if (bits != 64 && !c->K32) {
     // vectors first: K32 exists only once everything the 32-bit path uses does
     for (double*& q : c->tv) {
       if (!q) {
-        if (int r = dalloc(&q, vec_n(c))) return r;
+        if (int r = dalloc(&q, vec_n(c))) {
+          if (new16) drop16();
+          return r;
+        }
         c->bytes += vec_n(c) * sizeof(double);
       }
     }
     const size_t n = (size_t)c->n_blk * Dims32Any(c->p);
     float* k32 = nullptr;
-    if (int r = dalloc(&k32, n)) return r;
+    if (int r = dalloc(&k32, n)) {
+      if (new16) drop16();
+      return r;
+    }
     c->K32 = k32;
     c->bytes += n * sizeof(float);
     if (int r = build_ortho(c)) {
       (void)hipFree(c->K32);
       c->K32 = nullptr;
       c->bytes -= n * sizeof(float);
+      if (new16) drop16();
       return r;
     }
   }

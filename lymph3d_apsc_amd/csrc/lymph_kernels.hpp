@@ -1776,6 +1776,16 @@ __global__ void k_inv_upper(const double* __restrict__ RM, double* __restrict__ 
   for (int i = 0; i <= j; ++i) Sp[pk_off(NP, i) + (j - i)] = x[i];
 }
 
+// The 48-bit apply keeps an operator entry k as its FP32 rounding hi and a
+// 16-bit correction t = (k - hi) / hi * 2^24 (|t| <= 1 since hi is k
+// rounded to nearest; FP16 keeps 11 significant bits of it), so that
+// hi (1 + t 2^-24) carries about 35 bits: a relative error of 2^-36 where
+// the FP32 rounding leaves 2^-25.
+__device__ __forceinline__ _Float16 split_residual(double k, float hi) {
+  if (hi == 0.0f) return (_Float16)0.0f;
+  return (_Float16)(float)((k - (double)hi) / (double)hi * 0x1p24);
+}
+
 // ~K_PQ = S_P^T K_PQ S_Q rounded to FP32 in the column-quad layout; one
 // workgroup per block, one component strip (Np rows) at a time:
 // T = X S_Q (per column component), Z = S_P^T T.
@@ -1784,7 +1794,8 @@ __global__ __launch_bounds__(256) void k_ortho_operator(const double* __restrict
                                                         const double* __restrict__ S,
                                                         const int* __restrict__ blk_row,
                                                         const int* __restrict__ col,
-                                                        float* __restrict__ K32, size_t b0) {
+                                                        float* __restrict__ K32,
+                                                        _Float16* __restrict__ K16, size_t b0) {
   using D = Dims<P>;
   constexpr int NP = D::NP, N3 = D::N3, NPK = D::NPK;
   __shared__ double SP[NPK], SQ[NPK];
@@ -1798,6 +1809,7 @@ __global__ __launch_bounds__(256) void k_ortho_operator(const double* __restrict
   }
   const double* Kb = K + b * D::BS;
   float* Ob = K32 + b * Dims32<P>::BS;
+  _Float16* Rb = K16 ? K16 + b * Dims32<P>::BS : nullptr;
   for (int i = 0; i < 3; ++i) {
     __syncthreads();
     for (int e = tid; e < NP * N3; e += blockDim.x) {
@@ -1816,13 +1828,19 @@ __global__ __launch_bounds__(256) void k_ortho_operator(const double* __restrict
       const int r = e / N3, c = e % N3;
       double s = 0.0;
       for (int k = 0; k <= r; ++k) s = s + SP[pk_off(NP, k) + (r - k)] * T[k * N3 + c];
-      Ob[kpos32(N3, i * NP + r, c)] = (float)s;
+      const float hi = (float)s;
+      const size_t o = kpos32(N3, i * NP + r, c);
+      Ob[o] = hi;
+      if (Rb) Rb[o] = split_residual(s, hi);
     }
   }
   // the floats that round the block stride up to a multiple of 4
   constexpr int PAD = (int)(Dims32<P>::BS - (size_t)N3 * N3);
   if constexpr (PAD > 0)
-    if (tid < PAD) Ob[(size_t)N3 * N3 + tid] = 0.0f;
+    if (tid < PAD) {
+      Ob[(size_t)N3 * N3 + tid] = 0.0f;
+      if (Rb) Rb[(size_t)N3 * N3 + tid] = (_Float16)0.0f;
+    }
 }
 
 // Mass-orthonormal transforms with one wave per polyhedron, lane i on row i
@@ -2065,6 +2083,167 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog32(const float* __restrict
   const size_t o = (size_t)Pp * N3;
   // time factor: one scalar, or per source group (ensembles whose instances
   // carry their own wavelet; the table is filled for this step)
+  const double gp = gsel ? gtab[gsel[Pp]] : g;
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) {
+    const int r = lane + 64 * k;
+    if (r < N3) {
+      const double w = -acc[k] + rs[o + r] + rd[o + r] * gp;
+      if (MODE == 0)
+        out[o + r] = c1 * w + 2.0 * u[o + r] - wa[o + r];
+      else
+        out[o + r] = c1 * w + u[o + r] + c2 * wa[o + r];
+    }
+  }
+}
+
+// The 48-bit apply: k_leapfrog32's walk over ~K (mass-orthonormal basis, no
+// solves) with each entry hi (1 + t 2^-24) (split_residual): hi x summed in
+// FP64 fused multiply-adds from the FP64 coefficients, the correction
+// (hi t) x in FP32 and scaled by 2^-24 once per block, then the block
+// products added in block order in FP64 and k_leapfrog32's update.  6 bytes
+// per operator entry; the state stays within about 1e-9 of the FP64 step's
+// over 1000 steps where the 32-bit apply drifts to 1e-5.
+typedef _Float16 lymph_hv4 __attribute__((ext_vector_type(4)));
+__device__ inline lymph_hv4 load_stream_h4(const _Float16* p) {
+  return __builtin_nontemporal_load(reinterpret_cast<const lymph_hv4*>(p));
+}
+
+template <int P, int MODE, int UNR, int WPP>
+__global__ __launch_bounds__(64 * WPP) void k_leapfrog48(const float* __restrict__ K,
+                                                   const _Float16* __restrict__ Kr,
+                                                   const int* __restrict__ row_ptr,
+                                                   const int* __restrict__ col,
+                                                   const double* __restrict__ rs,
+                                                   const double* __restrict__ rd,
+                                                   const double* __restrict__ u,
+                                                   const double* __restrict__ wa,
+                                                   double* __restrict__ out, double g, double c1,
+                                                   double c2, const int* __restrict__ gsel,
+                                                   const double* __restrict__ gtab, int n_poly,
+                                                   const int* __restrict__ order) {
+  using D = Dims<P>;
+  constexpr int N3 = D::N3, NQUAD = Dims32<P>::NQ, TAIL = Dims32<P>::T;
+  constexpr int RPL = (N3 + 63) / 64;
+  extern __shared__ double sblk[];  // WPP > 1: per-block products [block][row]
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int Pp = step_poly(order);
+  if (Pp >= n_poly) return;  // padding workgroups of the last XCD share
+  double acc[RPL];
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) acc[k] = 0.0;
+  const int b0 = row_ptr[Pp], b1 = row_ptr[Pp + 1];
+  for (int b = b0 + wv; b < b1; b += WPP) {
+    const int cb = __builtin_amdgcn_readfirstlane(col[b]);
+    const double* __restrict__ xv = u + (size_t)cb * N3;
+    const float* __restrict__ Kb = K + (size_t)b * Dims32<P>::BS;
+    const _Float16* __restrict__ Rb = Kr + (size_t)b * Dims32<P>::BS;
+    double xl[RPL];
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) {
+      const int r = lane + 64 * k;
+      xl[k] = r < N3 ? xv[r] : 0.0;
+    }
+    double s[RPL];
+    float sc[RPL];
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) {
+      s[k] = 0.0;
+      sc[k] = 0.0f;
+    }
+#pragma unroll 1
+    for (int q0 = 0; q0 < NQUAD; q0 += UNR) {
+      float4 kv[UNR][RPL];
+      lymph_hv4 tv[UNR][RPL];
+#pragma unroll
+      for (int uu = 0; uu < UNR; ++uu) {
+        const int q = q0 + uu;
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) {
+          const int r = lane + 64 * k;
+          if ((NQUAD % UNR == 0 || q < NQUAD) && r < N3) {
+            kv[uu][k] = load_stream(Kb + ((size_t)q * N3 + r) * 4);
+            tv[uu][k] = load_stream_h4(Rb + ((size_t)q * N3 + r) * 4);
+          }
+        }
+      }
+#pragma unroll
+      for (int uu = 0; uu < UNR; ++uu) {
+        const int q = q0 + uu;
+        if (NQUAD % UNR == 0 || q < NQUAD) {
+          double xq[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int c = 4 * q + j;
+            const double src = (RPL > 1 && (c >> 6)) ? xl[RPL - 1] : xl[0];
+            xq[j] = (c < N3) ? lane_bcast(src, c & 63) : 0.0;
+          }
+#pragma unroll
+          for (int k = 0; k < RPL; ++k) {
+            const int r = lane + 64 * k;
+            if (r < N3) {
+              const float4 h = kv[uu][k];
+              const lymph_hv4 t = tv[uu][k];
+              s[k] = fma((double)h.x, xq[0], s[k]);
+              s[k] = fma((double)h.y, xq[1], s[k]);
+              s[k] = fma((double)h.z, xq[2], s[k]);
+              s[k] = fma((double)h.w, xq[3], s[k]);
+              sc[k] = fmaf(h.x * (float)t.x, (float)xq[0], sc[k]);
+              sc[k] = fmaf(h.y * (float)t.y, (float)xq[1], sc[k]);
+              sc[k] = fmaf(h.z * (float)t.z, (float)xq[2], sc[k]);
+              sc[k] = fmaf(h.w * (float)t.w, (float)xq[3], sc[k]);
+            }
+          }
+        }
+      }
+    }
+    if constexpr (TAIL > 0) {  // last N3 % 4 columns, stored row by row
+      constexpr int C0 = 4 * NQUAD;
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) {
+        const int r = lane + 64 * k;
+#pragma unroll
+        for (int j = 0; j < TAIL; ++j) {
+          const int c = C0 + j;
+          const double src = (RPL > 1 && (c >> 6)) ? xl[RPL - 1] : xl[0];
+          const double xt = lane_bcast(src, c & 63);
+          if (r < N3) {
+            const size_t o = (size_t)C0 * N3 + (size_t)r * TAIL + j;
+            const float h = Kb[o];
+            s[k] = fma((double)h, xt, s[k]);
+            sc[k] = fmaf(h * (float)Rb[o], (float)xt, sc[k]);
+          }
+        }
+      }
+    }
+    if constexpr (WPP == 1) {
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) {
+        const double v = s[k] + (double)sc[k] * 0x1p-24;
+        acc[k] = (b == b0) ? v : acc[k] + v;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) {
+        const int r = lane + 64 * k;
+        if (r < N3) sblk[(size_t)(b - b0) * N3 + r] = s[k] + (double)sc[k] * 0x1p-24;
+      }
+    }
+  }
+  if constexpr (WPP > 1) {
+    __syncthreads();
+    if (wv != 0) return;  // no barrier follows in this kernel
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) {
+      const int r = lane + 64 * k;
+      if (r < N3) {
+        double a = sblk[r];
+        for (int j = 1; j < b1 - b0; ++j) a = a + sblk[(size_t)j * N3 + r];
+        acc[k] = a;
+      }
+    }
+  }
+  const size_t o = (size_t)Pp * N3;
   const double gp = gsel ? gtab[gsel[Pp]] : g;
 #pragma unroll
   for (int k = 0; k < RPL; ++k) {

@@ -123,8 +123,8 @@ class Ensemble:
             ctx.set_keep_adg(False)  # traces only: no DG-norm error, half the build memory
             ctx.set_build("factored")
             ctx.make_matrices_free()
-            if apply_bits == 32:
-                ctx.set_apply_precision(32)
+            if apply_bits != 64:  # 32- or 48-bit apply: the FP64 blocks are not needed after it
+                ctx.set_apply_precision(apply_bits)
                 ctx.release_fp64_operator()
             # sources and receivers located in each instance, ids offset into the union
             tet_off = np.cumsum([0] + [m.n_tet for m in meshes])

@@ -220,7 +220,9 @@ class Context:
         return nv.value
 
     def set_apply_precision(self, bits):
-        """64 (default) or 32: precision of the step's operator apply."""
+        """Precision of the step's operator apply: 64 (default), 32 (FP32
+        blocks in the mass-orthonormal basis, FP32 sums) or 48 (those FP32
+        blocks plus a 16-bit correction per entry, FP64 sums)."""
         check(lib().lymph_set_apply_precision(self.h, bits), "lymph_set_apply_precision")
         self.apply_bits = bits
 
@@ -261,9 +263,10 @@ class Context:
 
     def step_bytes_for(self, bits):
         """Algorithmic bytes of one step with the operator in `bits`
-        precision: the blocks, five vectors, and for the 64-bit apply the
-        packed mass factors (the 32-bit apply works in the mass-orthonormal
-        basis and solves nothing)."""
+        precision (64, 32, or 48: 4 bytes per entry plus a 2-byte
+        correction): the blocks, five vectors, and for the 64-bit apply the
+        packed mass factors (the 32- and 48-bit applies work in the
+        mass-orthonormal basis and solve nothing)."""
         n3, Np = self.n3, self.Np
         factors = 8 * self.n_poly * Np * (Np + 1) // 2 if bits == 64 else 0
         return (bits // 8) * self.n_blk * n3 * n3 + factors + 8 * 5 * self.n_poly * n3
