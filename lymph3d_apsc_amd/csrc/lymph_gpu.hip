@@ -277,20 +277,20 @@ template <> struct StepVar32<4> {
 template <int P> constexpr int n_var32() { return (int)(sizeof(StepVar32<P>::unr) / sizeof(int)); }
 // 48-bit apply (k_leapfrog48): column quads per load group, waves per polyhedron
 template <int P> struct StepVar48;
-template <> struct StepVar48<1> {
-  static constexpr int unr[] = {3, 3, 3};
+template <> struct StepVar48<1> {  // quads per load group: even (the corrections come in quad pairs)
+  static constexpr int unr[] = {4, 4, 4};
   static constexpr int wpp[] = {1, 2, 4};
 };
 template <> struct StepVar48<2> {
-  static constexpr int unr[] = {7, 7, 7};
+  static constexpr int unr[] = {8, 8, 8};
   static constexpr int wpp[] = {1, 2, 4};
 };
 template <> struct StepVar48<3> {
-  static constexpr int unr[] = {5, 15, 5, 15, 5, 15};
+  static constexpr int unr[] = {6, 16, 6, 16, 6, 16};
   static constexpr int wpp[] = {1, 1, 2, 2, 4, 4};
 };
 template <> struct StepVar48<4> {
-  static constexpr int unr[] = {13, 13, 13};
+  static constexpr int unr[] = {14, 14, 14};
   static constexpr int wpp[] = {1, 2, 4};
 };
 template <int P> constexpr int n_var48() { return (int)(sizeof(StepVar48<P>::unr) / sizeof(int)); }
@@ -375,6 +375,11 @@ int check_launch(const char* what) {
 }
 
 unsigned grid1(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+size_t Dims16Any(int p) {  // Dims16<p>::BS
+  const size_t n3 = (size_t)(p + 1) * (p + 2) * (p + 3) / 2;
+  const size_t npr = (n3 / 4 + 1) / 2;
+  return (npr * n3 * 8 + n3 * (n3 % 4) + 7) / 8 * 8;
+}
 size_t Dims32Any(int p) {  // Dims32<p>::BS
   const size_t n3 = (size_t)(p + 1) * (p + 2) * (p + 3) / 2;
   return (n3 * n3 + 3) / 4 * 4;
@@ -734,6 +739,8 @@ int tilde_rhs(LymphCtx* c) {
 // ~K from the FP64 blocks and the mass factors (K32, the tilde vectors and,
 // for the 48-bit apply, K16 allocated by the caller)
 int build_ortho(LymphCtx* c) {
+  if (c->K16)  // the corrections' unused slots (an odd count of quads) read as zeros
+    HIP_TRY(hipMemsetAsync(c->K16, 0, (size_t)c->n_blk * Dims16Any(c->p) * sizeof(_Float16), c->stream));
   double* S = nullptr;
   const size_t npk = (size_t)c->Np * (c->Np + 1) / 2;
   if (int rc = dalloc(&S, (size_t)c->n_poly * npk)) return rc;
@@ -1632,7 +1639,7 @@ int lymph_set_apply_precision(LymphCtx* c, int bits) {
   if (bits == 48 && !c->K16 && !c->K)
     return fail(LYMPH_ERR_STATE, "the 48-bit corrections need the FP64 K blocks: rebuild the operator");
   if (bits != c->apply_bits) drop_graph(c);  // a captured solve steps the other apply
-  const size_t n16 = (size_t)c->n_blk * Dims32Any(c->p);
+  const size_t n16 = (size_t)c->n_blk * Dims16Any(c->p);
   auto drop16 = [&]() {  // corrections allocated by this call but never written
     (void)hipFree(c->K16);
     c->K16 = nullptr;

@@ -70,6 +70,25 @@ struct Dims32 {
   static constexpr int NQ = N3 / 4, T = N3 % 4;
   static constexpr size_t BS = ((size_t)N3 * N3 + 3) / 4 * 4;
 };
+// The 48-bit apply's 16-bit corrections: column quads taken in pairs, a
+// row's 8 values of quads 2k, 2k+1 contiguous (one 16-byte load), then the
+// N3 % 4 tail columns row by row; blocks padded to 8 values.
+template <int P>
+struct Dims16 {
+  static constexpr int N3 = 3 * np_of(P);
+  static constexpr int NQ = N3 / 4, T = N3 % 4, NPR = (NQ + 1) / 2;
+  static constexpr size_t TAIL0 = (size_t)NPR * N3 * 8;
+  static constexpr size_t BS = (TAIL0 + (size_t)N3 * T + 7) / 8 * 8;
+};
+LYMPH_HD size_t kpos16(int N3, int r, int c) {
+  const int c0 = N3 / 4 * 4;
+  if (c < c0) {
+    const int q = c >> 2;
+    return ((size_t)(q >> 1) * N3 + r) * 8 + (q & 1) * 4 + (c & 3);
+  }
+  const int npr = (N3 / 4 + 1) / 2;
+  return (size_t)npr * N3 * 8 + (size_t)r * (N3 % 4) + (c - c0);
+}
 LYMPH_HD size_t kpos32(int N3, int r, int c) {
   const int c0 = N3 / 4 * 4;
   if (c < c0) return ((size_t)(c >> 2) * N3 + r) * 4 + (c & 3);
@@ -1809,7 +1828,7 @@ __global__ __launch_bounds__(256) void k_ortho_operator(const double* __restrict
   }
   const double* Kb = K + b * D::BS;
   float* Ob = K32 + b * Dims32<P>::BS;
-  _Float16* Rb = K16 ? K16 + b * Dims32<P>::BS : nullptr;
+  _Float16* Rb = K16 ? K16 + b * Dims16<P>::BS : nullptr;  // unused slots zeroed by the host
   for (int i = 0; i < 3; ++i) {
     __syncthreads();
     for (int e = tid; e < NP * N3; e += blockDim.x) {
@@ -1831,16 +1850,13 @@ __global__ __launch_bounds__(256) void k_ortho_operator(const double* __restrict
       const float hi = (float)s;
       const size_t o = kpos32(N3, i * NP + r, c);
       Ob[o] = hi;
-      if (Rb) Rb[o] = split_residual(s, hi);
+      if (Rb) Rb[kpos16(N3, i * NP + r, c)] = split_residual(s, hi);
     }
   }
   // the floats that round the block stride up to a multiple of 4
   constexpr int PAD = (int)(Dims32<P>::BS - (size_t)N3 * N3);
   if constexpr (PAD > 0)
-    if (tid < PAD) {
-      Ob[(size_t)N3 * N3 + tid] = 0.0f;
-      if (Rb) Rb[(size_t)N3 * N3 + tid] = (_Float16)0.0f;
-    }
+    if (tid < PAD) Ob[(size_t)N3 * N3 + tid] = 0.0f;
 }
 
 // Mass-orthonormal transforms with one wave per polyhedron, lane i on row i
@@ -2105,6 +2121,10 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog32(const float* __restrict
 // per operator entry; the state stays within about 1e-9 of the FP64 step's
 // over 1000 steps where the 32-bit apply drifts to 1e-5.
 typedef _Float16 lymph_hv4 __attribute__((ext_vector_type(4)));
+typedef _Float16 lymph_hv8 __attribute__((ext_vector_type(8)));
+__device__ inline lymph_hv8 load_stream_h8(const _Float16* p) {
+  return __builtin_nontemporal_load(reinterpret_cast<const lymph_hv8*>(p));
+}
 __device__ inline lymph_hv4 load_stream_h4(const _Float16* p) {
   return __builtin_nontemporal_load(reinterpret_cast<const lymph_hv4*>(p));
 }
@@ -2125,6 +2145,7 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog48(const float* __restrict
   using D = Dims<P>;
   constexpr int N3 = D::N3, NQUAD = Dims32<P>::NQ, TAIL = Dims32<P>::T;
   constexpr int RPL = (N3 + 63) / 64;
+  static_assert(UNR % 2 == 0, "whole quad pairs per load group");
   extern __shared__ double sblk[];  // WPP > 1: per-block products [block][row]
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int Pp = step_poly(order);
@@ -2137,7 +2158,7 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog48(const float* __restrict
     const int cb = __builtin_amdgcn_readfirstlane(col[b]);
     const double* __restrict__ xv = u + (size_t)cb * N3;
     const float* __restrict__ Kb = K + (size_t)b * Dims32<P>::BS;
-    const _Float16* __restrict__ Rb = Kr + (size_t)b * Dims32<P>::BS;
+    const _Float16* __restrict__ Rb = Kr + (size_t)b * Dims16<P>::BS;
     double xl[RPL];
 #pragma unroll
     for (int k = 0; k < RPL; ++k) {
@@ -2152,25 +2173,33 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog48(const float* __restrict
       sc[k] = 0.0f;
     }
 #pragma unroll 1
-    for (int q0 = 0; q0 < NQUAD; q0 += UNR) {
+    for (int q0 = 0; q0 < NQUAD; q0 += UNR) {  // UNR even: whole quad pairs
       float4 kv[UNR][RPL];
-      lymph_hv4 tv[UNR][RPL];
+      lymph_hv8 tv[UNR / 2][RPL];
 #pragma unroll
       for (int uu = 0; uu < UNR; ++uu) {
         const int q = q0 + uu;
 #pragma unroll
         for (int k = 0; k < RPL; ++k) {
           const int r = lane + 64 * k;
-          if ((NQUAD % UNR == 0 || q < NQUAD) && r < N3) {
+          if (q < NQUAD && r < N3) {
             kv[uu][k] = load_stream(Kb + ((size_t)q * N3 + r) * 4);
-            tv[uu][k] = load_stream_h4(Rb + ((size_t)q * N3 + r) * 4);
+            if ((uu & 1) == 0) {
+              const _Float16* tp = Rb + ((size_t)(q >> 1) * N3 + r) * 8;
+              if (q + 1 < NQUAD) {
+                tv[uu / 2][k] = load_stream_h8(tp);
+              } else {  // the last quad of an odd count: half a pair
+                const lymph_hv4 h = load_stream_h4(tp);
+                tv[uu / 2][k] = lymph_hv8{h.x, h.y, h.z, h.w, 0, 0, 0, 0};
+              }
+            }
           }
         }
       }
 #pragma unroll
       for (int uu = 0; uu < UNR; ++uu) {
         const int q = q0 + uu;
-        if (NQUAD % UNR == 0 || q < NQUAD) {
+        if (q < NQUAD) {
           double xq[4];
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
@@ -2183,15 +2212,16 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog48(const float* __restrict
             const int r = lane + 64 * k;
             if (r < N3) {
               const float4 h = kv[uu][k];
-              const lymph_hv4 t = tv[uu][k];
+              const lymph_hv8 t8 = tv[uu / 2][k];
+              const int o4 = (uu & 1) * 4;
               s[k] = fma((double)h.x, xq[0], s[k]);
               s[k] = fma((double)h.y, xq[1], s[k]);
               s[k] = fma((double)h.z, xq[2], s[k]);
               s[k] = fma((double)h.w, xq[3], s[k]);
-              sc[k] = fmaf(h.x * (float)t.x, (float)xq[0], sc[k]);
-              sc[k] = fmaf(h.y * (float)t.y, (float)xq[1], sc[k]);
-              sc[k] = fmaf(h.z * (float)t.z, (float)xq[2], sc[k]);
-              sc[k] = fmaf(h.w * (float)t.w, (float)xq[3], sc[k]);
+              sc[k] = fmaf(h.x * (float)t8[o4 + 0], (float)xq[0], sc[k]);
+              sc[k] = fmaf(h.y * (float)t8[o4 + 1], (float)xq[1], sc[k]);
+              sc[k] = fmaf(h.z * (float)t8[o4 + 2], (float)xq[2], sc[k]);
+              sc[k] = fmaf(h.w * (float)t8[o4 + 3], (float)xq[3], sc[k]);
             }
           }
         }
@@ -2208,25 +2238,24 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog48(const float* __restrict
           const double src = (RPL > 1 && (c >> 6)) ? xl[RPL - 1] : xl[0];
           const double xt = lane_bcast(src, c & 63);
           if (r < N3) {
-            const size_t o = (size_t)C0 * N3 + (size_t)r * TAIL + j;
-            const float h = Kb[o];
+            const float h = Kb[(size_t)C0 * N3 + (size_t)r * TAIL + j];
+            const float t = (float)Rb[Dims16<P>::TAIL0 + (size_t)r * TAIL + j];
             s[k] = fma((double)h, xt, s[k]);
-            sc[k] = fmaf(h * (float)Rb[o], (float)xt, sc[k]);
+            sc[k] = fmaf(h * t, (float)xt, sc[k]);
           }
         }
       }
     }
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) s[k] = s[k] + (double)sc[k] * 0x1p-24;
     if constexpr (WPP == 1) {
 #pragma unroll
-      for (int k = 0; k < RPL; ++k) {
-        const double v = s[k] + (double)sc[k] * 0x1p-24;
-        acc[k] = (b == b0) ? v : acc[k] + v;
-      }
+      for (int k = 0; k < RPL; ++k) acc[k] = (b == b0) ? s[k] : acc[k] + s[k];
     } else {
 #pragma unroll
       for (int k = 0; k < RPL; ++k) {
         const int r = lane + 64 * k;
-        if (r < N3) sblk[(size_t)(b - b0) * N3 + r] = s[k] + (double)sc[k] * 0x1p-24;
+        if (r < N3) sblk[(size_t)(b - b0) * N3 + r] = s[k];
       }
     }
   }
