@@ -203,6 +203,17 @@ def ensemble_leg(n_inst, steps, plain_sample, f0=5.0, seed0=1000):
     t_ens64 = time.perf_counter() - t0
     ens64.close()
     del ens64
+    # and with the 48-bit apply (FP32 blocks plus 16-bit corrections, FP64 sums)
+    t0 = time.perf_counter()
+    ens48 = E.Ensemble(inst, dt, steps, every=1, f0=f0, apply_bits=48, autotune=True)
+    t_setup48 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tr48 = ens48.run()  # one replay
+    t_ens48 = time.perf_counter() - t0
+    ens48.close()
+    del ens48
+    errs48 = [float(np.linalg.norm(tr48[k] - tr64[k]) / max(np.linalg.norm(tr64[k]), 1e-300))
+              for k in range(n_inst)]
     # plain path on a sample: FP64, one instance at a time, uncaptured
     sample = list(range(0, n_inst, max(1, n_inst // plain_sample)))[:plain_sample]
     t_plain, t_plain_setup, errs = 0.0, 0.0, []
@@ -248,6 +259,14 @@ def ensemble_leg(n_inst, steps, plain_sample, f0=5.0, seed0=1000):
             "fp32_trace_rel_l2_vs_this_mean": float(np.mean(errs64)),
             "note": "the same instances, graph and launches with the FP64 apply: the gain of "
                     "batching alone; the FP32 ensemble's further gain is its halved bytes"},
+        "fp48_ensemble": {
+            "value": n_inst / t_ens48, "unit": "ensemble solves/s", "seconds_per_ensemble_solve": t_ens48,
+            "setup_s": t_setup48, "solves_per_s_including_setup": n_inst / (t_setup48 + t_ens48),
+            "speedup_vs_plain": (n_inst / t_ens48) / plain_rate,
+            "trace_rel_l2_vs_fp64_ensemble_max": max(errs48),
+            "trace_rel_l2_vs_fp64_ensemble_mean": float(np.mean(errs48)),
+            "note": "the same instances with the 48-bit apply (FP32 blocks plus a 16-bit "
+                    "correction per entry, FP64 sums), against the FP64 ensemble's traces"},
         "note": "solve = zero state, first step, then `steps` leapfrog steps recording 64 "
                 "receivers per step; value excludes the one-time setup (operator build, "
                 "transforms, RHS, capture), which solves_per_s_including_setup adds for one "
