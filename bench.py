@@ -639,13 +639,16 @@ def main():
         ms48 = ctx.time_steps(prob, a.steps, t_start, dt) / a.steps
         u48 = ctx.get_slot(_lib.SLOT_U_CUR)
         B48 = ctx.step_bytes_for(48)
+        traffic48, traffic48_src = pmc_traffic(a.n, p, "k_leapfrog48")
         l2_48 = float(np.sqrt(ctx.compute_error_l2_matrix_free(u48, u)))
         res["fp48_apply"] = {
             "value": dofs / (ms48 * 1e-3), "unit": "DOF-updates/s", "ms_per_step": ms48,
             "speedup_vs_fp64": ms_step / ms48,
             "roofline": {"bound": "hbm", "achieved": B48 / (ms48 * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": B48 / (ms48 * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                         "bytes_per_step": B48, "kernel": f"k_leapfrog48<{p},0>"},
+                         "bytes_per_step": B48, "kernel": f"k_leapfrog48<{p},0>",
+                         "traffic": traffic48, "traffic_source": traffic48_src,
+                         "traffic_over_algorithmic": (traffic48 / B48) if traffic48 else None},
             "ms_per_step_includes": f"the basis transforms of one stepping call of {a.steps} steps",
             "rel_l2_diff_vs_fp64": float(np.linalg.norm(u48 - u) / np.linalg.norm(u)),
             "err_l2": float(np.sqrt(ctx.compute_error_l2_matrix_free(u48, uex))),
