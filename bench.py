@@ -184,6 +184,7 @@ def ensemble_leg(n_inst, steps, plain_sample, f0=5.0, seed0=1000):
     t0 = time.perf_counter()
     ens = E.Ensemble(inst, dt, steps, every=1, f0=f0, apply_bits=32, autotune=True)
     t_setup = time.perf_counter() - t0
+    setup_parts = dict(ens.setup_seconds)
     ens.run()  # first replay (untimed)
     t0 = time.perf_counter()
     tr = ens.run()
@@ -245,7 +246,7 @@ def ensemble_leg(n_inst, steps, plain_sample, f0=5.0, seed0=1000):
                               for g in groups)),
         "groups": groups, "seconds_per_ensemble_solve": t_ens,
         "step_GBps": sum(g["step_bytes"] for g in groups) * (steps + 1) / t_ens / 1e9,
-        "setup_s": t_setup, "instance_generation_s": t_gen,
+        "setup_s": t_setup, "setup_seconds_by_phase": setup_parts, "instance_generation_s": t_gen,
         "plain_fp64": {"value": plain_rate, "unit": "solves/s", "sample_instances": len(sample),
                        "seconds": t_plain, "setup_seconds": t_plain_setup},
         "speedup_vs_plain": ens_rate / plain_rate,

@@ -106,35 +106,70 @@ def make_instance(seed, n_range=(10, 20), p_choices=(2, 3), n_receivers=64, sigm
                 receivers=rec, f0=f0)
 
 
+def _locate_each(meshes, points):
+    """meshes[k].locate(points[k]) for every k, with one query per distinct
+    geometry (instances of one cube size share it): the same tets, one
+    locator call per size instead of one per instance."""
+    out = [None] * len(meshes)
+    by_geom = {}
+    for k, m in enumerate(meshes):
+        by_geom.setdefault((id(m.xyz), id(m.tet_vert)), []).append(k)
+    for ks in by_geom.values():
+        pts = [np.asarray(points[k], np.float64).reshape(-1, 3) for k in ks]
+        tets = meshes[ks[0]].locate(np.concatenate(pts))
+        cut = np.cumsum([0] + [len(q) for q in pts])
+        for j, k in enumerate(ks):
+            out[k] = tets[cut[j]:cut[j + 1]]
+    return out
+
+
 class Ensemble:
     """Instances grouped by degree; one context per group on the GPU.  Each
     instance's Ricker wavelet uses its own f0 (instance["f0"], default `f0`)
     through per-group time factors computed on the device each step."""
 
     def __init__(self, instances, dt, nsteps, every=1, f0=5.0, apply_bits=32, autotune=True):
+        import time
         self.instances = list(instances)
         self.dt, self.nsteps, self.every, self.f0 = dt, nsteps, every, f0
         self.groups = []
+        # seconds per set-up phase, summed over the groups (device work is
+        # synchronised at each phase's end)
+        self.setup_seconds = dict(union=0.0, context=0.0, operator=0.0, precision=0.0,
+                                  sources_receivers=0.0, autotune=0.0, capture=0.0)
+        tick = [time.perf_counter()]
+
+        def lap(phase, ctx=None):
+            if ctx is not None:
+                ctx.sync()
+            now = time.perf_counter()
+            self.setup_seconds[phase] += now - tick[0]
+            tick[0] = now
+
         for p in sorted({ins["p"] for ins in self.instances}):
             members = [i for i, ins in enumerate(self.instances) if ins["p"] == p]
             meshes = [self.instances[i]["mesh"] for i in members]
             um = union(meshes)
+            lap("union")
             ctx = Context(um, p)
             ctx.set_keep_adg(False)  # traces only: no DG-norm error, half the build memory
             ctx.set_build("factored")
+            lap("context", ctx)
             ctx.make_matrices_free()
+            lap("operator", ctx)
             if apply_bits != 64:  # 32- or 48-bit apply: the FP64 blocks are not needed after it
                 ctx.set_apply_precision(apply_bits)
                 ctx.release_fp64_operator()
+            lap("precision", ctx)
             # sources and receivers located in each instance, ids offset into the union
             tet_off = np.cumsum([0] + [m.n_tet for m in meshes])
             src = np.concatenate([self.instances[i]["source"] for i in members])
             frc = np.concatenate([self.instances[i]["force"] for i in members])
-            stet = np.concatenate([m.locate(self.instances[i]["source"]) + tet_off[k]
-                                   for k, (i, m) in enumerate(zip(members, meshes))])
+            stet = np.concatenate([t + tet_off[k] for k, t in enumerate(
+                _locate_each(meshes, [self.instances[i]["source"] for i in members]))])
             rec = np.concatenate([self.instances[i]["receivers"] for i in members])
-            rtet = np.concatenate([m.locate(self.instances[i]["receivers"]) + tet_off[k]
-                                   for k, (i, m) in enumerate(zip(members, meshes))])
+            rtet = np.concatenate([t + tet_off[k] for k, t in enumerate(
+                _locate_each(meshes, [self.instances[i]["receivers"] for i in members]))])
             ctx.set_ricker(f0)
             # each instance keeps its own wavelet (f0, t0 = 1.2/f0)
             f0s = np.array([self.instances[i].get("f0", f0) for i in members], np.float64)
@@ -143,9 +178,12 @@ class Ensemble:
             ctx.make_rhs_free(_lib.PROB_RICKER, _lib.SLOT_RHS_DYN)
             ctx.add_point_sources(src, frc, tets=stet.astype(np.int32))
             ctx.set_receivers(rec, max_records=nsteps // every, tets=rtet.astype(np.int32))
+            lap("sources_receivers", ctx)
             if autotune:
-                ctx.autotune(reps=5)
+                ctx.autotune(reps=2)
+            lap("autotune", ctx)
             ctx.capture_solve(_lib.PROB_RICKER, dt, nsteps, every)
+            lap("capture", ctx)
             nrec = [len(self.instances[i]["receivers"]) for i in members]
             self.groups.append(dict(p=p, members=members, ctx=ctx, rec_split=np.cumsum([0] + nrec)))
         self.n_dofs = sum(g["ctx"].n_poly * g["ctx"].n3 for g in self.groups)

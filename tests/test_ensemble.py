@@ -125,3 +125,16 @@ def test_face_material_convention_and_field_smoothness():
     a_rough, a_smooth = _asymmetry(rough), _asymmetry(smooth)
     assert a_rough > 0.1
     assert a_smooth < 0.4 * a_rough
+
+
+def test_grouped_location_equals_per_instance():
+    """Sources and receivers located with one query per cube size give the
+    tets that each instance's own query gives (here the grouped queries take
+    the k-d tree path and the single ones the exhaustive one)."""
+    from lymph3d_apsc_amd import ensemble as E
+    inst = [E.make_instance(300 + s, n_range=(8, 9), n_receivers=9) for s in range(12)]
+    meshes = [i["mesh"] for i in inst]
+    pts = [np.concatenate([i["source"], i["receivers"]]) for i in inst]
+    grouped = E._locate_each(meshes, pts)
+    for m, q, t in zip(meshes, pts, grouped):
+        assert np.array_equal(t, m.locate(q))
