@@ -1136,7 +1136,7 @@ int lymph_build_operator(LymphCtx* c) {
     for (int p0 = 0; p0 < c->n_poly; p0 += chunk) {
       const int n = std::min(chunk, c->n_poly - p0);
       if (c->build_mode == LYMPH_BUILD_FACTORED)
-        k_build_factored<P><<<n, 256, 0, c->stream>>>(c->md, c->alpha, c->theta, c->K, c->A, c->M,
+        k_build_factored<P><<<n, factored_threads<P>(), 0, c->stream>>>(c->md, c->alpha, c->theta, c->K, c->A, c->M,
                                                       c->Mm, p0);
       else
         k_build_operator<P><<<n, 256, 0, c->stream>>>(c->md, c->alpha, c->theta, c->K, c->A, c->M,
