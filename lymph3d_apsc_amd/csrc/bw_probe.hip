@@ -1,5 +1,5 @@
 // bw_probe.hip -- STREAM-style copy and read kernels used by bench.py to
-// measure the achievable HBM bandwidth of the box (reported beside the 8 TB/s
+// measure the achievable HBM bandwidth of this GPU (reported beside the 8 TB/s
 // spec) and to calibrate the FETCH_SIZE / WRITE_SIZE counters.
 // Not part of the PolyDG path; built into liblymph_bwprobe.so.
 #include <hip/hip_runtime.h>
