@@ -369,7 +369,15 @@ struct LymphMlp {
   int adam_t = 0;
   float* ws = nullptr;  // split-k partial sums, sized by the largest split GEMM seen
   size_t ws_floats = 0;
+  // training's sample order and per-step losses, grown to the longest call
+  int* dorder = nullptr;
+  size_t dorder_cap = 0;
+  float* dloss = nullptr;
+  size_t dloss_cap = 0;
   std::vector<void*> owned;
+  // every kernel and copy of this network runs on its own stream, so that
+  // networks trained from different host threads overlap on the GPU
+  hipStream_t stream = nullptr;
 };
 
 namespace {
@@ -403,7 +411,7 @@ int gemm(LymphMlp* m, GemmArgs a) {
   if (S > 1) {
     const size_t need = (size_t)S * a.M * a.N;
     if (need > m->ws_floats) {  // grown once, on the first call of a larger shape
-      (void)hipDeviceSynchronize();
+      (void)hipStreamSynchronize(m->stream);
       if (m->ws) (void)hipFree(m->ws);
       m->ws = nullptr;
       m->ws_floats = 0;
@@ -415,10 +423,10 @@ int gemm(LymphMlp* m, GemmArgs a) {
   }
   dim3 grid(blocks(a.N, TM), blocks(a.M, TM), S);
   if (big)
-    k_gemm<A_KC, B_KC, EPI, 128><<<grid, 256>>>(a);
+    k_gemm<A_KC, B_KC, EPI, 128><<<grid, 256, 0, m->stream>>>(a);
   else
-    k_gemm<A_KC, B_KC, EPI, 64><<<grid, 256>>>(a);
-  if (S > 1) k_splitk<EPI><<<blocks((size_t)a.M * a.N, 256), 256>>>(a, S);
+    k_gemm<A_KC, B_KC, EPI, 64><<<grid, 256, 0, m->stream>>>(a);
+  if (S > 1) k_splitk<EPI><<<blocks((size_t)a.M * a.N, 256), 256, 0, m->stream>>>(a, S);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : fail(LYMPH_MLP_ERR_HIP, std::string("k_gemm: ") + hipGetErrorString(e));
 }
@@ -449,8 +457,8 @@ int backward(LymphMlp* m, int B, const float* T, const int* idx, float* loss_out
   const int NL = m->w[m->L];
   const size_t ny = (size_t)B * NL;
   const unsigned nb = blocks(ny, kLossPer);
-  k_loss<<<nb, 256>>>(m->Y, NL, T, idx, B, 2.0f / (float)ny, m->dY, m->part);
-  k_sum<<<1, 256>>>(m->part, (int)nb, 1.0f / (float)ny, loss_out);
+  k_loss<<<nb, 256, 0, m->stream>>>(m->Y, NL, T, idx, B, 2.0f / (float)ny, m->dY, m->part);
+  k_sum<<<1, 256, 0, m->stream>>>(m->part, (int)nb, 1.0f / (float)ny, loss_out);
   bf16_t* dZ = m->dY;
   for (int l = m->L - 1; l >= 0; --l) {
     const int N = m->w[l + 1], K = m->w[l];
@@ -460,7 +468,7 @@ int backward(LymphMlp* m, int B, const float* T, const int* idx, float* loss_out
     a.B = m->H[l]; a.ldb = K;
     a.C32 = m->G + m->woff[l]; a.ldc = K;
     if (int rc = gemm<false, false, EPI_F32>(m, a)) return rc;
-    k_colsum<<<blocks(N, 64), 256>>>(dZ, B, N, m->G + m->boff[l]);
+    k_colsum<<<blocks(N, 64), 256, 0, m->stream>>>(dZ, B, N, m->G + m->boff[l]);
     if (l > 0) {  // dZ_{l-1} = (dZ_l W_l) * gelu'(Z_{l-1})
       bf16_t* nxt = (dZ == m->dZa) ? m->dZb : m->dZa;
       GemmArgs d{};
@@ -487,9 +495,12 @@ const char* lymph_mlp_last_error(void) { return g_err.c_str(); }
 
 int lymph_mlp_destroy(LymphMlp* m) {
   if (!m) return 0;
-  (void)hipDeviceSynchronize();
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
   for (void* p : m->owned) (void)hipFree(p);
   if (m->ws) (void)hipFree(m->ws);
+  if (m->dorder) (void)hipFree(m->dorder);
+  if (m->dloss) (void)hipFree(m->dloss);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
   return 0;
 }
@@ -505,6 +516,10 @@ int lymph_mlp_create(int n_layers, const int32_t* widths, int max_batch, LymphMl
   if (hipGetDeviceCount(&dev) != hipSuccess || dev == 0)
     return fail(LYMPH_MLP_ERR_HIP, "no HIP device: liblymph_mlp needs an MI355X (gfx950)");
   auto* m = new LymphMlp;
+  if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete m;
+    return fail(LYMPH_MLP_ERR_HIP, "hipStreamCreate failed");
+  }
   m->L = n_layers;
   m->max_batch = max_batch;
   m->w.assign(widths, widths + n_layers + 1);
@@ -543,8 +558,11 @@ int lymph_mlp_create(int n_layers, const int32_t* widths, int max_batch, LymphMl
     lymph_mlp_destroy(m);
     return rc;
   }
-  if (hipMemset(m->P, 0, off * 4) != hipSuccess || hipMemset(m->Mo, 0, off * 4) != hipSuccess ||
-      hipMemset(m->Vo, 0, off * 4) != hipSuccess || hipMemset(m->Wb, 0, off * 2) != hipSuccess) {
+  if (hipMemsetAsync(m->P, 0, off * 4, m->stream) != hipSuccess ||
+      hipMemsetAsync(m->Mo, 0, off * 4, m->stream) != hipSuccess ||
+      hipMemsetAsync(m->Vo, 0, off * 4, m->stream) != hipSuccess ||
+      hipMemsetAsync(m->Wb, 0, off * 2, m->stream) != hipSuccess ||
+      hipStreamSynchronize(m->stream) != hipSuccess) {
     lymph_mlp_destroy(m);
     return fail(LYMPH_MLP_ERR_HIP, "hipMemset failed");
   }
@@ -561,12 +579,12 @@ int lymph_mlp_n_params(const LymphMlp* m, int64_t* n) {
 int lymph_mlp_set_params(LymphMlp* m, const float* p) {
   if (int rc = need(m)) return rc;
   if (!p) return fail(LYMPH_MLP_ERR_ARG, "null params");
-  HIP_TRY(hipMemcpy(m->P, p, m->n_params * 4, hipMemcpyHostToDevice));
-  k_to_bf16<<<blocks(m->n_params, 256), 256>>>(m->P, m->Wb, m->n_params);
+  HIP_TRY(hipMemcpyAsync(m->P, p, m->n_params * 4, hipMemcpyHostToDevice, m->stream));
+  k_to_bf16<<<blocks(m->n_params, 256), 256, 0, m->stream>>>(m->P, m->Wb, m->n_params);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemset(m->Mo, 0, m->n_params * 4));
-  HIP_TRY(hipMemset(m->Vo, 0, m->n_params * 4));
-  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemsetAsync(m->Mo, 0, m->n_params * 4, m->stream));
+  HIP_TRY(hipMemsetAsync(m->Vo, 0, m->n_params * 4, m->stream));
+  HIP_TRY(hipStreamSynchronize(m->stream));
   m->adam_t = 0;
   return 0;
 }
@@ -574,7 +592,8 @@ int lymph_mlp_set_params(LymphMlp* m, const float* p) {
 int lymph_mlp_get_params(const LymphMlp* m, float* p) {
   if (int rc = need(m)) return rc;
   if (!p) return fail(LYMPH_MLP_ERR_ARG, "null params");
-  HIP_TRY(hipMemcpy(p, m->P, m->n_params * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpyAsync(p, m->P, m->n_params * 4, hipMemcpyDeviceToHost, m->stream));
+  HIP_TRY(hipStreamSynchronize(m->stream));
   return 0;
 }
 
@@ -582,7 +601,7 @@ int lymph_mlp_set_data(LymphMlp* m, int n, const float* x, const float* y) {
   if (int rc = need(m)) return rc;
   if (n < 1 || !x || !y) return fail(LYMPH_MLP_ERR_ARG, "empty data set");
   const size_t nx = (size_t)n * m->w[0], ny = (size_t)n * m->w[m->L];
-  (void)hipDeviceSynchronize();
+  (void)hipStreamSynchronize(m->stream);
   for (float** q : {&m->X, &m->T})
     if (*q) {
       for (auto& o : m->owned)
@@ -592,8 +611,9 @@ int lymph_mlp_set_data(LymphMlp* m, int n, const float* x, const float* y) {
     }
   if (int rc = dalloc(m, &m->X, nx)) return rc;
   if (int rc = dalloc(m, &m->T, ny)) return rc;
-  HIP_TRY(hipMemcpy(m->X, x, nx * 4, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(m->T, y, ny * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpyAsync(m->X, x, nx * 4, hipMemcpyHostToDevice, m->stream));
+  HIP_TRY(hipMemcpyAsync(m->T, y, ny * 4, hipMemcpyHostToDevice, m->stream));
+  HIP_TRY(hipStreamSynchronize(m->stream));
   m->n_data = n;
   return 0;
 }
@@ -607,36 +627,44 @@ int lymph_mlp_train(LymphMlp* m, int n_steps, int batch, const int32_t* order, f
   if (n_steps < 0 || !order) return fail(LYMPH_MLP_ERR_ARG, "bad steps/order");
   for (size_t i = 0; i < (size_t)n_steps * batch; ++i)
     if (order[i] < 0 || order[i] >= m->n_data) return fail(LYMPH_MLP_ERR_ARG, "sample index out of range");
-  int* dorder = nullptr;
-  float* dloss = nullptr;
-  HIP_TRY(hipMalloc(&dorder, sizeof(int) * std::max<size_t>(1, (size_t)n_steps * batch)));
-  if (hipMalloc(&dloss, sizeof(float) * std::max(1, n_steps)) != hipSuccess) {
-    (void)hipFree(dorder);
-    return fail(LYMPH_MLP_ERR_HIP, "hipMalloc failed");
+  const size_t n_order = std::max<size_t>(1, (size_t)n_steps * batch);
+  const size_t n_loss = std::max<size_t>(1, (size_t)n_steps);
+  if (n_order > m->dorder_cap || n_loss > m->dloss_cap) {  // grown, then reused by later calls
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    if (m->dorder) (void)hipFree(m->dorder);
+    if (m->dloss) (void)hipFree(m->dloss);
+    m->dorder = nullptr;
+    m->dloss = nullptr;
+    m->dorder_cap = m->dloss_cap = 0;
+    HIP_TRY(hipMalloc(&m->dorder, sizeof(int) * n_order));
+    HIP_TRY(hipMalloc(&m->dloss, sizeof(float) * n_loss));
+    m->dorder_cap = n_order;
+    m->dloss_cap = n_loss;
   }
-  HIP_TRY(hipMemcpy(dorder, order, sizeof(int) * (size_t)n_steps * batch, hipMemcpyHostToDevice));
+  int* dorder = m->dorder;
+  float* dloss = m->dloss;
+  HIP_TRY(hipMemcpyAsync(dorder, order, sizeof(int) * (size_t)n_steps * batch, hipMemcpyHostToDevice,
+                         m->stream));
   int rc = 0;
   for (int s = 0; s < n_steps && !rc; ++s) {
     const int* idx = dorder + (size_t)s * batch;
-    k_gather<<<blocks((size_t)batch * m->w[0], 256), 256>>>(m->X, m->w[0], idx, batch, m->H[0]);
+    k_gather<<<blocks((size_t)batch * m->w[0], 256), 256, 0, m->stream>>>(m->X, m->w[0], idx, batch, m->H[0]);
     rc = forward(m, batch);
     if (!rc) rc = backward(m, batch, m->T, idx, dloss + s);
     if (!rc) {
       m->adam_t++;
       const float c1 = 1.0f / (1.0f - std::pow(beta1, (float)m->adam_t));
       const float c2 = 1.0f / (1.0f - std::pow(beta2, (float)m->adam_t));
-      k_adam<<<blocks(m->n_params, 256), 256>>>(m->P, m->Mo, m->Vo, m->G, m->Wb, m->n_params, lr,
+      k_adam<<<blocks(m->n_params, 256), 256, 0, m->stream>>>(m->P, m->Mo, m->Vo, m->G, m->Wb, m->n_params, lr,
                                                 beta1, beta2, eps, c1, c2);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) rc = fail(LYMPH_MLP_ERR_HIP, std::string("k_adam: ") + hipGetErrorString(e));
     }
   }
   if (!rc && losses && n_steps > 0 &&
-      hipMemcpy(losses, dloss, sizeof(float) * n_steps, hipMemcpyDeviceToHost) != hipSuccess)
+      hipMemcpyAsync(losses, dloss, sizeof(float) * n_steps, hipMemcpyDeviceToHost, m->stream) != hipSuccess)
     rc = fail(LYMPH_MLP_ERR_HIP, "loss copy failed");
-  (void)hipDeviceSynchronize();
-  (void)hipFree(dorder);
-  (void)hipFree(dloss);
+  if (hipStreamSynchronize(m->stream) != hipSuccess && !rc) rc = fail(LYMPH_MLP_ERR_HIP, "training failed");
   return rc;
 }
 
@@ -656,20 +684,21 @@ int lymph_mlp_gradients(LymphMlp* m, int batch, const float* x, const float* y, 
   std::vector<int> id(batch);
   for (int i = 0; i < batch; ++i) id[i] = i;
   int rc = 0;
-  if (hipMemcpy(dx, x, nx * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(dy, y, ny * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(m->idx, id.data(), 4 * batch, hipMemcpyHostToDevice) != hipSuccess)
+  if (hipMemcpyAsync(dx, x, nx * 4, hipMemcpyHostToDevice, m->stream) != hipSuccess ||
+      hipMemcpyAsync(dy, y, ny * 4, hipMemcpyHostToDevice, m->stream) != hipSuccess ||
+      hipMemcpyAsync(m->idx, id.data(), 4 * batch, hipMemcpyHostToDevice, m->stream) != hipSuccess)
     rc = fail(LYMPH_MLP_ERR_HIP, "upload failed");
   if (!rc) {
-    k_gather<<<blocks(nx, 256), 256>>>(dx, m->w[0], m->idx, batch, m->H[0]);
+    k_gather<<<blocks(nx, 256), 256, 0, m->stream>>>(dx, m->w[0], m->idx, batch, m->H[0]);
     rc = forward(m, batch);
   }
   if (!rc) rc = backward(m, batch, dy, m->idx, m->loss);
-  if (!rc && grads && hipMemcpy(grads, m->G, m->n_params * 4, hipMemcpyDeviceToHost) != hipSuccess)
+  if (!rc && grads &&
+      hipMemcpyAsync(grads, m->G, m->n_params * 4, hipMemcpyDeviceToHost, m->stream) != hipSuccess)
     rc = fail(LYMPH_MLP_ERR_HIP, "gradient copy failed");
-  if (!rc && loss && hipMemcpy(loss, m->loss, 4, hipMemcpyDeviceToHost) != hipSuccess)
+  if (!rc && loss && hipMemcpyAsync(loss, m->loss, 4, hipMemcpyDeviceToHost, m->stream) != hipSuccess)
     rc = fail(LYMPH_MLP_ERR_HIP, "loss copy failed");
-  (void)hipDeviceSynchronize();
+  (void)hipStreamSynchronize(m->stream);
   (void)hipFree(dx);
   (void)hipFree(dy);
   return rc;
@@ -682,20 +711,23 @@ int lymph_mlp_predict(LymphMlp* m, int n, const float* x, float* y) {
   float* dx = nullptr;
   HIP_TRY(hipMalloc(&dx, (size_t)std::max(1, n) * w0 * 4));
   int rc = 0;
-  if (n > 0 && hipMemcpy(dx, x, (size_t)n * w0 * 4, hipMemcpyHostToDevice) != hipSuccess)
+  if (n > 0 &&
+      hipMemcpyAsync(dx, x, (size_t)n * w0 * 4, hipMemcpyHostToDevice, m->stream) != hipSuccess)
     rc = fail(LYMPH_MLP_ERR_HIP, "upload failed");
   std::vector<int> id(m->max_batch);
   for (int i = 0; i < m->max_batch; ++i) id[i] = i;
-  if (!rc && hipMemcpy(m->idx, id.data(), 4 * m->max_batch, hipMemcpyHostToDevice) != hipSuccess)
+  if (!rc &&
+      hipMemcpyAsync(m->idx, id.data(), 4 * m->max_batch, hipMemcpyHostToDevice, m->stream) != hipSuccess)
     rc = fail(LYMPH_MLP_ERR_HIP, "upload failed");
   for (int b0 = 0; b0 < n && !rc; b0 += m->max_batch) {
     const int B = std::min(m->max_batch, n - b0);
-    k_gather<<<blocks((size_t)B * w0, 256), 256>>>(dx + (size_t)b0 * w0, w0, m->idx, B, m->H[0]);
+    k_gather<<<blocks((size_t)B * w0, 256), 256, 0, m->stream>>>(dx + (size_t)b0 * w0, w0, m->idx, B, m->H[0]);
     rc = forward(m, B);
-    if (!rc && hipMemcpy(y + (size_t)b0 * NL, m->Y, (size_t)B * NL * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    if (!rc && hipMemcpyAsync(y + (size_t)b0 * NL, m->Y, (size_t)B * NL * 4, hipMemcpyDeviceToHost,
+                              m->stream) != hipSuccess)
       rc = fail(LYMPH_MLP_ERR_HIP, "download failed");
   }
-  (void)hipDeviceSynchronize();
+  (void)hipStreamSynchronize(m->stream);
   (void)hipFree(dx);
   return rc;
 }
@@ -705,20 +737,20 @@ int lymph_mlp_time_predict(LymphMlp* m, int n, int reps, double* ms) {
   if (!m->X || n < 1 || n > m->n_data || reps < 1) return fail(LYMPH_MLP_ERR_ARG, "bad n/reps or no data");
   std::vector<int> id(m->max_batch);
   for (int i = 0; i < m->max_batch; ++i) id[i] = i;
-  HIP_TRY(hipMemcpy(m->idx, id.data(), 4 * m->max_batch, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpyAsync(m->idx, id.data(), 4 * m->max_batch, hipMemcpyHostToDevice, m->stream));
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   int rc = 0;
-  HIP_TRY(hipEventRecord(e0, 0));
+  HIP_TRY(hipEventRecord(e0, m->stream));
   for (int r = 0; r < reps && !rc; ++r)
     for (int b0 = 0; b0 < n && !rc; b0 += m->max_batch) {
       const int B = std::min(m->max_batch, n - b0);
-      k_gather<<<blocks((size_t)B * m->w[0], 256), 256>>>(m->X + (size_t)b0 * m->w[0], m->w[0],
+      k_gather<<<blocks((size_t)B * m->w[0], 256), 256, 0, m->stream>>>(m->X + (size_t)b0 * m->w[0], m->w[0],
                                                           m->idx, B, m->H[0]);
       rc = forward(m, B);
     }
-  HIP_TRY(hipEventRecord(e1, 0));
+  HIP_TRY(hipEventRecord(e1, m->stream));
   HIP_TRY(hipEventSynchronize(e1));
   float f = 0.f;
   HIP_TRY(hipEventElapsedTime(&f, e0, e1));

@@ -99,6 +99,35 @@ def solve(thetas, n, p, steps, every, rec, apply_bits=32, chunk=512, log=None):
     return out, t_run, t_setup
 
 
+def train_networks(nets, n, epochs, batch, lr, seed, log=None):
+    """Each network trained on its n examples (set_data done) for `epochs`
+    epochs of shuffled batches (its own seeded order), Adam at lr (x0.3 from
+    60% of the epochs, x0.1 from 85%).  The networks run on their own
+    streams from one host thread each, so their small GEMMs overlap on the
+    GPU; each network's arithmetic is that of a run on its own.  Returns the
+    mean loss per epoch of every network."""
+    from concurrent.futures import ThreadPoolExecutor
+    steps_per_epoch = n // batch
+
+    def one(i):
+        net = nets[i]
+        rng = np.random.default_rng(seed + 1000 + i)
+        losses = []
+        for ep in range(epochs):
+            frac = ep / max(1, epochs - 1)
+            rate = lr if frac < 0.6 else (lr * 0.3 if frac < 0.85 else lr * 0.1)
+            order = rng.permutation(n)[:steps_per_epoch * batch].astype(np.int32)
+            losses.append(float(net.train(order, batch, rate).mean()))
+        return losses
+
+    with ThreadPoolExecutor(max_workers=len(nets)) as ex:
+        hist = list(ex.map(one, range(len(nets))))
+    if log:
+        for i, losses in enumerate(hist):
+            log(f"net {i}: loss {losses[0]:.3e} -> {losses[-1]:.3e}")
+    return hist
+
+
 class Surrogate:
     """Ensemble of networks on standardised inputs/outputs."""
 
@@ -115,22 +144,9 @@ class Surrogate:
         self.y_mean = y.mean(axis=0)
         self.y_std = max(float(np.sqrt(np.mean((y - self.y_mean) ** 2))), 1e-300)
         yn = ((y - self.y_mean) / self.y_std).astype(np.float32)
-        n = len(x)
-        steps_per_epoch = n // batch
-        hist = []
-        for i, net in enumerate(self.nets):
+        for net in self.nets:
             net.set_data(x, yn)
-            rng = np.random.default_rng(self.seed + 1000 + i)
-            losses = []
-            for ep in range(epochs):
-                frac = ep / max(1, epochs - 1)
-                rate = lr if frac < 0.6 else (lr * 0.3 if frac < 0.85 else lr * 0.1)
-                order = rng.permutation(n)[:steps_per_epoch * batch].astype(np.int32)
-                losses.append(float(net.train(order, batch, rate).mean()))
-            hist.append(losses)
-            if log:
-                log(f"net {i}: loss {losses[0]:.3e} -> {losses[-1]:.3e}")
-        return hist
+        return train_networks(self.nets, len(x), epochs, batch, lr, self.seed, log)
 
     def predict_all(self, thetas):
         x = ((thetas - IN_MEAN) / IN_STD).astype(np.float32)
@@ -254,22 +270,9 @@ class ReceiverSurrogate:
         c = (y - self.y_mean) @ self.basis
         self.c_std = max(float(np.sqrt(np.mean(c ** 2))), 1e-300)
         cn = (c / self.c_std).astype(np.float32)
-        n = len(x)
-        steps_per_epoch = n // batch
-        hist = []
-        for i, net in enumerate(self.nets):
+        for net in self.nets:
             net.set_data(x, cn)
-            rng = np.random.default_rng(self.seed + 1000 + i)
-            losses = []
-            for ep in range(epochs):
-                frac = ep / max(1, epochs - 1)
-                rate = lr if frac < 0.6 else (lr * 0.3 if frac < 0.85 else lr * 0.1)
-                order = rng.permutation(n)[:steps_per_epoch * batch].astype(np.int32)
-                losses.append(float(net.train(order, batch, rate).mean()))
-            hist.append(losses)
-            if log:
-                log(f"net {i}: loss {losses[0]:.3e} -> {losses[-1]:.3e}")
-        return hist
+        return train_networks(self.nets, len(x), epochs, batch, lr, self.seed, log)
 
     def _decode(self, c, dist, N):
         y = (c.astype(np.float64) * self.c_std) @ self.basis.T + self.y_mean
