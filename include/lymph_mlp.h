@@ -19,6 +19,12 @@
  * matrix cores; pre-activations, outputs, gradients of the parameters, the
  * loss and Adam state in FP32.  Loss = mean over batch and outputs of
  * (y - t)^2.
+ *
+ * Streams: each network runs its kernels and copies on a HIP stream of its
+ * own (created with it) and every call returns with that stream drained.
+ * Calls on different networks may come from different host threads at the
+ * same time and then overlap on the GPU; calls on one network must not
+ * overlap.  The error message of lymph_mlp_last_error() is shared.
  */
 #ifndef LYMPH_MLP_H
 #define LYMPH_MLP_H
