@@ -118,6 +118,8 @@ def train_networks(nets, n, epochs, batch, lr, seed, log=None):
             rate = lr if frac < 0.6 else (lr * 0.3 if frac < 0.85 else lr * 0.1)
             order = rng.permutation(n)[:steps_per_epoch * batch].astype(np.int32)
             losses.append(float(net.train(order, batch, rate).mean()))
+            if log and (ep + 1) % max(1, epochs // 10) == 0:
+                log(f"net {i}: epoch {ep + 1}/{epochs}, loss {losses[-1]:.3e}")
         return losses
 
     with ThreadPoolExecutor(max_workers=len(nets)) as ex:
