@@ -268,7 +268,7 @@ int lymph_release_fp64_operator(LymphCtx* ctx);
 /* Modes of lymph_set_symmetric_storage. */
 #define LYMPH_SYM_OFF 0       /* full block rows (default) */
 #define LYMPH_SYM_FP64 1      /* FP64 step reads each face pair once */
-#define LYMPH_SYM_FP64_FP32 2 /* and, at p = 3, so does the 32-bit apply */
+#define LYMPH_SYM_FP64_FP32 2 /* and, at p = 3, so do the 32- and 48-bit applies */
 
 /* Face pairs stored once (no reference counterpart; an operator storage
  * choice behind TIME_STEP_MATRIX_FREE, src/matrix_free.f90:1230-1385).  With
@@ -284,8 +284,9 @@ int lymph_release_fp64_operator(LymphCtx* ctx);
  * returns LYMPH_ERR_ARG and the full rows stay in use.  Needs p <= 3 and the
  * FP64 blocks (not released); lymph_build_operator drops the stored pairs;
  * on = LYMPH_SYM_OFF frees them.  Only with on = LYMPH_SYM_FP64_FP32 (p = 3)
- * does the 32-bit apply (lymph_set_apply_precision) keep its blocks once too;
- * with LYMPH_SYM_FP64 it keeps its full rows.  A failing call leaves no stored
+ * do the 32- and 48-bit applies (lymph_set_apply_precision) keep their blocks
+ * once too (the 48-bit apply its 16-bit corrections as well); with
+ * LYMPH_SYM_FP64 they keep their full rows.  A failing call leaves no stored
  * pairs switched on.  While the step in use reads stored pairs,
  * lymph_autotune returns LYMPH_ERR_STATE (the full-row variants are not run).
  * The DG norm is unaffected. */

@@ -125,8 +125,9 @@ struct LymphCtx {
   int4* pinfo = nullptr;    // per stored pair block: full-row slot, P, Q, slot of (Q, P)
   double* Sinv = nullptr;   // R^-1 of each mass block, packed like R (pairs-once FP64 update)
   float* Ks32 = nullptr;    // the stored blocks of the 32-bit operator (p = 3)
-  size_t ks32_bytes = 0;
-  bool sym32_wanted = false;  // LYMPH_SYM_FP64_FP32: the 32-bit apply stores pairs once too
+  _Float16* Ks16 = nullptr; // and their 48-bit corrections, once K16 exists
+  size_t ks32_bytes = 0;    // Ks32 and Ks16
+  bool sym32_wanted = false;  // LYMPH_SYM_FP64_FP32: the 32- and 48-bit applies store pairs once too
   double* prod = nullptr;   // per-block products of one step (n_blk x 3Np)
   size_t ks_bytes = 0;
   int* blk_row = nullptr;   // row polyhedron of each block
@@ -444,6 +445,20 @@ int launch_step(LymphCtx* c, int mode, double g, double c1, double c2, const dou
       } else {
         return fail(LYMPH_ERR_STATE, "32-bit stored pairs need 3Np a multiple of 12");
       }
+    } else if (c->apply_bits == 48 && c->Ks16) {
+      if constexpr (3 * np_of(P) % 12 == 0 && 3 * np_of(P) <= 64) {
+        k_sym_products48<P><<<step_grid(c->n_poly), 64, 0, c->stream>>>(
+            c->Ks32, c->Ks16, c->md.row_ptr, c->md.col, c->sidx, c->tdst, u, c->prod, c->n_poly, c->order);
+        if (int rc = check_launch("k_sym_products48")) return rc;
+        if (mode == 0)
+          k_sym_update32<P, 0, double><<<step_grid(c->n_poly), 64, 0, c->stream>>>(
+              c->prod, c->md.row_ptr, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->n_poly, c->order);
+        else
+          k_sym_update32<P, 1, double><<<step_grid(c->n_poly), 64, 0, c->stream>>>(
+              c->prod, c->md.row_ptr, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->n_poly, c->order);
+      } else {
+        return fail(LYMPH_ERR_STATE, "48-bit stored pairs need 3Np a multiple of 12");
+      }
     } else if (c->apply_bits == 48) {
       if (mode == 0)
         launch_leapfrog48<P, 0>(c->step_variant, c->n_poly, c->stream, c->K32, c->K16, c->md.row_ptr,
@@ -523,14 +538,17 @@ void drop_graph(LymphCtx* c) {
 
 // Drop the once-stored face pairs (operator rebuilt, storage switched off).
 int sym_off(LymphCtx* c) {
-  if (!c->Ks && !c->sidx && !c->tdst && !c->prod && !c->ssrc && !c->Ks32 && !c->Sinv && !c->pinfo)
+  if (!c->Ks && !c->sidx && !c->tdst && !c->prod && !c->ssrc && !c->Ks32 && !c->Ks16 && !c->Sinv &&
+      !c->pinfo)
     return 0;
   HIP_TRY(hipStreamSynchronize(c->stream));
   drop_graph(c);
   if (c->ssrc) HIP_TRY(hipFree(c->ssrc));
   if (c->Ks32) HIP_TRY(hipFree(c->Ks32));
+  if (c->Ks16) HIP_TRY(hipFree(c->Ks16));
   c->ssrc = nullptr;
   c->Ks32 = nullptr;
+  c->Ks16 = nullptr;
   c->bytes -= c->ks32_bytes;
   c->ks32_bytes = 0;
   if (c->Ks) HIP_TRY(hipFree(c->Ks));
@@ -706,27 +724,46 @@ int sym_on(LymphCtx* c, double tol, double* asym) {
   return 0;
 }
 
-// The stored pairs of the 32-bit operator, once both it and the stored list
-// exist and LYMPH_SYM_FP64_FP32 asked for them (p = 3: whole quads per
-// chunk); otherwise the 32-bit apply keeps its full rows.  On failure no
+// The stored pairs of the 32-bit operator (and of its 48-bit corrections
+// once K16 exists), once both it and the stored list exist and
+// LYMPH_SYM_FP64_FP32 asked for them (p = 3: whole quads per chunk);
+// otherwise the 32- and 48-bit applies keep their full rows.  On failure no
 // partial copy is kept.
 int sym32_build(LymphCtx* c) {
-  if (!c->sym32_wanted || !c->ssrc || !c->K32 || c->Ks32 || c->p != 3) return 0;
-  const size_t bs = (((size_t)c->n3 * c->n3 + 3) / 4) * 4;
+  if (!c->sym32_wanted || !c->ssrc || !c->K32 || c->p != 3) return 0;
   const int n_s = c->n_poly + (c->n_blk - c->n_poly) / 2;
-  float* ks = nullptr;
-  if (int rc = dalloc(&ks, (size_t)n_s * bs)) return rc;
-  k_gather_blocks<float><<<4096, 256, 0, c->stream>>>(c->K32, ks, c->ssrc, n_s, bs);
-  int rc = check_launch("k_gather_blocks");
-  if (!rc && hipStreamSynchronize(c->stream) != hipSuccess)
-    rc = fail(LYMPH_ERR_HIP, "gather of the 32-bit stored pairs failed");
-  if (rc) {
-    (void)hipFree(ks);
-    return rc;
+  if (!c->Ks32) {
+    const size_t bs = (size_t)Dims32Any(c->p);
+    float* ks = nullptr;
+    if (int rc = dalloc(&ks, (size_t)n_s * bs)) return rc;
+    k_gather_blocks<float><<<4096, 256, 0, c->stream>>>(c->K32, ks, c->ssrc, n_s, bs);
+    int rc = check_launch("k_gather_blocks");
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess)
+      rc = fail(LYMPH_ERR_HIP, "gather of the 32-bit stored pairs failed");
+    if (rc) {
+      (void)hipFree(ks);
+      return rc;
+    }
+    c->Ks32 = ks;
+    c->ks32_bytes += (size_t)n_s * bs * sizeof(float);
+    c->bytes += (size_t)n_s * bs * sizeof(float);
   }
-  c->Ks32 = ks;
-  c->ks32_bytes = (size_t)n_s * bs * sizeof(float);
-  c->bytes += c->ks32_bytes;
+  if (c->K16 && !c->Ks16) {
+    const size_t bs = (size_t)Dims16Any(c->p);
+    _Float16* ks = nullptr;
+    if (int rc = dalloc(&ks, (size_t)n_s * bs)) return rc;
+    k_gather_blocks<_Float16><<<4096, 256, 0, c->stream>>>(c->K16, ks, c->ssrc, n_s, bs);
+    int rc = check_launch("k_gather_blocks");
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess)
+      rc = fail(LYMPH_ERR_HIP, "gather of the 48-bit stored corrections failed");
+    if (rc) {
+      (void)hipFree(ks);
+      return rc;
+    }
+    c->Ks16 = ks;
+    c->ks32_bytes += (size_t)n_s * bs * sizeof(_Float16);
+    c->bytes += (size_t)n_s * bs * sizeof(_Float16);
+  }
   return 0;
 }
 
@@ -853,6 +890,7 @@ int lymph_ctx_destroy(LymphCtx* c) {
   if (c->prod) (void)hipFree(c->prod);
   if (c->ssrc) (void)hipFree(c->ssrc);
   if (c->Ks32) (void)hipFree(c->Ks32);
+  if (c->Ks16) (void)hipFree(c->Ks16);
   if (c->Sinv) (void)hipFree(c->Sinv);
   if (c->pinfo) (void)hipFree(c->pinfo);
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
@@ -1560,7 +1598,8 @@ int lymph_autotune(LymphCtx* c, int reps, int* n_variants, int* unr, int* xmode,
   if (reps < 1) return fail(LYMPH_ERR_ARG, "reps < 1");
   // with face pairs stored once the step runs k_sym_* whatever the variant:
   // timing the variants would only time those kernels again
-  if ((c->apply_bits == 64 && c->Ks) || (c->apply_bits == 32 && c->Ks32))
+  if ((c->apply_bits == 64 && c->Ks) || (c->apply_bits == 32 && c->Ks32) ||
+      (c->apply_bits == 48 && c->Ks16))
     return fail(LYMPH_ERR_STATE, "face pairs are stored once: the full-row variants are not in use");
   return dispatch_p(c->p, [&](auto pc) -> int {
     constexpr int P = decltype(pc)::value;
@@ -1686,8 +1725,8 @@ int lymph_set_apply_precision(LymphCtx* c, int bits) {
       return r;
     }
   }
-  if (bits == 32)
-    if (int rc = sym32_build(c)) return rc;  // pairs stored once: the 32-bit copy too
+  if (bits != 64)
+    if (int rc = sym32_build(c)) return rc;  // pairs stored once: the 32- and 48-bit copies too
   c->apply_bits = bits;
   c->step_variant = 0;
   return 0;

@@ -1305,7 +1305,8 @@ __global__ __launch_bounds__(64) void k_sym_update(const double* __restrict__ pr
 // column runs over ascending rows, the chain Q's full row would run, so where
 // the two rounded blocks are exact transposes the products equal the full
 // rows' bit for bit.  k_sym_update32 adds them in block order in FP64 and
-// applies k_leapfrog32's update.  (Summing the transposed products in three
+// applies k_leapfrog32's update (PT = double: the 48-bit apply's products,
+// k_sym_products48).  (Summing the transposed products in three
 // row groups instead made the state drift from FP64 faster over long runs,
 // increasingly with the mesh size: 7x the full rows' drift after 1000 steps
 // at 40^3.)
@@ -1385,8 +1386,8 @@ __global__ __launch_bounds__(64) void k_sym_products32(const float* __restrict__
   }
 }
 
-template <int P, int MODE>
-__global__ __launch_bounds__(64) void k_sym_update32(const float* __restrict__ prod,
+template <int P, int MODE, typename PT = float>
+__global__ __launch_bounds__(64) void k_sym_update32(const PT* __restrict__ prod,
                                                      const int* __restrict__ row_ptr,
                                                      const double* __restrict__ rs,
                                                      const double* __restrict__ rd,
@@ -2297,6 +2298,127 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog48(const float* __restrict
       else
         out[o + r] = c1 * w + u[o + r] + c2 * wa[o + r];
     }
+  }
+}
+
+// The pairs-once step of the 48-bit apply: k_sym_products32's walk over the
+// stored list (one wave per polyhedron, self blocks whole and multiplied by
+// rows, pair blocks also transposed through LDS) with k_leapfrog48's
+// arithmetic, products stored as doubles (prod_stride(3Np) per block) and
+// summed by k_sym_update32<P, MODE, double>.  The row product is
+// k_leapfrog48's block product; the transposed product of column c runs the
+// chain Q's own row would run over block (Q, P) -- hi x_P[r] in FP64 fused
+// multiply-adds and (hi t) x_P[r] in FP32 over ascending r, joined with the
+// 2^-24 scale -- from the entries hi and hi t that the row lanes leave in
+// LDS, so where the two stored blocks are exact transposes it equals the
+// full rows' product bit for bit.  p = 3 (3Np = 60: whole quads per chunk,
+// no tail columns).
+template <int P>
+__global__ __launch_bounds__(64) void k_sym_products48(const float* __restrict__ Ks,
+                                                       const _Float16* __restrict__ Kr,
+                                                       const int* __restrict__ row_ptr,
+                                                       const int* __restrict__ col,
+                                                       const int* __restrict__ sidx,
+                                                       const int* __restrict__ tdst,
+                                                       const double* __restrict__ u,
+                                                       double* __restrict__ prod, int n_poly,
+                                                       const int* __restrict__ order) {
+  constexpr int N3 = Dims32<P>::N3, NQ = Dims32<P>::NQ, NPR = Dims16<P>::NPR;
+  constexpr int PS = prod_stride(N3);
+  static_assert(N3 <= 64 && N3 % 12 == 0 && Dims32<P>::T == 0, "one lane per row, whole quads per chunk");
+  constexpr int CW = N3 / 3, CQ = CW / 4;
+  constexpr int LDT = 65;  // every lane writes its row (lanes past the rows into padding); odd stride
+  __shared__ float sH[CW * LDT], sC[CW * LDT];  // hi and hi t of one chunk, column-major
+  __shared__ __align__(16) double sU[N3];       // u_P, read by the column chains
+  __shared__ __align__(16) double sX[N3];                      // u_Q of the current block, read by the rows
+  const int lane = threadIdx.x;
+  const bool row = lane < N3;
+  const int rl = row ? lane : N3 - 1;  // lanes past the rows load the last row's (cached) lines
+  const int Pp = step_poly(order);
+  if (Pp >= n_poly) return;  // padding workgroups of the last XCD share
+  const int b0 = row_ptr[Pp], b1 = row_ptr[Pp + 1];
+  const int lcb = lane < b1 - b0 ? col[b0 + lane] : 0;
+  const int lsi = lane < b1 - b0 ? sidx[b0 + lane] : 0;
+  const int ltd = lane < b1 - b0 ? tdst[b0 + lane] : 0;
+  if (row) sU[lane] = u[(size_t)Pp * N3 + lane];  // ordered before its reads by the chunks' syncs
+  for (int b = b0; b < b1; ++b) {
+    const int j = b - b0;
+    const int si = j < 64 ? __builtin_amdgcn_readlane(lsi, j) : __builtin_amdgcn_readfirstlane(sidx[b]);
+    if (si < 0) continue;  // written by the neighbour's wave
+    const int cb = j < 64 ? __builtin_amdgcn_readlane(lcb, j) : __builtin_amdgcn_readfirstlane(col[b]);
+    const int td = j < 64 ? __builtin_amdgcn_readlane(ltd, j) : __builtin_amdgcn_readfirstlane(tdst[b]);
+    const float* __restrict__ Kb = Ks + (size_t)si * Dims32<P>::BS;
+    const _Float16* __restrict__ Rb = Kr + (size_t)si * Dims16<P>::BS;
+    const double xl = u[(size_t)cb * N3 + rl];
+    float4 kq[NQ];
+    lymph_hv8 tq[NPR];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) kq[q] = load_stream(Kb + ((size_t)q * N3 + rl) * 4);
+#pragma unroll
+    for (int k = 0; k < NPR; ++k) {
+      const _Float16* tp = Rb + ((size_t)k * N3 + rl) * 8;
+      if (2 * k + 1 < NQ) {
+        tq[k] = load_stream_h8(tp);
+      } else {  // the last quad of an odd count: half a pair
+        const lymph_hv4 h = load_stream_h4(tp);
+        tq[k] = lymph_hv8{h.x, h.y, h.z, h.w, 0, 0, 0, 0};
+      }
+    }
+    // three chunks of CW columns: the row product's chain continues over the
+    // chunk's quads and (pair blocks) each entry's hi and hi t go to LDS as
+    // they are formed, then the chunk's columns run their chains
+    const bool pair = cb != Pp;
+    wave_lds_sync();  // the previous block's reads of sX are done
+    if (row) sX[lane] = xl;
+    wave_lds_sync();
+    double s = 0.0, st = 0.0;
+    float sc = 0.0f;
+#pragma unroll
+    for (int h = 0; h < 3; ++h) {
+      if (pair) wave_lds_sync();  // the previous chunk's reads are done
+#pragma unroll
+      for (int k = 0; k < CQ; ++k) {
+        const int q = h * CQ + k;
+        const double2 xa = *reinterpret_cast<const double2*>(sX + 4 * q);  // broadcasts
+        const double2 xb = *reinterpret_cast<const double2*>(sX + 4 * q + 2);
+        const double xq[4] = {xa.x, xa.y, xb.x, xb.y};
+        const float hv[4] = {kq[q].x, kq[q].y, kq[q].z, kq[q].w};
+        const lymph_hv8 t8 = tq[q >> 1];
+        const int o4 = (q & 1) * 4;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const float hc = hv[j] * (float)t8[o4 + j];
+          s = fma((double)hv[j], xq[j], s);
+          sc = fmaf(hc, (float)xq[j], sc);
+          if (pair) {
+            sH[(4 * k + j) * LDT + lane] = hv[j];
+            sC[(4 * k + j) * LDT + lane] = hc;
+          }
+        }
+      }
+      // the chunk's row chain is complete here: otherwise the compiler defers
+      // it past the column chains and keeps every broadcast u_Q live
+      asm volatile("" : "+v"(s), "+v"(sc));
+      if (!pair) continue;  // self block: no partner
+      wave_lds_sync();
+      if (lane >= h * CW && lane < (h + 1) * CW) {  // column's chain over ascending rows
+        const float* __restrict__ hp = sH + (lane - h * CW) * LDT;
+        const float* __restrict__ cp = sC + (lane - h * CW) * LDT;
+        double v = 0.0;
+        float vc = 0.0f;
+#pragma unroll
+        for (int r = 0; r < N3; ++r) {
+          const double ur = sU[r];  // the same address in every lane: a broadcast
+          v = fma((double)hp[r], ur, v);
+          vc = fmaf(cp[r], (float)ur, vc);
+          if (r % 6 == 5) __builtin_amdgcn_sched_barrier(0);  // bounded loads in flight
+        }
+        st = v + (double)vc * 0x1p-24;
+      }
+    }
+    s = s + (double)sc * 0x1p-24;
+    if (lane < PS) prod[(size_t)b * PS + lane] = row ? s : 0.0;  // zeros past the rows
+    if (pair && lane < PS) prod[(size_t)td * PS + lane] = st;
   }
 }
 

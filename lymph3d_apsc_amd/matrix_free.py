@@ -246,8 +246,8 @@ class Context:
         """Keep every face pair's block once (K_QP applied as the transpose of
         K_PQ) for the FP64 step; the operator must pass the device's pair
         check (SIPG, one material per face).  With fp32=True (p = 3) the
-        32-bit apply keeps its blocks once too; otherwise it keeps its full
-        rows.  Returns the largest max|K_PQ - K_QP^T| / max|K_PQ| found (0
+        32- and 48-bit applies keep their blocks (and the 48-bit corrections)
+        once too; otherwise they keep their full rows.  Returns the largest max|K_PQ - K_QP^T| / max|K_PQ| found (0
         when switching off)."""
         mode = 0 if not on else (2 if fp32 else 1)
         asym = C.c_double()

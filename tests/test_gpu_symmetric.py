@@ -9,7 +9,9 @@ full runs of the reference's decks within four times the known-answer
 tolerance of test_gpu_parity (the factored build's margin) and within 1e-11
 of the oracle's final vector.  Operators that are not symmetric (NIPG,
 per-tet materials) and p = 4 are refused with LYMPH_ERR_ARG and the full rows
-stay in use."""
+stay in use.  The 32- and 48-bit applies with pairs stored once are held to
+their full-row steps' bounds against the oracle (1e-5, 1e-10) and to their
+full rows' long-run drift."""
 import os
 
 import numpy as np
@@ -279,3 +281,103 @@ def test_fp32_pairs_once_long_run_drift():
     d_pairs = np.linalg.norm(res["pairs32"] - res["fp64"]) / nrm
     say(f"  drift vs fp64: full rows {d_full:.3e}, pairs once {d_pairs:.3e}")
     assert d_full < 1e-4 and abs(d_pairs - d_full) < 0.02 * d_full, (d_full, d_pairs)
+
+
+@pytest.mark.parametrize("name,first", [("cube3agg", "pairs"), ("cube3agg", "fp48"), ("grown", "pairs")])
+def test_48bit_apply_with_pairs_stored_once(name, first):
+    """The 48-bit apply with face pairs stored once (p = 3, fp32=True): the
+    stored FP32 blocks and their 16-bit corrections are allocated, the step's
+    acceleration term is within 1e-10 of the FP64 oracle (the 48-bit full
+    rows' bound) and within 1e-11 of the full-row 48-bit step (its products
+    are the full rows' bit for bit where the two sides' blocks round alike),
+    and the device loop and first step agree with the full rows to 1e-10."""
+    say(f"[symmetric] 48-bit apply {name}, {first} first")
+    m, p = _mesh(name), 3
+    o = oracle_for(m, p)
+    o.build_operator()
+    rng = np.random.default_rng(13)
+    n = m.n_poly * 3 * o.Np
+    u0, u1 = rng.standard_normal(n), rng.standard_normal(n)
+    dt, g = 1e-3, 0.37
+    ref = o.step(g, dt, u0, u1, o.rhs(L.PROB_NULL), o.rhs(L.PROB_REF_MMS))
+    acc = lambda a: a - (2 * u1 - u0)  # noqa: E731
+    with L.Context(m, p) as ctx:
+        _prepared(ctx)
+        ctx.set_apply_precision(48)
+        full48 = ctx.time_step_matrix_free(g, dt, u0, u1)
+        bytes48 = ctx.device_bytes
+        ctx.set_apply_precision(64)
+        if first == "pairs":
+            ctx.set_symmetric_storage(True, fp32=True)
+            ctx.set_apply_precision(48)
+        else:
+            ctx.set_apply_precision(48)
+            ctx.set_symmetric_storage(True, fp32=True)
+        n_st = m.n_poly + (ctx.n_blk - m.n_poly) // 2
+        assert ctx.device_bytes - bytes48 >= n_st * (3 * o.Np) ** 2 * (8 + 4 + 2)
+        with pytest.raises(L.LymphError) as e:
+            ctx.autotune(reps=2)
+        assert e.value.code == _lib.ERR_STATE
+        sym48 = ctx.time_step_matrix_free(g, dt, u0, u1)
+        e_or, e_full = rel(acc(sym48), acc(ref)), rel(acc(sym48), acc(full48))
+        say(f"  acceleration: vs oracle {e_or:.2e}, vs full-row 48-bit {e_full:.2e}")
+        assert e_or < 1e-10, e_or
+        assert e_full < 1e-11, e_full
+        ctx.set_slot(_lib.SLOT_U_PREV, u0)
+        ctx.set_slot(_lib.SLOT_U_CUR, u1)
+        ctx.step_n(L.PROB_REF_MMS, 20, 0.0, 1e-4)
+        a = ctx.get_slot(_lib.SLOT_U_CUR)
+        first_sym = ctx.first_time_step_matrix_free(g, dt, u0, u1)
+        ctx.set_symmetric_storage(False)
+        assert ctx.device_bytes == bytes48
+        ctx.set_slot(_lib.SLOT_U_PREV, u0)
+        ctx.set_slot(_lib.SLOT_U_CUR, u1)
+        ctx.step_n(L.PROB_REF_MMS, 20, 0.0, 1e-4)
+        b = ctx.get_slot(_lib.SLOT_U_CUR)
+        first_full = ctx.first_time_step_matrix_free(g, dt, u0, u1)
+        assert rel(a, b) < 1e-10, rel(a, b)
+        assert rel(first_sym - u0, first_full - u0) < 1e-10
+
+
+def test_48bit_pairs_once_long_run_drift_and_capture():
+    """1000 steps of the 48-bit apply from the same start on the 4^3
+    agglomerated cube at p = 3: with face pairs stored once the state is as
+    far from the FP64 state as the full rows' (within 5%, both under 5e-8);
+    a captured solve with the stored pairs replays to the uncaptured steps'
+    state within 1e-12."""
+    say("[symmetric] 48-bit pairs once, 1000-step drift")
+    m, p = M.cube_mesh(4, agglomerate=True), 3
+    dt = 0.2 * (1 / 4) / (np.sqrt(3.0) * p * p)
+    with L.Context(m, p) as ctx:
+        ctx.set_build("factored")
+        ctx.make_matrices_free()
+        res = {}
+        for key in ("fp64", "full48", "pairs48"):
+            ctx.set_symmetric_storage(False)
+            ctx.set_apply_precision(64)
+            ctx.run(L.PROB_CUBE_WAVE, dt, 1e30, max_steps=1)
+            if key != "fp64":
+                ctx.set_apply_precision(48)
+            if key == "pairs48":
+                ctx.set_symmetric_storage(True, fp32=True)
+            ctx.step_n(L.PROB_CUBE_WAVE, 1000, dt, dt)
+            res[key] = ctx.get_slot(_lib.SLOT_U_CUR)
+        # captured against uncaptured, both with the stored pairs (from rest, Ricker point force)
+        src, force = np.array([[0.45, 0.55, 0.5]]), np.array([[0.0, 0.0, 1.0]])
+        t = ctx.start_from_rest(dt, src, force, f0=5.0)
+        ctx.step_n(L.PROB_RICKER, 30, t, dt)
+        u_eager = ctx.get_slot(_lib.SLOT_U_CUR)
+        ctx.start_from_rest(dt, src, force, f0=5.0)
+        ctx.capture_solve(L.PROB_RICKER, dt, 30, every=1)
+        ctx.replay()
+        ctx.sync()
+        u_graph = ctx.get_slot(_lib.SLOT_U_CUR)
+        ctx.set_symmetric_storage(False)
+    nrm = np.linalg.norm(res["fp64"])
+    d_full = np.linalg.norm(res["full48"] - res["fp64"]) / nrm
+    d_pairs = np.linalg.norm(res["pairs48"] - res["fp64"]) / nrm
+    say(f"  drift vs fp64: full rows {d_full:.3e}, pairs once {d_pairs:.3e}")
+    assert d_full < 5e-8 and d_pairs < 5e-8, (d_full, d_pairs)
+    assert abs(d_pairs - d_full) < 0.05 * d_full, (d_full, d_pairs)
+    assert np.abs(u_eager).max() > 0
+    assert rel(u_graph, u_eager) < 1e-12, rel(u_graph, u_eager)
