@@ -375,6 +375,26 @@ def pairs_once_bytes(ctx):
                 + ctx.n_poly * npk + 5 * ctx.n_poly * n3) + 12 * ctx.n_blk
 
 
+def pairs_once_tilde_bytes(ctx, bits):
+    """Algorithmic bytes of one 32- or 48-bit step with face pairs stored once
+    (p = 3): the pair blocks (FP32, and for 48 bits their 16-bit corrections)
+    and the self blocks' packed upper triangles read once -- quad q's rows
+    0..4q+3, and per pair of quads its rows 0..8k+7 (4 halves per row for the
+    last quad of an odd count) -- the per-block products (FP32 for 32 bits,
+    FP64 for 48) written and read back, five FP64 vectors and three int32
+    indices per block."""
+    n3 = ctx.n3
+    nq = n3 // 4
+    n_pair = (ctx.n_blk - ctx.n_poly) // 2
+    self32 = sum(4 * (4 * q + 4) for q in range(nq))
+    self16 = sum((8 if 2 * k + 1 < nq else 4) * min(8 * k + 8, n3) for k in range((nq + 1) // 2))
+    blocks = 4 * (n_pair * n3 * n3 + ctx.n_poly * self32)
+    if bits == 48:
+        blocks += 2 * (n_pair * n3 * n3 + ctx.n_poly * self16)
+    prod = (4 if bits == 32 else 8) * 2 * ctx.n_blk * n3
+    return blocks + prod + 8 * 5 * ctx.n_poly * n3 + 12 * ctx.n_blk
+
+
 def degree_sweep(steps=200):
     """Step throughput at p = 1..4 on agglomerated cubes of about 4 M DOFs
     (70^3, 52^3, 40^3, 33^3 cubes), FP64 and 32-bit apply, each autotuned,
@@ -610,11 +630,7 @@ def main():
             ctx.set_symmetric_storage(True, fp32=True)
             ms32s = ctx.time_steps(prob, a.steps, t_start, dt) / a.steps
             u32s = ctx.get_slot(_lib.SLOT_U_CUR)
-            n3 = 3 * ctx.Np
-            n_st = ctx.n_poly + (ctx.n_blk - ctx.n_poly) // 2
-            # stored FP32 blocks once, per-block FP32 products written and read, five FP64
-            # vectors, three int32 indices per block
-            B32s = 4 * n_st * n3 * n3 + 4 * 2 * ctx.n_blk * n3 + 8 * 5 * ctx.n_poly * n3 + 12 * ctx.n_blk
+            B32s = pairs_once_tilde_bytes(ctx, 32)
             traffic32s, traffic32s_src = pmc_traffic(a.n, p, "k_sym32")
             res["fp32_apply"]["pairs_once"] = {
                 "value": dofs / (ms32s * 1e-3), "unit": "DOF-updates/s", "ms_per_step": ms32s,
@@ -667,11 +683,7 @@ def main():
             ctx.set_symmetric_storage(True, fp32=True)
             ms48s = ctx.time_steps(prob, a.steps, t_start, dt) / a.steps
             u48s = ctx.get_slot(_lib.SLOT_U_CUR)
-            n3 = 3 * ctx.Np
-            n_st = ctx.n_poly + (ctx.n_blk - ctx.n_poly) // 2
-            # stored FP32 blocks and 16-bit corrections once, per-block FP64 products written and
-            # read, five FP64 vectors, three int32 indices per block
-            B48s = 6 * n_st * n3 * n3 + 8 * 2 * ctx.n_blk * n3 + 8 * 5 * ctx.n_poly * n3 + 12 * ctx.n_blk
+            B48s = pairs_once_tilde_bytes(ctx, 48)
             traffic48s, traffic48s_src = pmc_traffic(a.n, p, "k_sym48")
             l2_48s = float(np.sqrt(ctx.compute_error_l2_matrix_free(u48s, u)))
             res["fp48_apply"]["pairs_once"] = {

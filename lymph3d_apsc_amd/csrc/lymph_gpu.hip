@@ -738,6 +738,11 @@ int sym32_build(LymphCtx* c) {
     if (int rc = dalloc(&ks, (size_t)n_s * bs)) return rc;
     k_gather_blocks<float><<<4096, 256, 0, c->stream>>>(c->K32, ks, c->ssrc, n_s, bs);
     int rc = check_launch("k_gather_blocks");
+    if (!rc) {  // the self blocks (stored first) keep their upper triangles
+      const size_t ne = (size_t)c->n_poly * self32_off(Dims32<3>::NQ);
+      k_pack_self32<3><<<grid1(ne, 256), 256, 0, c->stream>>>(c->K32, ks, c->ssrc, c->n_poly);
+      rc = check_launch("k_pack_self32");
+    }
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess)
       rc = fail(LYMPH_ERR_HIP, "gather of the 32-bit stored pairs failed");
     if (rc) {
@@ -754,6 +759,11 @@ int sym32_build(LymphCtx* c) {
     if (int rc = dalloc(&ks, (size_t)n_s * bs)) return rc;
     k_gather_blocks<_Float16><<<4096, 256, 0, c->stream>>>(c->K16, ks, c->ssrc, n_s, bs);
     int rc = check_launch("k_gather_blocks");
+    if (!rc) {
+      const size_t ne = (size_t)c->n_poly * self16_off(Dims32<3>::N3, Dims16<3>::NPR);
+      k_pack_self16<3><<<grid1(ne, 256), 256, 0, c->stream>>>(c->K16, ks, c->ssrc, c->n_poly);
+      rc = check_launch("k_pack_self16");
+    }
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess)
       rc = fail(LYMPH_ERR_HIP, "gather of the 48-bit stored corrections failed");
     if (rc) {

@@ -1296,20 +1296,72 @@ __global__ __launch_bounds__(64) void k_sym_update(const double* __restrict__ pr
   }
 }
 
-// The pairs-once step of the 32-bit apply (mass-orthonormal basis, no
-// solves): ~K_QP = R_Q^-T K_PQ^T R_P^-1 = ~K_PQ^T, so the same stored list
-// applies to the FP32 blocks (column quads, 3Np a multiple of 12 so that each
-// of the three chunks is whole quads: p = 3).  Products and per-block sums in
-// FP32 as k_leapfrog32, stored as floats (prod_stride(3Np) per block, so a
-// 32-bit product moves half the bytes of a 64-bit one); the transposed product of a
-// column runs over ascending rows, the chain Q's full row would run, so where
-// the two rounded blocks are exact transposes the products equal the full
-// rows' bit for bit.  k_sym_update32 adds them in block order in FP64 and
-// applies k_leapfrog32's update (PT = double: the 48-bit apply's products,
-// k_sym_products48).  (Summing the transposed products in three
-// row groups instead made the state drift from FP64 faster over long runs,
-// increasingly with the mesh size: 7x the full rows' drift after 1000 steps
-// at 40^3.)
+// The pairs-once steps of the 32- and 48-bit applies (mass-orthonormal
+// basis, no solves): ~K_QP = R_Q^-T K_PQ^T R_P^-1 = ~K_PQ^T, so the stored
+// list of the FP64 step applies to the FP32 blocks (column quads, 3Np a
+// multiple of 12 so that each of three chunks is whole quads: p = 3).  A
+// self block keeps its upper triangle by column quads: quad q's rows
+// 0..4q+3 from float self32_off(q), row r at self32_off(q) + 4r (and the
+// 48-bit corrections by quad pairs: pair k's rows 0..min(8k+7, 3Np-1) from
+// half self16_off(k), row r at self16_off(k) + self16_w(k) r, 8 halves per
+// row, 4 for the last quad of an odd count); entries below the diagonal that
+// a quad's rows carry are zeroed on load.
+//
+// One wave per polyhedron walks its row's stored blocks.  Lane i holds row i
+// of the block; the block's entries go to LDS in three chunks of CW columns,
+// and lane c runs column c's chain over ascending rows r with u_P[r] -- the
+// chain Q's own full row runs over block (Q, P), so where the two rounded
+// blocks are exact transposes the transposed product equals the full rows'
+// bit for bit.  Then each lane's row chain over ascending columns: from zero
+// for a pair block, and for a self block continuing its own column chain
+// (the strict upper triangle's column r holds row r's entries left of the
+// diagonal, in the full row's order), so a self block's product is also the
+// full rows' product bit for bit.  (Summing the transposed products in three
+// row groups instead made the 32-bit state drift from FP64 faster over long
+// runs, increasingly with the mesh size: 7x the full rows' drift after 1000
+// steps at 40^3.)
+//
+// 32-bit: products and per-block sums in FP32 as k_leapfrog32, stored as
+// floats (prod_stride(3Np) per block, half the bytes of a 64-bit product);
+// k_sym_update32 adds them in block order in FP64 and applies k_leapfrog32's
+// update (PT = double: the 48-bit apply's products, k_sym_products48).
+LYMPH_HD constexpr int self32_off(int q) { return 8 * q * (q + 1); }
+LYMPH_HD constexpr int self16_rows(int n3, int k) { return 8 * k + 8 < n3 ? 8 * k + 8 : n3; }
+LYMPH_HD constexpr int self16_w(int n3, int k) { return 2 * k + 1 < n3 / 4 ? 8 : 4; }
+LYMPH_HD constexpr int self16_off(int n3, int k) {
+  int o = 0;
+  for (int i = 0; i < k; ++i) o += self16_w(n3, i) * self16_rows(n3, i);
+  return o;
+}
+
+// A self block's upper triangle in the packed layouts above, from the
+// full-row block of slot src[s] (s < n_self): one thread per packed entry.
+template <int P>
+__global__ void k_pack_self32(const float* __restrict__ K, float* __restrict__ Ks,
+                              const int* __restrict__ src, int n_self) {
+  constexpr int N3 = Dims32<P>::N3, NE = self32_off(Dims32<P>::NQ);
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (size_t)n_self * NE) return;
+  const int s = (int)(id / NE), e = (int)(id % NE);
+  int q = 0;
+  while (self32_off(q + 1) <= e) ++q;
+  const int r = (e - self32_off(q)) >> 2, c = 4 * q + (e & 3);
+  Ks[(size_t)s * Dims32<P>::BS + e] = K[(size_t)src[s] * Dims32<P>::BS + kpos32(N3, r, c)];
+}
+template <int P>
+__global__ void k_pack_self16(const _Float16* __restrict__ K, _Float16* __restrict__ Ks,
+                              const int* __restrict__ src, int n_self) {
+  constexpr int N3 = Dims32<P>::N3, NE = self16_off(N3, Dims16<P>::NPR);
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (size_t)n_self * NE) return;
+  const int s = (int)(id / NE), e = (int)(id % NE);
+  int k = 0;
+  while (self16_off(N3, k + 1) <= e) ++k;
+  const int w = self16_w(N3, k), r = (e - self16_off(N3, k)) / w, h = (e - self16_off(N3, k)) % w;
+  const int c = 8 * k + h;  // quad 2k + h / 4, column h % 4 of it
+  Ks[(size_t)s * Dims16<P>::BS + e] = K[(size_t)src[s] * Dims16<P>::BS + kpos16(N3, r, c)];
+}
+
 template <int P>
 __global__ __launch_bounds__(64) void k_sym_products32(const float* __restrict__ Ks,
                                                        const int* __restrict__ row_ptr,
@@ -1321,68 +1373,86 @@ __global__ __launch_bounds__(64) void k_sym_products32(const float* __restrict__
                                                        const int* __restrict__ order) {
   constexpr int N3 = Dims32<P>::N3, NQ = Dims32<P>::NQ;
   static_assert(N3 <= 64 && N3 % 12 == 0, "one lane per row, whole quads per chunk");
+  static_assert(self32_off(NQ) <= (int)Dims32<P>::BS, "packed self block fits its slot");
   constexpr int CW = N3 / 3, CQ = CW / 4;
-  constexpr int LDT = N3 + 1;
-  __shared__ float sT[CW * LDT];
+  // one chunk row by row (lanes past the rows write padding rows): a lane
+  // stores its quads with 16-byte writes, and a column read over 20 lanes
+  // touches 20 distinct banks
+  __shared__ __align__(16) float sT[64 * CW];
+  __shared__ __align__(16) float sX[N3];  // u_Q of the current block, read by the row chains
   const int lane = threadIdx.x;
   const bool row = lane < N3;
+  const int rl = row ? lane : N3 - 1;  // lanes past the rows load the last row's (cached) lines
   const int Pp = step_poly(order);
   if (Pp >= n_poly) return;  // padding workgroups of the last XCD share
   const int b0 = row_ptr[Pp], b1 = row_ptr[Pp + 1];
   const int lcb = lane < b1 - b0 ? col[b0 + lane] : 0;
   const int lsi = lane < b1 - b0 ? sidx[b0 + lane] : 0;
   const int ltd = lane < b1 - b0 ? tdst[b0 + lane] : 0;
-  const float up = row ? (float)u[(size_t)Pp * N3 + lane] : 0.0f;
+  const float up = (float)u[(size_t)Pp * N3 + rl];
   for (int b = b0; b < b1; ++b) {
     const int j = b - b0;
     const int si = j < 64 ? __builtin_amdgcn_readlane(lsi, j) : __builtin_amdgcn_readfirstlane(sidx[b]);
     if (si < 0) continue;  // written by the neighbour's wave
     const int cb = j < 64 ? __builtin_amdgcn_readlane(lcb, j) : __builtin_amdgcn_readfirstlane(col[b]);
     const int td = j < 64 ? __builtin_amdgcn_readlane(ltd, j) : __builtin_amdgcn_readfirstlane(tdst[b]);
+    const bool pair = cb != Pp;
     const float* __restrict__ Kb = Ks + (size_t)si * Dims32<P>::BS;
-    const float xl = row ? (float)u[(size_t)cb * N3 + lane] : 0.0f;
+    const float xl = (float)u[(size_t)cb * N3 + rl];
+    // the self blocks' row clamps and masks are formed per block: hoisted
+    // out of the loop they would hold 60 masks in scalar registers
+    int rb = rl;
+    asm volatile("" : "+v"(rb));
     float4 kq[NQ];
+    if (pair) {
 #pragma unroll
-    for (int q = 0; q < NQ; ++q)
-      kq[q] = row ? load_stream(Kb + ((size_t)q * N3 + lane) * 4) : make_float4(0.f, 0.f, 0.f, 0.f);
-    float s = 0.0f;
+      for (int q = 0; q < NQ; ++q) kq[q] = load_stream(Kb + ((size_t)q * N3 + rl) * 4);
+    } else {  // the upper triangle's rows only
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      float xq[4];
+      for (int q = 0; q < NQ; ++q)
+        kq[q] = load_stream(Kb + self32_off(q) + 4 * (rb < 4 * q + 3 ? rb : 4 * q + 3));
 #pragma unroll
-      for (int k = 0; k < 4; ++k)
-        xq[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xl), 4 * q + k));
-      s = s + kq[q].x * xq[0];
-      s = s + kq[q].y * xq[1];
-      s = s + kq[q].z * xq[2];
-      s = s + kq[q].w * xq[3];
+      for (int q = 0; q < NQ; ++q) {
+        if (4 * q + 0 < rb) kq[q].x = 0.0f;
+        if (4 * q + 1 < rb) kq[q].y = 0.0f;
+        if (4 * q + 2 < rb) kq[q].z = 0.0f;
+        if (4 * q + 3 < rb) kq[q].w = 0.0f;
+      }
     }
-    if (lane < prod_stride(N3)) prod[(size_t)b * prod_stride(N3) + lane] = s;  // zeros past the rows
-    if (cb == Pp) continue;  // self block: no partner
     float st = 0.0f;
 #pragma unroll
     for (int h = 0; h < 3; ++h) {
-      wave_lds_sync();  // the previous chunk's reads are done
-      if (row) {
+      wave_lds_sync();  // the previous chunk's (and block's) reads are done
+      if (h == 0 && row) sX[lane] = xl;
 #pragma unroll
-        for (int k = 0; k < CQ; ++k) {
-          const float4 v = kq[h * CQ + k];
-          sT[(4 * k) * LDT + lane] = v.x * up;
-          sT[(4 * k + 1) * LDT + lane] = v.y * up;
-          sT[(4 * k + 2) * LDT + lane] = v.z * up;
-          sT[(4 * k + 3) * LDT + lane] = v.w * up;
-        }
+      for (int k = 0; k < CQ; ++k) {
+        const float4 v = kq[h * CQ + k];
+        *reinterpret_cast<float4*>(sT + lane * CW + 4 * k) =
+            make_float4(v.x * up, v.y * up, v.z * up, v.w * up);
       }
+      // a self block's diagonal belongs to its rows only (a wave's LDS
+      // writes land in program order)
+      if (!pair && lane >= h * CW && lane < (h + 1) * CW) sT[lane * CW + (lane - h * CW)] = 0.0f;
       wave_lds_sync();
-      if (lane >= h * CW && lane < (h + 1) * CW) {  // column's products over ascending rows
-        const float* __restrict__ colp = sT + (lane - h * CW) * LDT;
+      if (lane >= h * CW && lane < (h + 1) * CW) {  // column's chain over ascending rows
+        const float* __restrict__ colp = sT + (lane - h * CW);
         float v = 0.0f;
 #pragma unroll
-        for (int r = 0; r < N3; ++r) v = v + colp[r];
+        for (int r = 0; r < N3; ++r) v = v + colp[r * CW];
         st = v;
       }
     }
-    if (lane < prod_stride(N3)) prod[(size_t)td * prod_stride(N3) + lane] = st;
+    float s = pair ? 0.0f : st;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const float4 xq = *reinterpret_cast<const float4*>(sX + 4 * q);  // a broadcast
+      s = s + kq[q].x * xq.x;
+      s = s + kq[q].y * xq.y;
+      s = s + kq[q].z * xq.z;
+      s = s + kq[q].w * xq.w;
+    }
+    if (lane < prod_stride(N3)) prod[(size_t)b * prod_stride(N3) + lane] = row ? s : 0.0f;
+    if (pair && lane < prod_stride(N3)) prod[(size_t)td * prod_stride(N3) + lane] = row ? st : 0.0f;
   }
 }
 
@@ -2302,17 +2372,18 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog48(const float* __restrict
 }
 
 // The pairs-once step of the 48-bit apply: k_sym_products32's walk over the
-// stored list (one wave per polyhedron, self blocks whole and multiplied by
-// rows, pair blocks also transposed through LDS) with k_leapfrog48's
-// arithmetic, products stored as doubles (prod_stride(3Np) per block) and
-// summed by k_sym_update32<P, MODE, double>.  The row product is
-// k_leapfrog48's block product; the transposed product of column c runs the
-// chain Q's own row would run over block (Q, P) -- hi x_P[r] in FP64 fused
-// multiply-adds and (hi t) x_P[r] in FP32 over ascending r, joined with the
-// 2^-24 scale -- from the entries hi and hi t that the row lanes leave in
-// LDS, so where the two stored blocks are exact transposes it equals the
-// full rows' product bit for bit.  p = 3 (3Np = 60: whole quads per chunk,
-// no tail columns).
+// stored list and its packed self blocks, with k_leapfrog48's arithmetic
+// (hi x in FP64 fused multiply-adds, (hi t) x in FP32, joined with the 2^-24
+// scale per block), products stored as doubles (prod_stride(3Np) per block)
+// and summed by k_sym_update32<P, MODE, double>.  Here the row chain of each
+// chunk runs first and writes the chunk's hi and hi t to LDS as it forms
+// them; the chunk's columns then run their chains over ascending rows with
+// u_P[r], the chain Q's own row runs over block (Q, P), so a pair block's
+// products are the full rows' bit for bit where the two stored blocks round
+// alike.  A self block's product is its upper triangle's row chain (diagonal
+// included) plus its strict upper triangle's column chain: the full row's
+// terms in another order, which moves its FP64 sum by rounding only.  p = 3
+// (3Np = 60: whole quads per chunk, no tail columns).
 template <int P>
 __global__ __launch_bounds__(64) void k_sym_products48(const float* __restrict__ Ks,
                                                        const _Float16* __restrict__ Kr,
@@ -2326,11 +2397,14 @@ __global__ __launch_bounds__(64) void k_sym_products48(const float* __restrict__
   constexpr int N3 = Dims32<P>::N3, NQ = Dims32<P>::NQ, NPR = Dims16<P>::NPR;
   constexpr int PS = prod_stride(N3);
   static_assert(N3 <= 64 && N3 % 12 == 0 && Dims32<P>::T == 0, "one lane per row, whole quads per chunk");
+  static_assert(self32_off(NQ) <= (int)Dims32<P>::BS, "packed self block fits its slot");
+  static_assert(self16_off(N3, NPR) <= (int)Dims16<P>::BS, "packed self corrections fit their slot");
   constexpr int CW = N3 / 3, CQ = CW / 4;
-  constexpr int LDT = 65;  // every lane writes its row (lanes past the rows into padding); odd stride
-  __shared__ float sH[CW * LDT], sC[CW * LDT];  // hi and hi t of one chunk, column-major
-  __shared__ __align__(16) double sU[N3];       // u_P, read by the column chains
-  __shared__ __align__(16) double sX[N3];                      // u_Q of the current block, read by the rows
+  // one chunk's hi and hi t row by row (lanes past the rows write padding
+  // rows): 16-byte writes, and a column read over 20 lanes hits 20 banks
+  __shared__ __align__(16) float sH[64 * CW], sC[64 * CW];
+  __shared__ __align__(16) double sU[N3];  // u_P, read by the column chains
+  __shared__ __align__(16) double sX[N3];  // u_Q of the current block, read by the row chains
   const int lane = threadIdx.x;
   const bool row = lane < N3;
   const int rl = row ? lane : N3 - 1;  // lanes past the rows load the last row's (cached) lines
@@ -2350,24 +2424,52 @@ __global__ __launch_bounds__(64) void k_sym_products48(const float* __restrict__
     const float* __restrict__ Kb = Ks + (size_t)si * Dims32<P>::BS;
     const _Float16* __restrict__ Rb = Kr + (size_t)si * Dims16<P>::BS;
     const double xl = u[(size_t)cb * N3 + rl];
+    const bool pair = cb != Pp;
+    // the self blocks' row clamps and masks are formed per block: hoisted
+    // out of the loop they would hold 60 masks in scalar registers
+    int rb = rl;
+    asm volatile("" : "+v"(rb));
     float4 kq[NQ];
     lymph_hv8 tq[NPR];
+    if (pair) {
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) kq[q] = load_stream(Kb + ((size_t)q * N3 + rl) * 4);
+      for (int q = 0; q < NQ; ++q) kq[q] = load_stream(Kb + ((size_t)q * N3 + rl) * 4);
 #pragma unroll
-    for (int k = 0; k < NPR; ++k) {
-      const _Float16* tp = Rb + ((size_t)k * N3 + rl) * 8;
-      if (2 * k + 1 < NQ) {
-        tq[k] = load_stream_h8(tp);
-      } else {  // the last quad of an odd count: half a pair
-        const lymph_hv4 h = load_stream_h4(tp);
-        tq[k] = lymph_hv8{h.x, h.y, h.z, h.w, 0, 0, 0, 0};
+      for (int k = 0; k < NPR; ++k) {
+        const _Float16* tp = Rb + ((size_t)k * N3 + rl) * 8;
+        if (2 * k + 1 < NQ) {
+          tq[k] = load_stream_h8(tp);
+        } else {  // the last quad of an odd count: half a pair
+          const lymph_hv4 h = load_stream_h4(tp);
+          tq[k] = lymph_hv8{h.x, h.y, h.z, h.w, 0, 0, 0, 0};
+        }
+      }
+    } else {  // the upper triangle's rows only
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+        kq[q] = load_stream(Kb + self32_off(q) + 4 * (rb < 4 * q + 3 ? rb : 4 * q + 3));
+#pragma unroll
+      for (int k = 0; k < NPR; ++k) {
+        const int last = self16_rows(N3, k) - 1;
+        const _Float16* tp = Rb + self16_off(N3, k) + self16_w(N3, k) * (rb < last ? rb : last);
+        if (2 * k + 1 < NQ) {
+          tq[k] = load_stream_h8(tp);
+        } else {
+          const lymph_hv4 h = load_stream_h4(tp);
+          tq[k] = lymph_hv8{h.x, h.y, h.z, h.w, 0, 0, 0, 0};
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        if (4 * q + 0 < rb) kq[q].x = 0.0f;
+        if (4 * q + 1 < rb) kq[q].y = 0.0f;
+        if (4 * q + 2 < rb) kq[q].z = 0.0f;
+        if (4 * q + 3 < rb) kq[q].w = 0.0f;
       }
     }
-    // three chunks of CW columns: the row product's chain continues over the
-    // chunk's quads and (pair blocks) each entry's hi and hi t go to LDS as
-    // they are formed, then the chunk's columns run their chains
-    const bool pair = cb != Pp;
+    // three chunks of CW columns: the row chain continues over the chunk's
+    // quads, each entry's hi and hi t going to LDS as they are formed, then
+    // the chunk's columns run their chains
     wave_lds_sync();  // the previous block's reads of sX are done
     if (row) sX[lane] = xl;
     wave_lds_sync();
@@ -2375,7 +2477,7 @@ __global__ __launch_bounds__(64) void k_sym_products48(const float* __restrict__
     float sc = 0.0f;
 #pragma unroll
     for (int h = 0; h < 3; ++h) {
-      if (pair) wave_lds_sync();  // the previous chunk's reads are done
+      wave_lds_sync();  // the previous chunk's reads are done
 #pragma unroll
       for (int k = 0; k < CQ; ++k) {
         const int q = h * CQ + k;
@@ -2385,40 +2487,44 @@ __global__ __launch_bounds__(64) void k_sym_products48(const float* __restrict__
         const float hv[4] = {kq[q].x, kq[q].y, kq[q].z, kq[q].w};
         const lymph_hv8 t8 = tq[q >> 1];
         const int o4 = (q & 1) * 4;
+        float hc[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const float hc = hv[j] * (float)t8[o4 + j];
+          hc[j] = hv[j] * (float)t8[o4 + j];
           s = fma((double)hv[j], xq[j], s);
-          sc = fmaf(hc, (float)xq[j], sc);
-          if (pair) {
-            sH[(4 * k + j) * LDT + lane] = hv[j];
-            sC[(4 * k + j) * LDT + lane] = hc;
-          }
+          sc = fmaf(hc[j], (float)xq[j], sc);
         }
+        *reinterpret_cast<float4*>(sH + lane * CW + 4 * k) = kq[q];
+        *reinterpret_cast<float4*>(sC + lane * CW + 4 * k) = make_float4(hc[0], hc[1], hc[2], hc[3]);
       }
       // the chunk's row chain is complete here: otherwise the compiler defers
       // it past the column chains and keeps every broadcast u_Q live
       asm volatile("" : "+v"(s), "+v"(sc));
-      if (!pair) continue;  // self block: no partner
+      // a self block's diagonal belongs to its rows only (a wave's LDS
+      // writes land in program order)
+      if (!pair && lane >= h * CW && lane < (h + 1) * CW) {
+        sH[lane * CW + (lane - h * CW)] = 0.0f;
+        sC[lane * CW + (lane - h * CW)] = 0.0f;
+      }
       wave_lds_sync();
       if (lane >= h * CW && lane < (h + 1) * CW) {  // column's chain over ascending rows
-        const float* __restrict__ hp = sH + (lane - h * CW) * LDT;
-        const float* __restrict__ cp = sC + (lane - h * CW) * LDT;
+        const float* __restrict__ hp = sH + (lane - h * CW);
+        const float* __restrict__ cp = sC + (lane - h * CW);
         double v = 0.0;
         float vc = 0.0f;
 #pragma unroll
         for (int r = 0; r < N3; ++r) {
           const double ur = sU[r];  // the same address in every lane: a broadcast
-          v = fma((double)hp[r], ur, v);
-          vc = fmaf(cp[r], (float)ur, vc);
-          if (r % 6 == 5) __builtin_amdgcn_sched_barrier(0);  // bounded loads in flight
+          v = fma((double)hp[r * CW], ur, v);
+          vc = fmaf(cp[r * CW], (float)ur, vc);
         }
         st = v + (double)vc * 0x1p-24;
       }
     }
     s = s + (double)sc * 0x1p-24;
+    if (!pair) s = s + st;  // the strict upper triangle's transposed product
     if (lane < PS) prod[(size_t)b * PS + lane] = row ? s : 0.0;  // zeros past the rows
-    if (pair && lane < PS) prod[(size_t)td * PS + lane] = st;
+    if (pair && lane < PS) prod[(size_t)td * PS + lane] = row ? st : 0.0;
   }
 }
 
