@@ -378,8 +378,8 @@ int check_launch(const char* what) {
 unsigned grid1(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 size_t Dims16Any(int p) {  // Dims16<p>::BS
   const size_t n3 = (size_t)(p + 1) * (p + 2) * (p + 3) / 2;
-  const size_t npr = (n3 / 4 + 1) / 2;
-  return (npr * n3 * 8 + n3 * (n3 % 4) + 7) / 8 * 8;
+  const size_t nq = n3 / 4;
+  return ((nq / 2) * n3 * 8 + (nq % 2) * n3 * 4 + n3 * (n3 % 4) + 7) / 8 * 8;
 }
 size_t Dims32Any(int p) {  // Dims32<p>::BS
   const size_t n3 = (size_t)(p + 1) * (p + 2) * (p + 3) / 2;

@@ -71,23 +71,27 @@ struct Dims32 {
   static constexpr size_t BS = ((size_t)N3 * N3 + 3) / 4 * 4;
 };
 // The 48-bit apply's 16-bit corrections: column quads taken in pairs, a
-// row's 8 values of quads 2k, 2k+1 contiguous (one 16-byte load), then the
-// N3 % 4 tail columns row by row; blocks padded to 8 values.
+// row's 8 values of quads 2k, 2k+1 contiguous (one 16-byte load); the last
+// quad of an odd count row by row at 4 values per row (one 8-byte load, no
+// half-empty slots to come in with the lines); then the N3 % 4 tail columns
+// row by row; blocks padded to 8 values (none at p = 1, 3).
 template <int P>
 struct Dims16 {
   static constexpr int N3 = 3 * np_of(P);
-  static constexpr int NQ = N3 / 4, T = N3 % 4, NPR = (NQ + 1) / 2;
-  static constexpr size_t TAIL0 = (size_t)NPR * N3 * 8;
+  static constexpr int NQ = N3 / 4, T = N3 % 4, NPF = NQ / 2, NPR = (NQ + 1) / 2;
+  static constexpr size_t HALF0 = (size_t)NPF * N3 * 8;
+  static constexpr size_t TAIL0 = HALF0 + (size_t)(NQ % 2) * N3 * 4;
   static constexpr size_t BS = (TAIL0 + (size_t)N3 * T + 7) / 8 * 8;
 };
 LYMPH_HD size_t kpos16(int N3, int r, int c) {
-  const int c0 = N3 / 4 * 4;
+  const int c0 = N3 / 4 * 4, nq = N3 / 4, npf = nq / 2;
+  const size_t half0 = (size_t)npf * N3 * 8;
   if (c < c0) {
     const int q = c >> 2;
-    return ((size_t)(q >> 1) * N3 + r) * 8 + (q & 1) * 4 + (c & 3);
+    if (q < 2 * npf) return ((size_t)(q >> 1) * N3 + r) * 8 + (q & 1) * 4 + (c & 3);
+    return half0 + (size_t)r * 4 + (c & 3);
   }
-  const int npr = (N3 / 4 + 1) / 2;
-  return (size_t)npr * N3 * 8 + (size_t)r * (N3 % 4) + (c - c0);
+  return half0 + (size_t)(nq % 2) * N3 * 4 + (size_t)r * (N3 % 4) + (c - c0);
 }
 LYMPH_HD size_t kpos32(int N3, int r, int c) {
   const int c0 = N3 / 4 * 4;
@@ -2269,11 +2273,10 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog48(const float* __restrict
           if (q < NQUAD && r < N3) {
             kv[uu][k] = load_stream(Kb + ((size_t)q * N3 + r) * 4);
             if ((uu & 1) == 0) {
-              const _Float16* tp = Rb + ((size_t)(q >> 1) * N3 + r) * 8;
               if (q + 1 < NQUAD) {
-                tv[uu / 2][k] = load_stream_h8(tp);
-              } else {  // the last quad of an odd count: half a pair
-                const lymph_hv4 h = load_stream_h4(tp);
+                tv[uu / 2][k] = load_stream_h8(Rb + ((size_t)(q >> 1) * N3 + r) * 8);
+              } else {  // the last quad of an odd count: 4 values per row
+                const lymph_hv4 h = load_stream_h4(Rb + Dims16<P>::HALF0 + (size_t)r * 4);
                 tv[uu / 2][k] = lymph_hv8{h.x, h.y, h.z, h.w, 0, 0, 0, 0};
               }
             }
@@ -2436,11 +2439,10 @@ __global__ __launch_bounds__(64) void k_sym_products48(const float* __restrict__
       for (int q = 0; q < NQ; ++q) kq[q] = load_stream(Kb + ((size_t)q * N3 + rl) * 4);
 #pragma unroll
       for (int k = 0; k < NPR; ++k) {
-        const _Float16* tp = Rb + ((size_t)k * N3 + rl) * 8;
         if (2 * k + 1 < NQ) {
-          tq[k] = load_stream_h8(tp);
-        } else {  // the last quad of an odd count: half a pair
-          const lymph_hv4 h = load_stream_h4(tp);
+          tq[k] = load_stream_h8(Rb + ((size_t)k * N3 + rl) * 8);
+        } else {  // the last quad of an odd count: 4 values per row
+          const lymph_hv4 h = load_stream_h4(Rb + Dims16<P>::HALF0 + (size_t)rl * 4);
           tq[k] = lymph_hv8{h.x, h.y, h.z, h.w, 0, 0, 0, 0};
         }
       }
