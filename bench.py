@@ -380,9 +380,10 @@ def pairs_once_tilde_bytes(ctx, bits):
     (p = 3): the pair blocks (FP32, and for 48 bits their 16-bit corrections)
     and the self blocks' packed upper triangles read once -- quad q's rows
     0..4q+3, and per pair of quads its rows 0..8k+7 (4 halves per row for the
-    last quad of an odd count) -- the per-block products (FP32 for 32 bits,
-    FP64 for 48) written and read back, five FP64 vectors and three int32
-    indices per block."""
+    last quad of an odd count) -- the pair blocks' transposed products (FP32
+    for 32 bits, FP64 for 48) and each polyhedron's summed row products (FP64)
+    written and read back, five FP64 vectors, and four int32 indices per
+    block (three read by the products pass, one by the update)."""
     n3 = ctx.n3
     nq = n3 // 4
     n_pair = (ctx.n_blk - ctx.n_poly) // 2
@@ -391,8 +392,8 @@ def pairs_once_tilde_bytes(ctx, bits):
     blocks = 4 * (n_pair * n3 * n3 + ctx.n_poly * self32)
     if bits == 48:
         blocks += 2 * (n_pair * n3 * n3 + ctx.n_poly * self16)
-    prod = (4 if bits == 32 else 8) * 2 * ctx.n_blk * n3
-    return blocks + prod + 8 * 5 * ctx.n_poly * n3 + 12 * ctx.n_blk
+    prod = (4 if bits == 32 else 8) * 2 * n_pair * n3 + 8 * 2 * ctx.n_poly * n3
+    return blocks + prod + 8 * 5 * ctx.n_poly * n3 + 16 * ctx.n_blk
 
 
 def degree_sweep(steps=200):

@@ -129,6 +129,7 @@ struct LymphCtx {
   size_t ks32_bytes = 0;    // Ks32 and Ks16
   bool sym32_wanted = false;  // LYMPH_SYM_FP64_FP32: the 32- and 48-bit applies store pairs once too
   double* prod = nullptr;   // per-block products of one step (n_blk x 3Np)
+  double* rowp = nullptr;   // 32/48-bit pairs once: per polyhedron, its stored blocks' row products summed
   size_t ks_bytes = 0;
   int* blk_row = nullptr;   // row polyhedron of each block
   int* order = nullptr;     // step dispatch order (step_grid(n_poly) entries)
@@ -434,28 +435,33 @@ int launch_step(LymphCtx* c, int mode, double g, double c1, double c2, const dou
       if constexpr (3 * np_of(P) % 12 == 0 && 3 * np_of(P) <= 64) {
         float* prod32 = reinterpret_cast<float*>(c->prod);  // the 32-bit products in the same buffer
         k_sym_products32<P><<<step_grid(c->n_poly), 64, 0, c->stream>>>(
-            c->Ks32, c->md.row_ptr, c->md.col, c->sidx, c->tdst, u, prod32, c->n_poly, c->order);
+            c->Ks32, c->md.row_ptr, c->md.col, c->sidx, c->tdst, u, prod32, c->rowp, c->n_poly, c->order);
         if (int rc = check_launch("k_sym_products32")) return rc;
         if (mode == 0)
           k_sym_update32<P, 0><<<step_grid(c->n_poly), 64, 0, c->stream>>>(
-              prod32, c->md.row_ptr, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->n_poly, c->order);
+              prod32, c->rowp, c->sidx, c->md.row_ptr, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->n_poly,
+              c->order);
         else
           k_sym_update32<P, 1><<<step_grid(c->n_poly), 64, 0, c->stream>>>(
-              prod32, c->md.row_ptr, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->n_poly, c->order);
+              prod32, c->rowp, c->sidx, c->md.row_ptr, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->n_poly,
+              c->order);
       } else {
         return fail(LYMPH_ERR_STATE, "32-bit stored pairs need 3Np a multiple of 12");
       }
     } else if (c->apply_bits == 48 && c->Ks16) {
       if constexpr (3 * np_of(P) % 12 == 0 && 3 * np_of(P) <= 64) {
         k_sym_products48<P><<<step_grid(c->n_poly), 64, 0, c->stream>>>(
-            c->Ks32, c->Ks16, c->md.row_ptr, c->md.col, c->sidx, c->tdst, u, c->prod, c->n_poly, c->order);
+            c->Ks32, c->Ks16, c->md.row_ptr, c->md.col, c->sidx, c->tdst, u, c->prod, c->rowp, c->n_poly,
+            c->order);
         if (int rc = check_launch("k_sym_products48")) return rc;
         if (mode == 0)
           k_sym_update32<P, 0, double><<<step_grid(c->n_poly), 64, 0, c->stream>>>(
-              c->prod, c->md.row_ptr, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->n_poly, c->order);
+              c->prod, c->rowp, c->sidx, c->md.row_ptr, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->n_poly,
+              c->order);
         else
           k_sym_update32<P, 1, double><<<step_grid(c->n_poly), 64, 0, c->stream>>>(
-              c->prod, c->md.row_ptr, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->n_poly, c->order);
+              c->prod, c->rowp, c->sidx, c->md.row_ptr, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->n_poly,
+              c->order);
       } else {
         return fail(LYMPH_ERR_STATE, "48-bit stored pairs need 3Np a multiple of 12");
       }
@@ -539,13 +545,15 @@ void drop_graph(LymphCtx* c) {
 // Drop the once-stored face pairs (operator rebuilt, storage switched off).
 int sym_off(LymphCtx* c) {
   if (!c->Ks && !c->sidx && !c->tdst && !c->prod && !c->ssrc && !c->Ks32 && !c->Ks16 && !c->Sinv &&
-      !c->pinfo)
+      !c->pinfo && !c->rowp)
     return 0;
   HIP_TRY(hipStreamSynchronize(c->stream));
   drop_graph(c);
   if (c->ssrc) HIP_TRY(hipFree(c->ssrc));
   if (c->Ks32) HIP_TRY(hipFree(c->Ks32));
   if (c->Ks16) HIP_TRY(hipFree(c->Ks16));
+  if (c->rowp) HIP_TRY(hipFree(c->rowp));
+  c->rowp = nullptr;
   c->ssrc = nullptr;
   c->Ks32 = nullptr;
   c->Ks16 = nullptr;
@@ -732,6 +740,12 @@ int sym_on(LymphCtx* c, double tol, double* asym) {
 int sym32_build(LymphCtx* c) {
   if (!c->sym32_wanted || !c->ssrc || !c->K32 || c->p != 3) return 0;
   const int n_s = c->n_poly + (c->n_blk - c->n_poly) / 2;
+  if (!c->rowp) {  // each polyhedron's summed row products
+    const size_t n = (size_t)c->n_poly * prod_stride(c->n3);
+    if (int rc = dalloc(&c->rowp, n)) return rc;
+    c->ks32_bytes += n * sizeof(double);
+    c->bytes += n * sizeof(double);
+  }
   if (!c->Ks32) {
     const size_t bs = (size_t)Dims32Any(c->p);
     float* ks = nullptr;
@@ -901,6 +915,7 @@ int lymph_ctx_destroy(LymphCtx* c) {
   if (c->ssrc) (void)hipFree(c->ssrc);
   if (c->Ks32) (void)hipFree(c->Ks32);
   if (c->Ks16) (void)hipFree(c->Ks16);
+  if (c->rowp) (void)hipFree(c->rowp);
   if (c->Sinv) (void)hipFree(c->Sinv);
   if (c->pinfo) (void)hipFree(c->pinfo);
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);

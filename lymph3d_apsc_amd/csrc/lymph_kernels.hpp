@@ -1325,10 +1325,16 @@ __global__ __launch_bounds__(64) void k_sym_update(const double* __restrict__ pr
 // runs, increasingly with the mesh size: 7x the full rows' drift after 1000
 // steps at 40^3.)
 //
-// 32-bit: products and per-block sums in FP32 as k_leapfrog32, stored as
-// floats (prod_stride(3Np) per block, half the bytes of a 64-bit product);
-// k_sym_update32 adds them in block order in FP64 and applies k_leapfrog32's
-// update (PT = double: the 48-bit apply's products, k_sym_products48).
+// A wave adds the row products of its stored blocks in FP64 (block order)
+// and stores that sum once, rowp[P]; only a pair block's transposed product
+// is stored per block, in the slot of block (Q, P) of Q's row.
+// k_sym_update32 adds to rowp[P] the transposed products of the blocks kept
+// in the neighbours' rows (sidx < 0), in block order, in FP64, and applies
+// k_leapfrog32's update.
+//
+// 32-bit: products and per-block sums in FP32 as k_leapfrog32, transposed
+// products stored as floats (prod_stride(3Np) per block, half the bytes of a
+// 64-bit product; PT = double: the 48-bit apply's, k_sym_products48).
 LYMPH_HD constexpr int self32_off(int q) { return 8 * q * (q + 1); }
 LYMPH_HD constexpr int self16_rows(int n3, int k) { return 8 * k + 8 < n3 ? 8 * k + 8 : n3; }
 LYMPH_HD constexpr int self16_w(int n3, int k) { return 2 * k + 1 < n3 / 4 ? 8 : 4; }
@@ -1373,7 +1379,8 @@ __global__ __launch_bounds__(64) void k_sym_products32(const float* __restrict__
                                                        const int* __restrict__ sidx,
                                                        const int* __restrict__ tdst,
                                                        const double* __restrict__ u,
-                                                       float* __restrict__ prod, int n_poly,
+                                                       float* __restrict__ prod,
+                                                       double* __restrict__ rowp, int n_poly,
                                                        const int* __restrict__ order) {
   constexpr int N3 = Dims32<P>::N3, NQ = Dims32<P>::NQ;
   static_assert(N3 <= 64 && N3 % 12 == 0, "one lane per row, whole quads per chunk");
@@ -1394,6 +1401,7 @@ __global__ __launch_bounds__(64) void k_sym_products32(const float* __restrict__
   const int lsi = lane < b1 - b0 ? sidx[b0 + lane] : 0;
   const int ltd = lane < b1 - b0 ? tdst[b0 + lane] : 0;
   const float up = (float)u[(size_t)Pp * N3 + rl];
+  double racc = 0.0;  // the row products of this wave's stored blocks, in block order
   for (int b = b0; b < b1; ++b) {
     const int j = b - b0;
     const int si = j < 64 ? __builtin_amdgcn_readlane(lsi, j) : __builtin_amdgcn_readfirstlane(sidx[b]);
@@ -1455,13 +1463,19 @@ __global__ __launch_bounds__(64) void k_sym_products32(const float* __restrict__
       s = s + kq[q].z * xq.z;
       s = s + kq[q].w * xq.w;
     }
-    if (lane < prod_stride(N3)) prod[(size_t)b * prod_stride(N3) + lane] = row ? s : 0.0f;
+    // the block's row chain ends here: left free, the compiler interleaves it
+    // with the next block's loads and holds both in registers
+    asm volatile("" : "+v"(s));
+    racc = racc + (double)s;
     if (pair && lane < prod_stride(N3)) prod[(size_t)td * prod_stride(N3) + lane] = row ? st : 0.0f;
   }
+  if (lane < prod_stride(N3)) rowp[(size_t)Pp * prod_stride(N3) + lane] = row ? racc : 0.0;
 }
 
 template <int P, int MODE, typename PT = float>
 __global__ __launch_bounds__(64) void k_sym_update32(const PT* __restrict__ prod,
+                                                     const double* __restrict__ rowp,
+                                                     const int* __restrict__ sidx,
                                                      const int* __restrict__ row_ptr,
                                                      const double* __restrict__ rs,
                                                      const double* __restrict__ rd,
@@ -1481,15 +1495,19 @@ __global__ __launch_bounds__(64) void k_sym_update32(const PT* __restrict__ prod
   const int b0 = row_ptr[Pp], b1 = row_ptr[Pp + 1];
   const size_t o = (size_t)Pp * N3;
   const double vs = rs[o + lane], vd = rd[o + lane], vu = u[o + lane], va = wa[o + lane];
-  double acc = 0.0;
+  // the wave's own row products (summed by the products pass), then the
+  // neighbours' transposed products in block order
+  double acc = rowp[(size_t)Pp * prod_stride(N3) + lane];
   for (int b = b0; b < b1; b += 8) {
     double v[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-      v[k] = b + k < b1 ? (double)prod[(size_t)(b + k) * prod_stride(N3) + lane] : 0.0;
+    for (int k = 0; k < 8; ++k) {
+      const bool theirs = b + k < b1 && sidx[b + k] < 0;
+      v[k] = theirs ? (double)prod[(size_t)(b + k) * prod_stride(N3) + lane] : 0.0;
+    }
 #pragma unroll
     for (int k = 0; k < 8; ++k)
-      if (b + k < b1) acc = (b + k == b0) ? v[k] : acc + v[k];
+      if (b + k < b1 && sidx[b + k] < 0) acc = acc + v[k];
   }
   const double gp = gsel ? gtab[gsel[Pp]] : g;
   const double w = -acc + vs + vd * gp;
@@ -2395,7 +2413,8 @@ __global__ __launch_bounds__(64) void k_sym_products48(const float* __restrict__
                                                        const int* __restrict__ sidx,
                                                        const int* __restrict__ tdst,
                                                        const double* __restrict__ u,
-                                                       double* __restrict__ prod, int n_poly,
+                                                       double* __restrict__ prod,
+                                                       double* __restrict__ rowp, int n_poly,
                                                        const int* __restrict__ order) {
   constexpr int N3 = Dims32<P>::N3, NQ = Dims32<P>::NQ, NPR = Dims16<P>::NPR;
   constexpr int PS = prod_stride(N3);
@@ -2418,6 +2437,7 @@ __global__ __launch_bounds__(64) void k_sym_products48(const float* __restrict__
   const int lsi = lane < b1 - b0 ? sidx[b0 + lane] : 0;
   const int ltd = lane < b1 - b0 ? tdst[b0 + lane] : 0;
   if (row) sU[lane] = u[(size_t)Pp * N3 + lane];  // ordered before its reads by the chunks' syncs
+  double racc = 0.0;  // the row products of this wave's stored blocks, in block order
   for (int b = b0; b < b1; ++b) {
     const int j = b - b0;
     const int si = j < 64 ? __builtin_amdgcn_readlane(lsi, j) : __builtin_amdgcn_readfirstlane(sidx[b]);
@@ -2525,9 +2545,11 @@ __global__ __launch_bounds__(64) void k_sym_products48(const float* __restrict__
     }
     s = s + (double)sc * 0x1p-24;
     if (!pair) s = s + st;  // the strict upper triangle's transposed product
-    if (lane < PS) prod[(size_t)b * PS + lane] = row ? s : 0.0;  // zeros past the rows
+    asm volatile("" : "+v"(s));  // as in k_sym_products32
+    racc = racc + s;
     if (pair && lane < PS) prod[(size_t)td * PS + lane] = row ? st : 0.0;
   }
+  if (lane < PS) rowp[(size_t)Pp * PS + lane] = row ? racc : 0.0;  // zeros past the rows
 }
 
 // L2 error terms du_i^T M_modal du_i per (polyhedron, component)
