@@ -1390,7 +1390,7 @@ __global__ __launch_bounds__(64) void k_sym_products32(const float* __restrict__
   // stores its quads with 16-byte writes, and a column read over 20 lanes
   // touches 20 distinct banks
   __shared__ __align__(16) float sT[64 * CW];
-  __shared__ __align__(16) float sX[N3];  // u_Q of the current block, read by the row chains
+  __shared__ __align__(16) float sX[64];  // u_Q of the current block, read by the row chains
   const int lane = threadIdx.x;
   const bool row = lane < N3;
   const int rl = row ? lane : N3 - 1;  // lanes past the rows load the last row's (cached) lines
@@ -1415,14 +1415,13 @@ __global__ __launch_bounds__(64) void k_sym_products32(const float* __restrict__
     // out of the loop they would hold 60 masks in scalar registers
     int rb = rl;
     asm volatile("" : "+v"(rb));
+    // one load path for both kinds (an address select per quad), so that the
+    // chunks wait on their own quads only
     float4 kq[NQ];
-    if (pair) {
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) kq[q] = load_stream(Kb + ((size_t)q * N3 + rl) * 4);
-    } else {  // the upper triangle's rows only
-#pragma unroll
-      for (int q = 0; q < NQ; ++q)
-        kq[q] = load_stream(Kb + self32_off(q) + 4 * (rb < 4 * q + 3 ? rb : 4 * q + 3));
+    for (int q = 0; q < NQ; ++q)
+      kq[q] = load_stream(Kb + (pair ? (q * N3 + rb) * 4 : self32_off(q) + 4 * (rb < 4 * q + 3 ? rb : 4 * q + 3)));
+    if (!pair) {  // the upper triangle's rows only
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         if (4 * q + 0 < rb) kq[q].x = 0.0f;
@@ -1435,7 +1434,7 @@ __global__ __launch_bounds__(64) void k_sym_products32(const float* __restrict__
 #pragma unroll
     for (int h = 0; h < 3; ++h) {
       wave_lds_sync();  // the previous chunk's (and block's) reads are done
-      if (h == 0 && row) sX[lane] = xl;
+      if (h == 0) sX[lane] = xl;  // lanes past the rows write padding
 #pragma unroll
       for (int k = 0; k < CQ; ++k) {
         const float4 v = kq[h * CQ + k];
@@ -2426,7 +2425,7 @@ __global__ __launch_bounds__(64) void k_sym_products48(const float* __restrict__
   // rows): 16-byte writes, and a column read over 20 lanes hits 20 banks
   __shared__ __align__(16) float sH[64 * CW], sC[64 * CW];
   __shared__ __align__(16) double sU[N3];  // u_P, read by the column chains
-  __shared__ __align__(16) double sX[N3];  // u_Q of the current block, read by the row chains
+  __shared__ __align__(16) double sX[64];  // u_Q of the current block, read by the row chains
   const int lane = threadIdx.x;
   const bool row = lane < N3;
   const int rl = row ? lane : N3 - 1;  // lanes past the rows load the last row's (cached) lines
@@ -2452,35 +2451,25 @@ __global__ __launch_bounds__(64) void k_sym_products48(const float* __restrict__
     // out of the loop they would hold 60 masks in scalar registers
     int rb = rl;
     asm volatile("" : "+v"(rb));
+    // one load path for both kinds (an address select per load), so that the
+    // chunks wait on their own loads only
     float4 kq[NQ];
     lymph_hv8 tq[NPR];
-    if (pair) {
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) kq[q] = load_stream(Kb + ((size_t)q * N3 + rl) * 4);
+    for (int q = 0; q < NQ; ++q)
+      kq[q] = load_stream(Kb + (pair ? (q * N3 + rb) * 4 : self32_off(q) + 4 * (rb < 4 * q + 3 ? rb : 4 * q + 3)));
 #pragma unroll
-      for (int k = 0; k < NPR; ++k) {
-        if (2 * k + 1 < NQ) {
-          tq[k] = load_stream_h8(Rb + ((size_t)k * N3 + rl) * 8);
-        } else {  // the last quad of an odd count: 4 values per row
-          const lymph_hv4 h = load_stream_h4(Rb + Dims16<P>::HALF0 + (size_t)rl * 4);
-          tq[k] = lymph_hv8{h.x, h.y, h.z, h.w, 0, 0, 0, 0};
-        }
+    for (int k = 0; k < NPR; ++k) {
+      const int last = self16_rows(N3, k) - 1;
+      const int self_at = self16_off(N3, k) + self16_w(N3, k) * (rb < last ? rb : last);
+      if (2 * k + 1 < NQ) {
+        tq[k] = load_stream_h8(Rb + (pair ? (k * N3 + rb) * 8 : self_at));
+      } else {  // the last quad of an odd count: 4 values per row
+        const lymph_hv4 h = load_stream_h4(Rb + (pair ? (int)Dims16<P>::HALF0 + rb * 4 : self_at));
+        tq[k] = lymph_hv8{h.x, h.y, h.z, h.w, 0, 0, 0, 0};
       }
-    } else {  // the upper triangle's rows only
-#pragma unroll
-      for (int q = 0; q < NQ; ++q)
-        kq[q] = load_stream(Kb + self32_off(q) + 4 * (rb < 4 * q + 3 ? rb : 4 * q + 3));
-#pragma unroll
-      for (int k = 0; k < NPR; ++k) {
-        const int last = self16_rows(N3, k) - 1;
-        const _Float16* tp = Rb + self16_off(N3, k) + self16_w(N3, k) * (rb < last ? rb : last);
-        if (2 * k + 1 < NQ) {
-          tq[k] = load_stream_h8(tp);
-        } else {
-          const lymph_hv4 h = load_stream_h4(tp);
-          tq[k] = lymph_hv8{h.x, h.y, h.z, h.w, 0, 0, 0, 0};
-        }
-      }
+    }
+    if (!pair) {  // the upper triangle's rows only
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         if (4 * q + 0 < rb) kq[q].x = 0.0f;
@@ -2493,7 +2482,7 @@ __global__ __launch_bounds__(64) void k_sym_products48(const float* __restrict__
     // quads, each entry's hi and hi t going to LDS as they are formed, then
     // the chunk's columns run their chains
     wave_lds_sync();  // the previous block's reads of sX are done
-    if (row) sX[lane] = xl;
+    sX[lane] = xl;  // lanes past the rows write padding
     wave_lds_sync();
     double s = 0.0, st = 0.0;
     float sc = 0.0f;
