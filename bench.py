@@ -398,9 +398,9 @@ def pairs_once_tilde_bytes(ctx, bits):
 
 def degree_sweep(steps=200):
     """Step throughput at p = 1..4 on agglomerated cubes of about 4 M DOFs
-    (70^3, 52^3, 40^3, 33^3 cubes), FP64 and 32-bit apply, each autotuned,
-    FP64 with face pairs stored once at p <= 3; and the time of both operator
-    builds (keep_adg off, K and M only)."""
+    (70^3, 52^3, 40^3, 33^3 cubes), FP64, 32- and 48-bit apply, each
+    autotuned, FP64 with face pairs stored once at p <= 3; and the time of
+    both operator builds (keep_adg off, K and M only)."""
     import lymph3d_apsc_amd as L
     from lymph3d_apsc_amd import _lib, mesh as M
     rows = []
@@ -421,7 +421,7 @@ def degree_sweep(steps=200):
             ctx.make_rhs_free(L.PROB_CUBE_WAVE, _lib.SLOT_RHS_DYN)
             row = dict(p=p, n=n, n_poly=m.n_poly, dofs=m.n_poly * ctx.n3, n_blk=ctx.n_blk,
                        operator_build_s=build_s)
-            for bits in (64, 32):
+            for bits in (64, 32, 48):
                 ctx.set_apply_precision(bits)
                 chosen, _ = ctx.autotune(reps=10)
                 ms = ctx.time_steps(L.PROB_CUBE_WAVE, steps, dt, dt) / steps

@@ -225,7 +225,7 @@ void launch_leapfrog(int variant, int grid, hipStream_t st, const double* K, con
         constexpr int G = 64 / (3 * np_of(P));
         const int groups = (grid + G - 1) / G;
         k_leapfrog_packed<P, MODE, 64, W><<<step_grid((groups + W - 1) / W), 64 * W, 0, st>>>(
-            K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid);
+            K, nullptr, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid);
         return;
       } else if constexpr (StepVar<P>::xs[I] == 3) {
         constexpr int G = 64 / (3 * np_of(P));
@@ -277,23 +277,29 @@ template <> struct StepVar32<4> {
   static constexpr int pk[] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 template <int P> constexpr int n_var32() { return (int)(sizeof(StepVar32<P>::unr) / sizeof(int)); }
-// 48-bit apply (k_leapfrog48): column quads per load group, waves per polyhedron
+// 48-bit apply (k_leapfrog48): column quads per load group, waves per
+// polyhedron (pk = 1: the packed step, 64/3Np polyhedra per wave, waves per
+// workgroup; p <= 2)
 template <int P> struct StepVar48;
 template <> struct StepVar48<1> {  // quads per load group: even (the corrections come in quad pairs)
-  static constexpr int unr[] = {4, 4, 4};
-  static constexpr int wpp[] = {1, 2, 4};
+  static constexpr int unr[] = {4, 4, 4, 1, 1};
+  static constexpr int wpp[] = {1, 2, 4, 1, 4};
+  static constexpr int pk[] = {0, 0, 0, 1, 1};
 };
 template <> struct StepVar48<2> {
-  static constexpr int unr[] = {8, 8, 8};
-  static constexpr int wpp[] = {1, 2, 4};
+  static constexpr int unr[] = {8, 8, 8, 1, 1};
+  static constexpr int wpp[] = {1, 2, 4, 1, 4};
+  static constexpr int pk[] = {0, 0, 0, 1, 1};
 };
 template <> struct StepVar48<3> {
   static constexpr int unr[] = {6, 16, 6, 16, 6, 16};
   static constexpr int wpp[] = {1, 1, 2, 2, 4, 4};
+  static constexpr int pk[] = {0, 0, 0, 0, 0, 0};
 };
 template <> struct StepVar48<4> {
   static constexpr int unr[] = {14, 14, 14};
   static constexpr int wpp[] = {1, 2, 4};
+  static constexpr int pk[] = {0, 0, 0};
 };
 template <int P> constexpr int n_var48() { return (int)(sizeof(StepVar48<P>::unr) / sizeof(int)); }
 // lymph_autotune reports into caller arrays of 16 entries
@@ -318,7 +324,7 @@ void launch_leapfrog32(int variant, int grid, hipStream_t st, const float* K, co
         constexpr int G = 64 / (3 * np_of(P));
         const int groups = (grid + G - 1) / G;
         k_leapfrog_packed<P, MODE, 32, W><<<step_grid((groups + W - 1) / W), 64 * W, 0, st>>>(
-            K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid);
+            K, nullptr, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid);
         return;
       } else if constexpr (StepVar32<P>::pk[I] == 2) {
         constexpr int G = 64 / (3 * np_of(P));
@@ -356,14 +362,23 @@ void launch_leapfrog48(int variant, int grid, hipStream_t st, const float* K, co
   if constexpr (I < n_var48<P>()) {
     if (variant == I) {
       constexpr int W = StepVar48<P>::wpp[I];
-      if (W > 1 && dyn > kMaxDynLds) {
-        launch_leapfrog48<P, MODE, 0>(0, grid, st, K, Kr, rp, col, rs, rd, u, wa, out, g, c1, c2,
-                                      gsel, gtab, ord, dyn);
+      if constexpr (StepVar48<P>::pk[I] == 1) {
+        static_assert(3 * np_of(P) <= 32, "packed step needs 3Np <= 32");
+        constexpr int G = 64 / (3 * np_of(P));
+        const int groups = (grid + G - 1) / G;
+        k_leapfrog_packed<P, MODE, 48, W><<<step_grid((groups + W - 1) / W), 64 * W, 0, st>>>(
+            K, Kr, rp, col, nullptr, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid);
+        return;
+      } else {
+        if (W > 1 && dyn > kMaxDynLds) {
+          launch_leapfrog48<P, MODE, 0>(0, grid, st, K, Kr, rp, col, rs, rd, u, wa, out, g, c1, c2,
+                                        gsel, gtab, ord, dyn);
+          return;
+        }
+        k_leapfrog48<P, MODE, StepVar48<P>::unr[I], W><<<step_grid(grid), 64 * W, W > 1 ? dyn : 0, st>>>(
+            K, Kr, rp, col, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid, ord);
         return;
       }
-      k_leapfrog48<P, MODE, StepVar48<P>::unr[I], W><<<step_grid(grid), 64 * W, W > 1 ? dyn : 0, st>>>(
-          K, Kr, rp, col, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid, ord);
-      return;
     }
     launch_leapfrog48<P, MODE, I + 1>(variant, grid, st, K, Kr, rp, col, rs, rd, u, wa, out, g, c1,
                                       c2, gsel, gtab, ord, dyn);
@@ -1649,7 +1664,9 @@ int lymph_autotune(LymphCtx* c, int reps, int* n_variants, int* unr, int* xmode,
       if (unr)
         unr[v] = bits == 64 ? StepVar<P>::unr[v]
                             : (bits == 32 ? StepVar32<P>::unr[v] : StepVar48<P>::unr[v]);
-      if (xmode) xmode[v] = bits == 64 ? StepVar<P>::xs[v] : (bits == 32 ? 1 + StepVar32<P>::pk[v] : 1);
+      if (xmode)
+        xmode[v] = bits == 64 ? StepVar<P>::xs[v]
+                              : 1 + (bits == 32 ? StepVar32<P>::pk[v] : StepVar48<P>::pk[v]);
       if (wpp) wpp[v] = w;
       if (w > 1 && !fits) {  // would run variant 0: not a candidate
         if (ms) ms[v] = -1.0;

@@ -93,6 +93,16 @@ LYMPH_HD size_t kpos16(int N3, int r, int c) {
   }
   return half0 + (size_t)(nq % 2) * N3 * 4 + (size_t)r * (N3 % 4) + (c - c0);
 }
+// The corrections' loads: 8 (a quad pair's row) or 4 (the last quad's row)
+// halves, non-temporal like the operator's own.
+typedef _Float16 lymph_hv4 __attribute__((ext_vector_type(4)));
+typedef _Float16 lymph_hv8 __attribute__((ext_vector_type(8)));
+__device__ inline lymph_hv8 load_stream_h8(const _Float16* p) {
+  return __builtin_nontemporal_load(reinterpret_cast<const lymph_hv8*>(p));
+}
+__device__ inline lymph_hv4 load_stream_h4(const _Float16* p) {
+  return __builtin_nontemporal_load(reinterpret_cast<const lymph_hv4*>(p));
+}
 LYMPH_HD size_t kpos32(int N3, int r, int c) {
   const int c0 = N3 / 4 * 4;
   if (c < c0) return ((size_t)(c >> 2) * N3 + r) * 4 + (c & 3);
@@ -1582,10 +1592,12 @@ __device__ inline void lsync() {
 // neighbour's coefficients go through LDS (one segment per polyhedron); the
 // block products are added in block order and the solves and update are the
 // ones of k_leapfrog, so results are bit-identical to it.  TB = 64 (FP64
-// operator in column pairs) or 32 (mass-orthonormal FP32 operator in column
-// quads, no solves).
+// operator in column pairs), 32 (mass-orthonormal FP32 operator in column
+// quads, no solves) or 48 (the FP32 operator plus its 16-bit corrections Kr,
+// with k_leapfrog48's arithmetic in its order: bit-identical to it).
 template <int P, int MODE, int TB, int WPW = 1>
 __global__ __launch_bounds__(64 * WPW) void k_leapfrog_packed(const void* __restrict__ Kv,
+                                                        const _Float16* __restrict__ Kr,
                                                         const int* __restrict__ row_ptr,
                                                         const int* __restrict__ col,
                                                         const double* __restrict__ Rp,
@@ -1631,8 +1643,10 @@ __global__ __launch_bounds__(64 * WPW) void k_leapfrog_packed(const void* __rest
     // the block's loads and the next neighbour value are issued before the
     // barrier, so they overlap the exchange of this block's coefficients
     double2 kd[TB == 64 ? NPAIR : 1];
-    float4 kf[TB == 32 ? NQUAD : 1];
-    float kt[TB == 32 && TAIL > 0 ? TAIL : 1];
+    float4 kf[TB != 64 ? NQUAD : 1];
+    float kt[TB != 64 && TAIL > 0 ? TAIL : 1];
+    lymph_hv8 kr[TB == 48 ? (NQUAD + 1) / 2 : 1];
+    float krt[TB == 48 && TAIL > 0 ? TAIL : 1];
     if (on) {
       if constexpr (TB == 64) {
         const double* Kb = static_cast<const double*>(Kv) + (size_t)b * D::BS;
@@ -1643,6 +1657,22 @@ __global__ __launch_bounds__(64 * WPW) void k_leapfrog_packed(const void* __rest
 #pragma unroll
         for (int q = 0; q < NQUAD; ++q) kf[q] = load_stream(Kb + ((size_t)q * N3 + r) * 4);
         if constexpr (TAIL > 0) load_tail<TAIL>(Kb + (size_t)4 * NQUAD * N3 + (size_t)r * TAIL, kt);
+        if constexpr (TB == 48) {
+          const _Float16* Rb = Kr + (size_t)b * Dims16<P>::BS;
+#pragma unroll
+          for (int k = 0; k < (NQUAD + 1) / 2; ++k) {
+            if (2 * k + 1 < NQUAD) {
+              kr[k] = load_stream_h8(Rb + ((size_t)k * N3 + r) * 8);
+            } else {  // the last quad of an odd count: 4 values per row
+              const lymph_hv4 h = load_stream_h4(Rb + Dims16<P>::HALF0 + (size_t)r * 4);
+              kr[k] = lymph_hv8{h.x, h.y, h.z, h.w, 0, 0, 0, 0};
+            }
+          }
+          if constexpr (TAIL > 0) {
+#pragma unroll
+            for (int j = 0; j < TAIL; ++j) krt[j] = (float)Rb[Dims16<P>::TAIL0 + (size_t)r * TAIL + j];
+          }
+        }
       }
       sx[buf][sub][r] = xn;
       if (j + 1 < cnt) xn = u[(size_t)col[b + 1] * N3 + r];
@@ -1657,6 +1687,34 @@ __global__ __launch_bounds__(64 * WPW) void k_leapfrog_packed(const void* __rest
           s = s + kd[c2i].x * sx[buf][sub][2 * c2i];
           if (!odd_tail) s = s + kd[c2i].y * sx[buf][sub][2 * c2i + 1];
         }
+      } else if constexpr (TB == 48) {  // k_leapfrog48's chains
+        double sd = 0.0;
+        float sc = 0.0f;
+#pragma unroll
+        for (int q = 0; q < NQUAD; ++q) {
+          double xq[4];
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj) xq[jj] = (4 * q + jj < N3) ? sx[buf][sub][4 * q + jj] : 0.0;
+          const lymph_hv8 t8 = kr[q >> 1];
+          const int o4 = (q & 1) * 4;
+          sd = fma((double)kf[q].x, xq[0], sd);
+          sd = fma((double)kf[q].y, xq[1], sd);
+          sd = fma((double)kf[q].z, xq[2], sd);
+          sd = fma((double)kf[q].w, xq[3], sd);
+          sc = fmaf(kf[q].x * (float)t8[o4 + 0], (float)xq[0], sc);
+          sc = fmaf(kf[q].y * (float)t8[o4 + 1], (float)xq[1], sc);
+          sc = fmaf(kf[q].z * (float)t8[o4 + 2], (float)xq[2], sc);
+          sc = fmaf(kf[q].w * (float)t8[o4 + 3], (float)xq[3], sc);
+        }
+        if constexpr (TAIL > 0) {
+#pragma unroll
+          for (int j = 0; j < TAIL; ++j) {
+            const double xt = sx[buf][sub][4 * NQUAD + j];
+            sd = fma((double)kt[j], xt, sd);
+            sc = fmaf(kt[j] * krt[j], (float)xt, sc);
+          }
+        }
+        s = sd + (double)sc * 0x1p-24;
       } else {
         float sf = 0.0f;
 #pragma unroll
@@ -2225,14 +2283,6 @@ __global__ __launch_bounds__(64 * WPP) void k_leapfrog32(const float* __restrict
 // products added in block order in FP64 and k_leapfrog32's update.  6 bytes
 // per operator entry; the state stays within about 1e-9 of the FP64 step's
 // over 1000 steps where the 32-bit apply drifts to 1e-5.
-typedef _Float16 lymph_hv4 __attribute__((ext_vector_type(4)));
-typedef _Float16 lymph_hv8 __attribute__((ext_vector_type(8)));
-__device__ inline lymph_hv8 load_stream_h8(const _Float16* p) {
-  return __builtin_nontemporal_load(reinterpret_cast<const lymph_hv8*>(p));
-}
-__device__ inline lymph_hv4 load_stream_h4(const _Float16* p) {
-  return __builtin_nontemporal_load(reinterpret_cast<const lymph_hv4*>(p));
-}
 
 template <int P, int MODE, int UNR, int WPP>
 __global__ __launch_bounds__(64 * WPP) void k_leapfrog48(const float* __restrict__ K,
