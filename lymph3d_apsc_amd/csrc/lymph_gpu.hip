@@ -236,7 +236,7 @@ void launch_leapfrog(int variant, int grid, hipStream_t st, const double* K, con
         }
         const int groups = (grid + G - 1) / G;
         k_leapfrog_joint<P, MODE, 64, W><<<step_grid((groups + W - 1) / W), 64 * W, W * G * dyn, st>>>(
-            K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid, (int)(G * dyn / 8));
+            K, nullptr, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid, (int)(G * dyn / 8));
         return;
       } else {
       if (W > 1 && dyn > kMaxDynLds) {
@@ -279,17 +279,17 @@ template <> struct StepVar32<4> {
 template <int P> constexpr int n_var32() { return (int)(sizeof(StepVar32<P>::unr) / sizeof(int)); }
 // 48-bit apply (k_leapfrog48): column quads per load group, waves per
 // polyhedron (pk = 1: the packed step, 64/3Np polyhedra per wave, waves per
-// workgroup; p <= 2)
+// workgroup; pk = 2: the same with the joint block walk; p <= 2)
 template <int P> struct StepVar48;
 template <> struct StepVar48<1> {  // quads per load group: even (the corrections come in quad pairs)
-  static constexpr int unr[] = {4, 4, 4, 1, 1};
-  static constexpr int wpp[] = {1, 2, 4, 1, 4};
-  static constexpr int pk[] = {0, 0, 0, 1, 1};
+  static constexpr int unr[] = {4, 4, 4, 1, 1, 1, 1};
+  static constexpr int wpp[] = {1, 2, 4, 1, 4, 1, 4};
+  static constexpr int pk[] = {0, 0, 0, 1, 1, 2, 2};
 };
 template <> struct StepVar48<2> {
-  static constexpr int unr[] = {8, 8, 8, 1, 1};
-  static constexpr int wpp[] = {1, 2, 4, 1, 4};
-  static constexpr int pk[] = {0, 0, 0, 1, 1};
+  static constexpr int unr[] = {8, 8, 8, 1, 1, 1, 1};
+  static constexpr int wpp[] = {1, 2, 4, 1, 4, 1, 4};
+  static constexpr int pk[] = {0, 0, 0, 1, 1, 2, 2};
 };
 template <> struct StepVar48<3> {
   static constexpr int unr[] = {6, 16, 6, 16, 6, 16};
@@ -335,7 +335,7 @@ void launch_leapfrog32(int variant, int grid, hipStream_t st, const float* K, co
         }
         const int groups = (grid + G - 1) / G;
         k_leapfrog_joint<P, MODE, 32, W><<<step_grid((groups + W - 1) / W), 64 * W, W * G * dyn, st>>>(
-            K, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid, (int)(G * dyn / 8));
+            K, nullptr, rp, col, Rp, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid, (int)(G * dyn / 8));
         return;
       } else {
       if (W > 1 && dyn > kMaxDynLds) {
@@ -368,6 +368,18 @@ void launch_leapfrog48(int variant, int grid, hipStream_t st, const float* K, co
         const int groups = (grid + G - 1) / G;
         k_leapfrog_packed<P, MODE, 48, W><<<step_grid((groups + W - 1) / W), 64 * W, 0, st>>>(
             K, Kr, rp, col, nullptr, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid);
+        return;
+      } else if constexpr (StepVar48<P>::pk[I] == 2) {
+        static_assert(3 * np_of(P) <= 32, "joint packed step needs 3Np <= 32");
+        constexpr int G = 64 / (3 * np_of(P));
+        if ((size_t)W * G * dyn > kMaxDynLds) {
+          launch_leapfrog48<P, MODE, 0>(0, grid, st, K, Kr, rp, col, rs, rd, u, wa, out, g, c1, c2,
+                                        gsel, gtab, ord, dyn);
+          return;
+        }
+        const int groups = (grid + G - 1) / G;
+        k_leapfrog_joint<P, MODE, 48, W><<<step_grid((groups + W - 1) / W), 64 * W, W * G * dyn, st>>>(
+            K, Kr, rp, col, nullptr, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, grid, (int)(G * dyn / 8));
         return;
       } else {
         if (W > 1 && dyn > kMaxDynLds) {

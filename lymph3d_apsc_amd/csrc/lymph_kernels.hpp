@@ -1777,10 +1777,12 @@ __global__ __launch_bounds__(64 * WPW) void k_leapfrog_packed(const void* __rest
 // streams G consecutive blocks and no cache line is left half read for a
 // later iteration (32-bit blocks of 576 B at p = 1 end mid-line).  Each
 // block's product goes to LDS; each row then adds its own blocks in block
-// order, so results are bit-identical to k_leapfrog.  Dynamic LDS: the
-// products of the wave's longest block range (G x longest row x 3Np doubles).
+// order, so results are bit-identical to k_leapfrog (TB = 48: to
+// k_leapfrog48, with Kr its 16-bit corrections).  Dynamic LDS: the products
+// of the wave's longest block range (G x longest row x 3Np doubles).
 template <int P, int MODE, int TB, int WPW = 1>
 __global__ __launch_bounds__(64 * WPW) void k_leapfrog_joint(const void* __restrict__ Kv,
+                                                       const _Float16* __restrict__ Kr,
                                                        const int* __restrict__ row_ptr,
                                                        const int* __restrict__ col,
                                                        const double* __restrict__ Rp,
@@ -1826,8 +1828,10 @@ __global__ __launch_bounds__(64 * WPW) void k_leapfrog_joint(const void* __restr
     const bool on = sub < G && bj < B1;
     const int buf = j & 1;
     double2 kd[TB == 64 ? NPAIR : 1];
-    float4 kf[TB == 32 ? NQUAD : 1];
-    float kt[TB == 32 && TAIL > 0 ? TAIL : 1];
+    float4 kf[TB != 64 ? NQUAD : 1];
+    float kt[TB != 64 && TAIL > 0 ? TAIL : 1];
+    lymph_hv8 kr[TB == 48 ? (NQUAD + 1) / 2 : 1];
+    float krt[TB == 48 && TAIL > 0 ? TAIL : 1];
     if (on) {
       if constexpr (TB == 64) {
         const double* Kb = static_cast<const double*>(Kv) + (size_t)bj * D::BS;
@@ -1838,6 +1842,22 @@ __global__ __launch_bounds__(64 * WPW) void k_leapfrog_joint(const void* __restr
 #pragma unroll
         for (int q = 0; q < NQUAD; ++q) kf[q] = load_stream(Kb + ((size_t)q * N3 + r) * 4);
         if constexpr (TAIL > 0) load_tail<TAIL>(Kb + (size_t)4 * NQUAD * N3 + (size_t)r * TAIL, kt);
+        if constexpr (TB == 48) {
+          const _Float16* Rb = Kr + (size_t)bj * Dims16<P>::BS;
+#pragma unroll
+          for (int kk = 0; kk < (NQUAD + 1) / 2; ++kk) {
+            if (2 * kk + 1 < NQUAD) {
+              kr[kk] = load_stream_h8(Rb + ((size_t)kk * N3 + r) * 8);
+            } else {  // the last quad of an odd count: 4 values per row
+              const lymph_hv4 h = load_stream_h4(Rb + Dims16<P>::HALF0 + (size_t)r * 4);
+              kr[kk] = lymph_hv8{h.x, h.y, h.z, h.w, 0, 0, 0, 0};
+            }
+          }
+          if constexpr (TAIL > 0) {
+#pragma unroll
+            for (int jj = 0; jj < TAIL; ++jj) krt[jj] = (float)Rb[Dims16<P>::TAIL0 + (size_t)r * TAIL + jj];
+          }
+        }
       }
       sx[buf][sub][r] = xn;
       if (bj + G < B1) xn = u[(size_t)col[bj + G] * N3 + r];
@@ -1852,6 +1872,34 @@ __global__ __launch_bounds__(64 * WPW) void k_leapfrog_joint(const void* __restr
           s = s + kd[c2i].x * sx[buf][sub][2 * c2i];
           if (!odd_tail) s = s + kd[c2i].y * sx[buf][sub][2 * c2i + 1];
         }
+      } else if constexpr (TB == 48) {  // k_leapfrog48's chains
+        double sd = 0.0;
+        float sc = 0.0f;
+#pragma unroll
+        for (int q = 0; q < NQUAD; ++q) {
+          double xq[4];
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj) xq[jj] = sx[buf][sub][4 * q + jj];
+          const lymph_hv8 t8 = kr[q >> 1];
+          const int o4 = (q & 1) * 4;
+          sd = fma((double)kf[q].x, xq[0], sd);
+          sd = fma((double)kf[q].y, xq[1], sd);
+          sd = fma((double)kf[q].z, xq[2], sd);
+          sd = fma((double)kf[q].w, xq[3], sd);
+          sc = fmaf(kf[q].x * (float)t8[o4 + 0], (float)xq[0], sc);
+          sc = fmaf(kf[q].y * (float)t8[o4 + 1], (float)xq[1], sc);
+          sc = fmaf(kf[q].z * (float)t8[o4 + 2], (float)xq[2], sc);
+          sc = fmaf(kf[q].w * (float)t8[o4 + 3], (float)xq[3], sc);
+        }
+        if constexpr (TAIL > 0) {
+#pragma unroll
+          for (int jj = 0; jj < TAIL; ++jj) {
+            const double xt = sx[buf][sub][4 * NQUAD + jj];
+            sd = fma((double)kt[jj], xt, sd);
+            sc = fmaf(kt[jj] * krt[jj], (float)xt, sc);
+          }
+        }
+        s = sd + (double)sc * 0x1p-24;
       } else {
         float sf = 0.0f;
 #pragma unroll
