@@ -206,13 +206,14 @@ int lymph_time_steps(LymphCtx* ctx, int problem, int nsteps, double t0,
 
 /* Time every variant of the step kernel for `reps` steps on copies of the
  * state slots, keep the fastest for this context and report them: for
- * variant i, unr[i] = column pairs (quads for the 32-bit apply) loaded per
- * unrolled group, xmode[i] = how the neighbour's coefficients reach the lanes
- * (0 scalar loads, 1 vector load + lane broadcast, 2 several polyhedra per
- * wave with the coefficients exchanged through LDS, p <= 2, 3 the same with the
- * wave's polyhedra walking their blocks jointly), wpp[i] = waves sharing one
- * polyhedron's blocks (their block products are added in block order, so
- * results do not depend on it), ms[i] = milliseconds per step (-1: not run,
+ * variant i, unr[i] = column pairs (quads for the 32- and 48-bit applies)
+ * loaded per unrolled group, xmode[i] = how the neighbour's coefficients
+ * reach the lanes (0 scalar loads, 1 vector load + lane broadcast, 2 several
+ * polyhedra per wave with the coefficients exchanged through LDS, p <= 2, 3
+ * the same with the wave's polyhedra walking their blocks jointly), wpp[i] =
+ * waves sharing one polyhedron's blocks (their block products are added in
+ * block order, so results do not depend on it; for xmode 2 and 3, waves per
+ * workgroup, each with its own polyhedra), ms[i] = milliseconds per step (-1: not run,
  * the longest block row does not fit the variant's LDS); arrays of at least
  * 16 entries, any may be NULL; *chosen = index of the variant kept.  No
  * reference counterpart (the reference has one CPU loop); results do not
