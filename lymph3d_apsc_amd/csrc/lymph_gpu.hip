@@ -509,10 +509,26 @@ int launch_step(LymphCtx* c, int mode, double g, double c1, double c2, const dou
     } else if (c->Ks) {
       if constexpr (3 * np_of(P) <= 64 && (3 * np_of(P)) % 2 == 0) {
         const int n_pair = (c->n_blk - c->n_poly) / 2;
-        const int self_per = (c->n_poly + 7) / 8, pair_per = (n_pair + 7) / 8;
-        k_sym_products<P><<<8u * (unsigned)(self_per + pair_per), 64, 0, c->stream>>>(
-            c->Ks, c->ssrc, c->pinfo, u, c->prod, c->n_poly, n_pair, self_per, pair_per);
-        if (int rc = check_launch("k_sym_products")) return rc;
+        if constexpr (3 * np_of(P) <= 32) {  // several stored blocks per wave
+          constexpr int G = 64 / (3 * np_of(P));
+          const int waves = (c->n_poly + n_pair + G - 1) / G;
+          k_sym_products_packed<P><<<step_grid(waves), 64, 0, c->stream>>>(
+              c->Ks, c->ssrc, c->pinfo, u, c->prod, c->n_poly, c->n_poly + n_pair);
+          if (int rc = check_launch("k_sym_products_packed")) return rc;
+          const unsigned ug = step_grid((c->n_poly + G - 1) / G);
+          if (mode == 0)
+            k_sym_update_packed<P, 0><<<ug, 64, 0, c->stream>>>(c->prod, c->md.row_ptr, c->Sinv, rs, rd, u,
+                                                                wa, out, g, c1, c2, gsel, gtab, c->n_poly);
+          else
+            k_sym_update_packed<P, 1><<<ug, 64, 0, c->stream>>>(c->prod, c->md.row_ptr, c->Sinv, rs, rd, u,
+                                                                wa, out, g, c1, c2, gsel, gtab, c->n_poly);
+          return check_launch("k_sym_update_packed");
+        } else {
+          const int self_per = (c->n_poly + 7) / 8, pair_per = (n_pair + 7) / 8;
+          k_sym_products<P><<<8u * (unsigned)(self_per + pair_per), 64, 0, c->stream>>>(
+              c->Ks, c->ssrc, c->pinfo, u, c->prod, c->n_poly, n_pair, self_per, pair_per);
+          if (int rc = check_launch("k_sym_products")) return rc;
+        }
         if (mode == 0)
           k_sym_update<P, 0><<<step_grid(c->n_poly), 64, 0, c->stream>>>(
               c->prod, c->md.row_ptr, c->Sinv, rs, rd, u, wa, out, g, c1, c2, gsel, gtab, c->n_poly,

@@ -1218,6 +1218,96 @@ __global__ __launch_bounds__(64) void k_sym_products(const double* __restrict__ 
   sym_block<P>(self, Ks + (size_t)sb * D::BS + 2 * lane, q.x, q.y, q.z, q.w, u, prod, sT, lane);
 }
 
+// k_sym_products at small degree (3Np <= 32: p = 1, 2): one wave carries
+// G = 64 / 3Np stored blocks, lane = (block, row), so short rows do not leave
+// lanes idle.  Per block the sums of sym_block in its order: the row product
+// K_PQ u_Q (FMA-free chain over ascending columns, u_Q broadcast from LDS),
+// then the block's products K_PQ[i][c] u_P[i] in LDS at once (a self block's
+// diagonal zeroed) and lane (block, c) adding column c over ascending rows;
+// so the two kernels agree bit for bit.  Stores cover the rows only (the
+// update reads no padding).
+template <int P>
+__global__ __launch_bounds__(64) void k_sym_products_packed(const double* __restrict__ Ks,
+                                                            const int* __restrict__ ssrc,
+                                                            const int4* __restrict__ pinfo,
+                                                            const double* __restrict__ u,
+                                                            double* __restrict__ prod, int n_poly,
+                                                            int n_stored) {
+  using D = Dims<P>;
+  constexpr int N3 = D::N3, NPAIR = D::NPAIR, PS = prod_stride(N3);
+  static_assert(N3 <= 32 && N3 % 2 == 0, "packed stored blocks need 3Np <= 32, whole column pairs");
+  static_assert(self_upper_off(NPAIR) <= D::BS, "packed self block fits its slot");
+  constexpr int G = 64 / N3;
+  __shared__ double sX[G][N3];          // u_Q of each block
+  __shared__ double sT[G][N3][N3 + 1];  // K_PQ[i][c] u_P[i]; odd row stride
+  const int lane = threadIdx.x, sub = lane / N3, r = lane % N3;
+  // XCD-aware: workgroups are dealt to the XCDs round robin; XCD x takes a
+  // contiguous eighth of the waves, hence of the stored blocks
+  const int per = (int)(gridDim.x >> 3);
+  const int w = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+  const int sb = w * G + sub;
+  const bool on = sub < G && sb < n_stored;
+  const bool self = sb < n_poly;
+  int b = 0, Pp = 0, Qq = 0, td = 0;
+  if (on) {
+    if (self) {
+      b = ssrc[sb];
+      Pp = Qq = sb;
+    } else {
+      const int4 q = pinfo[sb - n_poly];
+      b = q.x;
+      Pp = q.y;
+      Qq = q.z;
+      td = q.w;
+    }
+  }
+  const double* __restrict__ Kb = Ks + (size_t)(on ? sb : 0) * D::BS;
+  double2 kv[NPAIR];
+#pragma unroll
+  for (int c2i = 0; c2i < NPAIR; ++c2i) {
+    const bool ld = on && (!self || r <= 2 * c2i + 1);
+    kv[c2i] = ld ? load_stream(Kb + (self ? self_upper_off(c2i) + 2 * r : ((size_t)c2i * N3 + r) * 2))
+                 : make_double2(0.0, 0.0);
+    if (self && r == 2 * c2i + 1) kv[c2i].x = 0.0;  // K_PP[r][r - 1]: below the diagonal
+  }
+  const double xl = on ? u[(size_t)Qq * N3 + r] : 0.0;
+  const double up = on ? u[(size_t)Pp * N3 + r] : 0.0;
+  if (on) sX[sub][r] = xl;
+  __syncthreads();
+  double s = 0.0;
+  if (on) {
+#pragma unroll
+    for (int c2i = 0; c2i < NPAIR; ++c2i) {
+      s = s + kv[c2i].x * sX[sub][2 * c2i];
+      s = s + kv[c2i].y * sX[sub][2 * c2i + 1];
+    }
+  }
+  if (on) {
+#pragma unroll
+    for (int c2i = 0; c2i < NPAIR; ++c2i) {
+      sT[sub][r][2 * c2i] = kv[c2i].x * up;
+      sT[sub][r][2 * c2i + 1] = kv[c2i].y * up;
+    }
+    if (self) sT[sub][r][r] = 0.0;  // the diagonal is in the row product only
+  }
+  __syncthreads();
+  double st = 0.0;
+  if (on) {
+    double v = 0.0;  // column r of this block over ascending rows
+#pragma unroll
+    for (int i = 0; i < N3; ++i) v = v + sT[sub][i][r];
+    st = v;
+  }
+  if (on) {
+    if (self) {
+      prod[(size_t)b * PS + r] = s + st;
+    } else {
+      prod[(size_t)b * PS + r] = s;
+      prod[(size_t)td * PS + r] = st;
+    }
+  }
+}
+
 // K_PP's upper triangle in the packed layout of k_sym_products, one thread
 // per stored entry: the self blocks listed in self_s (stored index s, taken
 // from full-row slot src[s]).
@@ -1307,6 +1397,82 @@ __global__ __launch_bounds__(64) void k_sym_update(const double* __restrict__ pr
       out[o + lane] = c1 * w + 2.0 * vu - va;
     else
       out[o + lane] = c1 * w + vu + c2 * va;
+  }
+}
+
+// k_sym_update at small degree (3Np <= 32: p = 1, 2): G = 64 / 3Np
+// polyhedra per wave, lane = (polyhedron, row), each with its own S and
+// vector in LDS; the sums of k_sym_update in its order, so the two agree bit
+// for bit.
+template <int P, int MODE>
+__global__ __launch_bounds__(64) void k_sym_update_packed(const double* __restrict__ prod,
+                                                          const int* __restrict__ row_ptr,
+                                                          const double* __restrict__ Sp,
+                                                          const double* __restrict__ rs,
+                                                          const double* __restrict__ rd,
+                                                          const double* __restrict__ u,
+                                                          const double* __restrict__ wa,
+                                                          double* __restrict__ out, double g,
+                                                          double c1, double c2,
+                                                          const int* __restrict__ gsel,
+                                                          const double* __restrict__ gtab, int n_poly) {
+  using D = Dims<P>;
+  constexpr int NP = D::NP, N3 = D::N3;
+  static_assert(N3 <= 32, "packed update needs 3Np <= 32");
+  static_assert(MODE == 0 || MODE == 1, "leapfrog and first step");
+  constexpr int G = 64 / N3;
+  __shared__ double sS[G][D::NPK];
+  __shared__ double sv[G][N3];
+  const int lane = threadIdx.x, sub = lane / N3, r = lane % N3;
+  const int P0 = step_poly() * G;
+  const int Pp = P0 + sub;
+  const bool act = sub < G && Pp < n_poly;
+  for (int k = lane; k < G * D::NPK; k += 64) {
+    const int sp = k / D::NPK, e = k % D::NPK;
+    if (P0 + sp < n_poly) sS[sp][e] = Sp[(size_t)(P0 + sp) * D::NPK + e];
+  }
+  const size_t o = (size_t)Pp * N3 + r;
+  double vs = 0.0, vd = 0.0, vu = 0.0, va = 0.0, acc = 0.0;
+  if (act) {
+    vs = rs[o];
+    vd = rd[o];
+    vu = u[o];
+    va = wa[o];
+    const int b0 = row_ptr[Pp], b1 = row_ptr[Pp + 1];
+    for (int b = b0; b < b1; b += 8) {  // the row's products, added in block order
+      double v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v[k] = b + k < b1 ? prod[(size_t)(b + k) * prod_stride(N3) + r] : 0.0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (b + k < b1) acc = (b + k == b0) ? v[k] : acc + v[k];
+    }
+  }
+  const int cc = r / NP, ii = r % NP;
+  const double* __restrict__ vc = &sv[act ? sub : 0][cc * NP];  // this lane's component
+  if (act) {
+    const double gp = gsel ? gtab[gsel[Pp]] : g;
+    sv[sub][r] = -acc + vs + vd * gp;
+  }
+  __syncthreads();  // sS, sv
+  double z = 0.0;  // z = S^T w: z_i = sum_{j <= i} S_ji w_j
+  if (act) {
+#pragma unroll
+    for (int j = 0; j < NP; ++j)
+      if (j <= ii) z = z + sS[sub][pk_off(NP, j) + (ii - j)] * vc[j];
+  }
+  __syncthreads();
+  if (act) sv[sub][r] = z;
+  __syncthreads();
+  if (act) {  // M^-1 w = S z: x_i = sum_{j >= i} S_ij z_j
+    double w = 0.0;
+#pragma unroll
+    for (int j = 0; j < NP; ++j)
+      if (j >= ii) w = w + sS[sub][pk_off(NP, ii) + (j - ii)] * vc[j];
+    if (MODE == 0)
+      out[o] = c1 * w + 2.0 * vu - va;
+    else
+      out[o] = c1 * w + vu + c2 * va;
   }
 }
 
