@@ -381,3 +381,29 @@ def test_48bit_pairs_once_long_run_drift_and_capture():
     assert abs(d_pairs - d_full) < 0.05 * d_full, (d_full, d_pairs)
     assert np.abs(u_eager).max() > 0
     assert rel(u_graph, u_eager) < 1e-12, rel(u_graph, u_eager)
+
+
+def test_48bit_pairs_once_survives_release():
+    """With the 48-bit apply on stored pairs, lymph_release_fp64_operator
+    frees the FP64 blocks and leaves the step's bits unchanged (the stored
+    FP32 blocks and corrections are kept); switching the storage off then
+    falls back to the 48-bit full rows, which were kept too."""
+    say("[symmetric] 48-bit pairs once after releasing the FP64 blocks")
+    m, p = M.cube_mesh(3, agglomerate=True), 3
+    rng = np.random.default_rng(21)
+    with L.Context(m, p) as ctx:
+        _prepared(ctx)
+        n = m.n_poly * ctx.n3
+        u0, u1 = rng.standard_normal(n), rng.standard_normal(n)
+        ctx.set_apply_precision(48)
+        full48 = ctx.time_step_matrix_free(0.37, 1e-3, u0, u1)
+        ctx.set_symmetric_storage(True, fp32=True)
+        a = ctx.time_step_matrix_free(0.37, 1e-3, u0, u1)
+        before = ctx.device_bytes
+        ctx.release_fp64_operator()
+        assert ctx.device_bytes < before
+        assert np.array_equal(ctx.time_step_matrix_free(0.37, 1e-3, u0, u1), a)
+        ctx.set_symmetric_storage(False)
+        assert np.array_equal(ctx.time_step_matrix_free(0.37, 1e-3, u0, u1), full48)
+        with pytest.raises(L.LymphError):
+            ctx.set_symmetric_storage(True, fp32=True)  # the pair check needs the FP64 blocks
