@@ -10,8 +10,14 @@ matrix-free path, on the GPU:
 With monitors (the deck's SAVEOUT path, :400-406 and :441) the loop also
 writes the EnSight monitor files every TMONITOR steps.
 
+With --precision 32 or 48 the operator is applied by the 32- or 48-bit
+apply (FP32 blocks in the mass-orthonormal basis; 48: plus a 16-bit
+correction per entry, FP64 sums); with --pairs-once each face pair's block is
+stored once (a symmetric operator: SIPG and the deck's forced single
+material; at p = 3 also for the 32- and 48-bit applies).
+
 Usage: python -m lymph3d_apsc_amd.driver <deck folder> [--max-steps N]
-           [--monitor DIR [--every N]]
+           [--monitor DIR [--every N]] [--precision 64|32|48] [--pairs-once]
 """
 import argparse
 import sys
@@ -84,9 +90,12 @@ def run_monitored(ctx, problem, dt, stop_time, every, out, max_steps=-1, strict=
 
 
 def run_deck(folder, max_steps=-1, p=None, verbose=True, problem=_lib.PROB_REF_MMS, monitor=None,
-             every=None):
+             every=None, precision=64, pairs_once=False):
     """Run a deck; with `monitor` a folder, also write the EnSight monitor
-    files there every `every` steps (default: the deck's TMONITOR)."""
+    files there every `every` steps (default: the deck's TMONITOR).
+    `precision` 32 or 48 selects that operator apply and `pairs_once` the
+    storage of each face pair's block once, both set up after the operator
+    build; the error norms stay FP64."""
     mesh = read_deck(folder)
     hmax = mesh.hmax()
     if not compute_cfl(hmax, mesh.dt):
@@ -94,6 +103,12 @@ def run_deck(folder, max_steps=-1, p=None, verbose=True, problem=_lib.PROB_REF_M
     deg = mesh.degree if p is None else p
     t0 = time.perf_counter()
     with Context(mesh, deg) as ctx:
+        if precision != 64 or pairs_once:
+            ctx.make_matrices_free()
+            if pairs_once:
+                ctx.set_symmetric_storage(True, fp32=precision != 64)
+            if precision != 64:
+                ctx.set_apply_precision(precision)
         if monitor:
             res = run_monitored(ctx, problem, mesh.dt, mesh.stop_time,
                                 int(every or mesh.monitor_every or 1), monitor, max_steps)
@@ -102,6 +117,7 @@ def run_deck(folder, max_steps=-1, p=None, verbose=True, problem=_lib.PROB_REF_M
     res["wall_s"] = time.perf_counter() - t0
     res["hmax"] = hmax
     res["dofs"] = mesh.n_dof(deg)
+    res["precision"], res["pairs_once"] = precision, bool(pairs_once)
     if verbose:
         print(" GRID SIZE: ", repr(hmax))
         print(" Global number of degrees of freedom: ", res["dofs"])
@@ -118,8 +134,13 @@ def main(argv=None):
     ap.add_argument("--degree", type=int, default=None)
     ap.add_argument("--monitor", default=None, help="folder for the EnSight monitor files")
     ap.add_argument("--every", type=int, default=None, help="steps between exports (default TMONITOR)")
+    ap.add_argument("--precision", type=int, choices=(64, 32, 48), default=64,
+                    help="operator apply: FP64 (default), 32-bit, or 48-bit (FP32 + 16-bit corrections)")
+    ap.add_argument("--pairs-once", action="store_true",
+                    help="store each face pair's block once (symmetric operators)")
     a = ap.parse_args(argv)
-    run_deck(a.deck, a.max_steps, a.degree, monitor=a.monitor, every=a.every)
+    run_deck(a.deck, a.max_steps, a.degree, monitor=a.monitor, every=a.every, precision=a.precision,
+             pairs_once=a.pairs_once)
     return 0
 
 
