@@ -644,6 +644,7 @@ def main():
                              "traffic_over_algorithmic": (traffic32s / B32s) if traffic32s else None},
                 "rel_diff_vs_full_rows": float(np.abs(u32s - u32).max() / np.abs(u32).max()),
                 "rel_l2_diff_vs_fp64": float(np.linalg.norm(u32s - u) / np.linalg.norm(u)),
+                "err_l2": float(np.sqrt(ctx.compute_error_l2_matrix_free(u32s, uex))),
             }
             ctx.set_symmetric_storage(False)
         log(f"fp32 apply: {ms32_step:.3f} ms/step, rel diff vs fp64 {res['fp32_apply']['rel_l2_diff_vs_fp64']:.2e}")
@@ -698,6 +699,7 @@ def main():
                              "traffic_over_algorithmic": (traffic48s / B48s) if traffic48s else None},
                 "rel_diff_vs_full_rows": float(np.abs(u48s - u48).max() / np.abs(u48).max()),
                 "rel_l2_diff_vs_fp64": float(np.linalg.norm(u48s - u) / np.linalg.norm(u)),
+                "err_l2": float(np.sqrt(ctx.compute_error_l2_matrix_free(u48s, uex))),
                 "l2_diff_vs_fp64_over_fp64_err_l2": l2_48s / e_l2,
             }
             ctx.set_symmetric_storage(False)
